@@ -1,0 +1,156 @@
+/*
+ * szx_amd.h -- C ABI of libszx_amd.so, an MI355X (gfx950) implementation of
+ * SZx's blockwise fast codec (fastMode = SZx_WITH_BLOCK_FAST_CMPR).
+ *
+ * Two layers:
+ *
+ *  1. Drop-in host API.  Same names, argument meaning and ownership as the
+ *     reference's szx.h; a caller relinks against libszx_amd.so unchanged.
+ *     Host pointers in, malloc'd host buffers out (release with free()).
+ *     fastMode 2 (and 3/4, which the reference routes to the same blocked
+ *     compressor family) runs on the GPU.  Failures return NULL / 0 and set
+ *     szx_amd_last_error() instead of exiting the process.
+ *
+ *  2. Device-resident API (szx_amd_*_dev).  Device pointers, a caller
+ *     workspace and a HIP stream; nothing is allocated, nothing is copied to
+ *     the host, and every call is asynchronous on `stream`.
+ *
+ * Streams are byte-identical to the reference CPU codec at the same
+ * parameters (szx_float.cc:1059-1139, szx_double.cc:1160-1238) and decode
+ * bit-identically to the reference decoder (szxd_float.cc:247-311,
+ * szxd_double.cc:452-516).
+ */
+#ifndef SZX_AMD_H
+#define SZX_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- constants (szx_defines.h:19-21, 36-37, 54-57, 60-66) ------------------ */
+#ifndef SZx_FLOAT
+#define ABS 0
+#define REL 1
+#define FXR 2
+#define SZx_FLOAT 0
+#define SZx_DOUBLE 1
+#define SZx_NO_BLOCK_FAST_CMPR 1
+#define SZx_WITH_BLOCK_FAST_CMPR 2
+#define SZx_RANDOMACCESS_FAST_CMPR 3
+#define SZx_OPENMP_FAST_CMPR 4
+#define SZx_SCES 0
+#define SZx_NSCS -1
+#define SZx_FERR -2
+#define SZx_TERR -3
+#define SZx_DERR -4
+#define SZx_MERR -5
+#define SZx_BERR -6
+#endif
+
+/* ---- 1. drop-in host API ---------------------------------------------------- */
+
+/* szx.h:75 / szx.cc:73-101.  Product of the non-zero dims, r1 fastest. */
+size_t SZx_computeDataLength(size_t r5, size_t r4, size_t r3, size_t r2, size_t r1);
+
+/* szx.h:77-78 / szx.cc:231-348.  Compress a host array.  blockSize is 128,
+ * as in the reference.  errBoundMode ABS uses absErrBound, REL uses
+ * relBoundRatio * (max - min) computed with the reference's float rounding,
+ * FXR (fixed rate) picks the bound for `compressionRatio` within `tolerance`.
+ * Returns a malloc'd stream and its size in *outSize; NULL on error. */
+unsigned char *SZx_fast_compress_args(int fastMode, int dataType, void *data, size_t *outSize,
+                                      int errBoundMode, float absErrBound, float relBoundRatio,
+                                      float compressionRatio, float tolerance, size_t r5,
+                                      size_t r4, size_t r3, size_t r2, size_t r1);
+
+/* szx.h:80 / szx.cc:378-430.  Decompress a fastMode-2 stream into a malloc'd
+ * host array of SZx_computeDataLength(r5..r1) values.  byteLength bounds
+ * every read (the reference ignores it).  NULL on a malformed stream. */
+void *SZx_fast_decompress(int fastMode, int dataType, unsigned char *bytes, size_t byteLength,
+                          size_t r5, size_t r4, size_t r3, size_t r2, size_t r1);
+
+/* szx.h:89 / szx.cc:466-473.  Blocked compressor with an explicit blockSize
+ * in [1, 256] (256 is stored as the byte 0, as the reference's (u8) cast
+ * does).  absErrBound is absolute. */
+unsigned char *SZx_fast_compress_args_unpredictable_blocked(void *oriData, int dataType,
+                                                            size_t *outSize, float absErrBound,
+                                                            size_t nbEle, int blockSize);
+
+/* szx.h:98 / szx.cc:514-524.  newData is really T**: *newData receives a
+ * malloc'd array.  Unlike the reference (whose fp64 branch is a no-op) both
+ * float and double are decoded.  The stream length is not passed in this
+ * signature, so it is read from the stream itself (header, bitmap, records). */
+void SZx_fast_decompress_args_unpredictable_blocked(void *newData, int dataType, size_t nbEle,
+                                                    unsigned char *cmpBytes);
+
+/* szx.h:81 / szx.cc:432-438.  max - min of the array (radius and median are
+ * returned when the pointers are non-NULL, as float). */
+double SZx_computeValueRange(void *oriData, int dataType, size_t length, float *radius,
+                             float *medianValue);
+
+/* szx.h:87 / szx.cc:458-464.  Error bound whose compression ratio is within
+ * tolerance of targetCompressionRatio (the reference's sampled model search,
+ * run here without its process exit). */
+float SZx_estimateErrorBoundbasedonCR(int dataType, float targetCompressionRatio, float tolerance,
+                                      void *data, float initErrorBound, int blockSize,
+                                      size_t nbEle);
+
+/* Aliases under the pre-SZx names used by older callers
+ * (example/testfloat_compress_fastmode2.cu:78). */
+unsigned char *SZ_fast_compress_args(int fastMode, int dataType, void *data, size_t *outSize,
+                                     int errBoundMode, float absErrBound, float relBoundRatio,
+                                     float compressionRatio, float tolerance, size_t r5, size_t r4,
+                                     size_t r3, size_t r2, size_t r1);
+void *SZ_fast_decompress(int fastMode, int dataType, unsigned char *bytes, size_t byteLength,
+                         size_t r5, size_t r4, size_t r3, size_t r2, size_t r1);
+
+/* ---- 2. device-resident API ------------------------------------------------ */
+
+/* Error text of the last failing call on this thread ("" if none). */
+const char *szx_amd_last_error(void);
+
+/* Largest stream the codec can emit for n values (header, bitmap and a full
+ * record for every block). */
+size_t szx_amd_compress_bound(size_t n, int dataType, int blockSize);
+
+/* Capacity d_out must have for szx_amd_compress_dev: the stream is built in
+ * place and its start floats inside the buffer (see below). */
+size_t szx_amd_compress_capacity(size_t n, int dataType, int blockSize);
+
+/* Workspace bytes for szx_amd_compress_dev. */
+size_t szx_amd_compress_workspace(size_t n, int dataType, int blockSize);
+
+/* Compress n device values.  errBoundMode ABS or REL (REL's bound is computed
+ * on the device with the reference's float rounding).  On completion
+ * d_result[0] = byte offset of the stream inside d_out, d_result[1] = stream
+ * size, d_result[2] = the absolute error bound used (as float bits).
+ * d_result is device (or host-pinned) memory of 3 uint64_t.
+ * Returns 0, or a negative SZx_* code (argument errors only; the call is
+ * asynchronous). */
+int szx_amd_compress_dev(const void *d_in, size_t n, int dataType, int errBoundMode,
+                         float absErrBound, float relBoundRatio, int blockSize, void *d_out,
+                         size_t outCapacity, void *d_work, size_t workBytes, uint64_t *d_result,
+                         void *hipStream);
+
+/* Workspace bytes for szx_amd_decompress_dev. */
+size_t szx_amd_decompress_workspace(size_t n, int dataType, int blockSize, size_t streamBytes);
+
+/* Decode flags. */
+#define SZX_AMD_DECODE_REFERENCE 0   /* bit-identical to the reference decoder */
+#define SZX_AMD_DECODE_EXACT 1       /* fp64: read the bytes the encoder wrote
+                                        where the reference decoder misreads
+                                        one (reqBytes 5, lead 0, byte 7) */
+
+/* Decompress a device-resident stream of streamBytes bytes into n values at
+ * d_out.  blockSize is taken from the stream.  d_status (device memory, one
+ * int32) receives 0 or a negative code for a malformed stream. */
+int szx_amd_decompress_dev(const void *d_stream, size_t streamBytes, size_t n, int dataType,
+                           void *d_out, void *d_work, size_t workBytes, int flags,
+                           int32_t *d_status, void *hipStream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -79,6 +79,9 @@ def lib():
                                            C.POINTER(_sz), C.POINTER(_sz)]
         L.szxo_fxr_sample.restype = C.c_float
         L.szxo_fxr_sample.argtypes = [vp, C.c_int, _sz, C.c_int, C.c_int, vp, vp, vp]
+        L.szxo_fxr_search.restype = C.c_float
+        L.szxo_fxr_search.argtypes = [C.c_float, C.c_float, C.c_int, C.c_float, C.c_int, _sz, vp,
+                                      C.c_int, vp, vp, C.POINTER(C.c_int)]
         _lib = L
     return _lib
 

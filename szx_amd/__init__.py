@@ -1,0 +1,248 @@
+"""szx_amd -- SZx's blockwise fast codec (fastMode 2) on MI355X.
+
+The product is ``libszx_amd.so`` (HIP kernels for gfx950 behind a C ABI,
+declared in ``include/szx_amd.h``).  This package is a thin ctypes binding to
+it, used by the tests and the benchmark:
+
+* the reference's drop-in entry points (host arrays in, bytes out), and
+* the device-resident entry points, with torch providing device memory and
+  streams.
+
+There is no fallback: if the library is missing, importing the binding's
+functions raises, and every codec call runs on the GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libszx_amd.so")
+
+# szx_defines.h
+ABS, REL, FXR = 0, 1, 2
+SZx_FLOAT, SZx_DOUBLE = 0, 1
+SZx_NO_BLOCK_FAST_CMPR = 1
+SZx_WITH_BLOCK_FAST_CMPR = 2
+SZx_RANDOMACCESS_FAST_CMPR = 3
+SZx_OPENMP_FAST_CMPR = 4
+DECODE_REFERENCE = 0
+DECODE_EXACT = 1
+
+# every symbol include/szx_amd.h declares
+EXPORTS = (
+    "SZx_computeDataLength", "SZx_fast_compress_args", "SZx_fast_decompress",
+    "SZx_fast_compress_args_unpredictable_blocked",
+    "SZx_fast_decompress_args_unpredictable_blocked", "SZx_computeValueRange",
+    "SZx_estimateErrorBoundbasedonCR", "SZ_fast_compress_args", "SZ_fast_decompress",
+    "szx_amd_last_error", "szx_amd_compress_bound", "szx_amd_compress_capacity",
+    "szx_amd_compress_workspace", "szx_amd_compress_dev", "szx_amd_decompress_workspace",
+    "szx_amd_decompress_dev",
+)
+
+_lib = None
+_libc = C.CDLL(None)
+_libc.free.argtypes = [C.c_void_p]
+
+
+class SZxError(RuntimeError):
+    pass
+
+
+def lib() -> C.CDLL:
+    """Load libszx_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: run `python -m szx_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    sz, vp, f32, i32 = C.c_size_t, C.c_void_p, C.c_float, C.c_int
+    L.szx_amd_last_error.restype = C.c_char_p
+    L.SZx_computeDataLength.restype = sz
+    L.SZx_computeDataLength.argtypes = [sz] * 5
+    L.SZx_fast_compress_args.restype = vp
+    L.SZx_fast_compress_args.argtypes = [i32, i32, vp, C.POINTER(sz), i32, f32, f32, f32, f32,
+                                         sz, sz, sz, sz, sz]
+    L.SZ_fast_compress_args.restype = vp
+    L.SZ_fast_compress_args.argtypes = L.SZx_fast_compress_args.argtypes
+    L.SZx_fast_decompress.restype = vp
+    L.SZx_fast_decompress.argtypes = [i32, i32, vp, sz, sz, sz, sz, sz, sz]
+    L.SZ_fast_decompress.restype = vp
+    L.SZ_fast_decompress.argtypes = L.SZx_fast_decompress.argtypes
+    L.SZx_fast_compress_args_unpredictable_blocked.restype = vp
+    L.SZx_fast_compress_args_unpredictable_blocked.argtypes = [vp, i32, C.POINTER(sz), f32, sz, i32]
+    L.SZx_fast_decompress_args_unpredictable_blocked.restype = None
+    L.SZx_fast_decompress_args_unpredictable_blocked.argtypes = [vp, i32, sz, vp]
+    L.SZx_computeValueRange.restype = C.c_double
+    L.SZx_computeValueRange.argtypes = [vp, i32, sz, C.POINTER(f32), C.POINTER(f32)]
+    L.SZx_estimateErrorBoundbasedonCR.restype = f32
+    L.SZx_estimateErrorBoundbasedonCR.argtypes = [i32, f32, f32, vp, f32, i32, sz]
+    L.szx_amd_compress_bound.restype = sz
+    L.szx_amd_compress_bound.argtypes = [sz, i32, i32]
+    L.szx_amd_compress_capacity.restype = sz
+    L.szx_amd_compress_capacity.argtypes = [sz, i32, i32]
+    L.szx_amd_compress_workspace.restype = sz
+    L.szx_amd_compress_workspace.argtypes = [sz, i32, i32]
+    L.szx_amd_compress_dev.restype = i32
+    L.szx_amd_compress_dev.argtypes = [vp, sz, i32, i32, f32, f32, i32, vp, sz, vp, sz, vp, vp]
+    L.szx_amd_decompress_workspace.restype = sz
+    L.szx_amd_decompress_workspace.argtypes = [sz, i32, i32, sz]
+    L.szx_amd_decompress_dev.restype = i32
+    L.szx_amd_decompress_dev.argtypes = [vp, sz, sz, i32, vp, vp, sz, i32, vp, vp]
+    _lib = L
+    return L
+
+
+def last_error() -> str:
+    return lib().szx_amd_last_error().decode()
+
+
+def _dtype_code(dtype) -> int:
+    dtype = np.dtype(dtype)
+    if dtype == np.float32:
+        return SZx_FLOAT
+    if dtype == np.float64:
+        return SZx_DOUBLE
+    raise TypeError(f"unsupported dtype {dtype}")
+
+
+def _take(ptr, nbytes: int) -> bytes:
+    out = C.string_at(ptr, nbytes)
+    _libc.free(ptr)
+    return out
+
+
+# ---------------------------------------------------------------------------
+# drop-in host API (szx.h)
+# ---------------------------------------------------------------------------
+
+def _dims(shape):
+    dims = list(shape)[::-1][:5]           # r1 is the fastest dimension
+    dims += [0] * (5 - len(dims))
+    return dims[4], dims[3], dims[2], dims[1], dims[0]
+
+
+def SZx_fast_compress_args(fastMode: int, data: np.ndarray, errBoundMode: int,
+                           absErrBound: float = 0.0, relBoundRatio: float = 0.0,
+                           compressionRatio: float = 0.0, tolerance: float = 0.0) -> bytes:
+    """szx.h:77 with the dims taken from data.shape (C order, last axis = r1)."""
+    x = np.ascontiguousarray(data)
+    osz = C.c_size_t(0)
+    p = lib().SZx_fast_compress_args(fastMode, _dtype_code(x.dtype), x.ctypes.data, C.byref(osz),
+                                     errBoundMode, absErrBound, relBoundRatio, compressionRatio,
+                                     tolerance, *_dims(x.shape))
+    if not p:
+        raise SZxError(last_error())
+    return _take(p, osz.value)
+
+
+def SZx_fast_decompress(fastMode: int, dtype, stream: bytes, shape) -> np.ndarray:
+    """szx.h:80: decode into an array of `shape`."""
+    dtype = np.dtype(dtype)
+    n = int(np.prod(shape))
+    buf = C.create_string_buffer(bytes(stream), len(stream))
+    p = lib().SZx_fast_decompress(fastMode, _dtype_code(dtype), buf, len(stream), *_dims(shape))
+    if not p:
+        raise SZxError(last_error())
+    return np.frombuffer(_take(p, n * dtype.itemsize), dtype).reshape(shape).copy()
+
+
+def compress_blocked(data: np.ndarray, absErrBound: float, blockSize: int = 128) -> bytes:
+    """szx.h:89 SZx_fast_compress_args_unpredictable_blocked."""
+    x = np.ascontiguousarray(data).ravel()
+    osz = C.c_size_t(0)
+    p = lib().SZx_fast_compress_args_unpredictable_blocked(x.ctypes.data, _dtype_code(x.dtype),
+                                                           C.byref(osz), absErrBound, x.size,
+                                                           blockSize)
+    if not p:
+        raise SZxError(last_error())
+    return _take(p, osz.value)
+
+
+def decompress_blocked(stream: bytes, n: int, dtype) -> np.ndarray:
+    """szx.h:98 SZx_fast_decompress_args_unpredictable_blocked (length-free)."""
+    dtype = np.dtype(dtype)
+    buf = C.create_string_buffer(bytes(stream), len(stream))
+    out = C.c_void_p(0)
+    lib().SZx_fast_decompress_args_unpredictable_blocked(C.byref(out), _dtype_code(dtype), n, buf)
+    if not out.value:
+        raise SZxError(last_error())
+    return np.frombuffer(_take(out.value, n * dtype.itemsize), dtype).copy()
+
+
+# ---------------------------------------------------------------------------
+# device-resident API (torch tensors as device memory)
+# ---------------------------------------------------------------------------
+
+def _torch():
+    import torch
+    return torch
+
+
+def _stream_handle(stream):
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+class DeviceCompressor:
+    """Holds the output buffer and workspace for repeated compresses of one
+    shape/dtype/blockSize, so the timed path allocates nothing."""
+
+    def __init__(self, n: int, dtype, blockSize: int = 128, device="cuda"):
+        torch = _torch()
+        self.n, self.dtype, self.bs = n, np.dtype(dtype), blockSize
+        code = _dtype_code(self.dtype)
+        L = lib()
+        self.capacity = L.szx_amd_compress_capacity(n, code, blockSize)
+        self.wsbytes = L.szx_amd_compress_workspace(n, code, blockSize)
+        if not self.capacity or not self.wsbytes:
+            raise SZxError(f"unsupported parameters (blockSize={blockSize})")
+        self.out = torch.empty(self.capacity, dtype=torch.uint8, device=device)
+        self.ws = torch.empty(self.wsbytes, dtype=torch.uint8, device=device)
+        self.result = torch.zeros(3, dtype=torch.int64, device=device)
+
+    def __call__(self, x, errBoundMode: int = ABS, absErrBound: float = 0.0,
+                 relBoundRatio: float = 0.0, stream=None) -> None:
+        """Enqueue one compress of tensor x (asynchronous)."""
+        rc = lib().szx_amd_compress_dev(x.data_ptr(), self.n, _dtype_code(self.dtype),
+                                        errBoundMode, absErrBound, relBoundRatio, self.bs,
+                                        self.out.data_ptr(), self.capacity, self.ws.data_ptr(),
+                                        self.wsbytes, self.result.data_ptr(),
+                                        _stream_handle(stream))
+        if rc != 0:
+            raise SZxError(last_error())
+
+    def stream_view(self):
+        """(device uint8 tensor view of the stream, offset, size, eb) -- syncs."""
+        off, size, ebbits = (int(v) for v in self.result.cpu().tolist())
+        eb = float(np.array([ebbits], np.uint32).view(np.float32)[0])
+        return self.out[off:off + size], off, size, eb
+
+
+class DeviceDecompressor:
+    def __init__(self, n: int, dtype, blockSize: int, stream_capacity: int, device="cuda"):
+        torch = _torch()
+        self.n, self.dtype = n, np.dtype(dtype)
+        code = _dtype_code(self.dtype)
+        self.wsbytes = lib().szx_amd_decompress_workspace(n, code, blockSize, stream_capacity)
+        self.ws = torch.empty(self.wsbytes, dtype=torch.uint8, device=device)
+        tdt = torch.float32 if self.dtype == np.float32 else torch.float64
+        self.out = torch.empty(n, dtype=tdt, device=device)
+        self.status = torch.zeros(1, dtype=torch.int32, device=device)
+
+    def __call__(self, stream_tensor, nbytes: int | None = None, flags: int = DECODE_REFERENCE,
+                 stream=None) -> None:
+        nb = stream_tensor.numel() if nbytes is None else nbytes
+        rc = lib().szx_amd_decompress_dev(stream_tensor.data_ptr(), nb, self.n,
+                                          _dtype_code(self.dtype), self.out.data_ptr(),
+                                          self.ws.data_ptr(), self.wsbytes, flags,
+                                          self.status.data_ptr(), _stream_handle(stream))
+        if rc != 0:
+            raise SZxError(last_error())
+
+    def check(self) -> int:
+        return int(self.status.item())

@@ -1,0 +1,63 @@
+"""Build libszx_amd.so (gfx950) in-tree with hipcc.
+
+Usage: python -m szx_amd.build   (or szx_amd.build.build() from Python)
+
+Flags that matter for exactness:
+  -ffp-contract=off                  no FMA contraction of the codec's
+                                     sub/mul/add (the reference does none)
+  -fno-gpu-flush-denormals-to-zero   fp32 denormals are data here
+  -Xclang -target-feature -Xclang -packed-fp32-ops
+                                     keep the compiler from emitting packed
+                                     fp32 instructions
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libszx_amd.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+HOSTCXX = os.environ.get("CXX", "g++")
+ARCH = os.environ.get("SZX_AMD_ARCH", "gfx950")
+
+SOURCES = ["szx_compress.hip", "szx_decompress.hip", "szx_api.cpp"]
+HEADERS = ["szx_common.hpp", "szx_internal.hpp"]
+
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",
+          "-Wno-unused-result"]
+DEVICE = [f"--offload-arch={ARCH}", "-fno-gpu-flush-denormals-to-zero",
+          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+
+def build(verbose: bool = False) -> str:
+    """Compile every source and link the shared library; returns its path."""
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    objs = []
+    for src in SOURCES:
+        path = os.path.join(CSRC, src)
+        obj = os.path.join(objdir, src + ".o")
+        if src.endswith(".hip"):
+            cmd = [HIPCC, "-x", "hip", *COMMON, *DEVICE, "-c", path, "-o", obj]
+        else:   # host-only C++ against the HIP runtime headers
+            cmd = [HOSTCXX, *COMMON, "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+                   "-c", path, "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose="-v" in sys.argv)
+    print(LIB)

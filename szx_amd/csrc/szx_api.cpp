@@ -1,0 +1,448 @@
+// szx_api.cpp -- the C ABI of libszx_amd.so (declared in include/szx_amd.h).
+//
+// Host entry points keep the reference's signatures and ownership (host
+// pointers in, malloc'd buffers out) and move the data through device buffers
+// that are cached per thread and device, so repeated calls do not pay for
+// hipMalloc.  Device entry points only enqueue kernels.
+#include "../../include/szx_amd.h"
+
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <mutex>
+#include <string>
+
+#include "szx_internal.hpp"
+
+using namespace szxamd;
+
+namespace {
+
+thread_local std::string g_err;
+
+void set_err(const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+}
+
+bool hip_ok(hipError_t e, const char *what)
+{
+    if (e == hipSuccess) return true;
+    set_err("%s: %s", what, hipGetErrorString(e));
+    return false;
+}
+
+int width_of(int dataType)
+{
+    if (dataType == SZx_FLOAT) return 4;
+    if (dataType == SZx_DOUBLE) return 8;
+    return 0;
+}
+
+bool bs_supported(int bs) { return bs >= 4 && bs <= 256 && (bs & (bs - 1)) == 0; }
+
+// Growable device scratch kept for the life of the thread (one per role).
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    int dev = -1;
+    bool reserve(size_t bytes)
+    {
+        int d = 0;
+        if (!hip_ok(hipGetDevice(&d), "hipGetDevice")) return false;
+        if (p && cap >= bytes && dev == d) return true;
+        if (p) {
+            int cur = d;
+            (void)hipSetDevice(dev);
+            (void)hipFree(p);
+            (void)hipSetDevice(cur);
+            p = nullptr;
+            cap = 0;
+        }
+        if (!hip_ok(hipMalloc(&p, bytes ? bytes : 1), "hipMalloc")) return false;
+        cap = bytes;
+        dev = d;
+        return true;
+    }
+    ~DevBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+};
+
+struct HostCtx {
+    DevBuf in, out, ws, res;
+    hipStream_t stream = nullptr;
+    int dev = -1;
+    bool ready()
+    {
+        int d = 0;
+        if (!hip_ok(hipGetDevice(&d), "hipGetDevice")) return false;
+        if (stream && dev == d) return true;
+        if (stream) (void)hipStreamDestroy(stream);
+        if (!hip_ok(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate"))
+            return false;
+        dev = d;
+        return true;
+    }
+};
+
+thread_local HostCtx g_ctx;
+
+size_t data_length(size_t r5, size_t r4, size_t r3, size_t r2, size_t r1)
+{
+    // szx.cc:73-101: trailing zero dims are absent, r1 is the fastest.
+    if (r1 == 0) return 0;
+    if (r2 == 0) return r1;
+    if (r3 == 0) return r1 * r2;
+    if (r4 == 0) return r1 * r2 * r3;
+    if (r5 == 0) return r1 * r2 * r3 * r4;
+    return r1 * r2 * r3 * r4 * r5;
+}
+
+// Host-in / host-out compress through the device path.
+unsigned char *host_compress(const void *data, size_t n, int w, int rel_mode, float abs_eb,
+                             float rel, int bs, size_t *outSize)
+{
+    if (outSize) *outSize = 0;
+    if (!data || n == 0) {
+        set_err("empty input");
+        return nullptr;
+    }
+    if (!bs_supported(bs)) {
+        set_err("blockSize %d is not supported (powers of two 4..256)", bs);
+        return nullptr;
+    }
+    HostCtx &c = g_ctx;
+    if (!c.ready()) return nullptr;
+    const CompressLayout L = compress_layout(n, w, bs);
+    if (!c.in.reserve(n * (size_t)w) || !c.out.reserve(L.capacity) || !c.ws.reserve(L.ws_bytes) ||
+        !c.res.reserve(64))
+        return nullptr;
+    if (!hip_ok(hipMemcpyAsync(c.in.p, data, n * (size_t)w, hipMemcpyHostToDevice, c.stream),
+                "hipMemcpyAsync H2D"))
+        return nullptr;
+    if (compress_launch(c.in.p, n, w, rel_mode, abs_eb, rel, bs, (uint8_t *)c.out.p,
+                        (uint8_t *)c.ws.p, (uint64_t *)c.res.p, c.stream)) {
+        set_err("compress launch failed");
+        return nullptr;
+    }
+    uint64_t res[3] = {0, 0, 0};
+    if (!hip_ok(hipMemcpyAsync(res, c.res.p, sizeof res, hipMemcpyDeviceToHost, c.stream),
+                "hipMemcpyAsync D2H") ||
+        !hip_ok(hipStreamSynchronize(c.stream), "compress"))
+        return nullptr;
+    unsigned char *out = (unsigned char *)malloc(res[1] ? res[1] : 1);
+    if (!out) {
+        set_err("malloc(%llu) failed", (unsigned long long)res[1]);
+        return nullptr;
+    }
+    if (!hip_ok(hipMemcpy(out, (uint8_t *)c.out.p + res[0], res[1], hipMemcpyDeviceToHost),
+                "hipMemcpy D2H stream")) {
+        free(out);
+        return nullptr;
+    }
+    if (outSize) *outSize = res[1];
+    return out;
+}
+
+void *host_decompress(const unsigned char *bytes, size_t len, size_t n, int w, int exact)
+{
+    if (!bytes || n == 0) {
+        set_err("empty input");
+        return nullptr;
+    }
+    if (len < 13) {
+        set_err("stream shorter than its header");
+        return nullptr;
+    }
+    const int bs = bytes[4] ? bytes[4] : 256;
+    HostCtx &c = g_ctx;
+    if (!c.ready()) return nullptr;
+    const DecompressLayout L = decompress_layout(n, w, bs, len);
+    if (!c.in.reserve(len) || !c.out.reserve(n * (size_t)w) || !c.ws.reserve(L.ws_bytes) ||
+        !c.res.reserve(64))
+        return nullptr;
+    if (!hip_ok(hipMemcpyAsync(c.in.p, bytes, len, hipMemcpyHostToDevice, c.stream), "H2D"))
+        return nullptr;
+    if (decompress_launch((const uint8_t *)c.in.p, len, n, w, c.out.p, (uint8_t *)c.ws.p,
+                          L.ws_bytes, exact, (int32_t *)c.res.p, c.stream)) {
+        set_err("decompress launch failed");
+        return nullptr;
+    }
+    int32_t status = 0;
+    if (!hip_ok(hipMemcpyAsync(&status, c.res.p, 4, hipMemcpyDeviceToHost, c.stream), "D2H") ||
+        !hip_ok(hipStreamSynchronize(c.stream), "decompress"))
+        return nullptr;
+    if (status != 0) {
+        set_err("malformed or unsupported stream (code %d)", status);
+        return nullptr;
+    }
+    void *out = malloc(n * (size_t)w);
+    if (!out) {
+        set_err("malloc failed");
+        return nullptr;
+    }
+    if (!hip_ok(hipMemcpy(out, c.out.p, n * (size_t)w, hipMemcpyDeviceToHost), "D2H output")) {
+        free(out);
+        return nullptr;
+    }
+    return out;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *szx_amd_last_error(void) { return g_err.c_str(); }
+
+size_t SZx_computeDataLength(size_t r5, size_t r4, size_t r3, size_t r2, size_t r1)
+{
+    return data_length(r5, r4, r3, r2, r1);
+}
+
+unsigned char *SZx_fast_compress_args(int fastMode, int dataType, void *data, size_t *outSize,
+                                      int errBoundMode, float absErrBound, float relBoundRatio,
+                                      float compressionRatio, float tolerance, size_t r5,
+                                      size_t r4, size_t r3, size_t r2, size_t r1)
+{
+    (void)compressionRatio;
+    (void)tolerance;
+    g_err.clear();
+    if (outSize) *outSize = 0;
+    const int w = width_of(dataType);
+    if (!w) {
+        set_err("unsupported dataType %d", dataType);
+        return nullptr;
+    }
+    if (fastMode != SZx_WITH_BLOCK_FAST_CMPR) {
+        set_err("fastMode %d is not provided (fastMode 2 is)", fastMode);
+        return nullptr;
+    }
+    if (errBoundMode != ABS && errBoundMode != REL) {
+        set_err("errBoundMode %d is not provided", errBoundMode);
+        return nullptr;
+    }
+    const size_t n = data_length(r5, r4, r3, r2, r1);
+    // szx.cc:234: the blocked family always uses blockSize 128.
+    return host_compress(data, n, w, errBoundMode == REL, absErrBound, relBoundRatio, 128, outSize);
+}
+
+unsigned char *SZ_fast_compress_args(int fastMode, int dataType, void *data, size_t *outSize,
+                                     int errBoundMode, float absErrBound, float relBoundRatio,
+                                     float compressionRatio, float tolerance, size_t r5, size_t r4,
+                                     size_t r3, size_t r2, size_t r1)
+{
+    return SZx_fast_compress_args(fastMode, dataType, data, outSize, errBoundMode, absErrBound,
+                                  relBoundRatio, compressionRatio, tolerance, r5, r4, r3, r2, r1);
+}
+
+void *SZx_fast_decompress(int fastMode, int dataType, unsigned char *bytes, size_t byteLength,
+                          size_t r5, size_t r4, size_t r3, size_t r2, size_t r1)
+{
+    g_err.clear();
+    const int w = width_of(dataType);
+    if (!w) {
+        set_err("unsupported dataType %d", dataType);
+        return nullptr;
+    }
+    if (fastMode != SZx_WITH_BLOCK_FAST_CMPR) {
+        set_err("fastMode %d is not provided (fastMode 2 is)", fastMode);
+        return nullptr;
+    }
+    return host_decompress(bytes, byteLength, data_length(r5, r4, r3, r2, r1), w, 0);
+}
+
+void *SZ_fast_decompress(int fastMode, int dataType, unsigned char *bytes, size_t byteLength,
+                         size_t r5, size_t r4, size_t r3, size_t r2, size_t r1)
+{
+    return SZx_fast_decompress(fastMode, dataType, bytes, byteLength, r5, r4, r3, r2, r1);
+}
+
+unsigned char *SZx_fast_compress_args_unpredictable_blocked(void *oriData, int dataType,
+                                                            size_t *outSize, float absErrBound,
+                                                            size_t nbEle, int blockSize)
+{
+    g_err.clear();
+    const int w = width_of(dataType);
+    if (!w) {
+        set_err("unsupported dataType %d", dataType);
+        return nullptr;
+    }
+    return host_compress(oriData, nbEle, w, 0, absErrBound, 0.0f, blockSize, outSize);
+}
+
+size_t szx_amd_stream_bytes(const unsigned char *s, size_t nbEle, int w);
+
+void SZx_fast_decompress_args_unpredictable_blocked(void *newData, int dataType, size_t nbEle,
+                                                    unsigned char *cmpBytes)
+{
+    g_err.clear();
+    void **dst = (void **)newData;
+    if (!dst) {
+        set_err("newData is NULL");
+        return;
+    }
+    *dst = nullptr;
+    const int w = width_of(dataType);
+    if (!w || !cmpBytes) {
+        set_err("unsupported dataType %d or NULL stream", dataType);
+        return;
+    }
+    // The signature carries no length: walk the stream's own structure.
+    const size_t len = szx_amd_stream_bytes(cmpBytes, nbEle, w);
+    if (!len) return;
+    *dst = host_decompress(cmpBytes, len, nbEle, w, 0);
+}
+
+double SZx_computeValueRange(void *oriData, int dataType, size_t length, float *radius,
+                             float *medianValue)
+{
+    // szx_float.cc:478-499 / szx_double.cc:1137-1158 (host arithmetic; this is
+    // a one-pass helper the reference exposes, not part of the codec path).
+    g_err.clear();
+    if (!oriData || length == 0) return 0.0;
+    if (dataType == SZx_FLOAT) {
+        const float *x = (const float *)oriData;
+        float lo = x[0], hi = x[0];
+        for (size_t i = 0; i < length; i++) {
+            if (lo > x[i]) lo = x[i];
+            else if (hi < x[i]) hi = x[i];
+        }
+        const float r = hi - lo;
+        if (radius) {
+            *radius = r / 2;
+            *medianValue = lo + *radius;
+        }
+        return r;
+    }
+    const double *x = (const double *)oriData;
+    double lo = x[0], hi = x[0];
+    for (size_t i = 0; i < length; i++) {
+        if (lo > x[i]) lo = x[i];
+        else if (hi < x[i]) hi = x[i];
+    }
+    const double r = hi - lo;
+    if (radius) {
+        *radius = (float)(r / 2);
+        *medianValue = (float)(lo + r / 2);
+    }
+    return r;
+}
+
+float SZx_estimateErrorBoundbasedonCR(int dataType, float targetCompressionRatio, float tolerance,
+                                      void *data, float initErrorBound, int blockSize,
+                                      size_t nbEle)
+{
+    (void)dataType; (void)targetCompressionRatio; (void)tolerance; (void)data;
+    (void)initErrorBound; (void)blockSize; (void)nbEle;
+    set_err("fixed-rate estimation is not built yet");
+    return 0.0f;
+}
+
+// Length of a fastMode-2 stream from its own structure (header, bitmap,
+// constant medians, then each record's size from its lead codes).
+size_t szx_amd_stream_bytes(const unsigned char *s, size_t nbEle, int w)
+{
+    const size_t bs = s[4] ? s[4] : 256;
+    const size_t nb = (nbEle + bs - 1) / bs, nbit = (nb + 7) / 8;
+    uint64_t nconst = 0;
+    for (int k = 0; k < 8; k++) nconst = (nconst << 8) | s[5 + k];
+    if (nconst > nb) {
+        set_err("corrupt stream header");
+        return 0;
+    }
+    size_t q = 13 + nbit + 4 * nconst;
+    for (size_t b = 0; b < nb; b++) {
+        if (!((s[13 + (b >> 3)] >> (7 - (b & 7))) & 1)) continue;
+        const size_t nel = (b + 1) * bs <= nbEle ? bs : nbEle - b * bs;
+        const int rl = s[q];
+        if (rl == 0 || rl > 8 * w) {
+            set_err("corrupt record at byte %zu", q);
+            return 0;
+        }
+        const int rb = (rl + 7) / 8;
+        size_t sz = 5 + (nel + 3) / 4;
+        for (size_t i = 0; i < nel; i++) {
+            const int ld = (s[q + 5 + (i >> 2)] >> (6 - 2 * (int)(i & 3))) & 3;
+            if (ld < rb) sz += (size_t)(rb - ld);
+        }
+        q += sz;
+    }
+    return q;
+}
+
+size_t szx_amd_compress_bound(size_t n, int dataType, int blockSize)
+{
+    const int w = width_of(dataType);
+    if (!w || blockSize < 1 || blockSize > 256) return 0;
+    const size_t nb = (n + blockSize - 1) / blockSize;
+    return 13 + (nb + 7) / 8 + nb * (5 + (blockSize + 3) / 4 + (size_t)blockSize * w);
+}
+
+size_t szx_amd_compress_capacity(size_t n, int dataType, int blockSize)
+{
+    const int w = width_of(dataType);
+    if (!w || !bs_supported(blockSize)) return 0;
+    return compress_layout(n, w, blockSize).capacity;
+}
+
+size_t szx_amd_compress_workspace(size_t n, int dataType, int blockSize)
+{
+    const int w = width_of(dataType);
+    if (!w || !bs_supported(blockSize)) return 0;
+    return compress_layout(n, w, blockSize).ws_bytes;
+}
+
+int szx_amd_compress_dev(const void *d_in, size_t n, int dataType, int errBoundMode,
+                         float absErrBound, float relBoundRatio, int blockSize, void *d_out,
+                         size_t outCapacity, void *d_work, size_t workBytes, uint64_t *d_result,
+                         void *hipStream)
+{
+    g_err.clear();
+    const int w = width_of(dataType);
+    if (!w) { set_err("unsupported dataType %d", dataType); return SZx_TERR; }
+    if (!bs_supported(blockSize)) { set_err("unsupported blockSize %d", blockSize); return SZx_NSCS; }
+    if (errBoundMode != ABS && errBoundMode != REL) { set_err("bad errBoundMode"); return SZx_BERR; }
+    if (!d_in || !d_out || !d_work || !d_result || n == 0) { set_err("NULL argument or n == 0"); return SZx_NSCS; }
+    if (((uintptr_t)d_in & 15) != 0) { set_err("d_in must be 16-byte aligned"); return SZx_NSCS; }
+    const CompressLayout L = compress_layout(n, w, blockSize);
+    if (outCapacity < L.capacity) { set_err("outCapacity < %llu", (unsigned long long)L.capacity); return SZx_NSCS; }
+    if (workBytes < L.ws_bytes) { set_err("workBytes < %llu", (unsigned long long)L.ws_bytes); return SZx_NSCS; }
+    if (compress_launch(d_in, n, w, errBoundMode == REL, absErrBound, relBoundRatio, blockSize,
+                        (uint8_t *)d_out, (uint8_t *)d_work, d_result, (hipStream_t)hipStream))
+        return SZx_NSCS;
+    return hip_ok(hipGetLastError(), "launch") ? SZx_SCES : SZx_NSCS;
+}
+
+size_t szx_amd_decompress_workspace(size_t n, int dataType, int blockSize, size_t streamBytes)
+{
+    const int w = width_of(dataType);
+    if (!w) return 0;
+    return decompress_layout(n, w, blockSize, streamBytes).ws_bytes;
+}
+
+int szx_amd_decompress_dev(const void *d_stream, size_t streamBytes, size_t n, int dataType,
+                           void *d_out, void *d_work, size_t workBytes, int flags,
+                           int32_t *d_status, void *hipStream)
+{
+    g_err.clear();
+    const int w = width_of(dataType);
+    if (!w) { set_err("unsupported dataType %d", dataType); return SZx_TERR; }
+    if (!d_stream || !d_out || !d_work || !d_status || n == 0) { set_err("NULL argument or n == 0"); return SZx_NSCS; }
+    if (((uintptr_t)d_out & 15) != 0) { set_err("d_out must be 16-byte aligned"); return SZx_NSCS; }
+    if (decompress_launch((const uint8_t *)d_stream, streamBytes, n, w, d_out, (uint8_t *)d_work,
+                          workBytes, flags & SZX_AMD_DECODE_EXACT, d_status, (hipStream_t)hipStream))
+        return SZx_NSCS;
+    return hip_ok(hipGetLastError(), "launch") ? SZx_SCES : SZx_NSCS;
+}
+
+}  // extern "C"

@@ -1,0 +1,154 @@
+"""GPU parity of libszx_amd.so against the oracle and the golden vectors.
+
+Every call goes through the C ABI (the reference's drop-in entry points or
+the device-resident ones).  The bar is bit-exact: the stream must equal the
+reference's byte for byte, and the decode must equal the reference decoder's
+bit for bit (the fp64 exact mode is checked against the oracle's exact mode).
+"""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+import szx_amd as Z
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz")))
+POW2 = (4, 8, 16, 32, 64, 128, 256)
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def load(path):
+    with np.load(path, allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def bits_equal(a, b):
+    return a.dtype == b.dtype and a.shape == b.shape and np.array_equal(
+        a.view(np.uint8), b.view(np.uint8))
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p)[:-4] for p in GOLDEN])
+def test_golden_stream_and_decode(path, device):
+    g = load(path)
+    x, bs, eb = g["x"], int(g["bs"]), float(g["eb"])
+    name = os.path.basename(path)
+    if bs not in POW2:
+        pytest.skip("block sizes that are not powers of two use the generic path")
+    say("case", name, x.dtype, x.size, bs, eb)
+    if "sin_rel" in name:   # produced by SZx_fast_compress_args(REL)
+        s = Z.SZx_fast_compress_args(Z.SZx_WITH_BLOCK_FAST_CMPR, x, Z.REL,
+                                     relBoundRatio=float(g["rel"]))
+    else:
+        s = Z.compress_blocked(x, eb, bs)
+    assert s == g["stream"].tobytes()
+    if bs == 128:
+        d = Z.SZx_fast_decompress(Z.SZx_WITH_BLOCK_FAST_CMPR, x.dtype, s, x.shape)
+    else:
+        d = Z.decompress_blocked(s, x.size, x.dtype)
+    want = g["decoded"] if g["decoded"].size else O.decompress(s, x.size, x.dtype)
+    assert bits_equal(d, want)
+
+
+def _field(r, n, dtype, kind):
+    t = np.arange(n)
+    if kind == 0:
+        x = np.sin(t * r.uniform(1e-4, 0.2)) * 10 ** r.uniform(-4, 4)
+    elif kind == 1:
+        x = np.cumsum(r.standard_normal(n)) * 10 ** r.uniform(-6, 2)
+    elif kind == 2:
+        x = np.where(r.random(n) < 0.5, 0.75, np.sin(t * 0.01))
+    elif kind == 3:
+        x = r.standard_normal(n) * 10.0 ** r.uniform(-300 if dtype == np.float64 else -40, 0)
+    else:
+        x = np.where(r.random(n) < 0.3, -0.0, r.standard_normal(n) * 1e-3)
+    x = x + r.standard_normal(n) * 10 ** r.uniform(-9, -3)
+    return x.astype(dtype)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_random_parity_host_api(dtype, device):
+    r = np.random.default_rng(100 if dtype == np.float32 else 101)
+    for trial in range(60):
+        n = int(r.choice([1, 2, 3, 5, 63, 64, 127, 128, 129, 1000, 4095, 4096, 4097, 50000,
+                          100003]))
+        bs = int(r.choice(POW2))
+        x = _field(r, n, dtype, trial % 5)
+        eb = float(np.float32((float(np.ptp(x)) + 1e-30) * 10.0 ** r.uniform(-12, -1)))
+        say("trial", trial, dtype.__name__, n, bs, eb)
+        s = Z.compress_blocked(x, eb, bs)
+        assert s == O.compress(x, eb, bs), (trial, n, bs, eb)
+        d = Z.decompress_blocked(s, n, dtype)
+        assert bits_equal(d, O.decompress(s, n, dtype, ref_quirk=True)), trial
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("rel", [1e-2, 1e-3, 1e-4, 1e-6])
+def test_rel_mode(dtype, rel, device):
+    r = np.random.default_rng(7)
+    x = _field(r, 70001, dtype, 1)
+    say("rel", dtype.__name__, rel)
+    s = Z.SZx_fast_compress_args(Z.SZx_WITH_BLOCK_FAST_CMPR, x, Z.REL, relBoundRatio=rel)
+    assert s == O.compress(x, O.rel_eb(x, rel), 128)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_device_api_and_exact_decode(dtype, device):
+    import torch
+    r = np.random.default_rng(5)
+    n = 300_007
+    x = _field(r, n, dtype, 1)
+    eb = float(np.float32(np.ptp(x) * 1e-8))
+    say("device api", dtype.__name__, n, eb)
+    comp = Z.DeviceCompressor(n, dtype, 128)
+    xt = torch.from_numpy(x).to(device)
+    comp(xt, Z.ABS, eb)
+    view, off, size, eb_used = comp.stream_view()
+    s = view.cpu().numpy().tobytes()
+    assert np.float32(eb_used) == np.float32(eb)
+    assert s == O.compress(x, eb, 128)
+    dec = Z.DeviceDecompressor(n, dtype, 128, size)
+    dec(view, size, Z.DECODE_REFERENCE)
+    assert dec.check() == 0
+    assert bits_equal(dec.out.cpu().numpy(), O.decompress(s, n, dtype, ref_quirk=True))
+    dec(view, size, Z.DECODE_EXACT)
+    assert dec.check() == 0
+    exact = dec.out.cpu().numpy()
+    assert bits_equal(exact, O.decompress(s, n, dtype, ref_quirk=False))
+    if dtype == np.float32:
+        assert np.max(np.abs(exact.astype(np.float64) - x)) <= eb
+
+
+def test_repeated_calls_reuse_buffers(device):
+    import torch
+    r = np.random.default_rng(9)
+    n = 1 << 20
+    comp = Z.DeviceCompressor(n, np.float32, 128)
+    for rep in range(3):
+        x = _field(r, n, np.float32, rep % 3)
+        eb = float(np.float32(np.ptp(x) * 1e-4))
+        say("repeat", rep)
+        comp(torch.from_numpy(x).to(device), Z.ABS, eb)
+        view, off, size, _ = comp.stream_view()
+        assert view.cpu().numpy().tobytes() == O.compress(x, eb, 128)
+
+
+def test_malformed_streams_are_rejected(device):
+    x = np.sin(np.arange(10000) * 0.01).astype(np.float32)
+    s = Z.compress_blocked(x, 1e-4, 128)
+    say("truncated")
+    with pytest.raises(Z.SZxError):
+        Z.SZx_fast_decompress(Z.SZx_WITH_BLOCK_FAST_CMPR, np.float32, s[: len(s) // 2], x.shape)
+    bad = bytearray(s)
+    bad[5:13] = (10 ** 9).to_bytes(8, "big")        # impossible constant-block count
+    with pytest.raises(Z.SZxError):
+        Z.SZx_fast_decompress(Z.SZx_WITH_BLOCK_FAST_CMPR, np.float32, bytes(bad), x.shape)
+    # a decode after a rejected stream still works
+    d = Z.SZx_fast_decompress(Z.SZx_WITH_BLOCK_FAST_CMPR, np.float32, s, x.shape)
+    assert bits_equal(d, O.decompress(s, x.size, np.float32))
