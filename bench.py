@@ -1,0 +1,247 @@
+#!/usr/bin/env python3
+"""Single-GPU throughput of the blockwise fast codec (SZx fastMode 2).
+
+Workload: one fp32 field of 512x512x512 (512 MiB) -- 64 random plane waves
+with integer wave vectors in [1,32], amplitude |k|^-2.5, Gaussian noise at
+1e-4 of the field's std, two boxes (30% of the volume together) holding the
+field mean -- compressed at a REL bound (default 1e-3, blockSize 128) and
+decompressed, all device-resident.
+
+A step is one compress of the whole field through the C ABI
+(szx_amd_compress_dev, REL mode: range pass + single-pass codec + header
+assembly).  value = uncompressed bytes / step time (GB/s).  Also reported:
+ABS-mode compress (the bound already known, as the reference's CLI times it),
+decompress with the compressor's record index and without it (rebuilt from
+the stream), roofline of the dominant kernel from HIP events recorded around
+it inside the library, a full-size parity check against the oracle, and the
+reference's own CPU codec timed on this host (compiled from its sources into
+oracle/_ref; the oracle itself if that is absent).
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--rel 1e-3]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "cmpr/decmpr GB/s + %HBM peak; fixed-rate solve time, learned vs bisection"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(x_host, eb, sample_values):
+    """Time the reference CPU codec on a bounded sample of the same field."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    xs = np.ascontiguousarray(x_host[:sample_values])
+    nbytes = xs.nbytes
+    out = {}
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+    def best(fn, reps=3):
+        fn()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return min(ts)
+
+    if O.ref_available():
+        kind = "reference"
+        t = best(lambda: O.ref_compress(xs, eb, 128))
+        s = O.ref_compress(xs, eb, 128)
+        td = best(lambda: O.ref_decompress(s, xs.size, np.float32))
+        t4 = best(lambda: O.ref_compress_args(xs, mode=0, abs_eb=eb, fast_mode=4))
+        out["serial"] = {"compress_gbs": nbytes / t / 1e9, "decompress_gbs": nbytes / td / 1e9,
+                         "cores": 1}
+        out["openmp"] = {"compress_gbs": nbytes / t4 / 1e9, "cores": threads}
+    else:
+        kind = "port"
+        t = best(lambda: O.compress(xs, eb, 128))
+        s = O.compress(xs, eb, 128)
+        td = best(lambda: O.decompress(s, xs.size, np.float32))
+        out["serial"] = {"compress_gbs": nbytes / t / 1e9, "decompress_gbs": nbytes / td / 1e9,
+                         "cores": 1}
+    return kind, out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--rel", type=float, default=1e-3)
+    ap.add_argument("--shape", default="512,512,512")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-parity", action="store_true")
+    ap.add_argument("--no-noindex", action="store_true", help="skip the index-free decode")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this codec build has no multi-GPU path "
+              "(fields shard by block ranges; see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import szx_amd as Z
+    from szx_amd.fields import sinusoid_field_torch
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    shape = tuple(int(v) for v in args.shape.split(","))
+    log(f"generating field {shape} fp32 on {torch.cuda.get_device_name(0)}")
+    x = sinusoid_field_torch(shape, seed=2, n_waves=64, kmax=32, slope=2.5, noise=1e-4,
+                             n_boxes=2, box_frac=0.30, box_value="mean", device=dev)
+    x = x.contiguous().view(-1)
+    n = x.numel()
+    nbytes = n * 4
+    torch.cuda.synchronize()
+
+    comp = Z.DeviceCompressor(n, np.float32, 128, device=dev)
+    comp(x, Z.REL, relBoundRatio=args.rel)
+    view, off, csize, eb = comp.stream_view()
+    dec = Z.DeviceDecompressor(n, np.float32, 128, comp.capacity, device=dev)
+    log(f"stream {csize} bytes (CR {nbytes / csize:.3f}), eb {eb:.6g}")
+
+    def timed(fn, steps):
+        torch.cuda.synchronize()
+        t0 = torch.cuda.Event(enable_timing=True)
+        t1 = torch.cuda.Event(enable_timing=True)
+        t0.record()
+        for _ in range(steps):
+            fn()
+        t1.record()
+        torch.cuda.synchronize()
+        return t0.elapsed_time(t1) / steps
+
+    rel_step = lambda: comp(x, Z.REL, relBoundRatio=args.rel)
+    abs_step = lambda: comp(x, Z.ABS, absErrBound=eb)
+    dec_step = lambda: dec(view, csize, Z.DECODE_REFERENCE, index=comp.index)
+
+    for _ in range(args.warmup):
+        rel_step()
+        dec_step()
+    torch.cuda.synchronize()
+
+    log(f"timing {args.steps} REL compress steps")
+    Z.timing_begin()
+    ms_rel = timed(rel_step, args.steps)
+    kt = Z.timing_end()
+    k_ms = kt["compress"][0] / max(1, kt["compress"][1])
+    _, _, csize2, _ = comp.stream_view()
+    assert csize2 == csize
+
+    log(f"timing {args.steps} ABS compress steps")
+    ms_abs = timed(abs_step, args.steps)
+
+    log(f"timing {args.steps} decompress steps (with record index)")
+    Z.timing_begin()
+    ms_dec = timed(dec_step, args.steps)
+    kd = Z.timing_end()
+    kd_ms = kd["decode"][0] / max(1, kd["decode"][1])
+    assert dec.check() == 0, "decoder rejected the stream"
+
+    ms_dec_noidx = None
+    if not args.no_noindex:
+        log("timing 1 decompress without the index (index rebuilt from the stream)")
+        ms_dec_noidx = timed(lambda: dec(view, csize, Z.DECODE_REFERENCE), 1)
+        assert dec.check() == 0
+
+    parity = None
+    xh = None
+    if not args.no_parity or not args.no_cpu:
+        xh = x.cpu().numpy()
+    if not args.no_parity:
+        log("parity check against the oracle at full size")
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        comp(x, Z.REL, relBoundRatio=args.rel)
+        s_gpu = comp.stream_view()[0].cpu().numpy().tobytes()
+        eb_o = O.rel_eb(xh, args.rel)
+        s_ora = O.compress(xh, eb_o, 128)
+        dec(view, csize, Z.DECODE_REFERENCE, index=comp.index)
+        d_gpu = dec.out.cpu().numpy()
+        d_ora = O.decompress(s_ora, n, np.float32)
+        dec(view, csize, Z.DECODE_REFERENCE)
+        d_gpu2 = dec.out.cpu().numpy() if not args.no_noindex else d_gpu
+        parity = {
+            "eb_identical": bool(np.float32(eb_o) == np.float32(eb)),
+            "stream_identical": s_gpu == s_ora,
+            "decode_identical": bool(np.array_equal(d_gpu.view(np.uint32), d_ora.view(np.uint32))),
+            "decode_noindex_identical": bool(np.array_equal(d_gpu2.view(np.uint32),
+                                                            d_ora.view(np.uint32))),
+            "max_abs_err": float(np.max(np.abs(d_gpu.astype(np.float64) - xh))),
+            "eb": eb,
+        }
+        parity["err_within_eb"] = parity["max_abs_err"] <= eb
+        log(f"parity {parity}")
+
+    cpu = None
+    if not args.no_cpu:
+        sample = min(n, 1 << 25)
+        log(f"CPU baseline on {sample} values")
+        kind, res = cpu_baseline(xh, eb, sample)
+        ref = res.get("openmp", res["serial"])
+        cpu = {"value": ref["compress_gbs"], "unit": "GB/s", "cores": ref["cores"],
+               "kind": kind,
+               "sample": f"first {sample} values ({sample * 4 / 2**20:.0f} MiB) of the same field, "
+                         f"ABS eb {eb:.6g}, blockSize 128, best of 3 after 1 warm-up; "
+                         + ("reference fastMode-4 OpenMP compress" if "openmp" in res
+                            else "oracle serial compress"),
+               "serial_fastmode2": res["serial"]}
+
+    alg_c = nbytes + csize
+    roof = {"bound": "hbm", "kernel": "k_compress", "achieved": alg_c / (k_ms * 1e-3) / 1e9,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": None,
+            "algorithmic_bytes": alg_c, "kernel_ms": k_ms}
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    line = {
+        "metric": METRIC,
+        "value": nbytes / (ms_rel * 1e-3) / 1e9,
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_rel,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{'x'.join(map(str, shape))} fp32 sinusoid field (64 waves, k<=32, "
+                               f"|k|^-2.5, noise 1e-4 std, 2 mean boxes 30%), REL {args.rel:g}, "
+                               f"blockSize 128, fastMode 2", "values": n, "rel": args.rel,
+                   "block_size": 128},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "compress_rel_gbs": nbytes / (ms_rel * 1e-3) / 1e9,
+        "compress_abs_gbs": nbytes / (ms_abs * 1e-3) / 1e9,
+        "compress_pct_hbm": alg_c / (ms_abs * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "decompress_gbs": nbytes / (ms_dec * 1e-3) / 1e9,
+        "decompress_pct_hbm": (nbytes + csize) / (ms_dec * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "decode_kernel_ms": kd_ms,
+        "decode_kernel_pct_hbm": (nbytes + csize) / (kd_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "decompress_noindex_gbs": (nbytes / (ms_dec_noidx * 1e-3) / 1e9) if ms_dec_noidx else None,
+        "compressed_bytes": csize,
+        "compression_ratio": nbytes / csize,
+        "eb": eb,
+        "parity": parity,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

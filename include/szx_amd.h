@@ -122,17 +122,25 @@ size_t szx_amd_compress_capacity(size_t n, int dataType, int blockSize);
 /* Workspace bytes for szx_amd_compress_dev. */
 size_t szx_amd_compress_workspace(size_t n, int dataType, int blockSize);
 
+/* uint64_t words of a record index for n values: per compressor tile (4096
+ * fp32 / 2048 fp64 values) the payload offset of the tile's first record and
+ * the number of constant blocks before the tile, plus a closing pair.  The
+ * index lives outside the stream (the stream stays byte-identical); handing
+ * it to szx_amd_decompress_dev spares the decoder the serial record walk. */
+size_t szx_amd_index_words(size_t n, int dataType);
+
 /* Compress n device values.  errBoundMode ABS or REL (REL's bound is computed
  * on the device with the reference's float rounding).  On completion
  * d_result[0] = byte offset of the stream inside d_out, d_result[1] = stream
  * size, d_result[2] = the absolute error bound used (as float bits).
- * d_result is device (or host-pinned) memory of 3 uint64_t.
+ * d_result is device (or host-pinned) memory of 3 uint64_t.  d_index, if not
+ * NULL, receives the record index (szx_amd_index_words() words).
  * Returns 0, or a negative SZx_* code (argument errors only; the call is
  * asynchronous). */
 int szx_amd_compress_dev(const void *d_in, size_t n, int dataType, int errBoundMode,
                          float absErrBound, float relBoundRatio, int blockSize, void *d_out,
                          size_t outCapacity, void *d_work, size_t workBytes, uint64_t *d_result,
-                         void *hipStream);
+                         uint64_t *d_index, void *hipStream);
 
 /* Workspace bytes for szx_amd_decompress_dev. */
 size_t szx_amd_decompress_workspace(size_t n, int dataType, int blockSize, size_t streamBytes);
@@ -144,11 +152,22 @@ size_t szx_amd_decompress_workspace(size_t n, int dataType, int blockSize, size_
                                         one (reqBytes 5, lead 0, byte 7) */
 
 /* Decompress a device-resident stream of streamBytes bytes into n values at
- * d_out.  blockSize is taken from the stream.  d_status (device memory, one
- * int32) receives 0 or a negative code for a malformed stream. */
+ * d_out.  blockSize must match the stream's (checked on the device).  d_index
+ * is the record index written by szx_amd_compress_dev, or NULL to rebuild it
+ * from the stream.  d_status (device memory, one int32) receives 0 or a
+ * negative code for a malformed stream or index. */
 int szx_amd_decompress_dev(const void *d_stream, size_t streamBytes, size_t n, int dataType,
-                           void *d_out, void *d_work, size_t workBytes, int flags,
-                           int32_t *d_status, void *hipStream);
+                           int blockSize, void *d_out, void *d_work, size_t workBytes,
+                           const uint64_t *d_index, int flags, int32_t *d_status,
+                           void *hipStream);
+
+/* Kernel timing for benchmarks: after szx_amd_timing_begin(), this thread's
+ * launches bracket their main kernels with HIP events on the launch stream;
+ * szx_amd_timing_end() waits for them and returns, per kernel kind
+ * (0 = compress, 1 = decode, 2 = record-index build), the summed milliseconds
+ * and the number of launches.  Arrays hold 3 entries. */
+void szx_amd_timing_begin(void);
+int szx_amd_timing_end(double *ms_sum, int *count);
 
 #ifdef __cplusplus
 }

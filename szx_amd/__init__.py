@@ -39,7 +39,8 @@ EXPORTS = (
     "SZx_estimateErrorBoundbasedonCR", "SZ_fast_compress_args", "SZ_fast_decompress",
     "szx_amd_last_error", "szx_amd_compress_bound", "szx_amd_compress_capacity",
     "szx_amd_compress_workspace", "szx_amd_compress_dev", "szx_amd_decompress_workspace",
-    "szx_amd_decompress_dev",
+    "szx_amd_decompress_dev", "szx_amd_index_words", "szx_amd_timing_begin",
+    "szx_amd_timing_end",
 )
 
 _lib = None
@@ -87,13 +88,37 @@ def lib() -> C.CDLL:
     L.szx_amd_compress_workspace.restype = sz
     L.szx_amd_compress_workspace.argtypes = [sz, i32, i32]
     L.szx_amd_compress_dev.restype = i32
-    L.szx_amd_compress_dev.argtypes = [vp, sz, i32, i32, f32, f32, i32, vp, sz, vp, sz, vp, vp]
+    L.szx_amd_compress_dev.argtypes = [vp, sz, i32, i32, f32, f32, i32, vp, sz, vp, sz, vp, vp,
+                                       vp]
+    L.szx_amd_index_words.restype = sz
+    L.szx_amd_index_words.argtypes = [sz, i32]
     L.szx_amd_decompress_workspace.restype = sz
     L.szx_amd_decompress_workspace.argtypes = [sz, i32, i32, sz]
     L.szx_amd_decompress_dev.restype = i32
-    L.szx_amd_decompress_dev.argtypes = [vp, sz, sz, i32, vp, vp, sz, i32, vp, vp]
+    L.szx_amd_decompress_dev.argtypes = [vp, sz, sz, i32, i32, vp, vp, sz, vp, i32, vp, vp]
+    L.szx_amd_timing_begin.restype = None
+    L.szx_amd_timing_begin.argtypes = []
+    L.szx_amd_timing_end.restype = i32
+    L.szx_amd_timing_end.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_int)]
     _lib = L
     return L
+
+
+KERNEL_KINDS = ("compress", "decode", "index")
+
+
+def timing_begin() -> None:
+    """Start bracketing this thread's main kernel launches with HIP events."""
+    lib().szx_amd_timing_begin()
+
+
+def timing_end() -> dict:
+    """{kind: (summed ms, launches)} for the launches since timing_begin()."""
+    ms = (C.c_double * 3)()
+    cnt = (C.c_int * 3)()
+    if lib().szx_amd_timing_end(ms, cnt) != 0:
+        raise SZxError(last_error())
+    return {k: (ms[i], cnt[i]) for i, k in enumerate(KERNEL_KINDS)}
 
 
 def last_error() -> str:
@@ -204,6 +229,8 @@ class DeviceCompressor:
         self.out = torch.empty(self.capacity, dtype=torch.uint8, device=device)
         self.ws = torch.empty(self.wsbytes, dtype=torch.uint8, device=device)
         self.result = torch.zeros(3, dtype=torch.int64, device=device)
+        self.index = torch.zeros(L.szx_amd_index_words(n, code), dtype=torch.int64,
+                                 device=device)
 
     def __call__(self, x, errBoundMode: int = ABS, absErrBound: float = 0.0,
                  relBoundRatio: float = 0.0, stream=None) -> None:
@@ -212,7 +239,7 @@ class DeviceCompressor:
                                         errBoundMode, absErrBound, relBoundRatio, self.bs,
                                         self.out.data_ptr(), self.capacity, self.ws.data_ptr(),
                                         self.wsbytes, self.result.data_ptr(),
-                                        _stream_handle(stream))
+                                        self.index.data_ptr(), _stream_handle(stream))
         if rc != 0:
             raise SZxError(last_error())
 
@@ -226,7 +253,7 @@ class DeviceCompressor:
 class DeviceDecompressor:
     def __init__(self, n: int, dtype, blockSize: int, stream_capacity: int, device="cuda"):
         torch = _torch()
-        self.n, self.dtype = n, np.dtype(dtype)
+        self.n, self.dtype, self.bs = n, np.dtype(dtype), blockSize
         code = _dtype_code(self.dtype)
         self.wsbytes = lib().szx_amd_decompress_workspace(n, code, blockSize, stream_capacity)
         self.ws = torch.empty(self.wsbytes, dtype=torch.uint8, device=device)
@@ -235,11 +262,14 @@ class DeviceDecompressor:
         self.status = torch.zeros(1, dtype=torch.int32, device=device)
 
     def __call__(self, stream_tensor, nbytes: int | None = None, flags: int = DECODE_REFERENCE,
-                 stream=None) -> None:
+                 index=None, stream=None) -> None:
+        """Enqueue one decode; `index` is the compressor's record index
+        (DeviceCompressor.index) or None to rebuild it from the stream."""
         nb = stream_tensor.numel() if nbytes is None else nbytes
         rc = lib().szx_amd_decompress_dev(stream_tensor.data_ptr(), nb, self.n,
-                                          _dtype_code(self.dtype), self.out.data_ptr(),
-                                          self.ws.data_ptr(), self.wsbytes, flags,
+                                          _dtype_code(self.dtype), self.bs, self.out.data_ptr(),
+                                          self.ws.data_ptr(), self.wsbytes,
+                                          None if index is None else index.data_ptr(), flags,
                                           self.status.data_ptr(), _stream_handle(stream))
         if rc != 0:
             raise SZxError(last_error())

@@ -97,6 +97,28 @@ struct HostCtx {
 
 thread_local HostCtx g_ctx;
 
+// Kernel timing (szx_amd_timing_*): events recorded around main kernels.
+struct TimingState {
+    bool on = false;
+    struct Pair { hipEvent_t a, b; int kind; };
+    Pair pairs[4096];
+    int npairs = 0;
+    hipEvent_t pending[KK_COUNT] = {};
+    hipEvent_t pool[8192];
+    int npool = 0, used = 0;
+    hipEvent_t get()
+    {
+        if (used < npool) return pool[used++];
+        if (npool >= 8192) return nullptr;
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        pool[npool++] = e;
+        used++;
+        return e;
+    }
+};
+thread_local TimingState g_tim;
+
 size_t data_length(size_t r5, size_t r4, size_t r3, size_t r2, size_t r1)
 {
     // szx.cc:73-101: trailing zero dims are absent, r1 is the fastest.
@@ -131,7 +153,7 @@ unsigned char *host_compress(const void *data, size_t n, int w, int rel_mode, fl
                 "hipMemcpyAsync H2D"))
         return nullptr;
     if (compress_launch(c.in.p, n, w, rel_mode, abs_eb, rel, bs, (uint8_t *)c.out.p,
-                        (uint8_t *)c.ws.p, (uint64_t *)c.res.p, c.stream)) {
+                        (uint8_t *)c.ws.p, (uint64_t *)c.res.p, nullptr, c.stream)) {
         set_err("compress launch failed");
         return nullptr;
     }
@@ -165,6 +187,10 @@ void *host_decompress(const unsigned char *bytes, size_t len, size_t n, int w, i
         return nullptr;
     }
     const int bs = bytes[4] ? bytes[4] : 256;
+    if (!bs_supported(bs)) {
+        set_err("blockSize %d is not supported (powers of two 4..256)", bs);
+        return nullptr;
+    }
     HostCtx &c = g_ctx;
     if (!c.ready()) return nullptr;
     const DecompressLayout L = decompress_layout(n, w, bs, len);
@@ -173,8 +199,8 @@ void *host_decompress(const unsigned char *bytes, size_t len, size_t n, int w, i
         return nullptr;
     if (!hip_ok(hipMemcpyAsync(c.in.p, bytes, len, hipMemcpyHostToDevice, c.stream), "H2D"))
         return nullptr;
-    if (decompress_launch((const uint8_t *)c.in.p, len, n, w, c.out.p, (uint8_t *)c.ws.p,
-                          L.ws_bytes, exact, (int32_t *)c.res.p, c.stream)) {
+    if (decompress_launch((const uint8_t *)c.in.p, len, n, w, bs, c.out.p, (uint8_t *)c.ws.p,
+                          L.ws_bytes, nullptr, exact, (int32_t *)c.res.p, c.stream)) {
         set_err("decompress launch failed");
         return nullptr;
     }
@@ -200,7 +226,54 @@ void *host_decompress(const unsigned char *bytes, size_t len, size_t n, int w, i
 
 }  // namespace
 
+namespace szxamd {
+void timing_mark_begin(int kind, hipStream_t st)
+{
+    TimingState &t = g_tim;
+    if (!t.on || kind < 0 || kind >= KK_COUNT) return;
+    hipEvent_t e = t.get();
+    if (e && hipEventRecord(e, st) == hipSuccess) t.pending[kind] = e;
+}
+void timing_mark_end(int kind, hipStream_t st)
+{
+    TimingState &t = g_tim;
+    if (!t.on || kind < 0 || kind >= KK_COUNT || !t.pending[kind] || t.npairs >= 4096) return;
+    hipEvent_t e = t.get();
+    if (e && hipEventRecord(e, st) == hipSuccess)
+        t.pairs[t.npairs++] = {t.pending[kind], e, kind};
+    t.pending[kind] = nullptr;
+}
+}  // namespace szxamd
+
 extern "C" {
+
+void szx_amd_timing_begin(void)
+{
+    g_tim.on = true;
+    g_tim.npairs = 0;
+    g_tim.used = 0;
+    for (int k = 0; k < KK_COUNT; k++) g_tim.pending[k] = nullptr;
+}
+
+int szx_amd_timing_end(double *ms_sum, int *count)
+{
+    TimingState &t = g_tim;
+    t.on = false;
+    for (int k = 0; k < KK_COUNT; k++) {
+        ms_sum[k] = 0.0;
+        count[k] = 0;
+    }
+    for (int i = 0; i < t.npairs; i++) {
+        if (!hip_ok(hipEventSynchronize(t.pairs[i].b), "hipEventSynchronize")) return -1;
+        float ms = 0.0f;
+        if (!hip_ok(hipEventElapsedTime(&ms, t.pairs[i].a, t.pairs[i].b), "hipEventElapsedTime"))
+            return -1;
+        ms_sum[t.pairs[i].kind] += ms;
+        count[t.pairs[i].kind] += 1;
+    }
+    t.npairs = 0;
+    return 0;
+}
 
 const char *szx_amd_last_error(void) { return g_err.c_str(); }
 
@@ -402,10 +475,16 @@ size_t szx_amd_compress_workspace(size_t n, int dataType, int blockSize)
     return compress_layout(n, w, blockSize).ws_bytes;
 }
 
+size_t szx_amd_index_words(size_t n, int dataType)
+{
+    const int w = width_of(dataType);
+    return w ? index_words(n, w) : 0;
+}
+
 int szx_amd_compress_dev(const void *d_in, size_t n, int dataType, int errBoundMode,
                          float absErrBound, float relBoundRatio, int blockSize, void *d_out,
                          size_t outCapacity, void *d_work, size_t workBytes, uint64_t *d_result,
-                         void *hipStream)
+                         uint64_t *d_index, void *hipStream)
 {
     g_err.clear();
     const int w = width_of(dataType);
@@ -418,7 +497,8 @@ int szx_amd_compress_dev(const void *d_in, size_t n, int dataType, int errBoundM
     if (outCapacity < L.capacity) { set_err("outCapacity < %llu", (unsigned long long)L.capacity); return SZx_NSCS; }
     if (workBytes < L.ws_bytes) { set_err("workBytes < %llu", (unsigned long long)L.ws_bytes); return SZx_NSCS; }
     if (compress_launch(d_in, n, w, errBoundMode == REL, absErrBound, relBoundRatio, blockSize,
-                        (uint8_t *)d_out, (uint8_t *)d_work, d_result, (hipStream_t)hipStream))
+                        (uint8_t *)d_out, (uint8_t *)d_work, d_result, d_index,
+                        (hipStream_t)hipStream))
         return SZx_NSCS;
     return hip_ok(hipGetLastError(), "launch") ? SZx_SCES : SZx_NSCS;
 }
@@ -431,17 +511,21 @@ size_t szx_amd_decompress_workspace(size_t n, int dataType, int blockSize, size_
 }
 
 int szx_amd_decompress_dev(const void *d_stream, size_t streamBytes, size_t n, int dataType,
-                           void *d_out, void *d_work, size_t workBytes, int flags,
-                           int32_t *d_status, void *hipStream)
+                           int blockSize, void *d_out, void *d_work, size_t workBytes,
+                           const uint64_t *d_index, int flags, int32_t *d_status, void *hipStream)
 {
     g_err.clear();
     const int w = width_of(dataType);
     if (!w) { set_err("unsupported dataType %d", dataType); return SZx_TERR; }
+    if (!bs_supported(blockSize)) { set_err("unsupported blockSize %d", blockSize); return SZx_NSCS; }
     if (!d_stream || !d_out || !d_work || !d_status || n == 0) { set_err("NULL argument or n == 0"); return SZx_NSCS; }
     if (((uintptr_t)d_out & 15) != 0) { set_err("d_out must be 16-byte aligned"); return SZx_NSCS; }
-    if (decompress_launch((const uint8_t *)d_stream, streamBytes, n, w, d_out, (uint8_t *)d_work,
-                          workBytes, flags & SZX_AMD_DECODE_EXACT, d_status, (hipStream_t)hipStream))
+    if (decompress_launch((const uint8_t *)d_stream, streamBytes, n, w, blockSize, d_out,
+                          (uint8_t *)d_work, workBytes, d_index, flags & SZX_AMD_DECODE_EXACT,
+                          d_status, (hipStream_t)hipStream)) {
+        set_err("workBytes too small or bad parameters");
         return SZx_NSCS;
+    }
     return hip_ok(hipGetLastError(), "launch") ? SZx_SCES : SZx_NSCS;
 }
 

@@ -38,6 +38,7 @@ struct CompressParams {
     TileState *tiles;
     unsigned *ticket;
     uint64_t *totals;     // [0] payload bytes, [1] constant blocks
+    uint64_t *index;      // optional record index: per tile {payload offset, constants before}
     uint64_t ntiles;
     uint64_t nbit;
 };
@@ -299,9 +300,17 @@ __global__ __launch_bounds__(256) void k_compress(CompressParams p)
         }
         s_excl[0] = ebytes;
         s_excl[1] = econst;
+        if (p.index) {
+            p.index[2 * tile] = ebytes;
+            p.index[2 * tile + 1] = econst;
+        }
         if (tile == p.ntiles - 1) {
             p.totals[0] = ebytes + tbytes;
             p.totals[1] = econst + tconst;
+            if (p.index) {
+                p.index[2 * p.ntiles] = ebytes + tbytes;
+                p.index[2 * p.ntiles + 1] = econst + tconst;
+            }
         }
     }
     __syncthreads();
@@ -457,11 +466,9 @@ __global__ void k_range_final(const T *x, const T *part, int nparts, float rel, 
 // host side
 // ---------------------------------------------------------------------------
 
-static constexpr int kSlabsF32 = 4;
-static constexpr int kSlabsF64 = 2;
+static constexpr int kSlabsF32 = TILE_VALUES_F32 / 1024;
+static constexpr int kSlabsF64 = TILE_VALUES_F64 / 1024;
 static constexpr int kRangeParts = 1024;
-
-static int tile_values(int w) { return 1024 * (w == 4 ? kSlabsF32 : kSlabsF64); }
 
 CompressLayout compress_layout(uint64_t n, int w, int bs)
 {
@@ -509,7 +516,8 @@ static int dispatch_bs(int bs, const CompressParams &p, uint64_t ntiles, hipStre
 }
 
 int compress_launch(const void *d_in, uint64_t n, int w, int rel_mode, float abs_eb, float rel,
-                    int bs, uint8_t *d_out, uint8_t *d_ws, uint64_t *d_result, hipStream_t st)
+                    int bs, uint8_t *d_out, uint8_t *d_ws, uint64_t *d_result, uint64_t *d_index,
+                    hipStream_t st)
 {
     const CompressLayout L = compress_layout(n, w, bs);
     unsigned long long *misc = reinterpret_cast<unsigned long long *>(d_ws + L.ws_misc);
@@ -544,10 +552,13 @@ int compress_launch(const void *d_in, uint64_t n, int w, int rel_mode, float abs
     p.tiles = reinterpret_cast<TileState *>(d_ws + L.ws_tiles);
     p.ticket = reinterpret_cast<unsigned *>(misc);
     p.totals = reinterpret_cast<uint64_t *>(misc + 2);
+    p.index = d_index;
     p.ntiles = L.ntiles;
     p.nbit = L.nbit;
+    timing_mark_begin(KK_COMPRESS, st);
     const int rc = w == 4 ? dispatch_bs<float>(bs, p, L.ntiles, st)
                           : dispatch_bs<double>(bs, p, L.ntiles, st);
+    timing_mark_end(KK_COMPRESS, st);
     if (rc) return rc;
     AssembleParams a{};
     a.out = d_out;

@@ -7,6 +7,24 @@
 
 namespace szxamd {
 
+// Values per tile (one workgroup of the compressor, and the unit of the
+// record index shared by the compressor and the decompressor).
+constexpr int TILE_VALUES_F32 = 4096;
+constexpr int TILE_VALUES_F64 = 2048;
+inline uint64_t tile_values(int w) { return w == 4 ? TILE_VALUES_F32 : TILE_VALUES_F64; }
+inline uint64_t index_words(uint64_t n, int w)
+{
+    uint64_t t = (n + tile_values(w) - 1) / tile_values(w);
+    return 2 * ((t ? t : 1) + 1);
+}
+
+// Optional kernel timing: when enabled on this thread, the launchers bracket
+// each main kernel (compress, decode, index build) with a pair of HIP events
+// on the launch stream.
+enum KernelKind { KK_COMPRESS = 0, KK_DECODE = 1, KK_INDEX = 2, KK_COUNT = 3 };
+void timing_mark_begin(int kind, hipStream_t st);
+void timing_mark_end(int kind, hipStream_t st);
+
 // Device memory layout of one compress call.
 struct CompressLayout {
     uint64_t nb;          // blocks
@@ -21,22 +39,24 @@ struct CompressLayout {
 CompressLayout compress_layout(uint64_t n, int w, int bs);
 
 // Enqueue a whole compress on `st`.  d_result receives
-// {stream offset in d_out, stream bytes, bound bits}.  Returns <0 for an
-// unsupported block size.
+// {stream offset in d_out, stream bytes, bound bits}; d_index (optional)
+// receives the record index.  Returns <0 for an unsupported block size.
 int compress_launch(const void *d_in, uint64_t n, int w, int rel_mode, float abs_eb, float rel,
-                    int bs, uint8_t *d_out, uint8_t *d_ws, uint64_t *d_result, hipStream_t st);
+                    int bs, uint8_t *d_out, uint8_t *d_ws, uint64_t *d_result, uint64_t *d_index,
+                    hipStream_t st);
 
 // Device memory layout of one decompress call.
 struct DecompressLayout {
-    uint64_t nb_max;      // blocks for the smallest supported block size
-    uint64_t ws_hdr, ws_info, ws_bytes;
+    uint64_t ws_hdr, ws_index, ws_bytes;
 };
 
-DecompressLayout decompress_layout(uint64_t n, int w, int bs_hint, uint64_t stream_bytes);
+DecompressLayout decompress_layout(uint64_t n, int w, int bs, uint64_t stream_bytes);
 
 // Enqueue a whole decompress on `st`; d_status receives 0 or a negative code.
-int decompress_launch(const uint8_t *d_stream, uint64_t stream_bytes, uint64_t n, int w,
-                      void *d_out, uint8_t *d_ws, uint64_t ws_bytes, int exact, int32_t *d_status,
-                      hipStream_t st);
+// d_index (optional) is the record index the compressor wrote; without it the
+// index is rebuilt from the stream.
+int decompress_launch(const uint8_t *d_stream, uint64_t stream_bytes, uint64_t n, int w, int bs,
+                      void *d_out, uint8_t *d_ws, uint64_t ws_bytes, const uint64_t *d_index,
+                      int exact, int32_t *d_status, hipStream_t st);
 
 }  // namespace szxamd

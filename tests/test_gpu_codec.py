@@ -114,9 +114,14 @@ def test_device_api_and_exact_decode(dtype, device):
     assert np.float32(eb_used) == np.float32(eb)
     assert s == O.compress(x, eb, 128)
     dec = Z.DeviceDecompressor(n, dtype, 128, size)
-    dec(view, size, Z.DECODE_REFERENCE)
+    ref_dec = O.decompress(s, n, dtype, ref_quirk=True)
+    dec(view, size, Z.DECODE_REFERENCE)                      # index rebuilt from the stream
     assert dec.check() == 0
-    assert bits_equal(dec.out.cpu().numpy(), O.decompress(s, n, dtype, ref_quirk=True))
+    assert bits_equal(dec.out.cpu().numpy(), ref_dec)
+    dec.out.zero_()
+    dec(view, size, Z.DECODE_REFERENCE, index=comp.index)    # the compressor's index
+    assert dec.check() == 0
+    assert bits_equal(dec.out.cpu().numpy(), ref_dec)
     dec(view, size, Z.DECODE_EXACT)
     assert dec.check() == 0
     exact = dec.out.cpu().numpy()
@@ -152,3 +157,37 @@ def test_malformed_streams_are_rejected(device):
     # a decode after a rejected stream still works
     d = Z.SZx_fast_decompress(Z.SZx_WITH_BLOCK_FAST_CMPR, np.float32, s, x.shape)
     assert bits_equal(d, O.decompress(s, x.size, np.float32))
+
+
+@pytest.mark.parametrize("bs", POW2)
+def test_index_path_all_block_sizes(bs, device):
+    import torch
+    r = np.random.default_rng(bs)
+    for dtype in (np.float32, np.float64):
+        n = int(r.integers(20000, 90000))
+        x = _field(r, n, dtype, int(r.integers(0, 5)))
+        eb = float(np.float32((float(np.ptp(x)) + 1e-30) * 10.0 ** r.uniform(-9, -2)))
+        say("index path", bs, dtype.__name__, n, eb)
+        comp = Z.DeviceCompressor(n, dtype, bs)
+        comp(torch.from_numpy(x).to(device), Z.ABS, eb)
+        view, off, size, _ = comp.stream_view()
+        s = view.cpu().numpy().tobytes()
+        assert s == O.compress(x, eb, bs)
+        dec = Z.DeviceDecompressor(n, dtype, bs, size)
+        dec(view, size, Z.DECODE_REFERENCE, index=comp.index)
+        assert dec.check() == 0
+        assert bits_equal(dec.out.cpu().numpy(), O.decompress(s, n, dtype, ref_quirk=True))
+
+
+def test_corrupt_index_is_rejected(device):
+    import torch
+    x = np.sin(np.arange(200000) * 0.001).astype(np.float32)
+    comp = Z.DeviceCompressor(x.size, np.float32, 128)
+    comp(torch.from_numpy(x).to(device), Z.ABS, 1e-5)
+    view, off, size, _ = comp.stream_view()
+    bad = comp.index.clone()
+    bad[4] += 3                      # shift one tile's start off its first record
+    dec = Z.DeviceDecompressor(x.size, np.float32, 128, size)
+    say("corrupt index")
+    dec(view, size, Z.DECODE_REFERENCE, index=bad)
+    assert dec.check() != 0
