@@ -19,12 +19,13 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libszx_amd.so")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HOSTCXX = os.environ.get("CXX", "g++")
 ARCH = os.environ.get("SZX_AMD_ARCH", "gfx950")
 
 SOURCES = ["szx_compress.hip", "szx_decompress.hip", "szx_api.cpp"]
-HEADERS = ["szx_common.hpp", "szx_internal.hpp"]
+HEADERS = ["szx_common.hpp", "szx_internal.hpp", "szx_wave.hpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",
           "-Wno-unused-result"]
@@ -44,7 +45,7 @@ def build(verbose: bool = False) -> str:
             cmd = [HIPCC, "-x", "hip", *COMMON, *DEVICE, "-c", path, "-o", obj]
         else:   # host-only C++ against the HIP runtime headers
             cmd = [HOSTCXX, *COMMON, "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
-                   "-c", path, "-o", obj]
+                   "-I" + INCLUDE, "-c", path, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)

@@ -4,7 +4,7 @@
 // pointers in, malloc'd buffers out) and move the data through device buffers
 // that are cached per thread and device, so repeated calls do not pay for
 // hipMalloc.  Device entry points only enqueue kernels.
-#include "../../include/szx_amd.h"
+#include "szx_amd.h"
 
 #include <hip/hip_runtime.h>
 #include <stdarg.h>

@@ -11,8 +11,8 @@
 //     XOR leading-byte encoding are computed in registers; the records of the
 //     tile are staged in LDS;
 //   * the tile's record bytes and constant-block count are published through
-//     a decoupled look-back (tickets give tiles in launch order, so every
-//     predecessor a tile waits on is already resident);
+//     a two-level decoupled look-back (tiles within 64-tile chunks, then
+//     chunk totals; the tile order is the launch order, blockIdx.x);
 //   * records are written at payload + (bytes of all earlier records).
 //
 // The payload start does not depend on the number of constant blocks: the
@@ -24,6 +24,7 @@
 // of the input plus one write of the stream.
 #include "szx_common.hpp"
 #include "szx_internal.hpp"
+#include "szx_wave.hpp"
 
 namespace szxamd {
 
@@ -35,8 +36,8 @@ struct CompressParams {
     uint8_t *payload;     // first record byte
     uint8_t *bitmap;      // scratch bitmap, nbit bytes
     uint32_t *cmed;       // scratch constant medians, big-endian words
-    TileState *tiles;
-    unsigned *ticket;
+    unsigned long long *tiles;   // per-tile look-back words
+    ChunkState *chunks;          // per-64-tile look-back words
     uint64_t *totals;     // [0] payload bytes, [1] constant blocks
     uint64_t *index;      // optional record index: per tile {payload offset, constants before}
     uint64_t ntiles;
@@ -45,7 +46,7 @@ struct CompressParams {
 
 template <typename T> struct SlabLoad;
 template <> struct SlabLoad<float> {
-    __device__ static void load4(const float *x, uint64_t e0, uint64_t n, float v[4])
+    __device__ static void load4(const float *x, uint64_t e0, uint64_t n, float *v)
     {
         if (e0 + 4 <= n) {
             float4 f = *reinterpret_cast<const float4 *>(x + e0);
@@ -56,7 +57,7 @@ template <> struct SlabLoad<float> {
     }
 };
 template <> struct SlabLoad<double> {
-    __device__ static void load4(const double *x, uint64_t e0, uint64_t n, double v[4])
+    __device__ static void load4(const double *x, uint64_t e0, uint64_t n, double *v)
     {
         if (e0 + 4 <= n) {
             double2 a = *reinterpret_cast<const double2 *>(x + e0);
@@ -100,144 +101,245 @@ __device__ inline uint32_t wave0_exclusive_scan(uint32_t *a, int lane)
     return __shfl(inc, 63, 64);
 }
 
-template <typename T, int BS, int SLABS>
+// What a lane keeps of one slab between encoding and staging.
+//   meta  = mid-byte offset in the record (12) | rb (4) << 12 |
+//           non-constant << 16 | reqLength (7) << 17
+//   code  = the lane's 2-bit lead codes (LV/4 bytes) | lead bytes of the
+//           block (7) << 16
+//   mbits = LV 4-bit mid-byte counts
+template <typename T, int LV> struct SlabState {
+    typename Word<T>::U s[LV];
+    uint32_t meta, code, mbits;
+    float hmed;
+};
+
+template <typename T> __device__ inline T vmin(T a, T b);
+template <typename T> __device__ inline T vmax(T a, T b);
+template <> __device__ inline float vmin<float>(float a, float b) { return __builtin_fminf(a, b); }
+template <> __device__ inline float vmax<float>(float a, float b) { return __builtin_fmaxf(a, b); }
+template <> __device__ inline double vmin<double>(double a, double b) { return __builtin_fmin(a, b); }
+template <> __device__ inline double vmax<double>(double a, double b) { return __builtin_fmax(a, b); }
+
+// Leading bytes (capped at 3) that s shares with the previous word: the
+// reference tests xr >> 8(W-3), >> 8(W-2), >> 8(W-1) against zero
+// (szx_float.cc:96-101, szx_double.cc:80-85), i.e. min(clz(xr) / 8, 3) on the
+// top 32 bits of the word.
+template <typename U> __device__ inline int lead_bytes(U xr)
+{
+    const uint32_t top = sizeof(U) == 4 ? (uint32_t)xr : (uint32_t)((uint64_t)xr >> 32);
+    const int lz = __builtin_clzg(top, 32);
+    return (lz >> 3) < 3 ? (lz >> 3) : 3;
+}
+
+// Order-preserving integer key of a float: signed comparison of keys is the
+// float order (with -0 < +0; NaNs land beyond +-inf).  Its own inverse.
+__device__ inline int32_t fkey(uint32_t b) { return (int32_t)(b ^ ((uint32_t)((int32_t)b >> 31) >> 1)); }
+constexpr int32_t FKEY_NEG_INF = (int32_t)0x807FFFFFu;   // fkey(-inf)
+constexpr int32_t FKEY_POS_INF = 0x7F800000;             // fkey(+inf)
+
+// Exact block min/max with the reference's NaN behaviour: NaNs never win a
+// comparison (szx_float.cc:327-333), so fmin/fmax over the values, except that
+// a NaN first value makes both NaN.
+template <typename T, int LV, int SEG, bool FULL>
+__device__ inline void minmax_float(const T (&v)[LV], const bool (&valid)[LV], T &mn, T &mx)
+{
+    using U = typename Word<T>::U;
+    mn = pos_inf<T>();
+    mx = -pos_inf<T>();
+#pragma unroll
+    for (int k = 0; k < LV; k++) {
+        if (FULL || valid[k]) {
+            mn = vmin(mn, v[k]);
+            mx = vmax(mx, v[k]);
+        }
+    }
+    mn = from_bits<T>(seg_reduce<SEG>(to_bits<T>(mn), [](U a, U b) {
+        return to_bits<T>(vmin(from_bits<T>(a), from_bits<T>(b)));
+    }));
+    mx = from_bits<T>(seg_reduce<SEG>(to_bits<T>(mx), [](U a, U b) {
+        return to_bits<T>(vmax(from_bits<T>(a), from_bits<T>(b)));
+    }));
+    const T first = from_bits<T>(seg_first<SEG>(to_bits<T>(v[0])));
+    if (first != first) { mn = first; mx = first; }
+}
+
+// Encode one slab: block statistics, reqLength, shifted words, lead codes and
+// mid-byte counts, all in registers.  FULL = every value of the slab exists.
+// recb = record bytes of the lane's block (0 if constant), isconst, med = the
+// block median as float.
+template <typename T, int BS, int LV, bool FULL>
+__device__ __forceinline__ void encode_slab(const T (&v)[LV], uint64_t e0, uint64_t n, int lane,
+                                            int lseg, int reqexp, float eb, SlabState<T, LV> &st,
+                                            uint32_t &recb, uint32_t &isconst, float &med_out)
+{
+    using U = typename Word<T>::U;
+    constexpr int W = Word<T>::W;
+    constexpr int SEG = BS / LV;
+    bool exists = true;
+    int nel = BS;
+    bool valid[LV];
+#pragma unroll
+    for (int k = 0; k < LV; k++) valid[k] = true;
+    if constexpr (!FULL) {
+        const uint64_t bstart = e0 - (uint64_t)LV * (uint64_t)lseg;
+        exists = bstart < n;
+        const uint64_t rem = exists ? n - bstart : 0;
+        nel = rem >= (uint64_t)BS ? BS : (int)rem;
+#pragma unroll
+        for (int k = 0; k < LV; k++) valid[k] = e0 + k < n;
+    }
+
+    T mn, mx;
+    if constexpr (W == 4) {
+        // integer keys reduce with DPP-fused v_min_i32 / v_max_i32; a block
+        // holding a NaN shows up as a key beyond +-inf and takes the exact
+        // float path (the whole wave, rarely)
+        int32_t kmn = 0x7FFFFFFF, kmx = (int32_t)0x80000000;
+#pragma unroll
+        for (int k = 0; k < LV; k++) {
+            const int32_t key = fkey(f32_bits(v[k]));
+            if (FULL || valid[k]) {
+                kmn = key < kmn ? key : kmn;
+                kmx = key > kmx ? key : kmx;
+            }
+        }
+        kmn = (int32_t)seg_reduce<SEG>((uint32_t)kmn, [](uint32_t a, uint32_t b) {
+            return (uint32_t)((int32_t)a < (int32_t)b ? (int32_t)a : (int32_t)b);
+        });
+        kmx = (int32_t)seg_reduce<SEG>((uint32_t)kmx, [](uint32_t a, uint32_t b) {
+            return (uint32_t)((int32_t)a > (int32_t)b ? (int32_t)a : (int32_t)b);
+        });
+        const bool has_nan = exists && (kmn < FKEY_NEG_INF || kmx > FKEY_POS_INF);
+        if (__builtin_expect(__ballot(has_nan) != 0, 0)) {
+            minmax_float<T, LV, SEG, FULL>(v, valid, mn, mx);
+        } else {
+            mn = bits_f32((uint32_t)fkey((uint32_t)kmn));
+            mx = bits_f32((uint32_t)fkey((uint32_t)kmx));
+        }
+    } else {
+        minmax_float<T, LV, SEG, FULL>(v, valid, mn, mx);
+    }
+
+    // radius, median, constant test (szx_float.cc:335-347,
+    // szx_double.cc:990-1003): median and radius are kept as float.
+    const T rad = (mx - mn) * (T)0.5;
+    const T med = mn + rad;
+    const float med_f = (float)med;
+    const float rad_f = (float)rad;
+    const bool nonconst = exists && !(rad <= (T)eb);
+
+    int clamped;
+    const int len = req_length<T>(rad_expo(rad_f), reqexp, clamped);
+    const int rb = (len + 7) >> 3;
+    const int shift = (-len) & 7;
+    const float hmed = clamped ? 0.0f : med_f;
+    // fp32 subtracts the stored median; fp64 keeps the unclamped one
+    // (szx_float.cc:60,90 vs szx_double.cc:38,78).
+    const T msub = W == 4 ? (T)hmed : (T)med_f;
+
+#pragma unroll
+    for (int k = 0; k < LV; k++) st.s[k] = to_bits<T>(v[k] - msub) >> shift;
+    U prev = seg_prev<SEG>(st.s[LV - 1], lane);
+
+    uint32_t code = 0, mbits = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < LV; k++) {
+        const int ld = lead_bytes<U>(st.s[k] ^ prev);
+        int m = rb - ld;
+        m = m > 0 ? m : 0;
+        uint32_t ldv = (uint32_t)ld;
+        if constexpr (!FULL) {
+            if (!valid[k]) { m = 0; ldv = 0; }
+        }
+        // element k -> byte k/4 of the lane's lead codes, bits 7-6 first
+        code |= ldv << (8 * (k >> 2) + 6 - 2 * (k & 3));
+        mbits |= (uint32_t)m << (4 * k);
+        tot += (uint32_t)m;
+        prev = st.s[k];
+    }
+    const uint32_t inc = seg_scan_incl<SEG>(tot, lane);
+    const uint32_t midtot = seg_last<SEG>(inc);
+    const uint32_t nlead = (uint32_t)(nel + 3) >> 2;
+
+    st.meta = (inc - tot) | ((uint32_t)rb << 12) | ((nonconst ? 1u : 0u) << 16) |
+              ((uint32_t)len << 17);
+    st.code = code | (nlead << 16);
+    st.mbits = mbits;
+    st.hmed = hmed;
+    recb = nonconst ? 5u + nlead + midtot : 0u;
+    isconst = exists && !nonconst ? 1u : 0u;
+    med_out = med_f;
+}
+
+// Write the m low bytes of t to LDS at byte `pos` (every payload byte of a
+// tile is written exactly once, so plain byte stores need no zeroing).
+template <typename U>
+__device__ inline void stage_bytes(uint8_t *stage, uint32_t pos, U t, int m)
+{
+#pragma unroll
+    for (int j = 0; j < (int)sizeof(U); j++)
+        if (j < m) stage[pos + j] = (uint8_t)(t >> (8 * j));
+}
+
+template <typename T, int BS, int LV, int SLABS>
 __global__ __launch_bounds__(256) void k_compress(CompressParams p)
 {
     using U = typename Word<T>::U;
     constexpr int W = Word<T>::W;
-    constexpr int SEG = BS / 4;           // lanes per block
-    constexpr int SLAB = 1024;            // values per slab (256 lanes x 4)
+    constexpr int SEG = BS / LV;          // lanes per block
+    constexpr int NLB = LV / 4;           // lead-code bytes per lane
+    constexpr int SLAB = 256 * LV;        // values per slab
     constexpr int TILE = SLAB * SLABS;
     constexpr int BPS = SLAB / BS;        // blocks per slab
     constexpr int NBLK = TILE / BS;       // blocks per tile (multiple of 8)
     constexpr int STAGE = TILE * W + NBLK * (5 + BS / 4);
-    static_assert(BS >= 4 && BS <= 256 && (BS & (BS - 1)) == 0, "power-of-two block size");
+    constexpr int STAGE_DW = (STAGE + 32 + 15) / 16 * 4;
+    static_assert(BS >= LV && BS <= 256 && (BS & (BS - 1)) == 0, "power-of-two block size");
     static_assert(NBLK % 8 == 0, "tiles own whole bitmap bytes");
     static_assert(STAGE < (1 << 20), "record bytes fit the packed scan word");
 
     __shared__ uint32_t s_pack[NBLK];     // record bytes | constant<<20 -> exclusive prefix
     __shared__ uint8_t s_state[NBLK];     // 1 = non-constant (bitmap bit)
     __shared__ float s_med[NBLK];         // medians (constant blocks)
-    __shared__ uint8_t s_stage[STAGE + 16];
+    __shared__ __attribute__((aligned(16))) uint32_t s_stage32[STAGE_DW];
     __shared__ unsigned long long s_excl[2];
     __shared__ uint32_t s_total;
-    __shared__ unsigned s_tile;
+    uint8_t *s_stage = reinterpret_cast<uint8_t *>(s_stage32);
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    const int lseg = tid % SEG;
-    if (tid == 0) s_tile = atomicAdd(p.ticket, 1u);
-    __syncthreads();
-    const uint64_t tile = s_tile;
+    const int lseg = tid & (SEG - 1);
+    const uint64_t tile = blockIdx.x;
     const float eb = p.eb_dev ? *p.eb_dev : p.eb;
     const int reqexp = req_expo(eb);
     const uint64_t n = p.n;
     const T *x = static_cast<const T *>(p.x);
+    const uint64_t t0 = tile * TILE;
+    const bool full = t0 + TILE <= n;
 
-    U s[SLABS][4];
-    uint32_t excl_b[SLABS];   // lane's first mid-byte within its record's mid-byte area
-    uint32_t code_b[SLABS];   // lane's lead-code byte
-    uint32_t mpack[SLABS];    // 4 x 8-bit mid-byte counts
-    int rb_[SLABS], nlead_[SLABS], len_[SLABS];
-    float hmed_[SLABS];
-    bool nonconst_[SLABS];
+    // issue every load of the tile before any arithmetic
+    T v[SLABS][LV];
+#pragma unroll
+    for (int u = 0; u < SLABS; u++)
+#pragma unroll
+        for (int q = 0; q < LV; q += 4)
+            SlabLoad<T>::load4(x, t0 + (uint64_t)u * SLAB + (uint64_t)LV * tid + q, n, &v[u][q]);
 
+    SlabState<T, LV> st[SLABS];
 #pragma unroll
     for (int u = 0; u < SLABS; u++) {
-        const uint64_t e0 = tile * TILE + (uint64_t)u * SLAB + 4u * (uint64_t)tid;
-        const uint64_t bstart = e0 - 4u * (uint64_t)lseg;
-        const bool exists = bstart < n;
-        const uint64_t rem = exists ? n - bstart : 0;
-        const int nel = rem >= (uint64_t)BS ? BS : (int)rem;
-        T v[4];
-        SlabLoad<T>::load4(x, e0, n, v);
-        bool valid[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) valid[k] = e0 + k < n;
-
-        // min / max: NaNs never win a comparison, as in the reference's
-        // sequential scan (szx_float.cc:327-333); a NaN first value poisons
-        // both, exactly as it does there.
-        T mn = pos_inf<T>(), mx = -pos_inf<T>();
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            if (valid[k]) {
-                mn = v[k] < mn ? v[k] : mn;
-                mx = v[k] > mx ? v[k] : mx;
-            }
-        }
-#pragma unroll
-        for (int m = 1; m < SEG; m <<= 1) {
-            T o = __shfl_xor(mn, m, SEG);
-            mn = o < mn ? o : mn;
-            o = __shfl_xor(mx, m, SEG);
-            mx = o > mx ? o : mx;
-        }
-        const T first = __shfl(v[0], 0, SEG);
-        if (first != first) { mn = first; mx = first; }
-
-        // radius, median, constant test (szx_float.cc:335-347,
-        // szx_double.cc:990-1003): median and radius are stored as float.
-        const T rad = (mx - mn) * (T)0.5;
-        const T med = mn + rad;
-        const float med_f = (float)med;
-        const float rad_f = (float)rad;
-        const bool nonconst = exists && !(rad <= (T)eb);
-
-        int clamped;
-        const int len = req_length<T>(rad_expo(rad_f), reqexp, clamped);
-        const int rb = (len + 7) >> 3;
-        const int shift = (8 - (len & 7)) & 7;
-        const float hmed = clamped ? 0.0f : med_f;
-        // fp32 subtracts the stored median; fp64 keeps the unclamped one
-        // (szx_float.cc:60,90 vs szx_double.cc:38,78).
-        const T msub = W == 4 ? (T)hmed : (T)med_f;
-
-#pragma unroll
-        for (int k = 0; k < 4; k++) s[u][k] = to_bits<T>(v[k] - msub) >> shift;
-        U prev = __shfl_up(s[u][3], 1, SEG);
-        if (lseg == 0) prev = 0;
-
-        uint32_t code = 0, mp = 0, tot = 0;
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const U xr = s[u][k] ^ prev;
-            int ld = 0;
-            if ((xr >> (8 * (W - 3))) == 0) ld = 3;
-            else if ((xr >> (8 * (W - 2))) == 0) ld = 2;
-            else if ((xr >> (8 * (W - 1))) == 0) ld = 1;
-            const int m = valid[k] && ld < rb ? rb - ld : 0;
-            if (valid[k]) code |= (uint32_t)ld << (6 - 2 * k);
-            mp |= (uint32_t)m << (8 * k);
-            tot += m;
-            prev = s[u][k];
-        }
-        // segment-inclusive scan of the lane's mid-byte count
-        uint32_t inc = tot;
-#pragma unroll
-        for (int d = 1; d < SEG; d <<= 1) {
-            uint32_t o = __shfl_up(inc, d, SEG);
-            if (lseg >= d) inc += o;
-        }
-        const uint32_t midtot = __shfl(inc, SEG - 1, SEG);
-        const int nlead = (nel + 3) >> 2;
-
-        excl_b[u] = inc - tot;
-        code_b[u] = code;
-        mpack[u] = mp;
-        rb_[u] = rb;
-        nlead_[u] = nlead;
-        len_[u] = len;
-        hmed_[u] = hmed;
-        nonconst_[u] = nonconst;
-
+        const uint64_t e0 = t0 + (uint64_t)u * SLAB + (uint64_t)LV * tid;
+        uint32_t recb, isconst;
+        float med;
+        if (full)
+            encode_slab<T, BS, LV, true>(v[u], e0, n, lane, lseg, reqexp, eb, st[u], recb, isconst, med);
+        else
+            encode_slab<T, BS, LV, false>(v[u], e0, n, lane, lseg, reqexp, eb, st[u], recb, isconst, med);
         if (lseg == 0) {
             const int lb = u * BPS + tid / SEG;
-            const uint32_t recb = nonconst ? 5u + (uint32_t)nlead + midtot : 0u;
-            const uint32_t isconst = exists && !nonconst ? 1u : 0u;
             s_pack[lb] = recb | (isconst << 20);
-            s_state[lb] = nonconst ? 1 : 0;
-            s_med[lb] = med_f;
+            s_state[lb] = recb ? 1 : 0;
+            s_med[lb] = med;
         }
     }
     __syncthreads();
@@ -248,68 +350,60 @@ __global__ __launch_bounds__(256) void k_compress(CompressParams p)
     }
     __syncthreads();
 
-    // Stage the tile's records in LDS (byte-addressed, tile-relative).
+    // Stage the tile's records in LDS (tile-relative byte offsets; each byte
+    // is written once, by the lane that owns it).
 #pragma unroll
     for (int u = 0; u < SLABS; u++) {
-        if (!nonconst_[u]) continue;
+        const uint32_t meta = st[u].meta;
+        if (!((meta >> 16) & 1)) continue;
         const int lb = u * BPS + tid / SEG;
         const uint32_t base = s_pack[lb] & 0xFFFFFu;
-        if (lseg == 0) {
-            const uint32_t hb = bswap32(f32_bits(hmed_[u]));
-            s_stage[base] = (uint8_t)len_[u];
+        const uint32_t nlead = st[u].code >> 16;
+        const int rb = (meta >> 12) & 15;
+        if (lseg == 0) {   // reqLength byte + big-endian median
+            const uint32_t hb = bswap32(f32_bits(st[u].hmed));
+            s_stage[base] = (uint8_t)(meta >> 17);
             s_stage[base + 1] = (uint8_t)hb;
             s_stage[base + 2] = (uint8_t)(hb >> 8);
             s_stage[base + 3] = (uint8_t)(hb >> 16);
             s_stage[base + 4] = (uint8_t)(hb >> 24);
         }
-        if (lseg < nlead_[u]) s_stage[base + 5 + lseg] = (uint8_t)code_b[u];
-        uint32_t pos = base + 5 + (uint32_t)nlead_[u] + excl_b[u];
-        const int lo = W - rb_[u];
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const int m = (mpack[u] >> (8 * k)) & 0xFF;
-            for (int j = 0; j < m; j++) s_stage[pos++] = (uint8_t)(s[u][k] >> (8 * (lo + j)));
+        for (int b = 0; b < NLB; b++)
+            if ((uint32_t)(lseg * NLB + b) < nlead)
+                s_stage[base + 5 + lseg * NLB + b] = (uint8_t)(st[u].code >> (8 * b));
+        uint32_t pos = base + 5 + nlead + (meta & 0xFFFu);
+        const int lo = W - rb;
+#pragma unroll
+        for (int k = 0; k < LV; k++) {
+            const int m = (st[u].mbits >> (4 * k)) & 15;
+            stage_bytes<U>(s_stage, pos, st[u].s[k] >> (8 * lo), m);
+            pos += m;
         }
     }
 
     // Decoupled look-back over (record bytes, constant blocks).
-    if (tid == 0) {
+    __syncthreads();
+    {
         const uint32_t tw = s_total;
         const unsigned long long tbytes = tw & 0xFFFFFu;
         const unsigned long long tconst = tw >> 20;
-        unsigned long long ebytes = 0, econst = 0;
-        TileState *ts = p.tiles;
-        if (tile == 0) {
-            ts[0].cnt_inc = tconst;
-            store_release(&ts[0].status, FLAG_INC | tbytes);
-        } else {
-            ts[tile].cnt_agg = tconst;
-            store_release(&ts[tile].status, FLAG_AGG | tbytes);
-            int64_t j = (int64_t)tile - 1;
-            while (j >= 0) {
-                const unsigned long long st = load_acquire(&ts[j].status);
-                const unsigned long long f = st & ~VAL_MASK;
-                if (f == 0) { __builtin_amdgcn_s_sleep(1); continue; }
-                ebytes += st & VAL_MASK;
-                if (f == FLAG_INC) { econst += ts[j].cnt_inc; break; }
-                econst += ts[j].cnt_agg;
-                j--;
-            }
-            ts[tile].cnt_inc = econst + tconst;
-            store_release(&ts[tile].status, FLAG_INC | (ebytes + tbytes));
-        }
-        s_excl[0] = ebytes;
-        s_excl[1] = econst;
-        if (p.index) {
-            p.index[2 * tile] = ebytes;
-            p.index[2 * tile + 1] = econst;
-        }
-        if (tile == p.ntiles - 1) {
-            p.totals[0] = ebytes + tbytes;
-            p.totals[1] = econst + tconst;
+        unsigned long long ebytes, econst;
+        if (tid < 64) lookback_2level(p.tiles, p.chunks, tile, p.ntiles, tbytes, tconst, ebytes, econst);
+        if (tid == 0) {
+            s_excl[0] = ebytes;
+            s_excl[1] = econst;
             if (p.index) {
-                p.index[2 * p.ntiles] = ebytes + tbytes;
-                p.index[2 * p.ntiles + 1] = econst + tconst;
+                p.index[2 * tile] = ebytes;
+                p.index[2 * tile + 1] = econst;
+            }
+            if (tile == p.ntiles - 1) {
+                p.totals[0] = ebytes + tbytes;
+                p.totals[1] = econst + tconst;
+                if (p.index) {
+                    p.index[2 * p.ntiles] = ebytes + tbytes;
+                    p.index[2 * p.ntiles + 1] = econst + tconst;
+                }
             }
         }
     }
@@ -319,22 +413,32 @@ __global__ __launch_bounds__(256) void k_compress(CompressParams p)
     const uint64_t econst = s_excl[1];
     const uint32_t tbytes = s_total & 0xFFFFFu;
 
-    // Copy the staged records out: aligned dwords inside, bytes at the edges
-    // (the neighbouring tiles own the other bytes of an edge dword).
+    // Copy the staged records out: 16-byte stores inside, bytes at the edges
+    // (the neighbouring tiles own the other bytes of an edge chunk).
     {
         uint8_t *dst = p.payload + ebytes;
-        const uintptr_t d0 = (uintptr_t)dst;
-        const uint32_t head = (uint32_t)((4 - (d0 & 3)) & 3);
+        const uint32_t head = (uint32_t)((16 - ((uintptr_t)dst & 15)) & 15);
         const uint32_t h = head < tbytes ? head : tbytes;
         if (tid < (int)h) dst[tid] = s_stage[tid];
-        const uint32_t nw = (tbytes - h) >> 2;
-        for (uint32_t w = tid; w < nw; w += 256) {
-            const uint32_t o = h + 4 * w;
-            const uint32_t v = (uint32_t)s_stage[o] | ((uint32_t)s_stage[o + 1] << 8) |
-                               ((uint32_t)s_stage[o + 2] << 16) | ((uint32_t)s_stage[o + 3] << 24);
-            *reinterpret_cast<uint32_t *>(dst + o) = v;
+        const uint32_t nchunk = (tbytes - h) >> 4;
+        const int r = h & 3;
+        for (uint32_t c = tid; c < nchunk; c += 256) {
+            const uint32_t o = h + 16 * c;             // stage offset of the chunk
+            const uint32_t *sw = s_stage32 + (o >> 2);
+            const uint32_t a0 = sw[0], a1 = sw[1], a2 = sw[2], a3 = sw[3];
+            const uint32_t a4 = r ? sw[4] : 0u;
+            uint4 val;
+            if (r) {
+                val.x = __builtin_amdgcn_alignbyte(a1, a0, r);
+                val.y = __builtin_amdgcn_alignbyte(a2, a1, r);
+                val.z = __builtin_amdgcn_alignbyte(a3, a2, r);
+                val.w = __builtin_amdgcn_alignbyte(a4, a3, r);
+            } else {
+                val = make_uint4(a0, a1, a2, a3);
+            }
+            *reinterpret_cast<uint4 *>(dst + o) = val;
         }
-        const uint32_t tail0 = h + 4 * nw;
+        const uint32_t tail0 = h + 16 * nchunk;
         if (tid < (int)(tbytes - tail0)) dst[tail0 + tid] = s_stage[tail0 + tid];
     }
 
@@ -358,7 +462,7 @@ __global__ __launch_bounds__(256) void k_compress(CompressParams p)
     }
 }
 
-// Zero the look-back state, ticket and totals (a kernel, not a memset, so the
+// Zero the look-back state and totals (a kernel, not a memset, so the
 // whole sequence can be captured in a HIP graph).
 __global__ void k_zero_u64(unsigned long long *p, uint64_t nwords)
 {
@@ -447,13 +551,28 @@ __global__ __launch_bounds__(256) void k_range_partial(const T *x, uint64_t n, T
 }
 
 template <typename T>
-__global__ void k_range_final(const T *x, const T *part, int nparts, float rel, float *eb_out)
+__global__ __launch_bounds__(256) void k_range_final(const T *x, const T *part, int nparts,
+                                                    float rel, float *eb_out)
 {
-    if (threadIdx.x != 0) return;
     T mn = pos_inf<T>(), mx = -pos_inf<T>();
-    for (int i = 0; i < nparts; i++) {
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
         mn = part[2 * i] < mn ? part[2 * i] : mn;
         mx = part[2 * i + 1] > mx ? part[2 * i + 1] : mx;
+    }
+    for (int m = 1; m < 64; m <<= 1) {
+        T o = __shfl_xor(mn, m, 64);
+        mn = o < mn ? o : mn;
+        o = __shfl_xor(mx, m, 64);
+        mx = o > mx ? o : mx;
+    }
+    __shared__ T sm[2][4];
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { sm[0][w] = mn; sm[1][w] = mx; }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    for (int k = 1; k < (int)(blockDim.x >> 6); k++) {
+        mn = sm[0][k] < mn ? sm[0][k] : mn;
+        mx = sm[1][k] > mx ? sm[1][k] : mx;
     }
     const T first = x[0];
     if (first != first) { mn = first; mx = first; }
@@ -466,8 +585,6 @@ __global__ void k_range_final(const T *x, const T *part, int nparts, float rel, 
 // host side
 // ---------------------------------------------------------------------------
 
-static constexpr int kSlabsF32 = TILE_VALUES_F32 / 1024;
-static constexpr int kSlabsF64 = TILE_VALUES_F64 / 1024;
 static constexpr int kRangeParts = 1024;
 
 CompressLayout compress_layout(uint64_t n, int w, int bs)
@@ -481,10 +598,11 @@ CompressLayout compress_layout(uint64_t n, int w, int bs)
     L.capacity = L.region + L.payload_max + 16;
     L.ntiles = ceil_div(n, (uint64_t)tile_values(w));
     if (L.ntiles == 0) L.ntiles = 1;
-    // workspace: tiles | ticket+totals (4 words) | eb (1 word) | range parts | bitmap | cmed
+    // workspace: tile words | chunk words | misc (totals, eb) | range parts | bitmap | cmed
     uint64_t off = 0;
-    L.ws_tiles = off;   off += L.ntiles * sizeof(TileState);
-    L.ws_misc = off;    off += 8 * sizeof(uint64_t);
+    L.ws_tiles = off;   off += L.ntiles * sizeof(unsigned long long);
+    L.ws_chunks = off;  off += ceil_div(L.ntiles, LB_CHUNK) * sizeof(ChunkState);
+    L.ws_misc = off;    off += 64 * sizeof(uint64_t);   // totals at +128 B, eb at +256 B
     L.ws_parts = off;   off += 2 * kRangeParts * sizeof(double);
     L.ws_bitmap = off;  off += (L.nbit + 15) & ~(uint64_t)15;
     L.ws_cmed = off;    off += (4 * L.nb + 15) & ~(uint64_t)15;
@@ -495,8 +613,11 @@ CompressLayout compress_layout(uint64_t n, int w, int bs)
 template <typename T, int BS>
 static void launch_main(const CompressParams &p, uint64_t ntiles, hipStream_t st)
 {
-    constexpr int SLABS = sizeof(T) == 4 ? kSlabsF32 : kSlabsF64;
-    hipLaunchKernelGGL((k_compress<T, BS, SLABS>), dim3((unsigned)ntiles), dim3(256), 0, st, p);
+    // 8 values per lane where a block spans at least 8 values, else 4
+    constexpr int LV = BS >= 8 ? 8 : 4;
+    constexpr int TILE = sizeof(T) == 4 ? TILE_VALUES_F32 : TILE_VALUES_F64;
+    constexpr int SLABS = TILE / (256 * LV);
+    hipLaunchKernelGGL((k_compress<T, BS, LV, SLABS>), dim3((unsigned)ntiles), dim3(256), 0, st, p);
 }
 
 template <typename T>
@@ -521,23 +642,23 @@ int compress_launch(const void *d_in, uint64_t n, int w, int rel_mode, float abs
 {
     const CompressLayout L = compress_layout(n, w, bs);
     unsigned long long *misc = reinterpret_cast<unsigned long long *>(d_ws + L.ws_misc);
-    const uint64_t zero_words = (L.ws_misc + 8 * sizeof(uint64_t)) / 8;
+    const uint64_t zero_words = (L.ws_misc + 64 * sizeof(uint64_t)) / 8;
     hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((zero_words + 255) / 256 < 1024 ? (zero_words + 255) / 256 : 1024)),
                        dim3(256), 0, st, reinterpret_cast<unsigned long long *>(d_ws), zero_words);
     float *eb_dev = nullptr;
     if (rel_mode) {
-        eb_dev = reinterpret_cast<float *>(misc + 4);
+        eb_dev = reinterpret_cast<float *>(misc + 32);
         if (w == 4) {
             float *part = reinterpret_cast<float *>(d_ws + L.ws_parts);
             hipLaunchKernelGGL(k_range_partial<float>, dim3(kRangeParts), dim3(256), 0, st,
                                static_cast<const float *>(d_in), n, part);
-            hipLaunchKernelGGL(k_range_final<float>, dim3(1), dim3(64), 0, st,
+            hipLaunchKernelGGL(k_range_final<float>, dim3(1), dim3(256), 0, st,
                                static_cast<const float *>(d_in), part, kRangeParts, rel, eb_dev);
         } else {
             double *part = reinterpret_cast<double *>(d_ws + L.ws_parts);
             hipLaunchKernelGGL(k_range_partial<double>, dim3(kRangeParts), dim3(256), 0, st,
                                static_cast<const double *>(d_in), n, part);
-            hipLaunchKernelGGL(k_range_final<double>, dim3(1), dim3(64), 0, st,
+            hipLaunchKernelGGL(k_range_final<double>, dim3(1), dim3(256), 0, st,
                                static_cast<const double *>(d_in), part, kRangeParts, rel, eb_dev);
         }
     }
@@ -549,9 +670,9 @@ int compress_launch(const void *d_in, uint64_t n, int w, int rel_mode, float abs
     p.payload = d_out + L.region;
     p.bitmap = d_ws + L.ws_bitmap;
     p.cmed = reinterpret_cast<uint32_t *>(d_ws + L.ws_cmed);
-    p.tiles = reinterpret_cast<TileState *>(d_ws + L.ws_tiles);
-    p.ticket = reinterpret_cast<unsigned *>(misc);
-    p.totals = reinterpret_cast<uint64_t *>(misc + 2);
+    p.tiles = reinterpret_cast<unsigned long long *>(d_ws + L.ws_tiles);
+    p.chunks = reinterpret_cast<ChunkState *>(d_ws + L.ws_chunks);
+    p.totals = reinterpret_cast<uint64_t *>(misc + 16);
     p.index = d_index;
     p.ntiles = L.ntiles;
     p.nbit = L.nbit;

@@ -33,7 +33,7 @@ struct CompressLayout {
     uint64_t payload_max; // largest possible payload
     uint64_t capacity;    // bytes d_out must hold
     uint64_t ntiles;
-    uint64_t ws_tiles, ws_misc, ws_parts, ws_bitmap, ws_cmed, ws_bytes;
+    uint64_t ws_tiles, ws_chunks, ws_misc, ws_parts, ws_bitmap, ws_cmed, ws_bytes;
 };
 
 CompressLayout compress_layout(uint64_t n, int w, int bs);
