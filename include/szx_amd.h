@@ -132,8 +132,10 @@ size_t szx_amd_index_words(size_t n, int dataType);
 /* Compress n device values.  errBoundMode ABS or REL (REL's bound is computed
  * on the device with the reference's float rounding).  On completion
  * d_result[0] = byte offset of the stream inside d_out, d_result[1] = stream
- * size, d_result[2] = the absolute error bound used (as float bits).
- * d_result is device (or host-pinned) memory of 3 uint64_t.  d_index, if not
+ * size, d_result[2] = the absolute error bound used (as float bits),
+ * d_result[3] = 0, or 1 if the call failed (a bounded wait between
+ * workgroups ran out; the stream is then invalid).
+ * d_result is device (or host-pinned) memory of 4 uint64_t.  d_index, if not
  * NULL, receives the record index (szx_amd_index_words() words).
  * Returns 0, or a negative SZx_* code (argument errors only; the call is
  * asynchronous). */

@@ -228,7 +228,7 @@ class DeviceCompressor:
             raise SZxError(f"unsupported parameters (blockSize={blockSize})")
         self.out = torch.empty(self.capacity, dtype=torch.uint8, device=device)
         self.ws = torch.empty(self.wsbytes, dtype=torch.uint8, device=device)
-        self.result = torch.zeros(3, dtype=torch.int64, device=device)
+        self.result = torch.zeros(4, dtype=torch.int64, device=device)
         self.index = torch.zeros(L.szx_amd_index_words(n, code), dtype=torch.int64,
                                  device=device)
 
@@ -245,7 +245,9 @@ class DeviceCompressor:
 
     def stream_view(self):
         """(device uint8 tensor view of the stream, offset, size, eb) -- syncs."""
-        off, size, ebbits = (int(v) for v in self.result.cpu().tolist())
+        off, size, ebbits, err = (int(v) for v in self.result.cpu().tolist())
+        if err:
+            raise SZxError("compress failed: a bounded wait between workgroups ran out")
         eb = float(np.array([ebbits], np.uint32).view(np.float32)[0])
         return self.out[off:off + size], off, size, eb
 

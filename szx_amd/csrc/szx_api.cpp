@@ -157,11 +157,15 @@ unsigned char *host_compress(const void *data, size_t n, int w, int rel_mode, fl
         set_err("compress launch failed");
         return nullptr;
     }
-    uint64_t res[3] = {0, 0, 0};
+    uint64_t res[4] = {0, 0, 0, 0};
     if (!hip_ok(hipMemcpyAsync(res, c.res.p, sizeof res, hipMemcpyDeviceToHost, c.stream),
                 "hipMemcpyAsync D2H") ||
         !hip_ok(hipStreamSynchronize(c.stream), "compress"))
         return nullptr;
+    if (res[3]) {
+        set_err("compress: a look-back wait overran its bound (device busy?)");
+        return nullptr;
+    }
     unsigned char *out = (unsigned char *)malloc(res[1] ? res[1] : 1);
     if (!out) {
         set_err("malloc(%llu) failed", (unsigned long long)res[1]);

@@ -124,9 +124,11 @@ __device__ inline void publish_pair(ChunkState *t, unsigned long long flag, unsi
 // have to travel through long chains of tiles, which matters because every
 // poll is a round trip across the XCDs (~1 us).
 //
-// Tiles must start in index order within an XCD (the hardware dispatches a
-// grid's workgroups in increasing order), so every tile waited on has been
-// dispatched and the loops end.
+// Tiles must start in index order (the hardware dispatches a grid's
+// workgroups in increasing order), so every tile waited on has been
+// dispatched and the loops end.  The waits are still bounded (LB_SPIN_LIMIT
+// polls, a second or more): on overrun the look-back reports failure and the
+// caller flags the call as failed instead of hanging.
 struct ChunkState {
     unsigned long long w[2];   // flag | bytes, flag | constants
 };
@@ -155,15 +157,18 @@ __device__ inline bool window_ready(bool ready, bool isinc, int lane, int &stop)
     return (nr & upto) == 0;
 }
 
+constexpr int LB_SPIN_LIMIT = 1 << 19;
+
 // Back-off between polls of look-back words that are not ready yet: every
 // poll is a memory-side round trip, and thousands of waiting waves polling
-// flat out would compete with the data traffic.
-__device__ inline void poll_backoff(int &round)
+// flat out would compete with the data traffic.  False once the wait has
+// overrun its bound.
+__device__ inline bool poll_backoff(int &round)
 {
     if (round < 2) __builtin_amdgcn_s_sleep(2);
     else if (round < 6) __builtin_amdgcn_s_sleep(16);
     else __builtin_amdgcn_s_sleep(64);
-    round++;
+    return ++round <= LB_SPIN_LIMIT;
 }
 
 __device__ inline unsigned long long wave_sum(unsigned long long v)
@@ -173,7 +178,7 @@ __device__ inline unsigned long long wave_sum(unsigned long long v)
     return v;
 }
 
-__device__ inline void lookback_2level(unsigned long long *tiles, ChunkState *chunks, uint64_t tile,
+__device__ inline bool lookback_2level(unsigned long long *tiles, ChunkState *chunks, uint64_t tile,
                                        uint64_t ntiles, unsigned long long agg_bytes,
                                        unsigned long long agg_cnt, unsigned long long &ex_bytes,
                                        unsigned long long &ex_cnt)
@@ -198,7 +203,7 @@ __device__ inline void lookback_2level(unsigned long long *tiles, ChunkState *ch
             const unsigned long long f = w & FLAG_MASK;
             int stop;
             if (!window_ready(f != 0, f == FLAG_INC, lane, stop)) {
-                poll_backoff(round);
+                if (!poll_backoff(round)) return false;
                 continue;
             }
             const unsigned long long v = lane <= stop ? (w & VAL_MASK) : 0ull;
@@ -231,7 +236,7 @@ __device__ inline void lookback_2level(unsigned long long *tiles, ChunkState *ch
             const unsigned long long fa = wa & FLAG_MASK, fb = wb & FLAG_MASK;
             int stop;
             if (!window_ready(fa != 0 && fa == fb, fa == FLAG_INC && fb == FLAG_INC, lane, stop)) {
-                poll_backoff(round);
+                if (!poll_backoff(round)) return false;
                 continue;
             }
             cx_b += wave_sum(lane <= stop ? (wa & VAL_MASK) : 0ull);
@@ -244,6 +249,7 @@ __device__ inline void lookback_2level(unsigned long long *tiles, ChunkState *ch
     }
     ex_bytes = cx_b + in_b;
     ex_cnt = cx_c + in_c;
+    return true;
 }
 
 }  // namespace szxamd

@@ -40,6 +40,7 @@ struct CompressParams {
     ChunkState *chunks;          // per-64-tile look-back words
     uint64_t *totals;     // [0] payload bytes, [1] constant blocks
     uint64_t *index;      // optional record index: per tile {payload offset, constants before}
+    int *err;             // set when a look-back wait overran its bound
     uint64_t ntiles;
     uint64_t nbit;
 };
@@ -388,8 +389,13 @@ __global__ __launch_bounds__(256) void k_compress(CompressParams p)
         const uint32_t tw = s_total;
         const unsigned long long tbytes = tw & 0xFFFFFu;
         const unsigned long long tconst = tw >> 20;
-        unsigned long long ebytes, econst;
-        if (tid < 64) lookback_2level(p.tiles, p.chunks, tile, p.ntiles, tbytes, tconst, ebytes, econst);
+        unsigned long long ebytes = 0, econst = 0;
+        if (tid < 64 &&
+            !lookback_2level(p.tiles, p.chunks, tile, p.ntiles, tbytes, tconst, ebytes, econst)) {
+            ebytes = 0;   // keeps every write in bounds; the call reports failure
+            econst = 0;
+            if (tid == 0) atomicExch(p.err, 1);
+        }
         if (tid == 0) {
             s_excl[0] = ebytes;
             s_excl[1] = econst;
@@ -482,7 +488,8 @@ struct AssembleParams {
     int bs;
     const float *eb_dev;
     float eb;
-    uint64_t *result;     // [0] stream offset, [1] stream size, [2] bound bits
+    uint64_t *result;     // [0] stream offset, [1] stream size, [2] bound bits, [3] error
+    const int *err;
 };
 
 __global__ void k_assemble(AssembleParams a)
@@ -501,6 +508,7 @@ __global__ void k_assemble(AssembleParams a)
         a.result[0] = start;
         a.result[1] = front + pbytes;
         a.result[2] = f32_bits(a.eb_dev ? *a.eb_dev : a.eb);
+        a.result[3] = (uint64_t)*a.err;
     }
     for (uint64_t i = gid; i < a.nbit; i += nthr) s[HEADER_BYTES + i] = a.bitmap[i];
     const uint8_t *cm = reinterpret_cast<const uint8_t *>(a.cmed);
@@ -602,7 +610,7 @@ CompressLayout compress_layout(uint64_t n, int w, int bs)
     uint64_t off = 0;
     L.ws_tiles = off;   off += L.ntiles * sizeof(unsigned long long);
     L.ws_chunks = off;  off += ceil_div(L.ntiles, LB_CHUNK) * sizeof(ChunkState);
-    L.ws_misc = off;    off += 64 * sizeof(uint64_t);   // totals at +128 B, eb at +256 B
+    L.ws_misc = off;    off += 64 * sizeof(uint64_t);   // totals at +128 B, eb at +256 B, error at +384 B
     L.ws_parts = off;   off += 2 * kRangeParts * sizeof(double);
     L.ws_bitmap = off;  off += (L.nbit + 15) & ~(uint64_t)15;
     L.ws_cmed = off;    off += (4 * L.nb + 15) & ~(uint64_t)15;
@@ -673,6 +681,7 @@ int compress_launch(const void *d_in, uint64_t n, int w, int rel_mode, float abs
     p.tiles = reinterpret_cast<unsigned long long *>(d_ws + L.ws_tiles);
     p.chunks = reinterpret_cast<ChunkState *>(d_ws + L.ws_chunks);
     p.totals = reinterpret_cast<uint64_t *>(misc + 16);
+    p.err = reinterpret_cast<int *>(misc + 48);
     p.index = d_index;
     p.ntiles = L.ntiles;
     p.nbit = L.nbit;
@@ -692,6 +701,7 @@ int compress_launch(const void *d_in, uint64_t n, int w, int rel_mode, float abs
     a.eb_dev = eb_dev;
     a.eb = abs_eb;
     a.result = d_result;
+    a.err = p.err;
     const uint64_t work = L.nbit > 4 * L.nb ? L.nbit : 4 * L.nb;
     uint64_t blocks = (work + 255) / 256;
     if (blocks < 1) blocks = 1;
