@@ -58,8 +58,11 @@ size_t SZx_computeDataLength(size_t r5, size_t r4, size_t r3, size_t r2, size_t 
 /* szx.h:77-78 / szx.cc:231-348.  Compress a host array.  blockSize is 128,
  * as in the reference.  errBoundMode ABS uses absErrBound, REL uses
  * relBoundRatio * (max - min) computed with the reference's float rounding,
- * FXR (fixed rate) picks the bound for `compressionRatio` within `tolerance`.
- * Returns a malloc'd stream and its size in *outSize; NULL on error. */
+ * FXR (fixed rate, any fastMode as in szx.cc:238-243) picks the bound the
+ * reference's fixed-rate compressor picks for `compressionRatio` within
+ * `tolerance` (szx_float.cc:921-949) and compresses with it, so the stream is
+ * the reference's.  Returns a malloc'd stream and its size in *outSize; NULL
+ * on error. */
 unsigned char *SZx_fast_compress_args(int fastMode, int dataType, void *data, size_t *outSize,
                                       int errBoundMode, float absErrBound, float relBoundRatio,
                                       float compressionRatio, float tolerance, size_t r5,
@@ -86,13 +89,19 @@ void SZx_fast_decompress_args_unpredictable_blocked(void *newData, int dataType,
                                                     unsigned char *cmpBytes);
 
 /* szx.h:81 / szx.cc:432-438.  max - min of the array (radius and median are
- * returned when the pointers are non-NULL, as float). */
+ * returned when the pointers are non-NULL, as float).  The extremes are found
+ * on the GPU with the reference's semantics (szx_float.cc:478-499: NaN if the
+ * first value is NaN, other NaNs skipped). */
 double SZx_computeValueRange(void *oriData, int dataType, size_t length, float *radius,
                              float *medianValue);
 
-/* szx.h:87 / szx.cc:458-464.  Error bound whose compression ratio is within
- * tolerance of targetCompressionRatio (the reference's sampled model search,
- * run here without its process exit). */
+/* szx.h:87 / szx.cc:458-464.  Error bound whose modelled compression ratio
+ * is within tolerance of targetCompressionRatio, searched from
+ * initErrorBound.  The reference's own entry reaches an estimator that exits
+ * the process (szx_float.cc:680); this runs the search its fixed-rate
+ * compressor uses (szx_float.cc:684-832, szx_double.cc:682-829) on a 1-in-10
+ * block sample, with the model sums computed on the GPU.  blockSize 1..256.
+ * Returns 0 and sets szx_amd_last_error() on failure. */
 float SZx_estimateErrorBoundbasedonCR(int dataType, float targetCompressionRatio, float tolerance,
                                       void *data, float initErrorBound, int blockSize,
                                       size_t nbEle);
@@ -162,6 +171,22 @@ int szx_amd_decompress_dev(const void *d_stream, size_t streamBytes, size_t n, i
                            int blockSize, void *d_out, void *d_work, size_t workBytes,
                            const uint64_t *d_index, int flags, int32_t *d_status,
                            void *hipStream);
+
+/* Workspace bytes for szx_amd_fxr_bound_dev (0 for unsupported arguments). */
+size_t szx_amd_fxr_workspace(size_t n, int dataType, int blockSize);
+
+/* The reference's fixed-rate bound choice (szx_float.cc:921-949 without the
+ * final compress; fp64 szx_double.cc:917-943) for n device values: a 1-in-10
+ * block sample, model sums on the GPU, the bracket-then-bisect search on the
+ * host.  initErrorBound <= 0 starts the search at 1% of the sampled value
+ * range, as the reference's fixed-rate compressor does.  *errorBound receives
+ * the bound (bit-identical to the reference's), *modelPasses (if not NULL)
+ * the number of model evaluations the search made.  Synchronous: it waits
+ * on `stream` once per model pass.  Returns 0 or a negative SZx_* code. */
+int szx_amd_fxr_bound_dev(const void *d_in, size_t n, int dataType, int blockSize,
+                          float targetCompressionRatio, float tolerance, float initErrorBound,
+                          void *d_work, size_t workBytes, float *errorBound, int *modelPasses,
+                          void *hipStream);
 
 /* Kernel timing for benchmarks: after szx_amd_timing_begin(), this thread's
  * launches bracket their main kernels with HIP events on the launch stream;

@@ -174,6 +174,23 @@ def fxr_error_bound(x: np.ndarray, target_cr: float, tol: float, bs: int = 128):
     return eb, ne.value
 
 
+def fxr_search_from(x: np.ndarray, target_cr: float, tol: float, init: float, bs: int = 128,
+                    rate: int = 10):
+    """The fixed-rate search from an explicit start bound; (eb, evaluations)."""
+    x = np.ascontiguousarray(x).ravel()
+    w = _width(x)
+    ns = x.size // (bs * rate)
+    buf = np.zeros((ns + 1) * bs, x.dtype)
+    med = np.zeros(ns + 1, np.float32)
+    rad = np.zeros(ns + 1, np.float32)
+    L = lib()
+    L.szxo_fxr_sample(_ptr(x), w, x.size, rate, bs, _ptr(buf), _ptr(med), _ptr(rad))
+    ne = C.c_int(0)
+    eb = L.szxo_fxr_search(float(target_cr), float(tol), rate, float(init), bs, x.size, _ptr(buf), w,
+                           _ptr(med), _ptr(rad), C.byref(ne))
+    return eb, ne.value
+
+
 # ---------------------------------------------------------------------------
 # the compiled reference (optional)
 # ---------------------------------------------------------------------------
@@ -227,13 +244,15 @@ def ref_compress(x: np.ndarray, eb: float, bs: int = 128) -> bytes:
 
 
 def ref_compress_args(x: np.ndarray, mode: int, abs_eb: float = 0.0, rel: float = 0.0,
-                      fast_mode: int = 2) -> bytes:
-    """SZx_fast_compress_args with 1-D dims (fixed blockSize 128)."""
+                      fast_mode: int = 2, cr: float = 0.0, tol: float = 0.0) -> bytes:
+    """SZx_fast_compress_args with 1-D dims (fixed blockSize 128).  For FXR the
+    reference writes into a buffer of N*W/cr*(1+2 tol) bytes: callers must
+    know the stream fits (it overflows the heap otherwise)."""
     x = np.ascontiguousarray(x).ravel()
     dt = 0 if x.dtype == np.float32 else 1
     osz = _sz(0)
     p = ref().SZx_fast_compress_args(fast_mode, dt, _ptr(x), C.byref(osz), mode, float(abs_eb),
-                                     float(rel), 0.0, 0.0, 0, 0, 0, 0, x.size)
+                                     float(rel), float(cr), float(tol), 0, 0, 0, 0, x.size)
     return _take(p, osz.value)
 
 

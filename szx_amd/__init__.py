@@ -40,7 +40,7 @@ EXPORTS = (
     "szx_amd_last_error", "szx_amd_compress_bound", "szx_amd_compress_capacity",
     "szx_amd_compress_workspace", "szx_amd_compress_dev", "szx_amd_decompress_workspace",
     "szx_amd_decompress_dev", "szx_amd_index_words", "szx_amd_timing_begin",
-    "szx_amd_timing_end",
+    "szx_amd_timing_end", "szx_amd_fxr_workspace", "szx_amd_fxr_bound_dev",
 )
 
 _lib = None
@@ -100,6 +100,11 @@ def lib() -> C.CDLL:
     L.szx_amd_timing_begin.argtypes = []
     L.szx_amd_timing_end.restype = i32
     L.szx_amd_timing_end.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_int)]
+    L.szx_amd_fxr_workspace.restype = sz
+    L.szx_amd_fxr_workspace.argtypes = [sz, i32, i32]
+    L.szx_amd_fxr_bound_dev.restype = i32
+    L.szx_amd_fxr_bound_dev.argtypes = [vp, sz, i32, i32, f32, f32, f32, vp, sz, C.POINTER(f32),
+                                        C.POINTER(i32), vp]
     _lib = L
     return L
 
@@ -198,6 +203,30 @@ def decompress_blocked(stream: bytes, n: int, dtype) -> np.ndarray:
     return np.frombuffer(_take(out.value, n * dtype.itemsize), dtype).copy()
 
 
+def SZx_computeValueRange(data: np.ndarray):
+    """szx.h:81: (range, radius, median) of the array."""
+    x = np.ascontiguousarray(data).ravel()
+    rad, med = C.c_float(0), C.c_float(0)
+    r = lib().SZx_computeValueRange(x.ctypes.data, _dtype_code(x.dtype), x.size, C.byref(rad),
+                                    C.byref(med))
+    if last_error():
+        raise SZxError(last_error())
+    return r, rad.value, med.value
+
+
+def SZx_estimateErrorBoundbasedonCR(data: np.ndarray, targetCompressionRatio: float,
+                                    tolerance: float, initErrorBound: float,
+                                    blockSize: int = 128) -> float:
+    """szx.h:87: the fixed-rate bound search from initErrorBound."""
+    x = np.ascontiguousarray(data).ravel()
+    eb = lib().SZx_estimateErrorBoundbasedonCR(_dtype_code(x.dtype), targetCompressionRatio,
+                                               tolerance, x.ctypes.data, initErrorBound,
+                                               blockSize, x.size)
+    if last_error():
+        raise SZxError(last_error())
+    return eb
+
+
 # ---------------------------------------------------------------------------
 # device-resident API (torch tensors as device memory)
 # ---------------------------------------------------------------------------
@@ -278,3 +307,28 @@ class DeviceDecompressor:
 
     def check(self) -> int:
         return int(self.status.item())
+
+
+class DeviceFixedRateBound:
+    """The reference's fixed-rate bound choice for device data (synchronous)."""
+
+    def __init__(self, n: int, dtype, blockSize: int = 128, device="cuda"):
+        torch = _torch()
+        self.n, self.dtype, self.bs = n, np.dtype(dtype), blockSize
+        self.wsbytes = lib().szx_amd_fxr_workspace(n, _dtype_code(self.dtype), blockSize)
+        if not self.wsbytes:
+            raise SZxError(f"unsupported parameters (blockSize={blockSize})")
+        self.ws = torch.empty(self.wsbytes, dtype=torch.uint8, device=device)
+
+    def __call__(self, x, targetCompressionRatio: float, tolerance: float,
+                 initErrorBound: float = 0.0, stream=None):
+        """(bound, model evaluations); initErrorBound <= 0 starts at 1% of the
+        sampled range, as the reference's fixed-rate compressor does."""
+        eb, ne = C.c_float(0), C.c_int(0)
+        rc = lib().szx_amd_fxr_bound_dev(x.data_ptr(), self.n, _dtype_code(self.dtype), self.bs,
+                                         targetCompressionRatio, tolerance, initErrorBound,
+                                         self.ws.data_ptr(), self.wsbytes, C.byref(eb),
+                                         C.byref(ne), _stream_handle(stream))
+        if rc != 0:
+            raise SZxError(last_error())
+        return eb.value, ne.value

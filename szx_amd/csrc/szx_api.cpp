@@ -15,6 +15,8 @@
 #include <mutex>
 #include <string>
 
+#include "szx_fxr.hpp"
+#include "szx_fxr_host.hpp"
 #include "szx_internal.hpp"
 
 using namespace szxamd;
@@ -79,7 +81,7 @@ struct DevBuf {
 };
 
 struct HostCtx {
-    DevBuf in, out, ws, res;
+    DevBuf in, out, ws, res, fx;
     hipStream_t stream = nullptr;
     int dev = -1;
     bool ready()
@@ -130,9 +132,53 @@ size_t data_length(size_t r5, size_t r4, size_t r3, size_t r2, size_t r1)
     return r1 * r2 * r3 * r4 * r5;
 }
 
-// Host-in / host-out compress through the device path.
+// The reference's fixed-rate bound (szx_float.cc:921-949 without the final
+// compress) for device data: sample and model sums on the GPU, the search on
+// the host.  With default_init the search starts at 1% of the sampled range,
+// as the reference's fixed-rate compressor does; otherwise at init.
+// Synchronizes `st` once per model pass.
+constexpr int FXR_RATE = 10;   // szx_float.cc:930: one block in 10 is sampled
+
+bool fxr_bound_device(const void *d_in, size_t n, int w, int bs, float target, float tol,
+                      bool default_init, float init, uint8_t *d_ws, hipStream_t st, float *eb_out,
+                      int *n_est)
+{
+    const FxrLayout L = fxr_layout(n, w, bs, FXR_RATE);
+    fxr_sample_launch(d_in, n, w, bs, FXR_RATE, d_ws, st);
+    if (default_init) {
+        float range = 0.0f;
+        if (!hip_ok(hipMemcpyAsync(&range, d_ws + L.ws_range, 4, hipMemcpyDeviceToHost, st), "D2H") ||
+            !hip_ok(hipStreamSynchronize(st), "fixed-rate sample"))
+            return false;
+        init = range * 1E-2;   // szx_float.cc:941 (float * double, rounded to float)
+    }
+    FxrEvaluator eval = [&](const float *ebs, int K, FxrSums *out) {
+        uint64_t sums[3 * FXR_MAX_CAND];
+        if (!hip_ok(hipMemcpyAsync(d_ws + L.ws_ebs, ebs, 4 * (size_t)K, hipMemcpyHostToDevice, st),
+                    "H2D bounds"))
+            return false;
+        fxr_estimate_launch(d_in, n, w, bs, FXR_RATE, d_ws, K, st);
+        if (!hip_ok(hipMemcpyAsync(sums, d_ws + L.ws_sums, 24 * (size_t)K, hipMemcpyDeviceToHost, st),
+                    "D2H sums") ||
+            !hip_ok(hipStreamSynchronize(st), "fixed-rate model"))
+            return false;
+        for (int k = 0; k < K; k++) out[k] = {sums[3 * k], sums[3 * k + 1], sums[3 * k + 2]};
+        return true;
+    };
+    int ne = 0;
+    if (!fxr_search(target, tol, init, w, bs, n, L.ns, eval, eb_out, &ne)) return false;
+    if (n_est) *n_est = ne;
+    return true;
+}
+
+struct FxrReq {
+    float target, tol;
+};
+
+// Host-in / host-out compress through the device path.  With fxr set the
+// bound is the reference's fixed-rate choice for (target ratio, tolerance).
 unsigned char *host_compress(const void *data, size_t n, int w, int rel_mode, float abs_eb,
-                             float rel, int bs, size_t *outSize)
+                             float rel, int bs, size_t *outSize, const FxrReq *fxr = nullptr)
 {
     if (outSize) *outSize = 0;
     if (!data || n == 0) {
@@ -152,6 +198,13 @@ unsigned char *host_compress(const void *data, size_t n, int w, int rel_mode, fl
     if (!hip_ok(hipMemcpyAsync(c.in.p, data, n * (size_t)w, hipMemcpyHostToDevice, c.stream),
                 "hipMemcpyAsync H2D"))
         return nullptr;
+    if (fxr) {
+        if (!c.fx.reserve(fxr_layout(n, w, bs, FXR_RATE).ws_bytes) ||
+            !fxr_bound_device(c.in.p, n, w, bs, fxr->target, fxr->tol, true, 0.0f,
+                              (uint8_t *)c.fx.p, c.stream, &abs_eb, nullptr))
+            return nullptr;
+        rel_mode = 0;
+    }
     if (compress_launch(c.in.p, n, w, rel_mode, abs_eb, rel, bs, (uint8_t *)c.out.p,
                         (uint8_t *)c.ws.p, (uint64_t *)c.res.p, nullptr, c.stream)) {
         set_err("compress launch failed");
@@ -291,14 +344,20 @@ unsigned char *SZx_fast_compress_args(int fastMode, int dataType, void *data, si
                                       float compressionRatio, float tolerance, size_t r5,
                                       size_t r4, size_t r3, size_t r2, size_t r1)
 {
-    (void)compressionRatio;
-    (void)tolerance;
     g_err.clear();
     if (outSize) *outSize = 0;
     const int w = width_of(dataType);
     if (!w) {
         set_err("unsupported dataType %d", dataType);
         return nullptr;
+    }
+    const size_t n = data_length(r5, r4, r3, r2, r1);
+    // szx.cc:234: the blocked family always uses blockSize 128.
+    if (errBoundMode == FXR) {
+        // szx.cc:238-243, 293-296: fixed rate takes the blocked fixed-rate
+        // compressor whatever fastMode says
+        const FxrReq req{compressionRatio, tolerance};
+        return host_compress(data, n, w, 0, 0.0f, 0.0f, 128, outSize, &req);
     }
     if (fastMode != SZx_WITH_BLOCK_FAST_CMPR) {
         set_err("fastMode %d is not provided (fastMode 2 is)", fastMode);
@@ -308,8 +367,6 @@ unsigned char *SZx_fast_compress_args(int fastMode, int dataType, void *data, si
         set_err("errBoundMode %d is not provided", errBoundMode);
         return nullptr;
     }
-    const size_t n = data_length(r5, r4, r3, r2, r1);
-    // szx.cc:234: the blocked family always uses blockSize 128.
     return host_compress(data, n, w, errBoundMode == REL, absErrBound, relBoundRatio, 128, outSize);
 }
 
@@ -383,46 +440,75 @@ void SZx_fast_decompress_args_unpredictable_blocked(void *newData, int dataType,
 double SZx_computeValueRange(void *oriData, int dataType, size_t length, float *radius,
                              float *medianValue)
 {
-    // szx_float.cc:478-499 / szx_double.cc:1137-1158 (host arithmetic; this is
-    // a one-pass helper the reference exposes, not part of the codec path).
+    // szx_float.cc:478-499 / szx_double.cc:1137-1158: min / max on the device,
+    // then the reference's scalar arithmetic (in T, radius and median rounded
+    // to float) on the two extremes.
     g_err.clear();
-    if (!oriData || length == 0) return 0.0;
-    if (dataType == SZx_FLOAT) {
-        const float *x = (const float *)oriData;
-        float lo = x[0], hi = x[0];
-        for (size_t i = 0; i < length; i++) {
-            if (lo > x[i]) lo = x[i];
-            else if (hi < x[i]) hi = x[i];
-        }
-        const float r = hi - lo;
+    const int w = width_of(dataType);
+    if (!w || !oriData || length == 0) {
+        set_err("unsupported dataType %d or empty input", dataType);
+        return 0.0;
+    }
+    HostCtx &c = g_ctx;
+    if (!c.ready() || !c.in.reserve(length * (size_t)w) || !c.ws.reserve(range_workspace_bytes()) ||
+        !c.res.reserve(64))
+        return 0.0;
+    double mm[2] = {0, 0};
+    if (!hip_ok(hipMemcpyAsync(c.in.p, oriData, length * (size_t)w, hipMemcpyHostToDevice, c.stream),
+                "H2D"))
+        return 0.0;
+    range_launch(c.in.p, length, w, (uint8_t *)c.ws.p, c.res.p, c.stream);
+    if (!hip_ok(hipMemcpyAsync(mm, c.res.p, 2 * (size_t)w, hipMemcpyDeviceToHost, c.stream), "D2H") ||
+        !hip_ok(hipStreamSynchronize(c.stream), "value range"))
+        return 0.0;
+    if (w == 4) {
+        float lo, hi;
+        memcpy(&lo, (const char *)mm, 4);
+        memcpy(&hi, (const char *)mm + 4, 4);
+        const float range = hi - lo;
         if (radius) {
-            *radius = r / 2;
+            *radius = range / 2;
             *medianValue = lo + *radius;
         }
-        return r;
+        return range;
     }
-    const double *x = (const double *)oriData;
-    double lo = x[0], hi = x[0];
-    for (size_t i = 0; i < length; i++) {
-        if (lo > x[i]) lo = x[i];
-        else if (hi < x[i]) hi = x[i];
-    }
-    const double r = hi - lo;
+    const double range = mm[1] - mm[0];
     if (radius) {
-        *radius = (float)(r / 2);
-        *medianValue = (float)(lo + r / 2);
+        *radius = range / 2;
+        *medianValue = mm[0] + *radius;
     }
-    return r;
+    return range;
 }
 
 float SZx_estimateErrorBoundbasedonCR(int dataType, float targetCompressionRatio, float tolerance,
                                       void *data, float initErrorBound, int blockSize,
                                       size_t nbEle)
 {
-    (void)dataType; (void)targetCompressionRatio; (void)tolerance; (void)data;
-    (void)initErrorBound; (void)blockSize; (void)nbEle;
-    set_err("fixed-rate estimation is not built yet");
-    return 0.0f;
+    // The reference's public entry (szx.cc:458-464) reaches an estimator that
+    // ends the process in its first call (szx_float.cc:680).  This runs the
+    // search its fixed-rate compressor uses (szx_float.cc:684-832) from the
+    // caller's initErrorBound instead.
+    g_err.clear();
+    const int w = width_of(dataType);
+    if (!w || !data || nbEle == 0) {
+        set_err("unsupported dataType %d or empty input", dataType);
+        return 0.0f;
+    }
+    if (blockSize < 1 || blockSize > 256) {
+        set_err("blockSize %d is not supported (1..256)", blockSize);
+        return 0.0f;
+    }
+    HostCtx &c = g_ctx;
+    if (!c.ready() || !c.in.reserve(nbEle * (size_t)w) ||
+        !c.fx.reserve(fxr_layout(nbEle, w, blockSize, FXR_RATE).ws_bytes))
+        return 0.0f;
+    if (!hip_ok(hipMemcpyAsync(c.in.p, data, nbEle * (size_t)w, hipMemcpyHostToDevice, c.stream), "H2D"))
+        return 0.0f;
+    float eb = 0.0f;
+    if (!fxr_bound_device(c.in.p, nbEle, w, blockSize, targetCompressionRatio, tolerance, false,
+                          initErrorBound, (uint8_t *)c.fx.p, c.stream, &eb, nullptr))
+        return 0.0f;
+    return eb;
 }
 
 // Length of a fastMode-2 stream from its own structure (header, bitmap,
@@ -531,6 +617,36 @@ int szx_amd_decompress_dev(const void *d_stream, size_t streamBytes, size_t n, i
         return SZx_NSCS;
     }
     return hip_ok(hipGetLastError(), "launch") ? SZx_SCES : SZx_NSCS;
+}
+
+size_t szx_amd_fxr_workspace(size_t n, int dataType, int blockSize)
+{
+    const int w = width_of(dataType);
+    if (!w || blockSize < 1 || blockSize > 256) return 0;
+    return fxr_layout(n, w, blockSize, FXR_RATE).ws_bytes;
+}
+
+int szx_amd_fxr_bound_dev(const void *d_in, size_t n, int dataType, int blockSize,
+                          float targetCompressionRatio, float tolerance, float initErrorBound,
+                          void *d_work, size_t workBytes, float *errorBound, int *modelPasses,
+                          void *hipStream)
+{
+    g_err.clear();
+    const int w = width_of(dataType);
+    if (!w || !d_in || n == 0 || !errorBound || blockSize < 1 || blockSize > 256) {
+        set_err("invalid arguments");
+        return SZx_NSCS;
+    }
+    if (workBytes < fxr_layout(n, w, blockSize, FXR_RATE).ws_bytes) {
+        set_err("workspace too small");
+        return SZx_MERR;
+    }
+    const bool dflt = !(initErrorBound > 0.0f);
+    if (!fxr_bound_device(d_in, n, w, blockSize, targetCompressionRatio, tolerance, dflt,
+                          initErrorBound, (uint8_t *)d_work, (hipStream_t)hipStream, errorBound,
+                          modelPasses))
+        return SZx_NSCS;
+    return SZx_SCES;
 }
 
 }  // extern "C"

@@ -558,9 +558,12 @@ __global__ __launch_bounds__(256) void k_range_partial(const T *x, uint64_t n, T
     }
 }
 
+// Folds the partials.  eb_out (if set) receives the REL bound; mm_out (if
+// set) receives {min, max} with the reference's semantics (NaN when x[0] is
+// NaN, NaNs elsewhere skipped).
 template <typename T>
 __global__ __launch_bounds__(256) void k_range_final(const T *x, const T *part, int nparts,
-                                                    float rel, float *eb_out)
+                                                    float rel, float *eb_out, T *mm_out)
 {
     T mn = pos_inf<T>(), mx = -pos_inf<T>();
     for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
@@ -586,7 +589,11 @@ __global__ __launch_bounds__(256) void k_range_final(const T *x, const T *part, 
     if (first != first) { mn = first; mx = first; }
     const T range = mx - mn;
     // fp32: fl32(range * rel); fp64: (float)(range * (double)rel)
-    *eb_out = (float)(range * (T)rel);
+    if (eb_out) *eb_out = (float)(range * (T)rel);
+    if (mm_out) {
+        mm_out[0] = mn;
+        mm_out[1] = mx;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -644,6 +651,27 @@ static int dispatch_bs(int bs, const CompressParams &p, uint64_t ntiles, hipStre
     return 0;
 }
 
+uint64_t range_workspace_bytes() { return 2 * kRangeParts * sizeof(double) + 256; }
+
+void range_launch(const void *d_in, uint64_t n, int w, uint8_t *d_ws, void *d_minmax, hipStream_t st)
+{
+    if (w == 4) {
+        float *part = reinterpret_cast<float *>(d_ws);
+        hipLaunchKernelGGL(k_range_partial<float>, dim3(kRangeParts), dim3(256), 0, st,
+                           static_cast<const float *>(d_in), n, part);
+        hipLaunchKernelGGL(k_range_final<float>, dim3(1), dim3(256), 0, st,
+                           static_cast<const float *>(d_in), part, kRangeParts, 0.0f,
+                           (float *)nullptr, static_cast<float *>(d_minmax));
+    } else {
+        double *part = reinterpret_cast<double *>(d_ws);
+        hipLaunchKernelGGL(k_range_partial<double>, dim3(kRangeParts), dim3(256), 0, st,
+                           static_cast<const double *>(d_in), n, part);
+        hipLaunchKernelGGL(k_range_final<double>, dim3(1), dim3(256), 0, st,
+                           static_cast<const double *>(d_in), part, kRangeParts, 0.0f,
+                           (float *)nullptr, static_cast<double *>(d_minmax));
+    }
+}
+
 int compress_launch(const void *d_in, uint64_t n, int w, int rel_mode, float abs_eb, float rel,
                     int bs, uint8_t *d_out, uint8_t *d_ws, uint64_t *d_result, uint64_t *d_index,
                     hipStream_t st)
@@ -661,13 +689,15 @@ int compress_launch(const void *d_in, uint64_t n, int w, int rel_mode, float abs
             hipLaunchKernelGGL(k_range_partial<float>, dim3(kRangeParts), dim3(256), 0, st,
                                static_cast<const float *>(d_in), n, part);
             hipLaunchKernelGGL(k_range_final<float>, dim3(1), dim3(256), 0, st,
-                               static_cast<const float *>(d_in), part, kRangeParts, rel, eb_dev);
+                               static_cast<const float *>(d_in), part, kRangeParts, rel, eb_dev,
+                               (float *)nullptr);
         } else {
             double *part = reinterpret_cast<double *>(d_ws + L.ws_parts);
             hipLaunchKernelGGL(k_range_partial<double>, dim3(kRangeParts), dim3(256), 0, st,
                                static_cast<const double *>(d_in), n, part);
             hipLaunchKernelGGL(k_range_final<double>, dim3(1), dim3(256), 0, st,
-                               static_cast<const double *>(d_in), part, kRangeParts, rel, eb_dev);
+                               static_cast<const double *>(d_in), part, kRangeParts, rel, eb_dev,
+                               (double *)nullptr);
         }
     }
     CompressParams p{};

@@ -1,0 +1,37 @@
+// szx_fxr.hpp -- host interface of the fixed-rate kernels (szx_fxr.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace szxamd {
+
+constexpr int FXR_MAX_CAND = 32;    // candidate bounds per model pass
+constexpr int FXR_MAX_WG = 1024;    // workgroups of a model pass
+constexpr int OCT_BINS = 512;       // radius bins per octave of bounds
+constexpr int OCT_WG = 1024;        // workgroups of an octave pass
+
+// Workspace of the sampled model: per sampled block median and radius,
+// per-workgroup partial sums, the candidate bounds, the folded sums and the
+// sampled value range.
+struct FxrLayout {
+    uint64_t ns;   // sampled blocks
+    uint64_t ws_med, ws_rad, ws_part, ws_ebs, ws_sums, ws_range, ws_bytes;
+};
+FxrLayout fxr_layout(uint64_t n, int w, int bs, int rate);
+
+// Sample the blocks and their range (float at ws + ws_range).
+void fxr_sample_launch(const void *x, uint64_t n, int w, int bs, int rate, uint8_t *ws, hipStream_t st);
+// Model sums for the K bounds at ws + ws_ebs -> ws + ws_sums (3 u64 each:
+// constant blocks, sum reqBytes, sum min(lead, reqBytes)).
+void fxr_estimate_launch(const void *x, uint64_t n, int w, int bs, int rate, uint8_t *ws, int K,
+                         hipStream_t st);
+
+// Exact rate over the octave of bounds with exponent reqexp: d_acc receives
+// 1 + 2 * OCT_BINS u64 (fixed bytes, then per radius bin count and record
+// bytes); ws holds octave_workspace_bytes().
+uint64_t octave_workspace_bytes();
+void rate_octave_launch(const void *x, uint64_t n, int w, int bs, int reqexp, uint8_t *ws,
+                        unsigned long long *d_acc, hipStream_t st);
+
+}  // namespace szxamd

@@ -45,6 +45,11 @@ int compress_launch(const void *d_in, uint64_t n, int w, int rel_mode, float abs
                     int bs, uint8_t *d_out, uint8_t *d_ws, uint64_t *d_result, uint64_t *d_index,
                     hipStream_t st);
 
+// Value range (computeValueRange_*, szx_float.cc:478-499): d_minmax receives
+// {min, max} as T; d_ws holds range_workspace_bytes().
+uint64_t range_workspace_bytes();
+void range_launch(const void *d_in, uint64_t n, int w, uint8_t *d_ws, void *d_minmax, hipStream_t st);
+
 // Device memory layout of one decompress call.
 struct DecompressLayout {
     uint64_t ws_hdr, ws_index, ws_bytes;

@@ -185,6 +185,33 @@ def test_fxr_search_vs_reference(dtype):
             assert np.float32(ours) == np.float32(theirs), (trial, target)
 
 
+@needs_ref
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_fxr_compress_vs_reference(dtype):
+    """The whole fixed-rate path (szx.cc:238-243): the oracle's bound and the
+    oracle's stream at that bound equal the reference's FXR output.  The
+    reference writes into N*W/cr*(1+2 tol) bytes, so it is only called where
+    the stream fits."""
+    r = np.random.default_rng(33)
+    w = np.dtype(dtype).itemsize
+    checked = 0
+    for trial in range(10):
+        n = int(r.integers(30000, 120000))
+        x = (np.sin(np.arange(n) * r.uniform(1e-4, 1e-2)) * 10 ** r.uniform(-2, 3)
+             + r.standard_normal(n) * 10 ** r.uniform(-7, -3)).astype(dtype)
+        for target in (4.0, 8.0, 16.0):
+            tol = 0.1
+            eb, _ = O.fxr_error_bound(x, target, tol, 128)
+            ours = O.compress(x, eb, 128)
+            cap = int(w * n / np.float32(target) * (1 + np.float32(tol) * 2))
+            if len(ours) + 64 > cap:   # margin for the float sizing
+                continue
+            theirs = O.ref_compress_args(x, mode=2, fast_mode=2, cr=target, tol=tol)
+            assert ours == theirs, (trial, target)
+            checked += 1
+    assert checked >= 5
+
+
 # ---- hand-worked known answers and format invariants ------------------------
 
 def _be32(v):
