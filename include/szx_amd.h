@@ -188,6 +188,80 @@ int szx_amd_fxr_bound_dev(const void *d_in, size_t n, int dataType, int blockSiz
                           void *d_work, size_t workBytes, float *errorBound, int *modelPasses,
                           void *hipStream);
 
+/* ---- 3. fixed rate on the exact codec ------------------------------------
+ * No counterpart in the reference: its FXR mode (above) hits a *modelled*
+ * ratio from a sample; these entry points hit a target rate of the real
+ * stream, 8 * stream bytes / n bits per value, within a relative tolerance,
+ * and leave that stream in d_out.  Every candidate is judged by the exact
+ * codec; reduced precision is used only inside the rate model. */
+
+#define SZX_AMD_RATE_BISECT 0    /* bisection on log2(eb), one compress per step */
+#define SZX_AMD_RATE_LEARNED 1   /* rate-model proposal, exact octave pass,
+                                    final compress; bisection if it misses */
+
+typedef struct {
+    float errorBound;      /* bound of the stream left in d_out */
+    double bitsPerValue;   /* its exact rate */
+    int codecPasses;       /* full compresses + octave (exact size) passes */
+    int featurePasses;     /* statistics passes (learned method) */
+    int hit;               /* 1 if |rate - target| <= tolerance * target */
+} szx_amd_rate_info;
+
+size_t szx_amd_fixed_rate_workspace(size_t n, int dataType, int blockSize);
+
+/* Fixed-rate compress of n device values (blockSize: powers of two 4..256).
+ * d_out / outCapacity / d_result / d_index as for szx_amd_compress_dev;
+ * d_modelParams: SZX_AMD_RATE_MODEL_PARAMS floats of the rate model (device
+ * memory; the learned method only).  Synchronous (waits on `stream` after
+ * each pass).  Returns 0 or a negative SZx_* code. */
+int szx_amd_fixed_rate_dev(const void *d_in, size_t n, int dataType, int blockSize,
+                           float targetBitsPerValue, float tolerance, int method,
+                           const float *d_modelParams, void *d_out, size_t outCapacity,
+                           void *d_work, size_t workBytes, uint64_t *d_result, uint64_t *d_index,
+                           szx_amd_rate_info *info, void *hipStream);
+
+/* Exact stream sizes for a whole octave of bounds [2^reqExpo, 2^(reqExpo+1)):
+ * d_acc receives 1 + 2 * SZX_AMD_OCTAVE_BINS u64 -- bytes fixed over the
+ * octave (always-constant blocks' medians and always-present records), then
+ * per radius bin k (radius in (2^e (1 + k/B), 2^e (1 + (k+1)/B)]) the block
+ * count and their record bytes.  The stream at eb_k = 2^e (1 + k/B) has
+ * 13 + ceil(nb/8) + fixed + 4 * (blocks in bins < k) + (records of bins >= k)
+ * bytes.  reqExpo in [-1074, 1023]. */
+#define SZX_AMD_OCTAVE_BINS 512
+size_t szx_amd_rate_octave_workspace(void);
+int szx_amd_rate_octave_dev(const void *d_in, size_t n, int dataType, int blockSize, int reqExpo,
+                            uint64_t *d_acc, void *d_work, size_t workBytes, void *hipStream);
+
+/* The rate model: an MLP 24 -> 64 -> 64 -> 1 (GELU) predicting a chunk's
+ * log2(bits per value) at a bound from the chunk's statistics.  Parameters
+ * are one FP32 array: W1[64][24], b1[64], W2[64][64] (out, in), b2[64],
+ * w3[64], b3.  The kernels run the hidden layers on BF16 matrix cores with
+ * FP32 accumulation. */
+#define SZX_AMD_RATE_MODEL_PARAMS 5825
+#define SZX_AMD_RATE_MODEL_INPUTS 24
+#define SZX_AMD_RATE_CHUNK 524288        /* values per statistics chunk */
+
+/* Per-chunk statistics (ceil(n / SZX_AMD_RATE_CHUNK) chunks). */
+size_t szx_amd_rate_stats_bytes(size_t n);
+size_t szx_amd_rate_stats_workspace(size_t n);
+int szx_amd_rate_stats_dev(const void *d_in, size_t n, int dataType, int blockSize, void *d_stats,
+                           void *d_work, size_t workBytes, void *hipStream);
+/* Model inputs for every (bound k, chunk c): d_X[(k * nchunks + c) * 24 ...]. */
+int szx_amd_rate_inputs_dev(const void *d_stats, size_t n, const float *d_ebs, int K, float *d_X,
+                            void *hipStream);
+/* d_y[i] = model(d_X[i]) for nSamples rows. */
+int szx_amd_mlp_forward_dev(const float *d_params, const float *d_X, size_t nSamples, float *d_y,
+                            void *hipStream);
+/* Gradient of mean((model(X) - T)^2) into d_grad[SZX_AMD_RATE_MODEL_PARAMS]
+ * and the loss into *d_loss. */
+size_t szx_amd_mlp_grad_workspace(void);
+int szx_amd_mlp_grad_dev(const float *d_params, const float *d_X, const float *d_T, size_t nSamples,
+                         float *d_grad, float *d_loss, void *d_work, size_t workBytes,
+                         void *hipStream);
+/* One bias-corrected Adam step (step >= 1) on the FP32 parameters. */
+int szx_amd_mlp_adam_dev(float *d_params, const float *d_grad, float *d_m, float *d_v, float lr,
+                         float beta1, float beta2, float eps, int step, void *hipStream);
+
 /* Kernel timing for benchmarks: after szx_amd_timing_begin(), this thread's
  * launches bracket their main kernels with HIP events on the launch stream;
  * szx_amd_timing_end() waits for them and returns, per kernel kind

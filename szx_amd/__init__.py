@@ -41,7 +41,22 @@ EXPORTS = (
     "szx_amd_compress_workspace", "szx_amd_compress_dev", "szx_amd_decompress_workspace",
     "szx_amd_decompress_dev", "szx_amd_index_words", "szx_amd_timing_begin",
     "szx_amd_timing_end", "szx_amd_fxr_workspace", "szx_amd_fxr_bound_dev",
+    "szx_amd_fixed_rate_workspace", "szx_amd_fixed_rate_dev", "szx_amd_rate_stats_bytes",
+    "szx_amd_rate_stats_workspace", "szx_amd_rate_stats_dev", "szx_amd_rate_inputs_dev",
+    "szx_amd_mlp_forward_dev", "szx_amd_mlp_grad_workspace", "szx_amd_mlp_grad_dev",
+    "szx_amd_mlp_adam_dev", "szx_amd_rate_octave_workspace", "szx_amd_rate_octave_dev",
 )
+OCTAVE_BINS = 512
+RATE_BISECT, RATE_LEARNED = 0, 1
+RATE_MODEL_PARAMS = 5825
+RATE_MODEL_INPUTS = 24
+RATE_CHUNK = 524288
+RATE_MODEL_PATH = os.path.join(HERE, "rate_model_fp32.bin")
+
+
+class RateInfo(C.Structure):
+    _fields_ = [("errorBound", C.c_float), ("bitsPerValue", C.c_double),
+                ("codecPasses", C.c_int), ("featurePasses", C.c_int), ("hit", C.c_int)]
 
 _lib = None
 _libc = C.CDLL(None)
@@ -105,6 +120,31 @@ def lib() -> C.CDLL:
     L.szx_amd_fxr_bound_dev.restype = i32
     L.szx_amd_fxr_bound_dev.argtypes = [vp, sz, i32, i32, f32, f32, f32, vp, sz, C.POINTER(f32),
                                         C.POINTER(i32), vp]
+    L.szx_amd_fixed_rate_workspace.restype = sz
+    L.szx_amd_fixed_rate_workspace.argtypes = [sz, i32, i32]
+    L.szx_amd_fixed_rate_dev.restype = i32
+    L.szx_amd_fixed_rate_dev.argtypes = [vp, sz, i32, i32, f32, f32, i32, vp, vp, sz, vp, sz, vp,
+                                         vp, C.POINTER(RateInfo), vp]
+    L.szx_amd_rate_octave_workspace.restype = sz
+    L.szx_amd_rate_octave_workspace.argtypes = []
+    L.szx_amd_rate_octave_dev.restype = i32
+    L.szx_amd_rate_octave_dev.argtypes = [vp, sz, i32, i32, i32, vp, vp, sz, vp]
+    L.szx_amd_rate_stats_bytes.restype = sz
+    L.szx_amd_rate_stats_bytes.argtypes = [sz]
+    L.szx_amd_rate_stats_workspace.restype = sz
+    L.szx_amd_rate_stats_workspace.argtypes = [sz]
+    L.szx_amd_rate_stats_dev.restype = i32
+    L.szx_amd_rate_stats_dev.argtypes = [vp, sz, i32, i32, vp, vp, sz, vp]
+    L.szx_amd_rate_inputs_dev.restype = i32
+    L.szx_amd_rate_inputs_dev.argtypes = [vp, sz, vp, i32, vp, vp]
+    L.szx_amd_mlp_forward_dev.restype = i32
+    L.szx_amd_mlp_forward_dev.argtypes = [vp, vp, sz, vp, vp]
+    L.szx_amd_mlp_grad_workspace.restype = sz
+    L.szx_amd_mlp_grad_workspace.argtypes = []
+    L.szx_amd_mlp_grad_dev.restype = i32
+    L.szx_amd_mlp_grad_dev.argtypes = [vp, vp, vp, sz, vp, vp, vp, sz, vp]
+    L.szx_amd_mlp_adam_dev.restype = i32
+    L.szx_amd_mlp_adam_dev.argtypes = [vp, vp, vp, vp, f32, f32, f32, f32, i32, vp]
     _lib = L
     return L
 
@@ -332,3 +372,130 @@ class DeviceFixedRateBound:
         if rc != 0:
             raise SZxError(last_error())
         return eb.value, ne.value
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise SZxError(last_error())
+
+
+class RateModel:
+    """The fixed-rate rate model on the device: forward, gradient and Adam
+    kernels of libszx_amd.so over an FP32 parameter tensor."""
+
+    def __init__(self, params=None, device="cuda"):
+        torch = _torch()
+        if params is None:
+            params = np.fromfile(RATE_MODEL_PATH, dtype="<f4")
+        p = np.asarray(params, np.float32).ravel()
+        if p.size != RATE_MODEL_PARAMS:
+            raise ValueError(f"expected {RATE_MODEL_PARAMS} parameters, got {p.size}")
+        self.params = torch.from_numpy(p.copy()).to(device)
+        self.device = device
+        self._ws = None
+
+    def forward(self, X, stream=None):
+        torch = _torch()
+        y = torch.empty(X.shape[0], dtype=torch.float32, device=X.device)
+        _check(lib().szx_amd_mlp_forward_dev(self.params.data_ptr(), X.data_ptr(), X.shape[0],
+                                             y.data_ptr(), _stream_handle(stream)))
+        return y
+
+    def grad(self, X, T, stream=None):
+        """(gradient tensor, loss tensor) of mean((model(X) - T)^2)."""
+        torch = _torch()
+        if self._ws is None:
+            self._ws = torch.empty(lib().szx_amd_mlp_grad_workspace(), dtype=torch.uint8,
+                                   device=X.device)
+        g = torch.empty(RATE_MODEL_PARAMS, dtype=torch.float32, device=X.device)
+        loss = torch.empty(1, dtype=torch.float32, device=X.device)
+        _check(lib().szx_amd_mlp_grad_dev(self.params.data_ptr(), X.data_ptr(), T.data_ptr(),
+                                          X.shape[0], g.data_ptr(), loss.data_ptr(),
+                                          self._ws.data_ptr(), self._ws.numel(),
+                                          _stream_handle(stream)))
+        return g, loss
+
+    def adam(self, grad, m, v, lr, step, beta1=0.9, beta2=0.999, eps=1e-8, stream=None):
+        _check(lib().szx_amd_mlp_adam_dev(self.params.data_ptr(), grad.data_ptr(), m.data_ptr(),
+                                          v.data_ptr(), lr, beta1, beta2, eps, step,
+                                          _stream_handle(stream)))
+
+
+def rate_stats(x, blockSize: int = 128, stream=None):
+    """Per-chunk statistics of device array x (raw bytes tensor)."""
+    torch = _torch()
+    n = x.numel()
+    code = SZx_FLOAT if x.dtype == torch.float32 else SZx_DOUBLE
+    st = torch.empty(lib().szx_amd_rate_stats_bytes(n), dtype=torch.uint8, device=x.device)
+    ws = torch.empty(lib().szx_amd_rate_stats_workspace(n), dtype=torch.uint8, device=x.device)
+    _check(lib().szx_amd_rate_stats_dev(x.data_ptr(), n, code, blockSize, st.data_ptr(),
+                                        ws.data_ptr(), ws.numel(), _stream_handle(stream)))
+    return st
+
+
+def rate_inputs(stats, n: int, ebs, stream=None):
+    """Model inputs [K * nchunks, 24] for the bounds in ebs (device float32)."""
+    torch = _torch()
+    nch = (n + RATE_CHUNK - 1) // RATE_CHUNK
+    X = torch.empty(ebs.numel() * nch, RATE_MODEL_INPUTS, dtype=torch.float32, device=ebs.device)
+    _check(lib().szx_amd_rate_inputs_dev(stats.data_ptr(), n, ebs.data_ptr(), ebs.numel(),
+                                         X.data_ptr(), _stream_handle(stream)))
+    return X
+
+
+def rate_octave(x, reqExpo: int, blockSize: int = 128, stream=None):
+    """Exact stream sizes (bytes) for the OCTAVE_BINS bounds of one octave:
+    (numpy sizes, numpy float32 bounds)."""
+    torch = _torch()
+    n = x.numel()
+    code = SZx_FLOAT if x.dtype == torch.float32 else SZx_DOUBLE
+    acc = torch.zeros(1 + 2 * OCTAVE_BINS, dtype=torch.int64, device=x.device)
+    ws = torch.empty(lib().szx_amd_rate_octave_workspace(), dtype=torch.uint8, device=x.device)
+    _check(lib().szx_amd_rate_octave_dev(x.data_ptr(), n, code, blockSize, reqExpo,
+                                         acc.data_ptr(), ws.data_ptr(), ws.numel(),
+                                         _stream_handle(stream)))
+    a = acc.cpu().numpy().astype(np.uint64)
+    nb = (n + blockSize - 1) // blockSize
+    cnt, rec = a[1::2], a[2::2]
+    const_before = np.concatenate([[0], np.cumsum(cnt)[:-1]])
+    rec_from = np.cumsum(rec[::-1])[::-1]
+    sizes = 13 + (nb + 7) // 8 + a[0] + 4 * const_before + rec_from
+    ebs = np.ldexp(np.float32(1) + np.arange(OCTAVE_BINS, dtype=np.float32) / np.float32(OCTAVE_BINS),
+                   reqExpo).astype(np.float32)
+    return sizes.astype(np.int64), ebs
+
+
+class FixedRateCompressor:
+    """Fixed-rate compress of one device field to a target bits/value."""
+
+    def __init__(self, n: int, dtype, blockSize: int = 128, model: RateModel | None = None,
+                 device="cuda"):
+        torch = _torch()
+        self.n, self.dtype, self.bs = n, np.dtype(dtype), blockSize
+        code = _dtype_code(self.dtype)
+        L = lib()
+        self.capacity = L.szx_amd_compress_capacity(n, code, blockSize)
+        self.wsbytes = L.szx_amd_fixed_rate_workspace(n, code, blockSize)
+        if not self.capacity or not self.wsbytes:
+            raise SZxError(f"unsupported parameters (blockSize={blockSize})")
+        self.out = torch.empty(self.capacity, dtype=torch.uint8, device=device)
+        self.ws = torch.empty(self.wsbytes, dtype=torch.uint8, device=device)
+        self.result = torch.zeros(4, dtype=torch.int64, device=device)
+        self.model = model
+
+    def __call__(self, x, targetBitsPerValue: float, tolerance: float = 0.02,
+                 method: int = RATE_LEARNED, stream=None) -> RateInfo:
+        info = RateInfo()
+        params = self.model.params.data_ptr() if (method == RATE_LEARNED and self.model) else None
+        _check(lib().szx_amd_fixed_rate_dev(x.data_ptr(), self.n, _dtype_code(self.dtype), self.bs,
+                                            targetBitsPerValue, tolerance, method, params,
+                                            self.out.data_ptr(), self.capacity, self.ws.data_ptr(),
+                                            self.wsbytes, self.result.data_ptr(), None,
+                                            C.byref(info), _stream_handle(stream)))
+        return info
+
+    def stream_view(self):
+        off, size, _, err = (int(v) for v in self.result.cpu().tolist())
+        if err:
+            raise SZxError("compress failed")
+        return self.out[off:off + size]

@@ -18,6 +18,8 @@
 #include "szx_fxr.hpp"
 #include "szx_fxr_host.hpp"
 #include "szx_internal.hpp"
+#include "szx_mlp.hpp"
+#include "szx_rate.hpp"
 
 using namespace szxamd;
 
@@ -646,6 +648,137 @@ int szx_amd_fxr_bound_dev(const void *d_in, size_t n, int dataType, int blockSiz
                           initErrorBound, (uint8_t *)d_work, (hipStream_t)hipStream, errorBound,
                           modelPasses))
         return SZx_NSCS;
+    return SZx_SCES;
+}
+
+size_t szx_amd_fixed_rate_workspace(size_t n, int dataType, int blockSize)
+{
+    const int w = width_of(dataType);
+    if (!w || !bs_supported(blockSize)) return 0;
+    return fixed_rate_workspace(n, w, blockSize);
+}
+
+int szx_amd_fixed_rate_dev(const void *d_in, size_t n, int dataType, int blockSize,
+                           float targetBitsPerValue, float tolerance, int method,
+                           const float *d_modelParams, void *d_out, size_t outCapacity,
+                           void *d_work, size_t workBytes, uint64_t *d_result, uint64_t *d_index,
+                           szx_amd_rate_info *info, void *hipStream)
+{
+    g_err.clear();
+    const int w = width_of(dataType);
+    if (!w || !d_in || n == 0 || !d_out || !d_result || !bs_supported(blockSize) ||
+        !(targetBitsPerValue > 0) || !(tolerance >= 0) ||
+        (method != SZX_AMD_RATE_BISECT && method != SZX_AMD_RATE_LEARNED) ||
+        (method == SZX_AMD_RATE_LEARNED && !d_modelParams)) {
+        set_err("invalid arguments");
+        return SZx_NSCS;
+    }
+    if (outCapacity < compress_layout(n, w, blockSize).capacity ||
+        workBytes < fixed_rate_workspace(n, w, blockSize)) {
+        set_err("output buffer or workspace too small");
+        return SZx_MERR;
+    }
+    const char *err = nullptr;
+    if (fixed_rate_solve(d_in, n, w, blockSize, targetBitsPerValue, tolerance, method, d_modelParams,
+                         (uint8_t *)d_out, (uint8_t *)d_work, d_result, d_index, info,
+                         (hipStream_t)hipStream, &err)) {
+        set_err("fixed rate: %s", err ? err : "device error");
+        return SZx_NSCS;
+    }
+    return SZx_SCES;
+}
+
+size_t szx_amd_rate_octave_workspace(void) { return octave_workspace_bytes(); }
+
+int szx_amd_rate_octave_dev(const void *d_in, size_t n, int dataType, int blockSize, int reqExpo,
+                            uint64_t *d_acc, void *d_work, size_t workBytes, void *hipStream)
+{
+    g_err.clear();
+    const int w = width_of(dataType);
+    if (!w || !d_in || n == 0 || !d_acc || blockSize < 1 || blockSize > 256 ||
+        workBytes < octave_workspace_bytes() || reqExpo < -1074 || reqExpo > 1023) {
+        set_err("invalid arguments");
+        return SZx_NSCS;
+    }
+    rate_octave_launch(d_in, n, w, blockSize, reqExpo, (uint8_t *)d_work,
+                       (unsigned long long *)d_acc, (hipStream_t)hipStream);
+    return SZx_SCES;
+}
+
+size_t szx_amd_rate_stats_bytes(size_t n)
+{
+    const size_t c = (n + RATE_CHUNK - 1) / RATE_CHUNK;
+    return (c ? c : 1) * sizeof(RateStats);
+}
+
+size_t szx_amd_rate_stats_workspace(size_t n) { return rate_stats_workspace_bytes(n); }
+
+int szx_amd_rate_stats_dev(const void *d_in, size_t n, int dataType, int blockSize, void *d_stats,
+                           void *d_work, size_t workBytes, void *hipStream)
+{
+    g_err.clear();
+    const int w = width_of(dataType);
+    if (!w || !d_in || n == 0 || !d_stats || blockSize < 1 || blockSize > 256 ||
+        workBytes < rate_stats_workspace_bytes(n)) {
+        set_err("invalid arguments");
+        return SZx_NSCS;
+    }
+    rate_stats_launch(d_in, n, w, blockSize, (uint8_t *)d_work, (RateStats *)d_stats,
+                      (hipStream_t)hipStream);
+    return SZx_SCES;
+}
+
+int szx_amd_rate_inputs_dev(const void *d_stats, size_t n, const float *d_ebs, int K, float *d_X,
+                            void *hipStream)
+{
+    g_err.clear();
+    if (!d_stats || !d_ebs || !d_X || K < 1 || n == 0) {
+        set_err("invalid arguments");
+        return SZx_NSCS;
+    }
+    const int nch = (int)((n + RATE_CHUNK - 1) / RATE_CHUNK);
+    rate_inputs_launch((const RateStats *)d_stats, nch, d_ebs, K, d_X, (hipStream_t)hipStream);
+    return SZx_SCES;
+}
+
+int szx_amd_mlp_forward_dev(const float *d_params, const float *d_X, size_t nSamples, float *d_y,
+                            void *hipStream)
+{
+    g_err.clear();
+    if (!d_params || !d_X || !d_y) {
+        set_err("invalid arguments");
+        return SZx_NSCS;
+    }
+    if (nSamples) mlp_forward_launch(d_params, d_X, nSamples, d_y, (hipStream_t)hipStream);
+    return SZx_SCES;
+}
+
+size_t szx_amd_mlp_grad_workspace(void) { return mlp_grad_workspace_bytes(); }
+
+int szx_amd_mlp_grad_dev(const float *d_params, const float *d_X, const float *d_T, size_t nSamples,
+                         float *d_grad, float *d_loss, void *d_work, size_t workBytes,
+                         void *hipStream)
+{
+    g_err.clear();
+    if (!d_params || !d_X || !d_T || !d_grad || !d_loss || nSamples == 0 ||
+        workBytes < mlp_grad_workspace_bytes()) {
+        set_err("invalid arguments");
+        return SZx_NSCS;
+    }
+    mlp_grad_launch(d_params, d_X, d_T, nSamples, d_grad, d_loss, (uint8_t *)d_work,
+                    (hipStream_t)hipStream);
+    return SZx_SCES;
+}
+
+int szx_amd_mlp_adam_dev(float *d_params, const float *d_grad, float *d_m, float *d_v, float lr,
+                         float beta1, float beta2, float eps, int step, void *hipStream)
+{
+    g_err.clear();
+    if (!d_params || !d_grad || !d_m || !d_v || step < 1) {
+        set_err("invalid arguments");
+        return SZx_NSCS;
+    }
+    mlp_adam_launch(d_params, d_grad, d_m, d_v, lr, beta1, beta2, eps, step, (hipStream_t)hipStream);
     return SZx_SCES;
 }
 

@@ -272,7 +272,7 @@ __global__ __launch_bounds__(256) void k_rate_octave(const T *x, uint64_t n, int
     __syncthreads();
     const uint64_t nb = (n + bs - 1) / bs;
     // 2^reqexp and 2^(reqexp+1) as doubles (exact for every float exponent)
-    const double lo_edge = __builtin_ldexp(1.0, reqexp), hi_edge = 2.0 * lo_edge;
+    const double lo_edge = ldexp(1.0, reqexp), hi_edge = 2.0 * lo_edge;
     unsigned long long fixed = 0;
     for (uint64_t i = blockIdx.x * 4ull + wv; i < nb; i += gridDim.x * 4ull) {
         const T *b = x + i * (uint64_t)bs;
@@ -296,9 +296,8 @@ __global__ __launch_bounds__(256) void k_rate_octave(const T *x, uint64_t n, int
             continue;
         }
         // bin k holds radii in (2^e (1 + k/B), 2^e (1 + (k+1)/B)]
-        const double t = (rd / lo_edge - 1.0) * OCT_BINS;
-        int k = (int)__builtin_ceil(t) - 1;
-        k = k < 0 ? 0 : (k >= OCT_BINS ? OCT_BINS - 1 : k);
+        const double t = (rd / lo_edge - 1.0) * OCT_BINS;   // in (0, OCT_BINS]
+        const int k = t >= OCT_BINS ? OCT_BINS - 1 : (t <= 1.0 ? 0 : (int)ceil(t) - 1);
         if (lane == 0) {
             atomicAdd(&s_acc[1 + 2 * k], 1ull);
             atomicAdd(&s_acc[2 + 2 * k], (unsigned long long)rec);
@@ -307,6 +306,175 @@ __global__ __launch_bounds__(256) void k_rate_octave(const T *x, uint64_t n, int
     if (lane == 0 && fixed) atomicAdd(&s_acc[0], fixed);
     __syncthreads();
     for (int k = threadIdx.x; k < WIDTH; k += 256) part[(uint64_t)blockIdx.x * WIDTH + k] = s_acc[k];
+}
+
+// ---------------------------------------------------------------------------
+// 3. per-chunk statistics and the rate model's inputs
+// ---------------------------------------------------------------------------
+
+// Per workgroup partial statistics of its slice of one chunk (RateStats
+// layout: doubles lo, hi, sum x, sum x^2, sum |d1|, sum d1^2, sum |d2|, then
+// u32 blocks, zero-radius blocks, radius-exponent histogram [256]).
+// d1 / d2 are first / second differences inside a codec block.
+template <typename T>
+__global__ __launch_bounds__(256) void k_rate_stats(const T *x, uint64_t n, int bs, uint64_t chunk,
+                                                    int wg_per_chunk, RateStats *part)
+{
+    __shared__ double s_d[4][7];
+    __shared__ unsigned s_hist[RATE_HIST];
+    __shared__ unsigned s_cnt[2];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int k = threadIdx.x; k < RATE_HIST; k += 256) s_hist[k] = 0;
+    if (threadIdx.x < 2) s_cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t c = blockIdx.x / wg_per_chunk, g = blockIdx.x % wg_per_chunk;
+    const uint64_t c0 = c * chunk, c1 = (c0 + chunk < n) ? c0 + chunk : n;
+    const uint64_t bpc = (c1 - c0 + bs - 1) / bs;   // blocks in the chunk
+    double lo = __builtin_inf(), hi = -__builtin_inf(), sx = 0, sxx = 0, sa = 0, sq = 0, s2 = 0;
+    for (uint64_t b = g * 4 + wv; b < bpc; b += (uint64_t)wg_per_chunk * 4) {
+        const uint64_t e0 = c0 + b * (uint64_t)bs;
+        const int cnt = (int)(c1 - e0 < (uint64_t)bs ? c1 - e0 : (uint64_t)bs);
+        T v[FX_MAXC];
+        load_block(x + e0, cnt, lane, v);
+        T blo, bhi;
+        block_min_max(x + e0, cnt, lane, v, blo, bhi);
+        const float r = (float)((bhi - blo) / 2);
+        if (lane == 0) {
+            atomicAdd(&s_cnt[0], 1u);
+            if (r == 0.0f) atomicAdd(&s_cnt[1], 1u);
+            atomicAdd(&s_hist[(f32_bits(r) >> 23) & 0xFF], 1u);
+        }
+        lo = (double)blo < lo ? (double)blo : lo;
+        hi = (double)bhi > hi ? (double)bhi : hi;
+        double carry1 = 0, carry0 = 0;   // last d1 and value of the previous column
+#pragma unroll
+        for (int q = 0; q < FX_MAXC; q++) {
+            if (64 * q >= cnt) break;
+            const int j = lane + 64 * q;
+            const double xv = (double)v[q];
+            double prev = __shfl_up(xv, 1, 64);
+            if (lane == 0) prev = carry0;
+            const double d1 = xv - prev;
+            double pd1 = __shfl_up(d1, 1, 64);
+            if (lane == 0) pd1 = carry1;
+            carry0 = __shfl(xv, 63, 64);
+            carry1 = __shfl(d1, 63, 64);
+            if (j < cnt) {
+                sx += xv;
+                sxx += xv * xv;
+                if (j >= 1) {
+                    sa += fabs(d1);
+                    sq += d1 * d1;
+                }
+                if (j >= 2) s2 += fabs(d1 - pd1);
+            }
+        }
+    }
+    double acc[7] = {lo, hi, sx, sxx, sa, sq, s2};
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) {
+        double o = __shfl_xor(acc[0], m, 64);
+        acc[0] = o < acc[0] ? o : acc[0];
+        o = __shfl_xor(acc[1], m, 64);
+        acc[1] = o > acc[1] ? o : acc[1];
+#pragma unroll
+        for (int k = 2; k < 7; k++) acc[k] += __shfl_xor(acc[k], m, 64);
+    }
+    if (lane == 0)
+        for (int k = 0; k < 7; k++) s_d[wv][k] = acc[k];
+    __syncthreads();
+    RateStats &o = part[blockIdx.x];
+    if (threadIdx.x == 0) {
+        double a[7];
+        for (int k = 0; k < 7; k++) a[k] = s_d[0][k];
+        for (int w = 1; w < 4; w++) {
+            a[0] = s_d[w][0] < a[0] ? s_d[w][0] : a[0];
+            a[1] = s_d[w][1] > a[1] ? s_d[w][1] : a[1];
+            for (int k = 2; k < 7; k++) a[k] += s_d[w][k];
+        }
+        for (int k = 0; k < 7; k++) o.d[k] = a[k];
+        o.blocks = s_cnt[0];
+        o.zero = s_cnt[1];
+        o.values = c1 - c0;
+    }
+    for (int k = threadIdx.x; k < RATE_HIST; k += 256) o.hist[k] = s_hist[k];
+}
+
+// Fold wg_per_chunk partials per chunk (one workgroup per chunk).
+__global__ void k_rate_stats_fold(const RateStats *part, int wg_per_chunk, RateStats *out)
+{
+    const RateStats *p = part + (uint64_t)blockIdx.x * wg_per_chunk;
+    RateStats &o = out[blockIdx.x];
+    for (int k = threadIdx.x; k < RATE_HIST; k += blockDim.x) {
+        unsigned h = 0;
+        for (int g = 0; g < wg_per_chunk; g++) h += p[g].hist[k];
+        o.hist[k] = h;
+    }
+    if (threadIdx.x == 0) {
+        double a[7];
+        for (int k = 0; k < 7; k++) a[k] = p[0].d[k];
+        unsigned blocks = p[0].blocks, zero = p[0].zero;
+        for (int g = 1; g < wg_per_chunk; g++) {
+            a[0] = p[g].d[0] < a[0] ? p[g].d[0] : a[0];
+            a[1] = p[g].d[1] > a[1] ? p[g].d[1] : a[1];
+            for (int k = 2; k < 7; k++) a[k] += p[g].d[k];
+            blocks += p[g].blocks;
+            zero += p[g].zero;
+        }
+        for (int k = 0; k < 7; k++) o.d[k] = a[k];
+        o.blocks = blocks;
+        o.zero = zero;
+        o.values = p[0].values;
+    }
+}
+
+__device__ inline float log2_ratio(double a, double r)
+{
+    const double q = a / r;
+    const double lo = 8.673617379884035e-19;   // 2^-60
+    return (float)(log2(q > lo ? q : lo) / 16.0);
+}
+
+// X[(k * nchunks + c)][24]: the rate model's inputs for chunk c at bound
+// ebs[k] (see rate_features for the meaning of each input).
+__global__ void k_rate_inputs(const RateStats *st, int nchunks, const float *ebs, int K, float *X)
+{
+    const uint64_t id = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (id >= (uint64_t)K * nchunks) return;
+    const int k = (int)(id / nchunks), c = (int)(id % nchunks);
+    const RateStats &s = st[c];
+    float *f = X + id * RATE_INPUTS;
+    const float eb = ebs[k];
+    const int e = req_expo(eb);
+    double range = s.d[1] - s.d[0];
+    if (!(range > 0)) range = 1e-300;
+    const double nv = (double)s.values;
+    const double mean = s.d[2] / nv;
+    double var = s.d[3] / nv - mean * mean;
+    if (var < 0) var = 0;
+    const double nd1 = nv > 1 ? nv - 1 : 1, nd2 = nv > 2 ? nv - 2 : 1;
+    const int er = (int)floor(log2(range));
+    const double le = log2((double)eb);
+    f[0] = (float)(e - er) / 16.0f;
+    f[1] = (float)(le - e);
+    f[2] = log2_ratio((double)eb, range);
+    f[3] = log2_ratio(sqrt(var), range);
+    f[4] = log2_ratio(s.d[4] / nd1, range);
+    f[5] = log2_ratio(sqrt(s.d[5] / nd1), range);
+    f[6] = log2_ratio(s.d[6] / nd2, range);
+    const float nb = s.blocks ? (float)s.blocks : 1.0f;
+    f[7] = (float)s.zero / nb;
+    // histogram of (radius exponent - bound exponent): <= -1, 0 .. 13, >= 14
+    float h[16];
+    for (int b = 0; b < 16; b++) h[b] = 0;
+    for (int b = 0; b < RATE_HIST; b++) {
+        const unsigned cnt = s.hist[b];
+        if (!cnt) continue;
+        const int d = (b - 127) - e;
+        const int bin = d <= -1 ? 0 : (d >= 14 ? 15 : d + 1);
+        h[bin] += (float)cnt;
+    }
+    for (int b = 0; b < 16; b++) f[8 + b] = h[b] / nb;
 }
 
 // ---------------------------------------------------------------------------
@@ -378,6 +546,38 @@ void fxr_estimate_launch(const void *x, uint64_t n, int w, int bs, int rate, uin
                            static_cast<const double *>(x), L.ns, (uint64_t)rate * bs, bs, med, rad, ebs,
                            K, part);
     sum_parts(part, g, 3 * K, sums, st);
+}
+
+uint64_t rate_stats_workspace_bytes(uint64_t n)
+{
+    const uint64_t nchunks = (n + RATE_CHUNK - 1) / RATE_CHUNK;
+    return (nchunks ? nchunks : 1) * RATE_WG_PER_CHUNK * sizeof(RateStats);
+}
+
+void rate_stats_launch(const void *x, uint64_t n, int w, int bs, uint8_t *ws, RateStats *d_stats,
+                       hipStream_t st)
+{
+    const uint64_t nchunks = (n + RATE_CHUNK - 1) / RATE_CHUNK;
+    RateStats *part = reinterpret_cast<RateStats *>(ws);
+    const unsigned g = (unsigned)(nchunks * RATE_WG_PER_CHUNK);
+    if (w == 4)
+        hipLaunchKernelGGL(k_rate_stats<float>, dim3(g), dim3(256), 0, st,
+                           static_cast<const float *>(x), n, bs, (uint64_t)RATE_CHUNK,
+                           RATE_WG_PER_CHUNK, part);
+    else
+        hipLaunchKernelGGL(k_rate_stats<double>, dim3(g), dim3(256), 0, st,
+                           static_cast<const double *>(x), n, bs, (uint64_t)RATE_CHUNK,
+                           RATE_WG_PER_CHUNK, part);
+    hipLaunchKernelGGL(k_rate_stats_fold, dim3((unsigned)nchunks), dim3(256), 0, st, part,
+                       RATE_WG_PER_CHUNK, d_stats);
+}
+
+void rate_inputs_launch(const RateStats *d_stats, int nchunks, const float *d_ebs, int K, float *d_X,
+                        hipStream_t st)
+{
+    const uint64_t total = (uint64_t)K * nchunks;
+    hipLaunchKernelGGL(k_rate_inputs, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, d_stats,
+                       nchunks, d_ebs, K, d_X);
 }
 
 uint64_t octave_workspace_bytes() { return (uint64_t)OCT_WG * (1 + 2 * OCT_BINS) * 8; }

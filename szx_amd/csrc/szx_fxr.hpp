@@ -34,4 +34,26 @@ uint64_t octave_workspace_bytes();
 void rate_octave_launch(const void *x, uint64_t n, int w, int bs, int reqexp, uint8_t *ws,
                         unsigned long long *d_acc, hipStream_t st);
 
+// Per-chunk statistics feeding the rate model (RATE_CHUNK values per chunk;
+// the last chunk may be shorter).  d: min, max, sum x, sum x^2, sum |d1|,
+// sum d1^2, sum |d2| (differences inside codec blocks); hist: codec blocks
+// per float radius exponent field (0..255).
+constexpr uint64_t RATE_CHUNK = 1ull << 19;
+constexpr int RATE_WG_PER_CHUNK = 16;
+constexpr int RATE_HIST = 256;
+constexpr int RATE_INPUTS = 24;
+struct RateStats {
+    double d[7];
+    uint64_t values;
+    unsigned blocks, zero;
+    unsigned hist[RATE_HIST];
+};
+
+uint64_t rate_stats_workspace_bytes(uint64_t n);
+void rate_stats_launch(const void *x, uint64_t n, int w, int bs, uint8_t *ws, RateStats *d_stats,
+                       hipStream_t st);
+// X[(k * nchunks + c)][RATE_INPUTS] for K bounds at d_ebs.
+void rate_inputs_launch(const RateStats *d_stats, int nchunks, const float *d_ebs, int K, float *d_X,
+                        hipStream_t st);
+
 }  // namespace szxamd

@@ -1,0 +1,369 @@
+// szx_rate.cpp -- fixed-rate compression on the exact codec: find a bound
+// whose stream has targetBitsPerValue (8 * stream bytes / n) within a
+// relative tolerance, and leave that stream in the output buffer.
+//
+// Two solvers, both judged by the real codec (every stream is exact codec
+// output; nothing is approximated in the codec arithmetic):
+//
+//  * bisection: bisect log2(eb) between range * 2^-D and the range, one full
+//    compress per step, until the rate is within tolerance.
+//
+//  * learned: one statistics pass gives per-chunk features; the rate MLP
+//    (szx_mlp.hip) predicts each chunk's log2 bits/value at a ladder of
+//    bounds and the predicted field rate picks a bound.  Its exponent names
+//    an octave of bounds; one octave pass (k_rate_octave) gives the exact
+//    stream size for OCT_BINS bounds spread over that octave, so the best
+//    bound in the octave is known exactly before compressing.  If the
+//    target lies outside the octave's sizes the neighbouring octave is
+//    tried (at most OCT_TRIES octave passes), then bisection takes over.
+//    The final compress runs at the chosen bound and its size is checked.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include <vector>
+
+#include "szx_amd.h"
+#include "szx_fxr.hpp"
+#include "szx_internal.hpp"
+#include "szx_mlp.hpp"
+#include "szx_rate.hpp"
+
+namespace szxamd {
+
+namespace {
+
+constexpr int K_MAX = 256;      // ladder of candidate bounds (quarter octaves)
+constexpr int OCT_TRIES = 3;
+constexpr int BISECT_MAX = 60;
+constexpr int STREAM_HEADER = 13;   // szx_float.cc:1091-1098
+
+struct RateLayout {
+    uint64_t comp, oct, acc, stats_ws, stats, ebs, X, y, range, minmax, bytes;
+    uint64_t nchunks;
+};
+
+RateLayout rate_layout(uint64_t n, int w, int bs)
+{
+    RateLayout L{};
+    const CompressLayout C = compress_layout(n, w, bs);
+    L.nchunks = (n + RATE_CHUNK - 1) / RATE_CHUNK;
+    if (!L.nchunks) L.nchunks = 1;
+    auto al = [](uint64_t v) { return (v + 255) & ~255ull; };
+    uint64_t off = 0;
+    L.comp = off;     off += al(C.ws_bytes);
+    L.oct = off;      off += al(octave_workspace_bytes());
+    L.acc = off;      off += al((1 + 2 * OCT_BINS) * 8);
+    L.stats_ws = off; off += al(rate_stats_workspace_bytes(n));
+    L.stats = off;    off += al(L.nchunks * sizeof(RateStats));
+    L.ebs = off;      off += al(K_MAX * 4);
+    L.X = off;        off += al((uint64_t)K_MAX * L.nchunks * RATE_INPUTS * 4);
+    L.y = off;        off += al((uint64_t)K_MAX * L.nchunks * 4);
+    L.range = off;    off += al(range_workspace_bytes());
+    L.minmax = off;   off += 256;
+    L.bytes = off;
+    return L;
+}
+
+struct Solver {
+    const void *x;
+    uint64_t n;
+    int w, bs;
+    float target, tol;
+    uint8_t *out, *ws;
+    uint64_t *result, *index;
+    hipStream_t st;
+    RateLayout L;
+    int passes = 0;
+    float last_eb = -1.0f;   // bound of the stream now in the output buffer
+    double last_bpv = 0.0;
+    float best_eb = 0;
+    double best_bpv = -1;
+    const char *err = nullptr;
+
+    bool ok(hipError_t e, const char *what)
+    {
+        if (e == hipSuccess) return true;
+        err = what;
+        return false;
+    }
+    bool within(double bpv) const { return fabs(bpv - target) <= (double)tol * target; }
+    void note(float eb, double bpv)
+    {
+        if (best_bpv < 0 || fabs(bpv - target) < fabs(best_bpv - target)) {
+            best_bpv = bpv;
+            best_eb = eb;
+        }
+    }
+
+    // One full compress at eb; returns the exact rate.
+    bool compress_at(float eb, double *bpv)
+    {
+        if (compress_launch(x, n, w, 0, eb, 0.0f, bs, out, ws + L.comp, result, index, st)) {
+            err = "compress launch";
+            return false;
+        }
+        uint64_t r[4];
+        if (!ok(hipMemcpyAsync(r, result, sizeof r, hipMemcpyDeviceToHost, st), "D2H result") ||
+            !ok(hipStreamSynchronize(st), "compress"))
+            return false;
+        if (r[3]) {
+            err = "compress: a look-back wait overran its bound";
+            return false;
+        }
+        passes++;
+        last_eb = eb;
+        *bpv = last_bpv = 8.0 * (double)r[1] / (double)n;
+        note(eb, *bpv);
+        return true;
+    }
+
+    // Exact stream sizes for the OCT_BINS bounds of octave e.
+    bool octave(int e, std::vector<double> &bpv, std::vector<float> &ebs)
+    {
+        e = e < -1074 ? -1074 : (e > 1023 ? 1023 : e);
+        rate_octave_launch(x, n, w, bs, e, ws + L.oct, (unsigned long long *)(ws + L.acc), st);
+        std::vector<unsigned long long> acc(1 + 2 * OCT_BINS);
+        if (!ok(hipMemcpyAsync(acc.data(), ws + L.acc, acc.size() * 8, hipMemcpyDeviceToHost, st),
+                "D2H octave") ||
+            !ok(hipStreamSynchronize(st), "octave pass"))
+            return false;
+        passes++;
+        const uint64_t nb = (n + bs - 1) / bs, nbit = (nb + 7) / 8;
+        // size(eb_k) = header + bitmap + fixed + 4 * (count of bins < k)
+        //              + records of bins >= k
+        uint64_t rec_all = 0;
+        for (int k = 0; k < OCT_BINS; k++) rec_all += acc[2 + 2 * k];
+        uint64_t const_part = 0, rec_part = rec_all;
+        bpv.assign(OCT_BINS, 0.0);
+        ebs.assign(OCT_BINS, 0.0f);
+        for (int k = 0; k < OCT_BINS; k++) {
+            const uint64_t size = STREAM_HEADER + nbit + acc[0] + const_part + rec_part;
+            bpv[k] = 8.0 * (double)size / (double)n;
+            ebs[k] = ldexpf(1.0f + (float)k / OCT_BINS, e);
+            const_part += 4 * acc[1 + 2 * k];
+            rec_part -= acc[2 + 2 * k];
+        }
+        return true;
+    }
+
+    // Octave search from exponent e; on success *eb is the best bound of the
+    // octave that holds the target (or the best seen when none does).
+    // Returns false only on a device error; *found says whether an octave
+    // bracketed the target.
+    bool octave_search(int e, float *eb, bool *found, int *lo_e, int *hi_e)
+    {
+        *found = false;
+        *lo_e = -100000;
+        *hi_e = 100000;
+        std::vector<double> bpv;
+        std::vector<float> ebs;
+        for (int t = 0; t < OCT_TRIES; t++) {
+            if (!octave(e, bpv, ebs)) return false;
+            // bpv is non-increasing in k
+            int bk = 0;
+            for (int k = 1; k < OCT_BINS; k++)
+                if (fabs(bpv[k] - target) < fabs(bpv[bk] - target)) bk = k;
+            if (bpv[OCT_BINS - 1] > target * (1.0 + tol)) {   // bounds too tight
+                *lo_e = e + 1 > *lo_e ? e + 1 : *lo_e;
+                e++;
+                continue;
+            }
+            if (bpv[0] < target * (1.0 - tol)) {              // bounds too loose
+                *hi_e = e < *hi_e ? e : *hi_e;
+                e--;
+                continue;
+            }
+            *eb = ebs[bk];
+            *found = true;
+            return true;
+        }
+        return true;
+    }
+
+    bool bisect(double lg_lo, double lg_hi, float *eb_out)
+    {
+        for (int i = 0; i < BISECT_MAX; i++) {
+            const double mid = 0.5 * (lg_lo + lg_hi);
+            const float eb = (float)exp2(mid);
+            double bpv;
+            if (!compress_at(eb, &bpv)) return false;
+            if (within(bpv)) {
+                *eb_out = eb;
+                return true;
+            }
+            if (bpv > target) lg_lo = mid;   // too many bits: loosen the bound
+            else lg_hi = mid;
+            if (lg_hi - lg_lo < 1e-7) break;
+        }
+        *eb_out = best_eb;
+        return true;
+    }
+
+    bool value_range(double *range)
+    {
+        range_launch(x, n, w, ws + L.range, ws + L.minmax, st);
+        double mm[2] = {0, 0};
+        if (!ok(hipMemcpyAsync(mm, ws + L.minmax, 2 * (size_t)w, hipMemcpyDeviceToHost, st), "D2H") ||
+            !ok(hipStreamSynchronize(st), "range"))
+            return false;
+        if (w == 4) {
+            float f[2];
+            memcpy(f, mm, 8);
+            *range = (double)f[1] - (double)f[0];
+        } else {
+            *range = mm[1] - mm[0];
+        }
+        return true;
+    }
+};
+
+}  // namespace
+
+uint64_t fixed_rate_workspace(uint64_t n, int w, int bs) { return rate_layout(n, w, bs).bytes; }
+
+int fixed_rate_solve(const void *d_in, uint64_t n, int w, int bs, float target, float tol, int method,
+                     const float *d_params, uint8_t *d_out, uint8_t *d_ws, uint64_t *d_result,
+                     uint64_t *d_index, szx_amd_rate_info *info, hipStream_t st, const char **err)
+{
+    Solver S;
+    S.x = d_in;
+    S.n = n;
+    S.w = w;
+    S.bs = bs;
+    S.target = target;
+    S.tol = tol;
+    S.out = d_out;
+    S.ws = d_ws;
+    S.result = d_result;
+    S.index = d_index;
+    S.st = st;
+    S.L = rate_layout(n, w, bs);
+    const double depth = w == 4 ? 40.0 : 64.0;   // octaves below the range searched
+    float eb = 0.0f;
+    double bpv = 0.0;
+    int feature_passes = 0;
+    bool done = false;
+
+    if (method == SZX_AMD_RATE_LEARNED) {
+        // features and the model's ladder of predictions
+        const int nch = (int)S.L.nchunks;
+        rate_stats_launch(d_in, n, w, bs, d_ws + S.L.stats_ws, (RateStats *)(d_ws + S.L.stats), st);
+        std::vector<RateStats> stats(nch);
+        if (!S.ok(hipMemcpyAsync(stats.data(), d_ws + S.L.stats, nch * sizeof(RateStats),
+                                 hipMemcpyDeviceToHost, st), "D2H stats") ||
+            !S.ok(hipStreamSynchronize(st), "statistics pass")) {
+            *err = S.err;
+            return -1;
+        }
+        feature_passes = 1;
+        double lo = stats[0].d[0], hi = stats[0].d[1];
+        for (int c = 1; c < nch; c++) {
+            lo = stats[c].d[0] < lo ? stats[c].d[0] : lo;
+            hi = stats[c].d[1] > hi ? stats[c].d[1] : hi;
+        }
+        double range = hi - lo;
+        if (!(range > 0)) range = 1.0;
+        const int K = (int)(4 * depth) < K_MAX ? (int)(4 * depth) : K_MAX;
+        std::vector<float> ladder(K);
+        for (int k = 0; k < K; k++) ladder[k] = (float)(range * exp2(-0.25 * k));
+        std::vector<float> y((size_t)K * nch);
+        if (!S.ok(hipMemcpyAsync(d_ws + S.L.ebs, ladder.data(), 4 * (size_t)K, hipMemcpyHostToDevice, st),
+                  "H2D ladder")) {
+            *err = S.err;
+            return -1;
+        }
+        rate_inputs_launch((const RateStats *)(d_ws + S.L.stats), nch, (const float *)(d_ws + S.L.ebs), K,
+                           (float *)(d_ws + S.L.X), st);
+        mlp_forward_launch(d_params, (const float *)(d_ws + S.L.X), (uint64_t)K * nch,
+                           (float *)(d_ws + S.L.y), st);
+        if (!S.ok(hipMemcpyAsync(y.data(), d_ws + S.L.y, y.size() * 4, hipMemcpyDeviceToHost, st),
+                  "D2H predictions") ||
+            !S.ok(hipStreamSynchronize(st), "rate model")) {
+            *err = S.err;
+            return -1;
+        }
+        // predicted field rate along the ladder (bits of the records,
+        // medians and bitmap per chunk, plus the 13-byte header)
+        std::vector<double> pred(K);
+        for (int k = 0; k < K; k++) {
+            double bits = 8.0 * STREAM_HEADER;
+            for (int c = 0; c < nch; c++) bits += exp2((double)y[(size_t)k * nch + c]) * (double)stats[c].values;
+            pred[k] = bits / (double)n;
+        }
+        // first rung whose predicted rate reaches the target; interpolate in
+        // log2(rate) between it and the rung before
+        double lg_eb = log2((double)ladder[K - 1]);
+        for (int k = 0; k < K; k++) {
+            if (pred[k] >= target) {
+                if (k == 0) {
+                    lg_eb = log2((double)ladder[0]);
+                } else {
+                    const double a = log2(pred[k - 1]), b = log2(pred[k]), t = log2((double)target);
+                    const double f = b > a ? (t - a) / (b - a) : 1.0;
+                    lg_eb = log2((double)ladder[k - 1]) - 0.25 * f;
+                }
+                break;
+            }
+        }
+        const int e0 = (int)floor(lg_eb);
+        bool found = false;
+        int lo_e, hi_e;
+        if (!S.octave_search(e0, &eb, &found, &lo_e, &hi_e)) {
+            *err = S.err;
+            return -1;
+        }
+        if (found) {
+            if (!S.compress_at(eb, &bpv)) {
+                *err = S.err;
+                return -1;
+            }
+            done = true;
+        } else {
+            // bisection inside what the octave passes ruled out
+            double glo = log2(range) - depth, ghi = log2(range) + 1;
+            if (lo_e > -100000) glo = glo > lo_e ? glo : (double)lo_e;
+            if (hi_e < 100000) ghi = ghi < hi_e ? ghi : (double)hi_e;
+            if (!(glo < ghi)) {
+                glo = log2(range) - depth;
+                ghi = log2(range) + 1;
+            }
+            if (!S.bisect(glo, ghi, &eb)) {
+                *err = S.err;
+                return -1;
+            }
+        }
+    } else {
+        double range;
+        if (!S.value_range(&range)) {
+            *err = S.err;
+            return -1;
+        }
+        if (!(range > 0)) range = 1.0;
+        if (!S.bisect(log2(range) - depth, log2(range) + 1, &eb)) {
+            *err = S.err;
+            return -1;
+        }
+    }
+    if (!done) {
+        // leave the stream of the chosen bound (the best seen when no bound
+        // was within tolerance) in the output buffer
+        if (!S.within(S.best_bpv)) eb = S.best_eb;
+        if (S.last_eb == eb) {
+            bpv = S.last_bpv;
+        } else if (!S.compress_at(eb, &bpv)) {
+            *err = S.err;
+            return -1;
+        }
+    }
+    if (info) {
+        info->errorBound = eb;
+        info->bitsPerValue = bpv;
+        info->codecPasses = S.passes;
+        info->featurePasses = feature_passes;
+        info->hit = S.within(bpv) ? 1 : 0;
+    }
+    return 0;
+}
+
+}  // namespace szxamd

@@ -299,3 +299,30 @@ def test_fp64_quirk_only_changes_rb5_records():
     assert rb5, "workload did not reach reqBytes 5"
     diff_blocks = set(np.nonzero(a != b)[0] // 128)
     assert diff_blocks and diff_blocks <= rb5
+
+
+# ---- the rate model's CPU oracle, pinned by finite differences ---------------
+
+def test_mlp_oracle_gradient_matches_finite_differences():
+    import mlp_oracle as M
+    r = np.random.default_rng(5)
+    P = M.init_params(1).astype(np.float64)
+    P[M.N_PARAMS - 1] = 0.3
+    X = r.standard_normal((40, M.N_IN))
+    T = r.standard_normal(40)
+    g, _ = M.grad(P, X, T, rounding=False)
+    h = 1e-6
+    for i in r.choice(M.N_PARAMS, 120, replace=False).tolist() + [M.N_PARAMS - 1, 1536, 5760]:
+        e = np.zeros_like(P)
+        e[i] = h
+        fd = (M.loss(P + e, X, T, rounding=False) - M.loss(P - e, X, T, rounding=False)) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 + 1e-5 * abs(fd), (i, fd, g[i])
+
+
+def test_mlp_oracle_bf16_rounding():
+    import mlp_oracle as M
+    a = np.array([1.0, 1.00390625, 1.005859375, -3.14159, 65504.0, 1e-30], np.float32)
+    b = M.bf16(a)
+    # bf16 keeps 8 significant bits: 1 + 2^-8 is a tie and rounds to even (1.0)
+    assert b[0] == 1.0 and b[1] == 1.0 and b[2] == 1.0078125
+    assert np.all(np.abs(b - a) <= np.abs(a) * 2.0 ** -8)
