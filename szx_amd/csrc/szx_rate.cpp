@@ -158,25 +158,41 @@ struct Solver {
         *hi_e = 100000;
         std::vector<double> bpv;
         std::vector<float> ebs;
+        int last_dir = 0;           // +1: moved to looser bounds, -1: tighter
+        float edge_eb = 0.0f;       // nearest bound of the previous octave
+        double edge_bpv = -1.0;
         for (int t = 0; t < OCT_TRIES; t++) {
             if (!octave(e, bpv, ebs)) return false;
             // bpv is non-increasing in k
             int bk = 0;
             for (int k = 1; k < OCT_BINS; k++)
                 if (fabs(bpv[k] - target) < fabs(bpv[bk] - target)) bk = k;
-            if (bpv[OCT_BINS - 1] > target * (1.0 + tol)) {   // bounds too tight
+            int dir = 0;
+            if (bpv[OCT_BINS - 1] > target * (1.0 + tol)) dir = +1;   // bounds too tight
+            else if (bpv[0] < target * (1.0 - tol)) dir = -1;         // bounds too loose
+            if (dir == 0) {
+                *eb = ebs[bk];
+                *found = true;
+                return true;
+            }
+            if (last_dir == -dir) {
+                // the target falls in the jump between two neighbouring
+                // octaves (every block's reqLength moves by one bit at the
+                // octave edge): no bound reaches it; take the nearer edge
+                *eb = fabs(bpv[bk] - target) <= fabs(edge_bpv - target) ? ebs[bk] : edge_eb;
+                *found = true;
+                return true;
+            }
+            edge_eb = ebs[bk];
+            edge_bpv = bpv[bk];
+            last_dir = dir;
+            if (dir > 0) {
                 *lo_e = e + 1 > *lo_e ? e + 1 : *lo_e;
                 e++;
-                continue;
-            }
-            if (bpv[0] < target * (1.0 - tol)) {              // bounds too loose
+            } else {
                 *hi_e = e < *hi_e ? e : *hi_e;
                 e--;
-                continue;
             }
-            *eb = ebs[bk];
-            *found = true;
-            return true;
         }
         return true;
     }

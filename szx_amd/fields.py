@@ -73,7 +73,11 @@ def sinusoid_field(shape, seed, n_waves=32, kmax=16, slope=2.0, noise=1e-3, n_bo
 
 def sinusoid_field_torch(shape, seed, n_waves=64, kmax=32, slope=2.5, noise=1e-4, n_boxes=2,
                          box_frac=0.30, box_value="mean", dtype="float32", device="cuda"):
-    """Device-resident version of sinusoid_field (same construction, torch)."""
+    """Device-resident version of sinusoid_field (same construction, torch).
+
+    sin(A_w + B_w) with A_w over the leading axes and B_w along the fastest
+    one is S_A C_B + C_A S_B, so the whole sum over waves is two
+    (leading cells x waves) @ (waves x fastest axis) products in float64."""
     import torch
     shape = tuple(int(s) for s in shape)
     rng = np.random.default_rng(seed)
@@ -81,20 +85,25 @@ def sinusoid_field_torch(shape, seed, n_waves=64, kmax=32, slope=2.5, noise=1e-4
     k, amp, phase = _waves(rng, ndim, n_waves, kmax, slope)
     gen = torch.Generator(device=device)
     gen.manual_seed(int(seed))
-    grids = [torch.arange(s, dtype=torch.float64, device=device) / s for s in shape]
-    f = torch.zeros(shape, dtype=torch.float64, device=device)
-    for w in range(n_waves):
-        a = torch.full(shape[:-1] + (1,), float(phase[w]), dtype=torch.float64, device=device)
-        for d in range(ndim - 1):
-            sh = [1] * ndim
-            sh[d] = shape[d]
-            a = a + 2.0 * np.pi * float(k[w, d]) * grids[d].reshape(sh)
-        b = 2.0 * np.pi * float(k[w, ndim - 1]) * grids[ndim - 1]
-        f += float(amp[w]) * (torch.sin(a) * torch.cos(b) + torch.cos(a) * torch.sin(b))
-        del a
+    f64 = torch.float64
+    kt = torch.tensor(k, dtype=f64, device=device)
+    lead = shape[:-1]
+    A = torch.tensor(phase, dtype=f64, device=device).reshape(n_waves, *([1] * len(lead)))
+    for d, s in enumerate(lead):
+        g = torch.arange(s, dtype=f64, device=device) / s
+        sh = [1] + [1] * len(lead)
+        sh[1 + d] = s
+        A = A + 2.0 * np.pi * kt[:, d].reshape(n_waves, *([1] * len(lead))) * g.reshape(sh)
+    A = A.reshape(n_waves, -1)
+    z = torch.arange(shape[-1], dtype=f64, device=device) / shape[-1]
+    B = 2.0 * np.pi * kt[:, ndim - 1:ndim] * z.reshape(1, -1)
+    at = torch.tensor(amp, dtype=f64, device=device).reshape(-1, 1)
+    f = (torch.sin(A).t().contiguous() @ (torch.cos(B) * at).contiguous()
+         + torch.cos(A).t().contiguous() @ (torch.sin(B) * at).contiguous()).reshape(shape)
+    del A, B
     std = float(f.std())
     if noise > 0:
-        f += torch.randn(shape, dtype=torch.float64, device=device, generator=gen) * (noise * std)
+        f += torch.randn(shape, dtype=f64, device=device, generator=gen) * (noise * std)
     fill = 0.0 if box_value == "zero" else float(f.mean())
     for box in _boxes(rng, shape, n_boxes, box_frac):
         f[box] = fill
