@@ -13,9 +13,12 @@ assembly).  value = uncompressed bytes / step time (GB/s).  Also reported:
 ABS-mode compress (the bound already known, as the reference's CLI times it),
 decompress with the compressor's record index and without it (rebuilt from
 the stream), roofline of the dominant kernel from HIP events recorded around
-it inside the library, a full-size parity check against the oracle, and the
+it inside the library, a full-size parity check against the oracle, the
 reference's own CPU codec timed on this host (compiled from its sources into
-oracle/_ref; the oracle itself if that is absent).
+oracle/_ref; the oracle itself if that is absent), and the fixed-rate solve
+on 64 fp32 256^3 fields at 1/2/4/8 bits per value: learned proposal vs
+codec-driven bisection (codec passes, wall time, rate error), with the
+reference's sampled-model bound judged by its real stream beside them.
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--rel 1e-3]
 """
@@ -76,6 +79,78 @@ def cpu_baseline(x_host, eb, sample_values):
     return kind, out
 
 
+def fixed_rate_eval(n_fields, dev, tol=0.02, targets=(1.0, 2.0, 4.0, 8.0)):
+    """Fixed-rate solve on fp32 256^3 fields (32 waves, k in [1,16], slope
+    U[1.5,4], noise sigma/std log-uniform [1e-6,1e-2], a 25% zero box; seeds
+    5000+i): learned proposal vs codec-driven bisection, plus the reference's
+    sampled-model FXR bound judged by the real stream.  Wall time per solve
+    includes every pass and host synchronisation."""
+    import numpy as np
+    import torch
+    import szx_amd as Z
+    from szx_amd.fields import sinusoid_field_torch
+    n = 256 ** 3
+    model = Z.RateModel(device=dev)
+    fr = Z.FixedRateCompressor(n, np.float32, 128, model=model, device=dev)
+    fxb = Z.DeviceFixedRateBound(n, np.float32, 128, device=dev)
+    comp = Z.DeviceCompressor(n, np.float32, 128, device=dev)
+    res = {m: {"passes": [], "ms": [], "err": [], "hit": []} for m in ("learned", "bisection",
+                                                                       "reference_model")}
+
+    def gen(i):
+        r = np.random.default_rng(5000 + i)
+        slope, noise = r.uniform(1.5, 4.0), 10 ** r.uniform(-6, -2)
+        return sinusoid_field_torch((256, 256, 256), seed=5000 + i, n_waves=32, kmax=16,
+                                    slope=slope, noise=noise, n_boxes=1, box_frac=0.25,
+                                    box_value="zero", device=dev).contiguous().view(-1)
+
+    x = gen(0)
+    fr(x, 2.0, tol, method=Z.RATE_LEARNED)   # warm-up
+    fr(x, 2.0, tol, method=Z.RATE_BISECT)
+    for i in range(n_fields):
+        x = gen(i)
+        torch.cuda.synchronize()
+        for t in targets:
+            for name, meth in (("learned", Z.RATE_LEARNED), ("bisection", Z.RATE_BISECT)):
+                t0 = time.perf_counter()
+                info = fr(x, t, tol, method=meth)
+                ms = (time.perf_counter() - t0) * 1e3
+                d = res[name]
+                d["passes"].append(info.codecPasses)
+                d["ms"].append(ms)
+                d["err"].append(abs(info.bitsPerValue - t) / t)
+                d["hit"].append(info.hit)
+            # the reference's fixed-rate mode: sampled model for CR = 32 / bpv
+            t0 = time.perf_counter()
+            eb, _ = fxb(x, 32.0 / t, tol)
+            comp(x, Z.ABS, absErrBound=eb)
+            size = comp.stream_view()[2]
+            ms = (time.perf_counter() - t0) * 1e3
+            bpv = 8.0 * size / n
+            d = res["reference_model"]
+            d["passes"].append(1)
+            d["ms"].append(ms)
+            d["err"].append(abs(bpv - t) / t)
+            d["hit"].append(int(abs(bpv - t) <= tol * t))
+        if (i + 1) % 16 == 0:
+            log(f"  fixed rate: {i + 1}/{n_fields} fields")
+    out = {"fields": n_fields, "targets_bpv": list(targets), "tolerance": tol,
+           "workload": "fp32 256^3 sinusoid fields (32 waves, k<=16, slope U[1.5,4], noise "
+                       "1e-6..1e-2 std, 25% zero box), seeds 5000+i"}
+    for name, d in res.items():
+        out[name] = {"mean_codec_passes": float(np.mean(d["passes"])),
+                     "mean_solve_ms": float(np.mean(d["ms"])),
+                     "median_solve_ms": float(np.median(d["ms"])),
+                     "hit_rate": float(np.mean(d["hit"])),
+                     "mean_rate_err": float(np.mean(d["err"])),
+                     "max_passes": int(np.max(d["passes"]))}
+    both = [a == b for a, b in zip(res["learned"]["hit"], res["bisection"]["hit"])]
+    out["learned_hits_where_bisection_hits"] = float(np.mean(
+        [a for a, b in zip(res["learned"]["hit"], res["bisection"]["hit"]) if b] or [0]))
+    out["same_hit_outcome"] = float(np.mean(both))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,6 +161,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--no-noindex", action="store_true", help="skip the index-free decode")
+    ap.add_argument("--fxr-fields", type=int, default=64,
+                    help="fixed-rate evaluation fields (0 skips it)")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run 3 plain elementwise copies of the input (y = x * 1, "
+                         "N*4 bytes read and written) to calibrate HBM byte counters")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -154,6 +234,14 @@ def main():
     kd_ms = kd["decode"][0] / max(1, kd["decode"][1])
     assert dec.check() == 0, "decoder rejected the stream"
 
+    if args.calib_copy:
+        log("calibration: 3 plain elementwise copies of the input")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            torch.mul(x, 1.0, out=y)
+        torch.cuda.synchronize()
+        del y
+
     ms_dec_noidx = None
     if not args.no_noindex:
         log("timing 1 decompress without the index (index rebuilt from the stream)")
@@ -203,6 +291,12 @@ def main():
                             else "oracle serial compress"),
                "serial_fastmode2": res["serial"]}
 
+    fixed = None
+    if args.fxr_fields > 0:
+        log(f"fixed-rate evaluation on {args.fxr_fields} fields")
+        fixed = fixed_rate_eval(args.fxr_fields, dev)
+        log(f"fixed rate {fixed}")
+
     alg_c = nbytes + csize
     roof = {"bound": "hbm", "kernel": "k_compress", "achieved": alg_c / (k_ms * 1e-3) / 1e9,
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": None,
@@ -239,6 +333,7 @@ def main():
         "compression_ratio": nbytes / csize,
         "eb": eb,
         "parity": parity,
+        "fixed_rate": fixed,
     }
     print(json.dumps(line), flush=True)
 

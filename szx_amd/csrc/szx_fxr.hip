@@ -25,15 +25,21 @@ namespace szxamd {
 // One wave per block; bs <= 256, so a lane holds at most 4 values.
 constexpr int FX_MAXC = 4;
 
+// Lanes past the block repeat its first value; columns past it are not loaded.
 template <typename T>
 __device__ inline void load_block(const T *b, int bs, int lane, T (&v)[FX_MAXC])
 {
+    v[0] = b[lane < bs ? lane : 0];
 #pragma unroll
-    for (int c = 0; c < FX_MAXC; c++) {
+    for (int c = 1; c < FX_MAXC; c++) {
         const int j = lane + 64 * c;
-        v[c] = j < bs ? b[j] : b[0];
+        v[c] = 64 * c < bs ? b[j < bs ? j : 0] : v[0];
     }
 }
+
+// Blocks a wave loads before working on them (memory-level parallelism of
+// the one-wave-per-block passes).
+constexpr int PF = 4;
 
 // Block min / max with the reference's sequential semantics
 // (szx_float.cc:268-276: `if (min > v) min = v; else if (max < v) max = v;`
@@ -119,19 +125,26 @@ __global__ __launch_bounds__(256) void k_fxr_sample(const T *x, uint64_t ns, uin
     __shared__ T s_lo[4], s_hi[4];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     T glo = __builtin_inf(), ghi = -__builtin_inf();
-    for (uint64_t i = blockIdx.x * 4ull + wv; i < ns; i += gridDim.x * 4ull) {
-        const T *b = x + i * stride;
-        T v[FX_MAXC];
-        load_block(b, bs, lane, v);
-        T lo, hi;
-        block_min_max(b, bs, lane, v, lo, hi);
-        const T r = (hi - lo) / 2;
-        if (lane == 0) {
-            rad[i] = (float)r;
-            med[i] = (float)(lo + r);
+    const uint64_t step = gridDim.x * 4ull;
+    for (uint64_t i0 = blockIdx.x * 4ull + wv; i0 < ns; i0 += PF * step) {
+        T v[PF][FX_MAXC];
+#pragma unroll
+        for (int p = 0; p < PF; p++)
+            if (i0 + p * step < ns) load_block(x + (i0 + p * step) * stride, bs, lane, v[p]);
+#pragma unroll
+        for (int p = 0; p < PF; p++) {
+            const uint64_t i = i0 + p * step;
+            if (i >= ns) break;
+            T lo, hi;
+            block_min_max(x + i * stride, bs, lane, v[p], lo, hi);
+            const T r = (hi - lo) / 2;
+            if (lane == 0) {
+                rad[i] = (float)r;
+                med[i] = (float)(lo + r);
+            }
+            glo = lo < glo ? lo : glo;
+            ghi = hi > ghi ? hi : ghi;
         }
-        glo = lo < glo ? lo : glo;
-        ghi = hi > ghi ? hi : ghi;
     }
     if (lane == 0) {
         s_lo[wv] = glo;
@@ -187,10 +200,17 @@ __global__ __launch_bounds__(256) void k_fxr_estimate(const T *x, uint64_t ns, u
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int k = threadIdx.x; k < 3 * K; k += 256) s_sum[k] = 0;
     __syncthreads();
-    for (uint64_t i = blockIdx.x * 4ull + wv; i < ns; i += gridDim.x * 4ull) {
-        const T *b = x + i * stride;
-        T v[FX_MAXC];
-        load_block(b, bs, lane, v);
+    const uint64_t step = gridDim.x * 4ull;
+    for (uint64_t i0 = blockIdx.x * 4ull + wv; i0 < ns; i0 += PF * step) {
+      T vp[PF][FX_MAXC];
+#pragma unroll
+      for (int p = 0; p < PF; p++)
+          if (i0 + p * step < ns) load_block(x + (i0 + p * step) * stride, bs, lane, vp[p]);
+#pragma unroll
+      for (int p = 0; p < PF; p++) {
+        const uint64_t i = i0 + p * step;
+        if (i >= ns) break;
+        const T (&v)[FX_MAXC] = vp[p];
         const float r = rad[i], m = med[i];
         for (int k = 0; k < K; k++) {
             const float eb = ebs[k];
@@ -208,28 +228,53 @@ __global__ __launch_bounds__(256) void k_fxr_estimate(const T *x, uint64_t ns, u
                 atomicAdd(&s_sum[3 * k + 2], ls);
             }
         }
+      }
     }
     __syncthreads();
     for (int k = threadIdx.x; k < 3 * K; k += 256)
         part[(uint64_t)blockIdx.x * 3 * K + k] = s_sum[k];
 }
 
-// out[c] = sum over g of part[g * width + c]; one thread per column.
-__global__ void k_sum_parts(const unsigned long long *part, int nparts, int width,
-                            unsigned long long *out)
+__global__ void k_fill_u64(unsigned long long *p, int n)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= width) return;
-    unsigned long long s = 0;
-    for (int g = 0; g < nparts; g++) s += part[(uint64_t)g * width + c];
-    out[c] = s;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// out[c] += sum over g of part[g * width + c] (out zeroed first).  A
+// workgroup folds 64 columns over a slice of the rows (4 rows per step, four
+// independent accumulators per thread) and adds its sums with one atomic per
+// column.
+constexpr int FOLD_SPLITS = 16;
+__global__ __launch_bounds__(256) void k_fold_parts(const unsigned long long *part, int nparts, int width,
+                                                    unsigned long long *out)
+{
+    __shared__ unsigned long long s[4][64];
+    const int col = blockIdx.x * 64 + (threadIdx.x & 63), rg = threadIdx.x >> 6;
+    unsigned long long a[4] = {0, 0, 0, 0};
+    if (col < width) {
+        const int rstep = FOLD_SPLITS * 4;
+        int r = blockIdx.y * 4 + rg;
+        for (; r + 3 * rstep < nparts; r += 4 * rstep)
+#pragma unroll
+            for (int u = 0; u < 4; u++) a[u] += part[(uint64_t)(r + u * rstep) * width + col];
+        for (; r < nparts; r += rstep) a[0] += part[(uint64_t)r * width + col];
+    }
+    s[rg][threadIdx.x & 63] = a[0] + a[1] + a[2] + a[3];
+    __syncthreads();
+    if (rg == 0 && col < width) {
+        const unsigned long long t = s[0][threadIdx.x] + s[1][threadIdx.x] + s[2][threadIdx.x] +
+                                     s[3][threadIdx.x];
+        if (t) atomicAdd(&out[col], t);
+    }
 }
 
 static void sum_parts(const unsigned long long *part, int nparts, int width, unsigned long long *out,
                       hipStream_t st)
 {
-    hipLaunchKernelGGL(k_sum_parts, dim3((width + 255) / 256), dim3(256), 0, st, part, nparts, width,
-                       out);
+    hipLaunchKernelGGL(k_fill_u64, dim3((width + 255) / 256), dim3(256), 0, st, out, width);
+    hipLaunchKernelGGL(k_fold_parts, dim3((width + 63) / 64, FOLD_SPLITS), dim3(256), 0, st, part, nparts,
+                       width, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -254,16 +299,16 @@ __device__ inline uint64_t record_bytes(const T (&v)[FX_MAXC], int n, int lane, 
     return 5 + (uint64_t)(n + 3) / 4 + (uint64_t)rb * n - leads;
 }
 
-// One wave per codec block.  acc layout (u64):
+// One wave per codec block.  acc layout (u64, zeroed before the launch):
 //   [0] bytes of constant blocks and records fixed for the whole octave
 //       (radius <= 2^e: always constant, 4 bytes; radius > 2^(e+1): always a
 //       record)
 //   [1 + 2k], [2 + 2k]: blocks with radius in bin k of (2^e, 2^(e+1)]:
 //       count, record bytes.
-// Per-workgroup partials; k_sum_parts folds them.
+// Each workgroup bins in LDS and adds its non-zero bins to acc.
 template <typename T>
 __global__ __launch_bounds__(256) void k_rate_octave(const T *x, uint64_t n, int bs, int reqexp,
-                                                     unsigned long long *part)
+                                                     unsigned long long *acc)
 {
     constexpr int WIDTH = 1 + 2 * OCT_BINS;
     __shared__ unsigned long long s_acc[WIDTH];
@@ -274,38 +319,53 @@ __global__ __launch_bounds__(256) void k_rate_octave(const T *x, uint64_t n, int
     // 2^reqexp and 2^(reqexp+1) as doubles (exact for every float exponent)
     const double lo_edge = ldexp(1.0, reqexp), hi_edge = 2.0 * lo_edge;
     unsigned long long fixed = 0;
-    for (uint64_t i = blockIdx.x * 4ull + wv; i < nb; i += gridDim.x * 4ull) {
-        const T *b = x + i * (uint64_t)bs;
-        const int cnt = (int)(n - i * (uint64_t)bs < (uint64_t)bs ? n - i * (uint64_t)bs : bs);
-        T v[FX_MAXC];
-        load_block(b, cnt, lane, v);
-        T lo, hi;
-        block_min_max(b, cnt, lane, v, lo, hi);
-        const T r = (hi - lo) / 2;
-        const T m = lo + r;
-        // the constant test compares the block radius (double for fp64) with
-        // the float bound widened to T
-        const double rd = (double)r;
-        if (rd <= lo_edge) {
-            fixed += 4;
-            continue;
+    const uint64_t step = gridDim.x * 4ull;
+    for (uint64_t i0 = blockIdx.x * 4ull + wv; i0 < nb; i0 += PF * step) {
+        T vp[PF][FX_MAXC];
+#pragma unroll
+        for (int p = 0; p < PF; p++) {
+            const uint64_t i = i0 + p * step;
+            if (i < nb) {
+                const int cnt = (int)(n - i * (uint64_t)bs < (uint64_t)bs ? n - i * (uint64_t)bs : bs);
+                load_block(x + i * (uint64_t)bs, cnt, lane, vp[p]);
+            }
         }
-        const uint64_t rec = record_bytes<T>(v, cnt, lane, (float)r, (float)m, reqexp);
-        if (!(rd <= hi_edge)) {   // also NaN radii: never constant
-            fixed += rec;
-            continue;
-        }
-        // bin k holds radii in (2^e (1 + k/B), 2^e (1 + (k+1)/B)]
-        const double t = (rd / lo_edge - 1.0) * OCT_BINS;   // in (0, OCT_BINS]
-        const int k = t >= OCT_BINS ? OCT_BINS - 1 : (t <= 1.0 ? 0 : (int)ceil(t) - 1);
-        if (lane == 0) {
-            atomicAdd(&s_acc[1 + 2 * k], 1ull);
-            atomicAdd(&s_acc[2 + 2 * k], (unsigned long long)rec);
+#pragma unroll
+        for (int p = 0; p < PF; p++) {
+            const uint64_t i = i0 + p * step;
+            if (i >= nb) break;
+            const T *b = x + i * (uint64_t)bs;
+            const int cnt = (int)(n - i * (uint64_t)bs < (uint64_t)bs ? n - i * (uint64_t)bs : bs);
+            const T (&v)[FX_MAXC] = vp[p];
+            T lo, hi;
+            block_min_max(b, cnt, lane, v, lo, hi);
+            const T r = (hi - lo) / 2;
+            const T m = lo + r;
+            // the constant test compares the block radius (double for fp64)
+            // with the float bound widened to T
+            const double rd = (double)r;
+            if (rd <= lo_edge) {
+                fixed += 4;
+                continue;
+            }
+            const uint64_t rec = record_bytes<T>(v, cnt, lane, (float)r, (float)m, reqexp);
+            if (!(rd <= hi_edge)) {   // also NaN radii: never constant
+                fixed += rec;
+                continue;
+            }
+            // bin k holds radii in (2^e (1 + k/B), 2^e (1 + (k+1)/B)]
+            const double t = (rd / lo_edge - 1.0) * OCT_BINS;   // in (0, OCT_BINS]
+            const int k = t >= OCT_BINS ? OCT_BINS - 1 : (t <= 1.0 ? 0 : (int)ceil(t) - 1);
+            if (lane == 0) {
+                atomicAdd(&s_acc[1 + 2 * k], 1ull);
+                atomicAdd(&s_acc[2 + 2 * k], (unsigned long long)rec);
+            }
         }
     }
     if (lane == 0 && fixed) atomicAdd(&s_acc[0], fixed);
     __syncthreads();
-    for (int k = threadIdx.x; k < WIDTH; k += 256) part[(uint64_t)blockIdx.x * WIDTH + k] = s_acc[k];
+    for (int k = threadIdx.x; k < WIDTH; k += 256)
+        if (s_acc[k]) atomicAdd(&acc[k], s_acc[k]);
 }
 
 // ---------------------------------------------------------------------------
@@ -331,11 +391,24 @@ __global__ __launch_bounds__(256) void k_rate_stats(const T *x, uint64_t n, int 
     const uint64_t c0 = c * chunk, c1 = (c0 + chunk < n) ? c0 + chunk : n;
     const uint64_t bpc = (c1 - c0 + bs - 1) / bs;   // blocks in the chunk
     double lo = __builtin_inf(), hi = -__builtin_inf(), sx = 0, sxx = 0, sa = 0, sq = 0, s2 = 0;
-    for (uint64_t b = g * 4 + wv; b < bpc; b += (uint64_t)wg_per_chunk * 4) {
+    const uint64_t step = (uint64_t)wg_per_chunk * 4;
+    for (uint64_t b0 = g * 4 + wv; b0 < bpc; b0 += PF * step) {
+      T vp[PF][FX_MAXC];
+#pragma unroll
+      for (int p = 0; p < PF; p++) {
+          const uint64_t b = b0 + p * step;
+          if (b < bpc) {
+              const uint64_t e0 = c0 + b * (uint64_t)bs;
+              load_block(x + e0, (int)(c1 - e0 < (uint64_t)bs ? c1 - e0 : (uint64_t)bs), lane, vp[p]);
+          }
+      }
+#pragma unroll
+      for (int p = 0; p < PF; p++) {
+        const uint64_t b = b0 + p * step;
+        if (b >= bpc) break;
         const uint64_t e0 = c0 + b * (uint64_t)bs;
         const int cnt = (int)(c1 - e0 < (uint64_t)bs ? c1 - e0 : (uint64_t)bs);
-        T v[FX_MAXC];
-        load_block(x + e0, cnt, lane, v);
+        const T (&v)[FX_MAXC] = vp[p];
         T blo, bhi;
         block_min_max(x + e0, cnt, lane, v, blo, bhi);
         const float r = (float)((bhi - blo) / 2);
@@ -369,6 +442,7 @@ __global__ __launch_bounds__(256) void k_rate_stats(const T *x, uint64_t n, int 
                 if (j >= 2) s2 += fabs(d1 - pd1);
             }
         }
+      }
     }
     double acc[7] = {lo, hi, sx, sxx, sa, sq, s2};
 #pragma unroll
@@ -435,17 +509,48 @@ __device__ inline float log2_ratio(double a, double r)
     return (float)(log2(q > lo ? q : lo) / 16.0);
 }
 
-// X[(k * nchunks + c)][24]: the rate model's inputs for chunk c at bound
-// ebs[k] (see rate_features for the meaning of each input).
-__global__ void k_rate_inputs(const RateStats *st, int nchunks, const float *ebs, int K, float *X)
+// The solver's ladder of candidate bounds, range * 2^(-k/4), k < K, from the
+// chunk statistics (one workgroup).
+__global__ void k_rate_ladder(const RateStats *st, int nchunks, int K, float *ebs)
 {
-    const uint64_t id = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    if (id >= (uint64_t)K * nchunks) return;
-    const int k = (int)(id / nchunks), c = (int)(id % nchunks);
+    __shared__ double s_range;
+    if (threadIdx.x == 0) {
+        double lo = st[0].d[0], hi = st[0].d[1];
+        for (int c = 1; c < nchunks; c++) {
+            lo = st[c].d[0] < lo ? st[c].d[0] : lo;
+            hi = st[c].d[1] > hi ? st[c].d[1] : hi;
+        }
+        const double r = hi - lo;
+        s_range = r > 0 ? r : 1.0;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < K; k += blockDim.x) ebs[k] = (float)(s_range * exp2(-0.25 * k));
+}
+
+// X[(k * nchunks + c)][24]: the rate model's inputs for chunk c at bound
+// ebs[k].  One workgroup per chunk: the chunk's radius-exponent histogram and
+// its prefix sums sit in LDS, then each thread fills the inputs of bounds.
+//   0   (exponent of eb - floor(log2 range)) / 16
+//   1   position of eb inside its octave, log2(eb) - exponent, in [0, 1)
+//   2   log2(eb / range) / 16
+//   3-6 log2(std, mean |d1|, rms d1, mean |d2| relative to the range) / 16
+//   7   fraction of blocks with zero radius
+//   8.. fraction of blocks per (radius exponent - eb exponent): <= -1, 0..13, >= 14
+__global__ __launch_bounds__(256) void k_rate_inputs(const RateStats *st, int nchunks, const float *ebs,
+                                                     int K, float *X)
+{
+    __shared__ unsigned s_pre[RATE_HIST + 1];   // s_pre[b] = blocks with exponent field < b
+    const int c = blockIdx.x;
     const RateStats &s = st[c];
-    float *f = X + id * RATE_INPUTS;
-    const float eb = ebs[k];
-    const int e = req_expo(eb);
+    if (threadIdx.x == 0) {
+        unsigned a = 0;
+        for (int b = 0; b < RATE_HIST; b++) {
+            s_pre[b] = a;
+            a += s.hist[b];
+        }
+        s_pre[RATE_HIST] = a;
+    }
+    __syncthreads();
     double range = s.d[1] - s.d[0];
     if (!(range > 0)) range = 1e-300;
     const double nv = (double)s.values;
@@ -454,27 +559,31 @@ __global__ void k_rate_inputs(const RateStats *st, int nchunks, const float *ebs
     if (var < 0) var = 0;
     const double nd1 = nv > 1 ? nv - 1 : 1, nd2 = nv > 2 ? nv - 2 : 1;
     const int er = (int)floor(log2(range));
-    const double le = log2((double)eb);
-    f[0] = (float)(e - er) / 16.0f;
-    f[1] = (float)(le - e);
-    f[2] = log2_ratio((double)eb, range);
-    f[3] = log2_ratio(sqrt(var), range);
-    f[4] = log2_ratio(s.d[4] / nd1, range);
-    f[5] = log2_ratio(sqrt(s.d[5] / nd1), range);
-    f[6] = log2_ratio(s.d[6] / nd2, range);
     const float nb = s.blocks ? (float)s.blocks : 1.0f;
-    f[7] = (float)s.zero / nb;
-    // histogram of (radius exponent - bound exponent): <= -1, 0 .. 13, >= 14
-    float h[16];
-    for (int b = 0; b < 16; b++) h[b] = 0;
-    for (int b = 0; b < RATE_HIST; b++) {
-        const unsigned cnt = s.hist[b];
-        if (!cnt) continue;
-        const int d = (b - 127) - e;
-        const int bin = d <= -1 ? 0 : (d >= 14 ? 15 : d + 1);
-        h[bin] += (float)cnt;
+    const float f3 = log2_ratio(sqrt(var), range), f4 = log2_ratio(s.d[4] / nd1, range);
+    const float f5 = log2_ratio(sqrt(s.d[5] / nd1), range), f6 = log2_ratio(s.d[6] / nd2, range);
+    auto below = [&](int field) {   // blocks whose exponent field is < field
+        return (float)s_pre[field < 0 ? 0 : (field > RATE_HIST ? RATE_HIST : field)];
+    };
+    for (int k = threadIdx.x; k < K; k += blockDim.x) {
+        float *f = X + ((uint64_t)k * nchunks + c) * RATE_INPUTS;
+        const float eb = ebs[k];
+        const int e = req_expo(eb);
+        const double le = log2((double)eb);
+        f[0] = (float)(e - er) / 16.0f;
+        f[1] = (float)(le - e);
+        f[2] = log2_ratio((double)eb, range);
+        f[3] = f3;
+        f[4] = f4;
+        f[5] = f5;
+        f[6] = f6;
+        f[7] = (float)s.zero / nb;
+        // exponent field b has radius exponent b - 127; d = b - 127 - e
+        const int base = e + 127;   // field of d = 0
+        f[8] = below(base) / nb;
+        for (int d = 0; d < 14; d++) f[9 + d] = (below(base + d + 1) - below(base + d)) / nb;
+        f[23] = ((float)s_pre[RATE_HIST] - below(base + 14)) / nb;
     }
-    for (int b = 0; b < 16; b++) f[8 + b] = h[b] / nb;
 }
 
 // ---------------------------------------------------------------------------
@@ -572,31 +681,36 @@ void rate_stats_launch(const void *x, uint64_t n, int w, int bs, uint8_t *ws, Ra
                        RATE_WG_PER_CHUNK, d_stats);
 }
 
+void rate_ladder_launch(const RateStats *d_stats, int nchunks, int K, float *d_ebs, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_rate_ladder, dim3(1), dim3(256), 0, st, d_stats, nchunks, K, d_ebs);
+}
+
 void rate_inputs_launch(const RateStats *d_stats, int nchunks, const float *d_ebs, int K, float *d_X,
                         hipStream_t st)
 {
-    const uint64_t total = (uint64_t)K * nchunks;
-    hipLaunchKernelGGL(k_rate_inputs, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, d_stats,
-                       nchunks, d_ebs, K, d_X);
+    hipLaunchKernelGGL(k_rate_inputs, dim3((unsigned)nchunks), dim3(256), 0, st, d_stats, nchunks,
+                       d_ebs, K, d_X);
 }
 
-uint64_t octave_workspace_bytes() { return (uint64_t)OCT_WG * (1 + 2 * OCT_BINS) * 8; }
+uint64_t octave_workspace_bytes() { return 256; }
 
 void rate_octave_launch(const void *x, uint64_t n, int w, int bs, int reqexp, uint8_t *ws,
                         unsigned long long *d_acc, hipStream_t st)
 {
-    auto *part = reinterpret_cast<unsigned long long *>(ws);
+    (void)ws;
+    constexpr int WIDTH = 1 + 2 * OCT_BINS;
+    hipLaunchKernelGGL(k_fill_u64, dim3((WIDTH + 255) / 256), dim3(256), 0, st, d_acc, WIDTH);
     const uint64_t nb = (n + bs - 1) / bs;
-    uint64_t g = (nb + 3) / 4;
+    uint64_t g = (nb + 4 * PF - 1) / (4 * PF);
     if (g < 1) g = 1;
     if (g > OCT_WG) g = OCT_WG;
     if (w == 4)
         hipLaunchKernelGGL(k_rate_octave<float>, dim3((unsigned)g), dim3(256), 0, st,
-                           static_cast<const float *>(x), n, bs, reqexp, part);
+                           static_cast<const float *>(x), n, bs, reqexp, d_acc);
     else
         hipLaunchKernelGGL(k_rate_octave<double>, dim3((unsigned)g), dim3(256), 0, st,
-                           static_cast<const double *>(x), n, bs, reqexp, part);
-    sum_parts(part, (int)g, 1 + 2 * OCT_BINS, d_acc, st);
+                           static_cast<const double *>(x), n, bs, reqexp, d_acc);
 }
 
 }  // namespace szxamd

@@ -9,7 +9,7 @@ namespace szxamd {
 constexpr int FXR_MAX_CAND = 32;    // candidate bounds per model pass
 constexpr int FXR_MAX_WG = 1024;    // workgroups of a model pass
 constexpr int OCT_BINS = 512;       // radius bins per octave of bounds
-constexpr int OCT_WG = 1024;        // workgroups of an octave pass
+constexpr int OCT_WG = 2048;        // workgroups of an octave pass
 
 // Workspace of the sampled model: per sampled block median and radius,
 // per-workgroup partial sums, the candidate bounds, the folded sums and the
@@ -29,7 +29,8 @@ void fxr_estimate_launch(const void *x, uint64_t n, int w, int bs, int rate, uin
 
 // Exact rate over the octave of bounds with exponent reqexp: d_acc receives
 // 1 + 2 * OCT_BINS u64 (fixed bytes, then per radius bin count and record
-// bytes); ws holds octave_workspace_bytes().
+// bytes); ws holds octave_workspace_bytes() (unused scratch, kept for the
+// interface).
 uint64_t octave_workspace_bytes();
 void rate_octave_launch(const void *x, uint64_t n, int w, int bs, int reqexp, uint8_t *ws,
                         unsigned long long *d_acc, hipStream_t st);
@@ -39,7 +40,7 @@ void rate_octave_launch(const void *x, uint64_t n, int w, int bs, int reqexp, ui
 // sum d1^2, sum |d2| (differences inside codec blocks); hist: codec blocks
 // per float radius exponent field (0..255).
 constexpr uint64_t RATE_CHUNK = 1ull << 19;
-constexpr int RATE_WG_PER_CHUNK = 16;
+constexpr int RATE_WG_PER_CHUNK = 32;
 constexpr int RATE_HIST = 256;
 constexpr int RATE_INPUTS = 24;
 struct RateStats {
@@ -52,6 +53,8 @@ struct RateStats {
 uint64_t rate_stats_workspace_bytes(uint64_t n);
 void rate_stats_launch(const void *x, uint64_t n, int w, int bs, uint8_t *ws, RateStats *d_stats,
                        hipStream_t st);
+// d_ebs[k] = field range * 2^(-k/4), k < K (range from the statistics).
+void rate_ladder_launch(const RateStats *d_stats, int nchunks, int K, float *d_ebs, hipStream_t st);
 // X[(k * nchunks + c)][RATE_INPUTS] for K bounds at d_ebs.
 void rate_inputs_launch(const RateStats *d_stats, int nchunks, const float *d_ebs, int K, float *d_X,
                         hipStream_t st);

@@ -262,49 +262,37 @@ int fixed_rate_solve(const void *d_in, uint64_t n, int w, int bs, float target, 
     bool done = false;
 
     if (method == SZX_AMD_RATE_LEARNED) {
-        // features and the model's ladder of predictions
+        // features and the model's ladder of predictions, one synchronisation
         const int nch = (int)S.L.nchunks;
-        rate_stats_launch(d_in, n, w, bs, d_ws + S.L.stats_ws, (RateStats *)(d_ws + S.L.stats), st);
-        std::vector<RateStats> stats(nch);
-        if (!S.ok(hipMemcpyAsync(stats.data(), d_ws + S.L.stats, nch * sizeof(RateStats),
-                                 hipMemcpyDeviceToHost, st), "D2H stats") ||
-            !S.ok(hipStreamSynchronize(st), "statistics pass")) {
-            *err = S.err;
-            return -1;
-        }
-        feature_passes = 1;
-        double lo = stats[0].d[0], hi = stats[0].d[1];
-        for (int c = 1; c < nch; c++) {
-            lo = stats[c].d[0] < lo ? stats[c].d[0] : lo;
-            hi = stats[c].d[1] > hi ? stats[c].d[1] : hi;
-        }
-        double range = hi - lo;
-        if (!(range > 0)) range = 1.0;
         const int K = (int)(4 * depth) < K_MAX ? (int)(4 * depth) : K_MAX;
-        std::vector<float> ladder(K);
-        for (int k = 0; k < K; k++) ladder[k] = (float)(range * exp2(-0.25 * k));
-        std::vector<float> y((size_t)K * nch);
-        if (!S.ok(hipMemcpyAsync(d_ws + S.L.ebs, ladder.data(), 4 * (size_t)K, hipMemcpyHostToDevice, st),
-                  "H2D ladder")) {
-            *err = S.err;
-            return -1;
-        }
-        rate_inputs_launch((const RateStats *)(d_ws + S.L.stats), nch, (const float *)(d_ws + S.L.ebs), K,
-                           (float *)(d_ws + S.L.X), st);
+        const RateStats *d_stats = (const RateStats *)(d_ws + S.L.stats);
+        float *d_ebs = (float *)(d_ws + S.L.ebs);
+        rate_stats_launch(d_in, n, w, bs, d_ws + S.L.stats_ws, (RateStats *)(d_ws + S.L.stats), st);
+        rate_ladder_launch(d_stats, nch, K, d_ebs, st);
+        rate_inputs_launch(d_stats, nch, d_ebs, K, (float *)(d_ws + S.L.X), st);
         mlp_forward_launch(d_params, (const float *)(d_ws + S.L.X), (uint64_t)K * nch,
                            (float *)(d_ws + S.L.y), st);
-        if (!S.ok(hipMemcpyAsync(y.data(), d_ws + S.L.y, y.size() * 4, hipMemcpyDeviceToHost, st),
+        std::vector<float> ladder(K), y((size_t)K * nch);
+        if (!S.ok(hipMemcpyAsync(ladder.data(), d_ebs, 4 * (size_t)K, hipMemcpyDeviceToHost, st),
+                  "D2H ladder") ||
+            !S.ok(hipMemcpyAsync(y.data(), d_ws + S.L.y, y.size() * 4, hipMemcpyDeviceToHost, st),
                   "D2H predictions") ||
             !S.ok(hipStreamSynchronize(st), "rate model")) {
             *err = S.err;
             return -1;
         }
+        feature_passes = 1;
+        const double range = (double)ladder[0];
         // predicted field rate along the ladder (bits of the records,
         // medians and bitmap per chunk, plus the 13-byte header)
         std::vector<double> pred(K);
         for (int k = 0; k < K; k++) {
             double bits = 8.0 * STREAM_HEADER;
-            for (int c = 0; c < nch; c++) bits += exp2((double)y[(size_t)k * nch + c]) * (double)stats[c].values;
+            for (int c = 0; c < nch; c++) {
+                const uint64_t vals = n - (uint64_t)c * RATE_CHUNK < RATE_CHUNK ? n - (uint64_t)c * RATE_CHUNK
+                                                                                : RATE_CHUNK;
+                bits += exp2((double)y[(size_t)k * nch + c]) * (double)vals;
+            }
             pred[k] = bits / (double)n;
         }
         // first rung whose predicted rate reaches the target; interpolate in
