@@ -133,10 +133,13 @@ size_t szx_amd_compress_workspace(size_t n, int dataType, int blockSize);
 
 /* uint64_t words of a record index for n values: per compressor tile (4096
  * fp32 / 2048 fp64 values) the payload offset of the tile's first record and
- * the number of constant blocks before the tile, plus a closing pair.  The
- * index lives outside the stream (the stream stays byte-identical); handing
- * it to szx_amd_decompress_dev spares the decoder the serial record walk. */
-size_t szx_amd_index_words(size_t n, int dataType);
+ * the number of constant blocks before the tile, plus a closing pair; then
+ * per block a uint16_t -- the record's start inside its tile's payload bytes,
+ * or a constant block's rank among its tile's constant blocks.  The index
+ * lives outside the stream (the stream stays byte-identical); handing it to
+ * szx_amd_decompress_dev lets every tile decode at once without walking the
+ * records. */
+size_t szx_amd_index_words(size_t n, int dataType, int blockSize);
 
 /* Compress n device values.  errBoundMode ABS or REL (REL's bound is computed
  * on the device with the reference's float rounding).  On completion

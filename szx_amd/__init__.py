@@ -106,7 +106,7 @@ def lib() -> C.CDLL:
     L.szx_amd_compress_dev.argtypes = [vp, sz, i32, i32, f32, f32, i32, vp, sz, vp, sz, vp, vp,
                                        vp]
     L.szx_amd_index_words.restype = sz
-    L.szx_amd_index_words.argtypes = [sz, i32]
+    L.szx_amd_index_words.argtypes = [sz, i32, i32]
     L.szx_amd_decompress_workspace.restype = sz
     L.szx_amd_decompress_workspace.argtypes = [sz, i32, i32, sz]
     L.szx_amd_decompress_dev.restype = i32
@@ -298,7 +298,7 @@ class DeviceCompressor:
         self.out = torch.empty(self.capacity, dtype=torch.uint8, device=device)
         self.ws = torch.empty(self.wsbytes, dtype=torch.uint8, device=device)
         self.result = torch.zeros(4, dtype=torch.int64, device=device)
-        self.index = torch.zeros(L.szx_amd_index_words(n, code), dtype=torch.int64,
+        self.index = torch.zeros(L.szx_amd_index_words(n, code, blockSize), dtype=torch.int64,
                                  device=device)
 
     def __call__(self, x, errBoundMode: int = ABS, absErrBound: float = 0.0,

@@ -567,10 +567,10 @@ size_t szx_amd_compress_workspace(size_t n, int dataType, int blockSize)
     return compress_layout(n, w, blockSize).ws_bytes;
 }
 
-size_t szx_amd_index_words(size_t n, int dataType)
+size_t szx_amd_index_words(size_t n, int dataType, int blockSize)
 {
     const int w = width_of(dataType);
-    return w ? index_words(n, w) : 0;
+    return (w && bs_supported(blockSize)) ? index_words(n, w, blockSize) : 0;
 }
 
 int szx_amd_compress_dev(const void *d_in, size_t n, int dataType, int errBoundMode,

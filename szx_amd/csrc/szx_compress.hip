@@ -40,9 +40,11 @@ struct CompressParams {
     ChunkState *chunks;          // per-64-tile look-back words
     uint64_t *totals;     // [0] payload bytes, [1] constant blocks
     uint64_t *index;      // optional record index: per tile {payload offset, constants before}
+    uint16_t *bidx;       // its per-block part (start in tile, or constant rank), or nullptr
     int *err;             // set when a look-back wait overran its bound
     uint64_t ntiles;
     uint64_t nbit;
+    uint64_t nb;
 };
 
 template <typename T> struct SlabLoad;
@@ -297,7 +299,7 @@ __global__ __launch_bounds__(256) void k_compress(CompressParams p)
     constexpr int STAGE_DW = (STAGE + 32 + 15) / 16 * 4;
     static_assert(BS >= LV && BS <= 256 && (BS & (BS - 1)) == 0, "power-of-two block size");
     static_assert(NBLK % 8 == 0, "tiles own whole bitmap bytes");
-    static_assert(STAGE < (1 << 20), "record bytes fit the packed scan word");
+    static_assert(STAGE < (1 << 16), "record starts fit the 16-bit block index");
 
     __shared__ uint32_t s_pack[NBLK];     // record bytes | constant<<20 -> exclusive prefix
     __shared__ uint8_t s_state[NBLK];     // 1 = non-constant (bitmap bit)
@@ -464,6 +466,16 @@ __global__ __launch_bounds__(256) void k_compress(CompressParams p)
 #pragma unroll
             for (int k = 0; k < 8; k++) b |= (uint32_t)s_state[8 * i + k] << (7 - k);
             p.bitmap[gbyte] = (uint8_t)b;
+        }
+    }
+    // per-block entries of the record index: the record's start inside the
+    // tile's payload bytes, or the constant block's rank among the tile's
+    // constant blocks (both fit 16 bits: a tile stages under 64 KiB)
+    if (p.bidx) {
+        for (int lb = tid; lb < NBLK; lb += 256) {
+            const uint64_t b = tile * NBLK + lb;
+            if (b < p.nb)
+                p.bidx[b] = (uint16_t)(s_state[lb] ? (s_pack[lb] & 0xFFFFFu) : (s_pack[lb] >> 20));
         }
     }
 }
@@ -713,6 +725,8 @@ int compress_launch(const void *d_in, uint64_t n, int w, int rel_mode, float abs
     p.totals = reinterpret_cast<uint64_t *>(misc + 16);
     p.err = reinterpret_cast<int *>(misc + 48);
     p.index = d_index;
+    p.bidx = d_index ? reinterpret_cast<uint16_t *>(d_index + 2 * (L.ntiles + 1)) : nullptr;
+    p.nb = L.nb;
     p.ntiles = L.ntiles;
     p.nbit = L.nbit;
     timing_mark_begin(KK_COMPRESS, st);

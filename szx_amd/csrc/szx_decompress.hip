@@ -3,16 +3,22 @@
 // A fastMode-2 stream has no block index: the start of non-constant record k
 // is the sum of the sizes of records 0..k-1, and a size is only known by
 // parsing that record's reqLength byte and lead codes (szxd_float.cc:276-283).
-// Decoding therefore works from a record index -- per compressor tile, the
-// payload offset of the tile's first record and the number of constant
-// blocks before it -- which this library's compressor emits as a by-product
-// of its look-back scan, and which k_index_* rebuilds for any other stream.
-// With the index every tile decodes independently:
+// Decoding therefore works from a record index kept outside the stream:
+// per compressor tile, the payload offset of the tile's first record and the
+// constant blocks before it, and per block a 16-bit entry (the record's start
+// inside the tile, or the constant block's rank inside the tile).  This
+// library's compressor writes it as a by-product of its scan; k_index_serial
+// rebuilds it for any other stream.  With the index every tile decodes
+// independently:
 //
-//   1. stage the tile's records in LDS and walk their sizes (a few dozen
-//      records, one wave);
+//   1. stage the tile's record bytes (16-byte loads) and its constant medians
+//      in LDS;
 //   2. decode every block in parallel, 4 values per lane, a block being an
-//      aligned group of bs/4 lanes (the mirror of the compressor's layout).
+//      aligned group of bs/4 lanes (the mirror of the compressor's layout);
+//   3. check the index against the records: a scan of the decoded record
+//      sizes must reproduce every start, and the tile's records must end
+//      exactly at the next tile's first record; a stream or index that fails
+//      is rejected (status < 0), and no access leaves the staged bytes.
 //
 // The decode of one record mirrors szxd_float.cc:34-244 / szxd_double.cc:25-449:
 // value i takes its top min(lead_i, rb) bytes from value i-1 and the rest of
@@ -21,14 +27,12 @@
 // (at most 3) copyable byte positions, and an exclusive segmented scan with
 // the last-writer-wins operator gives every lane the bytes it inherits.
 #include "szx_common.hpp"
+#include "szx_index.hpp"
 #include "szx_internal.hpp"
 
 namespace szxamd {
 
 // Header check: block size, counts and the bitmap/median region must fit.
-struct DecodeHeader {
-    uint64_t bs, nb, nbit, nconst, q0;
-};
 
 __global__ void k_check_header(const uint8_t *s, uint64_t len, uint64_t n, int bs_expect,
                                DecodeHeader *h, int32_t *status)
@@ -49,47 +53,6 @@ __global__ void k_check_header(const uint8_t *s, uint64_t len, uint64_t n, int b
     }
     h->bs = bs; h->nb = nb; h->nbit = nbit; h->nconst = nconst;
     h->q0 = HEADER_BYTES + nbit + 4 * nconst;
-    *status = rc;
-}
-
-// Record index of a foreign stream by the reference's serial walk
-// (szxd_float.cc:276-283), one lane: per tile of `tile_blocks` blocks, the
-// payload offset of its first record and the constant blocks before it.
-__global__ void k_index_serial(const uint8_t *s, uint64_t len, uint64_t n, int w,
-                               uint64_t tile_blocks, const DecodeHeader *h, uint64_t *index,
-                               int32_t *status)
-{
-    if (threadIdx.x != 0 || blockIdx.x != 0 || *status != 0) return;
-    const uint64_t bs = h->bs, nb = h->nb, q0 = h->q0;
-    uint64_t q = q0, k0 = 0;
-    int32_t rc = 0;
-    for (uint64_t b = 0; b < nb; b++) {
-        if (b % tile_blocks == 0) {
-            index[2 * (b / tile_blocks)] = q - q0;
-            index[2 * (b / tile_blocks) + 1] = k0;
-        }
-        if (!((s[HEADER_BYTES + (b >> 3)] >> (7 - (b & 7))) & 1)) {
-            k0++;
-            continue;
-        }
-        const uint64_t nel = (b + 1) * bs <= n ? bs : n - b * bs;
-        const uint64_t nl = (nel + 3) >> 2;
-        if (q + 5 + nl > len) { rc = -15; break; }
-        const int rl = s[q];
-        if (rl == 0 || rl > 8 * w) { rc = -16; break; }
-        const int rb = (rl + 7) >> 3;
-        uint64_t sz = 5 + nl;
-        for (uint64_t i = 0; i < nel; i++) {
-            const int ld = (s[q + 5 + (i >> 2)] >> (6 - 2 * (int)(i & 3))) & 3;
-            if (ld < rb) sz += (uint64_t)(rb - ld);
-        }
-        if (q + sz > len) { rc = -17; break; }
-        q += sz;
-    }
-    if (!rc && k0 != h->nconst) rc = -18;
-    const uint64_t nt = (nb + tile_blocks - 1) / tile_blocks;
-    index[2 * (nt ? nt : 1)] = q - q0;
-    index[2 * (nt ? nt : 1) + 1] = k0;
     *status = rc;
 }
 
@@ -133,18 +96,47 @@ __device__ inline float be_f32(const uint8_t *p)
                     (uint32_t)p[3]);
 }
 
+// Exclusive prefix sum over the 64 lanes of a wave.
+__device__ inline uint32_t wave_excl_scan(uint32_t v, int lane)
+{
+    uint32_t inc = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t o = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += o;
+    }
+    return inc - v;
+}
+
+// Copy bytes [src, src + nbytes) of global memory into LDS at dst + (src & 15)
+// (dst 16-byte aligned): 16-byte loads for the aligned chunks that lie wholly
+// inside [lo, hi) -- the caller's buffer -- and single bytes for the rest.
+__device__ inline void stage_bytes(uint8_t *dst, const uint8_t *src, uint32_t nbytes,
+                                   const uint8_t *lo, const uint8_t *hi, int tid)
+{
+    const uintptr_t a = (uintptr_t)src & ~(uintptr_t)15;
+    const uint32_t head = (uint32_t)((uintptr_t)src - a);
+    const uint32_t nchunk = (head + nbytes + 15) / 16;
+    for (uint32_t c = tid; c < nchunk; c += 256) {
+        const uint8_t *g = reinterpret_cast<const uint8_t *>(a + 16 * (uintptr_t)c);
+        if (g >= lo && g + 16 <= hi)
+            *reinterpret_cast<uint4 *>(dst + 16 * c) = *reinterpret_cast<const uint4 *>(g);
+    }
+    // the first and last chunks may be partial: one byte per thread
+    if (tid < 32 && nchunk) {
+        const uint32_t c = tid < 16 ? 0 : nchunk - 1;
+        const uint32_t k = tid & 15;
+        const uint8_t *g = reinterpret_cast<const uint8_t *>(a + 16 * (uintptr_t)c) + k;
+        if (g >= lo && g < hi) dst[16 * c + k] = *g;
+    }
+}
+
 // Decode one tile (the compressor's tile: TILE values, NBLK blocks).
-//   1. stage the tile's record bytes [index[t], index[t+1]) in LDS;
-//   2. wave 0 walks the tile's records (sizes from the lead codes, 64 lanes
-//      summing one record's codes per step) -> record start per block;
-//   3. every lane decodes 4 values per slab, a block being BS/4 lanes.
-// Every record must lie inside the staged span and the walk must end exactly
-// at the next tile's first record; otherwise the stream (or index) is
-// rejected and nothing reads out of bounds.
 template <typename T, int BS, int SLABS>
 __global__ __launch_bounds__(256) void k_decode_tiles(const uint8_t *s, uint64_t len, uint64_t n,
                                                      const DecodeHeader *h, const uint64_t *index,
-                                                     int32_t *status, T *out, int exact)
+                                                     const uint16_t *bidx, int32_t *status, T *out,
+                                                     int exact)
 {
     using U = typename Word<T>::U;
     constexpr int W = Word<T>::W;
@@ -154,9 +146,12 @@ __global__ __launch_bounds__(256) void k_decode_tiles(const uint8_t *s, uint64_t
     constexpr int BPS = SLAB / BS;
     constexpr int NBLK = TILE / BS;
     constexpr int STAGE = TILE * W + NBLK * (5 + BS / 4);
+    constexpr int PER_LANE = (NBLK + 63) / 64;   // blocks per lane in the check scan
 
-    __shared__ uint8_t st[STAGE + 16];
-    __shared__ uint32_t s_start[NBLK];   // record start, or constant rank within the tile
+    __shared__ __attribute__((aligned(16))) uint8_t st[STAGE + 48];
+    __shared__ __attribute__((aligned(16))) uint8_t s_med[4 * NBLK + 48];
+    __shared__ uint16_t s_start[NBLK];   // record start, or constant rank within the tile
+    __shared__ uint16_t s_size[NBLK];    // decoded record size (0: constant block)
     __shared__ uint8_t s_state[NBLK];
     __shared__ int s_err;
 
@@ -170,63 +165,30 @@ __global__ __launch_bounds__(256) void k_decode_tiles(const uint8_t *s, uint64_t
     const uint64_t off = q0 + index[2 * tile];
     const uint64_t end = q0 + index[2 * tile + 2];
     const uint64_t cbase = index[2 * tile + 1];
-    if (end < off || end > len || end - off > (uint64_t)STAGE || cbase > nconst) {
+    const uint64_t cend = index[2 * tile + 3];
+    if (end < off || end > len || end - off > (uint64_t)STAGE || cend < cbase ||
+        cend - cbase > (uint64_t)NBLK || cend > nconst) {
         if (tid == 0) atomicMin(status, -20);
         return;
     }
     const uint32_t tb = (uint32_t)(end - off);
+    const uint32_t nct = (uint32_t)(cend - cbase);
+    const uint8_t *rsrc = s + off;
+    const uint8_t *msrc = s + HEADER_BYTES + nbit + 4 * cbase;   // inside [0, q0) <= len
+    const uint32_t rdel = (uint32_t)((uintptr_t)rsrc & 15), mdel = (uint32_t)((uintptr_t)msrc & 15);
     if (tid == 0) s_err = 0;
-    for (uint32_t i = tid; i < tb; i += 256) st[i] = s[off + i];
+    stage_bytes(st, rsrc, tb, s, s + len, tid);
+    stage_bytes(s_med, msrc, 4 * nct, s, s + len, tid);
     for (int lb = tid; lb < NBLK; lb += 256) {
         const uint64_t b = gb0 + lb;
         s_state[lb] = b < nb ? (s[HEADER_BYTES + (b >> 3)] >> (7 - (b & 7))) & 1 : 0;
+        s_start[lb] = b < nb ? bidx[b] : 0;
+        s_size[lb] = 0;
     }
     __syncthreads();
 
-    if (tid < 64) {
-        uint32_t pos = 0, crank = 0;
-        int err = 0;
-        for (int lb = 0; lb < NBLK; lb++) {
-            const uint64_t b = gb0 + lb;
-            if (b >= nb) break;
-            if (!s_state[lb]) {
-                if (lane == 0) s_start[lb] = crank;
-                crank++;
-                continue;
-            }
-            const uint64_t rem = n - b * BS;
-            const int nel = rem >= (uint64_t)BS ? BS : (int)rem;
-            const int nlead = (nel + 3) >> 2;
-            if (pos + 5 + (uint32_t)nlead > tb) { err = 1; break; }
-            const int rl = st[pos];
-            if (rl == 0 || rl > 8 * W) { err = 1; break; }
-            const int rb = (rl + 7) >> 3;
-            uint32_t part = 0;
-            for (int j = lane; j < nlead; j += 64) {
-                const uint32_t c = st[pos + 5 + j];
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    const int ld = (c >> (6 - 2 * k)) & 3;
-                    if (4 * j + k < nel && ld < rb) part += (uint32_t)(rb - ld);
-                }
-            }
-#pragma unroll
-            for (int m = 1; m < 64; m <<= 1) part += __shfl_xor(part, m, 64);
-            if (lane == 0) s_start[lb] = pos;
-            pos += 5 + (uint32_t)nlead + part;
-            if (pos > tb) { err = 1; break; }
-        }
-        if (!err && pos != tb) err = 1;
-        if (!err && cbase + crank > nconst) err = 1;
-        if (lane == 0 && err) s_err = 1;
-    }
-    __syncthreads();
-    if (s_err) {
-        if (tid == 0) atomicMin(status, -21);
-        return;
-    }
-
-    const uint8_t *cmed = s + HEADER_BYTES + nbit;
+    const uint8_t *recs = st + rdel;
+    const uint8_t *meds = s_med + mdel;
 #pragma unroll
     for (int u = 0; u < SLABS; u++) {
         const uint64_t e0 = tile * TILE + (uint64_t)u * SLAB + 4u * (uint64_t)tid;
@@ -235,22 +197,34 @@ __global__ __launch_bounds__(256) void k_decode_tiles(const uint8_t *s, uint64_t
         if (bstart >= n) continue;   // whole segment
         T v[4];
         if (!s_state[lb]) {
-            const float med = be_f32(cmed + 4 * (cbase + s_start[lb]));
+            const uint32_t rank = s_start[lb];
+            float med = 0.0f;
+            if (rank < nct) {
+                const uint8_t *p = meds + 4 * rank;
+                med = bits_f32(((uint32_t)p[0] << 24) | ((uint32_t)p[1] << 16) |
+                               ((uint32_t)p[2] << 8) | (uint32_t)p[3]);
+            } else if (lseg == 0) {
+                s_err = 1;
+            }
 #pragma unroll
             for (int k = 0; k < 4; k++) v[k] = (T)med;
             Store4<T>::store(out, e0, n, v);
             continue;
         }
-        const uint8_t *rec = st + s_start[lb];
         const uint64_t rem = n - bstart;
         const int nel = rem >= (uint64_t)BS ? BS : (int)rem;
         const int nlead = (nel + 3) >> 2;
-        const int rl = rec[0];
-        const int rb = (rl + 7) >> 3;
+        const uint32_t start = s_start[lb];
+        // the record's fixed part must lie in the staged bytes
+        const bool head_ok = start + 5u + (uint32_t)nlead <= tb;
+        const uint8_t *rec = recs + (head_ok ? start : 0u);
+        const int rl = head_ok ? rec[0] : 1;
+        const bool rl_ok = head_ok && rl >= 1 && rl <= 8 * W;
+        const int rb = rl_ok ? (rl + 7) >> 3 : 1;
         const int shift = (8 - (rl & 7)) & 7;
         const int lo = W - rb;
         const float med = be_f32(rec + 1);
-        const uint32_t code = lseg < nlead ? rec[5 + lseg] : 0u;
+        const uint32_t code = (rl_ok && lseg < nlead) ? rec[5 + lseg] : 0u;
 
         int lk[4];
         uint32_t tot = 0;
@@ -266,7 +240,13 @@ __global__ __launch_bounds__(256) void k_decode_tiles(const uint8_t *s, uint64_t
             const uint32_t o = __shfl_up(inc, d, SEG);
             if (lseg >= d) inc += o;
         }
-        const uint8_t *src = rec + 5 + nlead + (inc - tot);
+        const uint32_t rsize = 5u + (uint32_t)nlead + __shfl(inc, SEG - 1, SEG);
+        const bool ok = rl_ok && start + rsize <= tb;
+        if (lseg == 0) {
+            if (ok) s_size[lb] = (uint16_t)rsize;
+            else s_err = 1;
+        }
+        const uint8_t *src = rec + 5 + nlead + (ok ? inc - tot : 0u);
 
         const U wmask = ~(U)0 >> shift;
         U e[4];
@@ -274,7 +254,7 @@ __global__ __launch_bounds__(256) void k_decode_tiles(const uint8_t *s, uint64_t
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             U ek = 0;
-            if (e0 + k < n) {
+            if (ok && e0 + k < n) {
                 const int m = rb - lk[k];
                 for (int j = 0; j < m; j++) ek |= (U)src[j] << (8 * (lo + j));
                 if constexpr (W == 8) {
@@ -315,6 +295,41 @@ __global__ __launch_bounds__(256) void k_decode_tiles(const uint8_t *s, uint64_t
         }
         Store4<T>::store(out, e0, n, v);
     }
+    __syncthreads();
+
+    // the index must be what the records say: starts are the exclusive scan
+    // of the record sizes, constant ranks count the constant blocks, and the
+    // tile's records and medians end where the next tile's begin
+    if (tid < 64) {
+        uint32_t bytes = 0, consts = 0;
+#pragma unroll
+        for (int k = 0; k < PER_LANE; k++) {
+            const int lb = lane * PER_LANE + k;
+            if (lb < NBLK && gb0 + lb < nb) {
+                if (s_state[lb]) bytes += s_size[lb];
+                else consts++;
+            }
+        }
+        uint32_t eb = wave_excl_scan(bytes, lane), ec = wave_excl_scan(consts, lane);
+        int bad = 0;
+#pragma unroll
+        for (int k = 0; k < PER_LANE; k++) {
+            const int lb = lane * PER_LANE + k;
+            if (lb < NBLK && gb0 + lb < nb) {
+                if (s_state[lb]) {
+                    bad |= s_start[lb] != eb;
+                    eb += s_size[lb];
+                } else {
+                    bad |= s_start[lb] != ec;
+                    ec++;
+                }
+            }
+        }
+        if (lane == 63) bad |= eb != tb || ec != nct;
+        if (__ballot(bad) && lane == 0) s_err = 1;
+    }
+    __syncthreads();
+    if (s_err && tid == 0) atomicMin(status, -21);
 }
 
 // ---------------------------------------------------------------------------
@@ -323,12 +338,11 @@ __global__ __launch_bounds__(256) void k_decode_tiles(const uint8_t *s, uint64_t
 
 DecompressLayout decompress_layout(uint64_t n, int w, int bs, uint64_t stream_bytes)
 {
-    (void)bs;
-    (void)stream_bytes;
     DecompressLayout L{};
     L.ws_hdr = 0;
     L.ws_index = 256;
-    L.ws_bytes = L.ws_index + 8 * index_words(n, w);
+    L.ws_build = (L.ws_index + 8 * index_words(n, w, bs) + 255) & ~255ull;
+    L.ws_bytes = L.ws_build + ix_layout(n, w, bs, stream_bytes).bytes;
     return L;
 }
 
@@ -338,8 +352,9 @@ static void launch_decode(const uint8_t *s, uint64_t len, uint64_t n, const Deco
                           uint64_t ntiles, hipStream_t st)
 {
     constexpr int SLABS = (sizeof(T) == 4 ? TILE_VALUES_F32 : TILE_VALUES_F64) / 1024;
+    const uint16_t *bidx = reinterpret_cast<const uint16_t *>(index + 2 * (ntiles + 1));
     hipLaunchKernelGGL((k_decode_tiles<T, BS, SLABS>), dim3((unsigned)ntiles), dim3(256), 0, st, s,
-                       len, n, h, index, status, static_cast<T *>(out), exact);
+                       len, n, h, index, bidx, status, static_cast<T *>(out), exact);
 }
 
 template <typename T>
@@ -374,9 +389,10 @@ int decompress_launch(const uint8_t *d_stream, uint64_t stream_bytes, uint64_t n
     const uint64_t *index = d_index;
     if (!index) {
         uint64_t *built = reinterpret_cast<uint64_t *>(d_ws + L.ws_index);
+        uint16_t *bbuilt = reinterpret_cast<uint16_t *>(built + 2 * (index_tiles(n, w) + 1));
         timing_mark_begin(KK_INDEX, st);
-        hipLaunchKernelGGL(k_index_serial, dim3(1), dim3(64), 0, st, d_stream, stream_bytes, n, w,
-                           tv / (uint64_t)bs, h, built, d_status);
+        index_build_launch(d_stream, stream_bytes, n, w, bs, h, built, bbuilt, d_ws + L.ws_build,
+                           d_status, st);
         timing_mark_end(KK_INDEX, st);
         index = built;
     }

@@ -1,0 +1,586 @@
+// szx_index.hip -- the record index of a fastMode-2 stream, built in
+// parallel from the stream alone (for streams that arrive without the index
+// this library's compressor writes).
+//
+// The payload is a chain: a record starting at byte p has size
+//   size(p) = 5 + NL + sum over its NL lead-code bytes of (rb - min(code, rb))
+// with rb = ceil(reqLength / 8), reqLength = byte p, NL = bs / 4 (every record
+// but the stream's last belongs to a full block).  Following the chain from
+// byte 0 is inherently serial, so it is cut into chunks of IX_C payload bytes:
+//
+//   A  k_ix_map     per chunk: size(p) for every byte p, then pointer doubling
+//                   in LDS gives, for every possible entry offset e (the first
+//                   record start at or after the chunk start lies less than one
+//                   record size R into the chunk), the exit offset into the
+//                   next chunk.  Chains from different entries merge as soon
+//                   as they meet a common record boundary, so almost every
+//                   chunk maps all valid entries to one exit ("converged") and
+//                   the next chunk's entry is known without this one's.
+//   B  k_ix_entry   entries: a chunk after a converged chunk starts at its
+//                   exit; a run of unconverged chunks is followed through their
+//                   stored exit tables by one thread.
+//   C  k_ix_walk    per chunk: walk the chain from the entry, keep the record
+//                   starts, count them.
+//   D  scans        records before each chunk; non-constant blocks before
+//                   each compressor tile.
+//   E  k_ix_scatter record starts in record order.
+//   F  k_ix_tail    the last record's true size (its block may be partial) and
+//                   the payload end.
+//   G  k_ix_tiles   the tile pairs and the per-block entries of the index.
+//
+// Every read stays inside [stream, stream + len); a chain that leaves the
+// stream or meets an impossible reqLength is marked invalid.  A stream whose
+// chain does not produce exactly the records its bitmap needs is rejected
+// (status -30..-33); the decoder re-checks every record against the index.
+#include "szx_common.hpp"
+#include "szx_index.hpp"
+#include "szx_internal.hpp"
+
+namespace szxamd {
+
+constexpr uint32_t IX_C = IX_CHUNK;        // payload bytes per chunk
+constexpr uint16_t IX_INV = 0xFFFF;        // invalid chain / position
+constexpr int IX_PAD = 96;                 // staged bytes past the chunk (5 + NL <= 69)
+constexpr uint32_t IX_PRE_PER = 17;                            // prefix entries per thread
+constexpr uint32_t IX_PRE = 256 * IX_PRE_PER;                  // >= IX_C + 5 + 64 + 1
+// staged words: 16-byte alignment head + every byte the prefix sums read
+constexpr uint32_t IX_WORDS = (16 + IX_PRE) / 4 + 2;
+static_assert(IX_PRE >= IX_C + IX_PAD, "prefix covers the staged chunk");
+constexpr int IX_PER = IX_C / 256;                             // chunk positions per thread
+
+__device__ inline uint32_t ld_u32_unaligned(const uint32_t *w, uint32_t byte)
+{
+    const uint32_t a = w[byte >> 2], b = w[(byte >> 2) + 1];
+    return __builtin_amdgcn_alignbyte(b, a, byte & 3);
+}
+
+// Stage bytes [src, src + cnt) into LDS words, zero-padded, at byte offset
+// (src & 15): aligned 16-byte chunks lying wholly inside the stream
+// [lo, hi) are copied with one load (their bytes outside the range are
+// stream bytes no step reads), the rest byte by byte; returns the offset.
+__device__ inline uint32_t ix_stage(uint32_t *bw, const uint8_t *src, uint32_t cnt, uint32_t words,
+                                    const uint8_t *lo, const uint8_t *hi)
+{
+    uint8_t *b = reinterpret_cast<uint8_t *>(bw);
+    for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) bw[i] = 0;
+    __syncthreads();
+    const uintptr_t a = (uintptr_t)src & ~(uintptr_t)15;
+    const uint32_t head = (uint32_t)((uintptr_t)src - a);
+    const uint32_t nchunk = (head + cnt + 15) / 16;
+    for (uint32_t c = threadIdx.x; c < nchunk; c += blockDim.x) {
+        const uint8_t *g = reinterpret_cast<const uint8_t *>(a + 16 * (uintptr_t)c);
+        if (g >= lo && g + 16 <= hi)
+            *reinterpret_cast<uint4 *>(b + 16 * c) = *reinterpret_cast<const uint4 *>(g);
+    }
+    // the first and last chunks may be partial: one byte per thread
+    if (threadIdx.x < 32 && nchunk) {
+        const uint32_t c = threadIdx.x < 16 ? 0 : nchunk - 1;
+        const uint32_t k = threadIdx.x & 15;
+        const uint8_t *g = reinterpret_cast<const uint8_t *>(a + 16 * (uintptr_t)c) + k;
+        if (g >= lo && g < hi) b[16 * c + k] = *g;
+    }
+    __syncthreads();
+    return head;
+}
+
+// Per lead-code byte c, packed in 16-bit fields: the sum over its four 2-bit
+// codes of min(code, 1), of min(code, 2) and of the code itself -- what a
+// record with rb = 1, 2 or >= 3 subtracts from 4 rb per lead byte.
+__device__ inline uint64_t lead_counts(uint32_t c)
+{
+    const uint32_t lo = c & 0x55u, hi = (c >> 1) & 0x55u;
+    const uint32_t f1 = __builtin_popcount(lo | hi);
+    const uint32_t f3 = __builtin_popcount(lo) + 2 * __builtin_popcount(hi);
+    const uint32_t f2 = f3 - __builtin_popcount(lo & hi);
+    return (uint64_t)f1 | ((uint64_t)f2 << 16) | ((uint64_t)f3 << 32);
+}
+
+// Chain step for every chunk byte: J[p] = p + size(p), or IX_INV; positions
+// at or past IX_C absorb.  The NL-byte window sums of lead counts come from
+// an exclusive prefix sum over the staged bytes (pre, IX_PRE entries), so a
+// step costs two reads.  avail: payload bytes present from the chunk start.
+__device__ inline void ix_steps(const uint32_t *bw, uint32_t base, uint64_t *pre, uint16_t *J,
+                                uint32_t dom, uint32_t avail, int nl, int w)
+{
+    const uint8_t *b = reinterpret_cast<const uint8_t *>(bw) + base;
+    __shared__ uint64_t s_part[256];
+    // per-thread runs of IX_PRE_PER bytes
+    const uint32_t q0 = threadIdx.x * IX_PRE_PER;
+    uint64_t cnt[IX_PRE_PER];   // loads first, then arithmetic (LDS stores below may alias)
+#pragma unroll
+    for (uint32_t i = 0; i < IX_PRE_PER; i++) cnt[i] = b[q0 + i];
+    uint64_t run = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < IX_PRE_PER; i++) {
+        cnt[i] = lead_counts((uint32_t)cnt[i]);
+        run += cnt[i];
+    }
+    s_part[threadIdx.x] = run;
+    __syncthreads();
+    for (int d = 1; d < 256; d <<= 1) {
+        const uint64_t v = threadIdx.x >= (unsigned)d ? s_part[threadIdx.x - d] : 0;
+        __syncthreads();
+        s_part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    run = s_part[threadIdx.x] - run;
+#pragma unroll
+    for (uint32_t i = 0; i < IX_PRE_PER; i++) {
+        pre[q0 + i] = run;
+        run += cnt[i];
+    }
+    __syncthreads();
+    // chunk positions: IX_PER per thread, all loads before any store
+    uint32_t rl[IX_PER];
+    uint64_t lo[IX_PER], hi[IX_PER];
+#pragma unroll
+    for (int i = 0; i < IX_PER; i++) {
+        const uint32_t p = threadIdx.x + 256 * i;
+        rl[i] = b[p];
+        lo[i] = pre[p + 5];
+        hi[i] = pre[p + 5 + nl];
+    }
+#pragma unroll
+    for (int i = 0; i < IX_PER; i++) {
+        const uint32_t p = threadIdx.x + 256 * i;
+        uint16_t j = IX_INV;
+        if (p + 5 + (uint32_t)nl <= avail && rl[i] != 0 && rl[i] <= 8u * w) {
+            const uint32_t rb = (rl[i] + 7) >> 3;
+            const uint32_t sh = rb >= 3 ? 32 : (rb == 2 ? 16 : 0);
+            const uint32_t sub = (uint32_t)((hi[i] - lo[i]) >> sh) & 0xFFFFu;
+            j = (uint16_t)(p + 5 + (uint32_t)nl + 4 * rb * (uint32_t)nl - sub);
+        }
+        J[p] = j;
+    }
+    for (uint32_t p = IX_C + threadIdx.x; p < dom; p += blockDim.x) J[p] = (uint16_t)p;
+}
+
+// A: exit map of every chunk.
+__global__ __launch_bounds__(256) void k_ix_map(IxParams x)
+{
+    __shared__ uint32_t bw[IX_WORDS];
+    __shared__ uint16_t J0[IX_C + IX_RMAX];
+    __shared__ union {
+        uint64_t pre[IX_PRE];
+        uint16_t j1[IX_C + IX_RMAX];
+    } u;
+    __shared__ unsigned s_min, s_max;
+    const uint64_t c = blockIdx.x;
+    const DecodeHeader &h = *x.h;
+    if (*x.status != 0) return;
+    const uint64_t P = x.len - h.q0;       // payload bytes present in the buffer
+    const uint64_t g0 = c * IX_C;
+    if (g0 >= P) {
+        if (threadIdx.x == 0) {
+            x.conv[c] = 1;
+            x.exits[c] = IX_INV;
+        }
+        return;
+    }
+    const uint32_t avail = (uint32_t)(P - g0 < IX_C + IX_PAD ? P - g0 : IX_C + IX_PAD);
+    const uint32_t base = ix_stage(bw, x.s + h.q0 + g0, avail, IX_WORDS, x.s, x.s + x.len);
+    const uint32_t dom = IX_C + x.rmax;
+    ix_steps(bw, base, u.pre, J0, dom, avail, x.nl, x.w);
+    __syncthreads();   // the prefix sums die here: their space holds the second table
+    if (threadIdx.x == 0) {
+        s_min = 0xFFFFFFFFu;
+        s_max = 0;
+    }
+    __syncthreads();
+    for (uint32_t p = IX_C + threadIdx.x; p < dom; p += blockDim.x) u.j1[p] = (uint16_t)p;
+    __syncthreads();
+    uint16_t *ja = J0, *jb = u.j1;   // current table, next table
+    for (int r = 0; r < x.rounds; r++) {
+        uint16_t v[IX_PER];
+#pragma unroll
+        for (int i = 0; i < IX_PER; i++) v[i] = ja[threadIdx.x + 256 * i];
+#pragma unroll
+        for (int i = 0; i < IX_PER; i++) v[i] = v[i] == IX_INV ? IX_INV : ja[v[i]];
+#pragma unroll
+        for (int i = 0; i < IX_PER; i++) jb[threadIdx.x + 256 * i] = v[i];
+        __syncthreads();
+        uint16_t *t = ja;
+        ja = jb;
+        jb = t;
+    }
+    // entries e < rmax: exit offset into the next chunk (wave min / max, then
+    // one LDS atomic per wave)
+    unsigned mn = 0xFFFFFFFFu, mx = 0;
+    for (uint32_t e = threadIdx.x; e < x.rmax; e += blockDim.x) {
+        const uint16_t j = ja[e];
+        if (j != IX_INV && j >= IX_C) {
+            const unsigned v = (unsigned)(j - IX_C);
+            mn = v < mn ? v : mn;
+            mx = v > mx ? v : mx;
+        }
+    }
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) {
+        const unsigned a = __shfl_xor(mn, m, 64), b = __shfl_xor(mx, m, 64);
+        mn = a < mn ? a : mn;
+        mx = b > mx ? b : mx;
+    }
+    if ((threadIdx.x & 63) == 0 && mn != 0xFFFFFFFFu) {
+        atomicMin(&s_min, mn);
+        atomicMax(&s_max, mx);
+    }
+    __syncthreads();
+    const bool conv = s_min == s_max || s_min == 0xFFFFFFFFu;
+    if (threadIdx.x == 0) {
+        x.conv[c] = conv ? 1 : 0;
+        x.exits[c] = s_min == 0xFFFFFFFFu ? IX_INV : (uint16_t)s_min;
+    }
+    if (!conv)
+        for (uint32_t e = threadIdx.x; e < x.rmax; e += blockDim.x) {
+            const uint16_t j = ja[e];
+            x.tables[c * x.rmax + e] = (j != IX_INV && j >= IX_C) ? (uint16_t)(j - IX_C) : IX_INV;
+        }
+}
+
+// B: entry offset of every chunk (one thread per run of unconverged chunks).
+__global__ void k_ix_entry(IxParams x)
+{
+    const uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (c >= x.nch || *x.status != 0) return;
+    if (c != 0 && !x.conv[c - 1]) return;   // inside a run: its start thread writes it
+    uint16_t e = c == 0 ? 0 : x.exits[c - 1];
+    for (uint64_t k = c; k < x.nch; k++) {
+        x.entry[k] = e;
+        if (x.conv[k]) break;
+        e = e == IX_INV || e >= x.rmax ? IX_INV : x.tables[k * x.rmax + e];
+    }
+}
+
+// C: record starts inside every chunk, from its entry.  The chain from the
+// entry is marked while the step table doubles: before round r every
+// position within 2^r steps of the entry is marked, and round r marks their
+// images under J^(2^r).  Marked positions below IX_C, in increasing order,
+// are the chunk's record starts.
+__global__ __launch_bounds__(256) void k_ix_walk(IxParams x)
+{
+    __shared__ uint32_t bw[IX_WORDS];
+    __shared__ uint16_t J0[IX_C + IX_RMAX];
+    __shared__ union {
+        uint64_t pre[IX_PRE];
+        uint16_t j1[IX_C + IX_RMAX];
+    } u;
+    __shared__ uint8_t on[IX_C + IX_RMAX];
+    __shared__ uint32_t s_part[256];
+    const uint64_t c = blockIdx.x;
+    const DecodeHeader &h = *x.h;
+    if (*x.status != 0) return;
+    const uint64_t P = x.len - h.q0;
+    const uint64_t g0 = c * IX_C;
+    if (g0 >= P) {
+        if (threadIdx.x == 0) x.cnt[c] = 0;
+        return;
+    }
+    const uint32_t avail = (uint32_t)(P - g0 < IX_C + IX_PAD ? P - g0 : IX_C + IX_PAD);
+    const uint32_t base = ix_stage(bw, x.s + h.q0 + g0, avail, IX_WORDS, x.s, x.s + x.len);
+    const uint32_t dom = IX_C + x.rmax;
+    ix_steps(bw, base, u.pre, J0, dom, avail, x.nl, x.w);
+    __syncthreads();
+    const uint32_t e0 = x.entry[c];
+    for (uint32_t p = threadIdx.x; p < dom; p += blockDim.x) {
+        on[p] = (p == e0) ? 1 : 0;
+        if (p >= IX_C) u.j1[p] = (uint16_t)p;
+    }
+    __syncthreads();
+    uint16_t *ja = J0, *jb = u.j1;
+    for (int r = 0; r < x.rounds; r++) {
+        uint16_t v[IX_PER], j2[IX_PER];
+        uint8_t m[IX_PER];
+#pragma unroll
+        for (int i = 0; i < IX_PER; i++) {
+            v[i] = ja[threadIdx.x + 256 * i];
+            m[i] = on[threadIdx.x + 256 * i];
+        }
+#pragma unroll
+        for (int i = 0; i < IX_PER; i++) j2[i] = v[i] == IX_INV ? IX_INV : ja[v[i]];
+#pragma unroll
+        for (int i = 0; i < IX_PER; i++) {
+            if (m[i] && v[i] != IX_INV) on[v[i]] = 1;
+            jb[threadIdx.x + 256 * i] = j2[i];
+        }
+        __syncthreads();
+        uint16_t *t = ja;
+        ja = jb;
+        jb = t;
+    }
+    // compaction of the marked positions below IX_C (16 per thread, in order)
+    constexpr uint32_t PER = IX_C / 256;
+    const uint32_t p0 = threadIdx.x * PER;
+    uint32_t k = 0;
+    for (uint32_t i = 0; i < PER; i++) k += on[p0 + i];
+    s_part[threadIdx.x] = k;
+    __syncthreads();
+    for (int d = 1; d < 256; d <<= 1) {
+        const uint32_t v = threadIdx.x >= (unsigned)d ? s_part[threadIdx.x - d] : 0;
+        __syncthreads();
+        s_part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t o = s_part[threadIdx.x] - k;
+    uint16_t *pos = x.pos + c * x.maxr;
+    for (uint32_t i = 0; i < PER; i++)
+        if (on[p0 + i]) {
+            if (o < x.maxr) pos[o] = (uint16_t)(p0 + i);
+            o++;
+        }
+    if (threadIdx.x == 255) x.cnt[c] = s_part[255] < x.maxr ? s_part[255] : x.maxr;
+}
+
+// Exclusive scan of n u32 values in three steps: per 1024-value block an
+// LDS scan (block sums aside), one workgroup over the block sums, then the
+// block offsets are added.  *total receives the sum.
+__device__ inline uint64_t block_excl_scan_1024(uint64_t v, uint64_t *s)
+{
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {
+        const uint64_t o = threadIdx.x >= (unsigned)d ? s[threadIdx.x - d] : 0;
+        __syncthreads();
+        s[threadIdx.x] += o;
+        __syncthreads();
+    }
+    return s[threadIdx.x] - v;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_blocks(const uint32_t *in, uint64_t n, uint64_t *out,
+                                                      uint64_t *bsum)
+{
+    __shared__ uint64_t s[1024];
+    const uint64_t i = blockIdx.x * 1024ull + threadIdx.x;
+    const uint64_t v = i < n ? in[i] : 0;
+    const uint64_t e = block_excl_scan_1024(v, s);
+    if (i < n) out[i] = e;
+    if (threadIdx.x == 1023) bsum[blockIdx.x] = s[1023];
+}
+
+__global__ __launch_bounds__(1024) void k_scan_tops(uint64_t *bsum, uint64_t nblk, uint64_t *total)
+{
+    __shared__ uint64_t s[1024];
+    const uint64_t per = (nblk + 1023) / 1024;
+    const uint64_t a = threadIdx.x * per, b = a + per < nblk ? a + per : nblk;
+    uint64_t sum = 0;
+    for (uint64_t i = a; i < b; i++) sum += bsum[i];
+    uint64_t run = block_excl_scan_1024(sum, s);
+    for (uint64_t i = a; i < b; i++) {
+        const uint64_t v = bsum[i];
+        bsum[i] = run;
+        run += v;
+    }
+    if (threadIdx.x == 1023 && total) *total = s[1023];
+}
+
+__global__ void k_scan_add(uint64_t *out, uint64_t n, const uint64_t *bsum)
+{
+    const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i < n) out[i] += bsum[i / 1024];
+}
+
+static void scan_u32(const uint32_t *in, uint64_t n, uint64_t *out, uint64_t *bsum, uint64_t *total,
+                     hipStream_t st)
+{
+    const uint64_t nblk = (n + 1023) / 1024;
+    hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nblk), dim3(1024), 0, st, in, n, out, bsum);
+    hipLaunchKernelGGL(k_scan_tops, dim3(1), dim3(1024), 0, st, bsum, nblk, total);
+    hipLaunchKernelGGL(k_scan_add, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, out, n, bsum);
+}
+
+// E: record starts (payload offsets) in record order.
+__global__ void k_ix_scatter(IxParams x)
+{
+    const uint64_t c = blockIdx.x;
+    if (*x.status != 0) return;
+    const uint64_t base = x.cbase[c];
+    const uint32_t k = x.cnt[c];
+    const uint64_t K = x.h->nb - x.h->nconst;
+    for (uint32_t i = threadIdx.x; i < k; i += blockDim.x)
+        if (base + i < K) x.rec[base + i] = c * IX_C + x.pos[c * x.maxr + i];
+}
+
+// Non-constant blocks per compressor tile (bitmap popcount).
+__global__ void k_ix_tile_counts(IxParams x)
+{
+    const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (t >= x.ntiles) return;
+    const DecodeHeader &h = *x.h;
+    const uint64_t b0 = t * x.tile_blocks, b1 = b0 + x.tile_blocks < h.nb ? b0 + x.tile_blocks : h.nb;
+    uint32_t c = 0;
+    for (uint64_t b = b0; b < b1; b++) c += (x.s[HEADER_BYTES + (b >> 3)] >> (7 - (b & 7))) & 1;
+    x.tcnt[t] = c;
+}
+
+// F: the last record's true size and the payload end; checks the record count.
+__global__ void k_ix_tail(IxParams x)
+{
+    if (threadIdx.x != 0 || *x.status != 0) return;
+    const DecodeHeader &h = *x.h;
+    const uint64_t K = h.nb - h.nconst;
+    const uint64_t P = x.len - h.q0;
+    if (*x.found < K) {
+        *x.status = -30;
+        return;
+    }
+    if (K == 0) {
+        *x.pend = 0;
+        return;
+    }
+    // the last non-constant block
+    uint64_t b = h.nb;
+    while (b > 0) {
+        b--;
+        if ((x.s[HEADER_BYTES + (b >> 3)] >> (7 - (b & 7))) & 1) break;
+    }
+    const uint64_t nel = (b + 1) * (uint64_t)x.bs <= x.n ? (uint64_t)x.bs : x.n - b * (uint64_t)x.bs;
+    const uint64_t nl = (nel + 3) / 4;
+    const uint64_t q = x.rec[K - 1];
+    if (q + 5 + nl > P) {
+        *x.status = -31;
+        return;
+    }
+    const uint8_t *r = x.s + h.q0 + q;
+    const uint32_t rl = r[0];
+    if (rl == 0 || rl > 8u * x.w) {
+        *x.status = -32;
+        return;
+    }
+    const uint32_t rb = (rl + 7) >> 3;
+    uint64_t sz = 5 + nl;
+    for (uint64_t i = 0; i < nel; i++) {
+        const uint32_t ld = (r[5 + (i >> 2)] >> (6 - 2 * (i & 3))) & 3;
+        if (ld < rb) sz += rb - ld;
+    }
+    if (q + sz > P) {
+        *x.status = -33;
+        return;
+    }
+    *x.pend = q + sz;
+}
+
+// G: tile pairs and per-block entries.  One workgroup per tile.
+__global__ __launch_bounds__(256) void k_ix_tiles(IxParams x)
+{
+    __shared__ uint32_t s_nc[1024 + 1];   // non-constant blocks before each block of the tile
+    const uint64_t t = blockIdx.x;
+    if (*x.status != 0) return;
+    const DecodeHeader &h = *x.h;
+    const uint64_t K = h.nb - h.nconst;
+    const uint64_t b0 = t * x.tile_blocks;
+    const uint64_t nbt = b0 + x.tile_blocks <= h.nb ? x.tile_blocks : h.nb - b0;
+    const uint64_t ord0 = x.tbase[t];
+    if (threadIdx.x == 0) {
+        uint32_t a = 0;
+        for (uint64_t i = 0; i < nbt; i++) {
+            s_nc[i] = a;
+            a += (x.s[HEADER_BYTES + ((b0 + i) >> 3)] >> (7 - ((b0 + i) & 7))) & 1;
+        }
+        s_nc[nbt] = a;
+        const uint64_t off = ord0 < K ? x.rec[ord0] : *x.pend;
+        x.index[2 * t] = off;
+        x.index[2 * t + 1] = b0 - ord0;
+        if (t == x.ntiles - 1) {
+            x.index[2 * x.ntiles] = *x.pend;
+            x.index[2 * x.ntiles + 1] = h.nconst;
+        }
+    }
+    __syncthreads();
+    const uint64_t off = ord0 < K ? x.rec[ord0] : *x.pend;
+    for (uint64_t i = threadIdx.x; i < nbt; i += blockDim.x) {
+        const uint64_t b = b0 + i;
+        const bool nc = (x.s[HEADER_BYTES + (b >> 3)] >> (7 - (b & 7))) & 1;
+        uint64_t v;
+        if (nc) {
+            const uint64_t ord = ord0 + s_nc[i];
+            v = ord < K ? x.rec[ord] - off : 0xFFFF;
+        } else {
+            v = i - s_nc[i];
+        }
+        x.bidx[b] = (uint16_t)(v > 0xFFFF ? 0xFFFF : v);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+IxLayout ix_layout(uint64_t n, int w, int bs, uint64_t len)
+{
+    IxLayout L{};
+    const uint64_t nb = (n + bs - 1) / bs;
+    L.nch = (len + IX_C - 1) / IX_C;
+    if (!L.nch) L.nch = 1;
+    L.nl = bs / 4;
+    L.rmax = 5 + (uint32_t)L.nl + (uint32_t)bs * (uint32_t)w;
+    L.maxr = IX_C / (5 + (uint32_t)L.nl) + 2;
+    L.ntiles = index_tiles(n, w);
+    auto al = [](uint64_t v) { return (v + 255) & ~255ull; };
+    uint64_t off = 0;
+    L.o_exits = off;  off += al(2 * L.nch);
+    L.o_conv = off;   off += al(L.nch);
+    L.o_entry = off;  off += al(2 * L.nch);
+    L.o_cnt = off;    off += al(4 * L.nch);
+    L.o_cbase = off;  off += al(8 * L.nch);
+    L.o_tables = off; off += al(2 * L.nch * L.rmax);
+    L.o_pos = off;    off += al(2 * L.nch * L.maxr);
+    L.o_rec = off;    off += al(8 * (nb ? nb : 1));
+    L.o_tcnt = off;   off += al(4 * L.ntiles);
+    L.o_tbase = off;  off += al(8 * L.ntiles);
+    L.o_misc = off;   off += 256;   // found, pend, scratch total
+    L.o_bsum = off;   off += al(8 * ((L.nch > L.ntiles ? L.nch : L.ntiles) / 1024 + 1));
+    L.bytes = off;
+    return L;
+}
+
+void index_build_launch(const uint8_t *s, uint64_t len, uint64_t n, int w, int bs, const DecodeHeader *h,
+                        uint64_t *index, uint16_t *bidx, uint8_t *ws, int32_t *status, hipStream_t st)
+{
+    const IxLayout L = ix_layout(n, w, bs, len);
+    IxParams x{};
+    x.s = s;
+    x.len = len;
+    x.n = n;
+    x.bs = bs;
+    x.w = w;
+    x.nl = (int)L.nl;
+    x.rmax = L.rmax;
+    x.maxr = L.maxr;
+    x.nch = L.nch;
+    x.ntiles = L.ntiles;
+    x.tile_blocks = tile_values(w) / (uint64_t)bs;
+    // doubling rounds: a chunk holds at most IX_C / (5 + NL) records
+    int rounds = 1;
+    while ((1u << rounds) < IX_C / (5 + (uint32_t)L.nl) + 2) rounds++;
+    x.rounds = rounds + 1;
+    x.h = h;
+    x.status = status;
+    x.exits = reinterpret_cast<uint16_t *>(ws + L.o_exits);
+    x.conv = ws + L.o_conv;
+    x.entry = reinterpret_cast<uint16_t *>(ws + L.o_entry);
+    x.cnt = reinterpret_cast<uint32_t *>(ws + L.o_cnt);
+    x.cbase = reinterpret_cast<uint64_t *>(ws + L.o_cbase);
+    x.tables = reinterpret_cast<uint16_t *>(ws + L.o_tables);
+    x.pos = reinterpret_cast<uint16_t *>(ws + L.o_pos);
+    x.rec = reinterpret_cast<uint64_t *>(ws + L.o_rec);
+    x.tcnt = reinterpret_cast<uint32_t *>(ws + L.o_tcnt);
+    x.tbase = reinterpret_cast<uint64_t *>(ws + L.o_tbase);
+    x.found = reinterpret_cast<uint64_t *>(ws + L.o_misc);
+    x.pend = reinterpret_cast<uint64_t *>(ws + L.o_misc + 8);
+    uint64_t *scratch = reinterpret_cast<uint64_t *>(ws + L.o_misc + 16);
+    uint64_t *bsum = reinterpret_cast<uint64_t *>(ws + L.o_bsum);
+    x.index = index;
+    x.bidx = bidx;
+    const unsigned nch = (unsigned)L.nch;
+    hipLaunchKernelGGL(k_ix_map, dim3(nch), dim3(256), 0, st, x);
+    hipLaunchKernelGGL(k_ix_entry, dim3((nch + 255) / 256), dim3(256), 0, st, x);
+    hipLaunchKernelGGL(k_ix_walk, dim3(nch), dim3(256), 0, st, x);
+    scan_u32(x.cnt, L.nch, x.cbase, bsum, x.found, st);
+    hipLaunchKernelGGL(k_ix_scatter, dim3(nch), dim3(256), 0, st, x);
+    hipLaunchKernelGGL(k_ix_tile_counts, dim3((unsigned)((L.ntiles + 255) / 256)), dim3(256), 0, st, x);
+    scan_u32(x.tcnt, L.ntiles, x.tbase, bsum, scratch, st);
+    hipLaunchKernelGGL(k_ix_tail, dim3(1), dim3(64), 0, st, x);
+    hipLaunchKernelGGL(k_ix_tiles, dim3((unsigned)L.ntiles), dim3(256), 0, st, x);
+}
+
+}  // namespace szxamd

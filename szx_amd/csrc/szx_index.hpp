@@ -1,0 +1,52 @@
+// szx_index.hpp -- parallel record-index build from a stream (szx_index.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "szx_internal.hpp"
+
+namespace szxamd {
+
+constexpr uint32_t IX_CHUNK = 4096;   // payload bytes per chunk
+constexpr uint32_t IX_RMAX = 2117;    // largest record: 5 + 64 + 256 * 8
+
+struct IxParams {
+    const uint8_t *s;
+    uint64_t len, n;
+    int bs, w, nl, rounds;
+    uint32_t rmax, maxr;
+    uint64_t nch, ntiles, tile_blocks;
+    const DecodeHeader *h;
+    int32_t *status;
+    uint16_t *exits;    // per chunk: exit offset when converged
+    uint8_t *conv;      // per chunk: all valid entries exit at one offset
+    uint16_t *entry;    // per chunk: offset of its first record
+    uint32_t *cnt;      // per chunk: records found
+    uint64_t *cbase;    // per chunk: records before it
+    uint16_t *tables;   // per unconverged chunk: exit for every entry
+    uint16_t *pos;      // per chunk: record starts
+    uint64_t *rec;      // per record: payload offset
+    uint32_t *tcnt;     // per tile: non-constant blocks
+    uint64_t *tbase;    // per tile: non-constant blocks before it
+    uint64_t *found;    // records found in total
+    uint64_t *pend;     // payload end
+    uint64_t *index;    // output: tile pairs
+    uint16_t *bidx;     // output: per-block entries
+};
+
+struct IxLayout {
+    uint64_t nch, ntiles, nl;
+    uint32_t rmax, maxr;
+    uint64_t o_exits, o_conv, o_entry, o_cnt, o_cbase, o_tables, o_pos, o_rec, o_tcnt, o_tbase,
+        o_misc, o_bsum, bytes;
+};
+
+IxLayout ix_layout(uint64_t n, int w, int bs, uint64_t len);
+
+// Build the record index of a stream of len bytes (header already checked
+// into *h).  ws holds ix_layout(...).bytes.
+void index_build_launch(const uint8_t *s, uint64_t len, uint64_t n, int w, int bs, const DecodeHeader *h,
+                        uint64_t *index, uint16_t *bidx, uint8_t *ws, int32_t *status, hipStream_t st);
+
+}  // namespace szxamd

@@ -12,10 +12,19 @@ namespace szxamd {
 constexpr int TILE_VALUES_F32 = 4096;
 constexpr int TILE_VALUES_F64 = 2048;
 inline uint64_t tile_values(int w) { return w == 4 ? TILE_VALUES_F32 : TILE_VALUES_F64; }
-inline uint64_t index_words(uint64_t n, int w)
+// Record index: per tile {payload offset of its first record, constant
+// blocks before it} plus a closing pair (u64), then per block a u16 -- the
+// record's start inside its tile's payload bytes (non-constant) or its rank
+// among the tile's constant blocks.
+inline uint64_t index_tiles(uint64_t n, int w)
 {
-    uint64_t t = (n + tile_values(w) - 1) / tile_values(w);
-    return 2 * ((t ? t : 1) + 1);
+    const uint64_t t = (n + tile_values(w) - 1) / tile_values(w);
+    return t ? t : 1;
+}
+inline uint64_t index_words(uint64_t n, int w, int bs)
+{
+    const uint64_t nb = (n + bs - 1) / bs;
+    return 2 * (index_tiles(n, w) + 1) + (nb + 3) / 4;
 }
 
 // Optional kernel timing: when enabled on this thread, the launchers bracket
@@ -50,9 +59,15 @@ int compress_launch(const void *d_in, uint64_t n, int w, int rel_mode, float abs
 uint64_t range_workspace_bytes();
 void range_launch(const void *d_in, uint64_t n, int w, uint8_t *d_ws, void *d_minmax, hipStream_t st);
 
+// Checked header of a stream (k_check_header): block size, blocks, bitmap
+// bytes, constant blocks, first payload byte.
+struct DecodeHeader {
+    uint64_t bs, nb, nbit, nconst, q0;
+};
+
 // Device memory layout of one decompress call.
 struct DecompressLayout {
-    uint64_t ws_hdr, ws_index, ws_bytes;
+    uint64_t ws_hdr, ws_index, ws_build, ws_bytes;
 };
 
 DecompressLayout decompress_layout(uint64_t n, int w, int bs, uint64_t stream_bytes);

@@ -191,3 +191,48 @@ def test_corrupt_index_is_rejected(device):
     say("corrupt index")
     dec(view, size, Z.DECODE_REFERENCE, index=bad)
     assert dec.check() != 0
+
+
+@pytest.mark.parametrize("bs", POW2)
+def test_index_free_decode(bs, device):
+    """Decode without the compressor's index: the record index is rebuilt in
+    parallel from the stream (chunk exit maps, chain marking, scans)."""
+    import torch
+    r = np.random.default_rng(1000 + bs)
+    for dtype in (np.float32, np.float64):
+        for trial in range(3):
+            n = int(r.choice([1, 5, 4097, 30011, 200003, 1 << 20]))
+            x = _field(r, n, dtype, int(r.integers(0, 5)))
+            if trial == 1:
+                x[: n // 2] = 0.5            # long constant run: few records
+            eb = float(np.float32((float(np.ptp(x)) + 1e-30) * 10.0 ** r.uniform(-9, -2)))
+            s = O.compress(x, eb, bs)
+            say("index-free", bs, dtype.__name__, n, trial, len(s))
+            want = O.decompress(s, n, dtype, ref_quirk=True)
+            st = torch.from_numpy(np.frombuffer(s, np.uint8).copy()).to(device)
+            dec = Z.DeviceDecompressor(n, dtype, bs, len(s) + 64)
+            dec(st, len(s), Z.DECODE_REFERENCE)
+            assert dec.check() == 0
+            assert bits_equal(dec.out.cpu().numpy(), want)
+            # a buffer longer than the stream (trailing bytes) decodes the same
+            padded = torch.cat([st, torch.full((64,), 0xAB, dtype=torch.uint8, device=device)])
+            dec(padded, len(s) + 64, Z.DECODE_REFERENCE)
+            assert dec.check() == 0
+            assert bits_equal(dec.out.cpu().numpy(), want)
+
+
+def test_index_free_rejects_damage(device):
+    import torch
+    x = np.sin(np.arange(300000) * 0.001).astype(np.float32)
+    s = bytearray(O.compress(x, 1e-6, 128))
+    dec = Z.DeviceDecompressor(x.size, np.float32, 128, len(s))
+    say("truncated payload")
+    st = torch.from_numpy(np.frombuffer(bytes(s), np.uint8).copy()).to(device)
+    dec(st[: len(s) - 100], len(s) - 100, Z.DECODE_REFERENCE)
+    assert dec.check() != 0
+    say("impossible reqLength")
+    q0 = 13 + (((x.size + 127) // 128) + 7) // 8 + 4 * int.from_bytes(s[5:13], "big")
+    s[q0] = 0                                # first record's reqLength byte
+    st = torch.from_numpy(np.frombuffer(bytes(s), np.uint8).copy()).to(device)
+    dec(st, len(s), Z.DECODE_REFERENCE)
+    assert dec.check() != 0
