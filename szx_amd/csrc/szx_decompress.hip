@@ -29,6 +29,7 @@
 #include "szx_common.hpp"
 #include "szx_index.hpp"
 #include "szx_internal.hpp"
+#include "szx_wave.hpp"
 
 namespace szxamd {
 
@@ -131,24 +132,160 @@ __device__ inline void stage_bytes(uint8_t *dst, const uint8_t *src, uint32_t nb
     }
 }
 
-// Decode one tile (the compressor's tile: TILE values, NBLK blocks).
-template <typename T, int BS, int SLABS>
+// Everything the slab decoder reads from the tile's LDS staging.
+struct TileLds {
+    const uint8_t *st;          // staged record bytes (16-byte aligned base)
+    uint32_t rdel;              // offset of the tile's first record byte in st
+    const uint8_t *meds;        // the tile's constant medians (big-endian)
+    uint32_t nct, tb;           // constant blocks, record bytes of the tile
+    const uint16_t *start;      // per block: record start / constant rank
+    uint16_t *size;             // per block: decoded record size
+    const uint8_t *state;       // per block: non-constant
+    int *err;
+};
+
+// Decode the LV values of one lane: lane lseg of the BS/LV lanes holding
+// block lb, first value e0.  FULL: the whole block lies inside the array.
+template <typename T, int BS, int LV, bool FULL>
+__device__ inline void decode_lane(const TileLds &L, uint64_t e0, uint64_t n, int lb, int lane, int lseg,
+                                   T *out, int exact)
+{
+    using U = typename Word<T>::U;
+    constexpr int W = Word<T>::W;
+    constexpr int SEG = BS / LV;
+    constexpr int NLB = LV / 4;          // lead-code bytes per lane
+    constexpr int NW = W == 4 ? 2 : 3;   // LDS words per gathered value
+    T v[LV];
+    if (!L.state[lb]) {
+        const uint32_t rank = L.start[lb];
+        float med = 0.0f;
+        if (rank < L.nct) {
+            const uint8_t *p = L.meds + 4 * rank;
+            med = bits_f32(((uint32_t)p[0] << 24) | ((uint32_t)p[1] << 16) | ((uint32_t)p[2] << 8) |
+                           (uint32_t)p[3]);
+        } else if (lseg == 0) {
+            *L.err = 1;
+        }
+#pragma unroll
+        for (int k = 0; k < LV; k++) v[k] = (T)med;
+    } else {
+        const uint64_t bstart = e0 - (uint64_t)LV * lseg;
+        const int nel = FULL ? BS : (n - bstart >= (uint64_t)BS ? BS : (int)(n - bstart));
+        const int nlead = (nel + 3) >> 2;
+        const uint32_t start = L.start[lb];
+        const bool head_ok = start + 5u + (uint32_t)nlead <= L.tb;   // fixed part staged
+        const uint8_t *rec = L.st + L.rdel + (head_ok ? start : 0u);
+        const int rl = head_ok ? rec[0] : 1;
+        const bool rl_ok = head_ok && rl >= 1 && rl <= 8 * W;
+        const int rb = rl_ok ? (rl + 7) >> 3 : 1;
+        const int shift = (8 - (rl & 7)) & 7;
+        const int lo = W - rb;
+        const float med = bits_f32(((uint32_t)rec[1] << 24) | ((uint32_t)rec[2] << 16) |
+                                   ((uint32_t)rec[3] << 8) | (uint32_t)rec[4]);
+        uint32_t code[NLB];
+#pragma unroll
+        for (int b = 0; b < NLB; b++)
+            code[b] = (rl_ok && (lseg * NLB + b) < nlead) ? rec[5 + lseg * NLB + b] : 0u;
+        int lk[LV];
+        uint32_t mk[LV], ofs[LV], tot = 0;
+#pragma unroll
+        for (int k = 0; k < LV; k++) {
+            const int ld = (code[k >> 2] >> (6 - 2 * (k & 3))) & 3;
+            lk[k] = ld < rb ? ld : rb;
+            const bool valid = FULL || e0 + k < n;
+            mk[k] = valid ? (uint32_t)(rb - lk[k]) : 0u;
+            ofs[k] = tot;
+            tot += mk[k];
+        }
+        const uint32_t inc = seg_scan_incl<SEG>(tot, lane);
+        const uint32_t rsize = 5u + (uint32_t)nlead + seg_last<SEG>(inc);
+        const bool ok = rl_ok && start + rsize <= L.tb;
+        if (lseg == 0) {
+            if (ok) L.size[lb] = (uint16_t)rsize;
+            else *L.err = 1;
+        }
+        // gather: value k's mk bytes start at run0 + ofs[k]; 2 (fp32) or 3
+        // (fp64) independent aligned word reads and a byte funnel each
+        const uint32_t run0 = (uint32_t)(rec - L.st) + 5u + (uint32_t)nlead + (ok ? inc - tot : 0u);
+        const uint32_t *stw = reinterpret_cast<const uint32_t *>(L.st);
+        uint32_t dw[LV][NW];
+#pragma unroll
+        for (int k = 0; k < LV; k++) {
+            const uint32_t wi = (run0 + (ok ? ofs[k] : 0u)) >> 2;
+#pragma unroll
+            for (int i = 0; i < NW; i++) dw[k][i] = stw[wi + i];
+        }
+        // pass 1 with a zero carry-in: t0[k] is value k's word if nothing came
+        // from the previous lane; inh[k] marks the top bytes value k still
+        // inherits from before the lane (copied through values 0..k)
+        const U wmask = ~(U)0 >> shift;
+        U t0[LV], inh[LV];
+        U tp = 0, ih = ~(U)0;
+#pragma unroll
+        for (int k = 0; k < LV; k++) {
+            const uint32_t sh = (run0 + (ok ? ofs[k] : 0u)) & 3u;
+            U xk;
+            if constexpr (W == 4)
+                xk = __builtin_amdgcn_alignbyte(dw[k][1], dw[k][0], sh);
+            else
+                xk = (U)__builtin_amdgcn_alignbyte(dw[k][1], dw[k][0], sh) |
+                     ((U)__builtin_amdgcn_alignbyte(dw[k][2], dw[k][1], sh) << 32);
+            const uint32_t m = ok ? mk[k] : 0u;
+            // the m stream bytes land on byte positions [W - rb, W - lk)
+            U ek = m ? (U)((xk << (8 * (W - m))) >> (8 * lk[k])) : (U)0;
+            if constexpr (W == 8) {
+                if (!exact && rb == 5 && lk[k] == 0 && m)   // szxd_double.cc:244 reads q[3]
+                    ek = (ek & ~((U)0xFF << 56)) | (((xk >> 24) & 0xFF) << 56);
+            }
+            ek &= wmask;
+            const U cm = lk[k] ? (~(U)0 << (8 * (W - lk[k]))) : (U)0;   // bytes copied from k-1
+            tp = ek | (tp & cm);
+            ih &= cm;
+            t0[k] = tp;
+            inh[k] = ih;
+        }
+        // the lane's record of the copyable bytes W-3..W-1: values and
+        // written flags (a byte is written if no longer inherited)
+        const uint32_t ih3 = (uint32_t)(ih >> (8 * (W - 3))) & 0xFFFFFFu;
+        const uint32_t wf = (ih3 & 0xFFu ? 0u : 1u) | (ih3 & 0xFF00u ? 0u : 2u) | (ih3 & 0xFF0000u ? 0u : 4u);
+        const uint32_t carry = ((uint32_t)(tp >> (8 * (W - 3))) & 0xFFFFFFu & ~ih3) | (wf << 24);
+        const uint32_t cinc = seg_scan_incl_op<SEG>(carry, lane, 0u, lw_combine);
+        const uint32_t cin = seg_prev<SEG>(cinc, lane);
+        const U cw = (U)(cin & 0xFFFFFFu) << (8 * (W - 3));
+        const T mT = (T)med;
+#pragma unroll
+        for (int k = 0; k < LV; k++) v[k] = from_bits<T>((t0[k] | (cw & inh[k])) << shift) + mT;
+    }
+    if (FULL) {
+#pragma unroll
+        for (int k = 0; k < LV; k += 4) Store4<T>::store(out, e0 + k, n, v + k);
+    } else {
+#pragma unroll
+        for (int k = 0; k < LV; k++)
+            if (e0 + k < n) out[e0 + k] = v[k];
+    }
+}
+
+// Decode one tile (the compressor's tile: TILE values, NBLK blocks); each
+// lane decodes LV consecutive values per slab of 256 * LV values.
+template <typename T, int BS, int LV>
 __global__ __launch_bounds__(256) void k_decode_tiles(const uint8_t *s, uint64_t len, uint64_t n,
                                                      const DecodeHeader *h, const uint64_t *index,
                                                      const uint16_t *bidx, int32_t *status, T *out,
                                                      int exact)
 {
-    using U = typename Word<T>::U;
     constexpr int W = Word<T>::W;
-    constexpr int SEG = BS / 4;
-    constexpr int SLAB = 1024;
-    constexpr int TILE = SLAB * SLABS;
+    constexpr int SEG = BS / LV;
+    constexpr int TILE = sizeof(T) == 4 ? TILE_VALUES_F32 : TILE_VALUES_F64;
+    constexpr int SLAB = 256 * LV;
+    constexpr int SLABS = TILE / SLAB;
     constexpr int BPS = SLAB / BS;
     constexpr int NBLK = TILE / BS;
     constexpr int STAGE = TILE * W + NBLK * (5 + BS / 4);
     constexpr int PER_LANE = (NBLK + 63) / 64;   // blocks per lane in the check scan
+    static_assert(SLABS >= 1 && TILE % SLAB == 0, "slabs tile the tile");
 
-    __shared__ __attribute__((aligned(16))) uint8_t st[STAGE + 48];
+    __shared__ __attribute__((aligned(16))) uint8_t st[STAGE + 64];
     __shared__ __attribute__((aligned(16))) uint8_t s_med[4 * NBLK + 48];
     __shared__ uint16_t s_start[NBLK];   // record start, or constant rank within the tile
     __shared__ uint16_t s_size[NBLK];    // decoded record size (0: constant block)
@@ -187,113 +324,18 @@ __global__ __launch_bounds__(256) void k_decode_tiles(const uint8_t *s, uint64_t
     }
     __syncthreads();
 
-    const uint8_t *recs = st + rdel;
-    const uint8_t *meds = s_med + mdel;
+    const TileLds L{st, rdel, s_med + mdel, nct, tb, s_start, s_size, s_state, &s_err};
+    const bool full = (tile + 1) * (uint64_t)TILE <= n;
 #pragma unroll
     for (int u = 0; u < SLABS; u++) {
-        const uint64_t e0 = tile * TILE + (uint64_t)u * SLAB + 4u * (uint64_t)tid;
+        const uint64_t e0 = tile * TILE + (uint64_t)u * SLAB + (uint64_t)LV * tid;
         const int lb = u * BPS + tid / SEG;
-        const uint64_t bstart = e0 - 4u * (uint64_t)lseg;
-        if (bstart >= n) continue;   // whole segment
-        T v[4];
-        if (!s_state[lb]) {
-            const uint32_t rank = s_start[lb];
-            float med = 0.0f;
-            if (rank < nct) {
-                const uint8_t *p = meds + 4 * rank;
-                med = bits_f32(((uint32_t)p[0] << 24) | ((uint32_t)p[1] << 16) |
-                               ((uint32_t)p[2] << 8) | (uint32_t)p[3]);
-            } else if (lseg == 0) {
-                s_err = 1;
-            }
-#pragma unroll
-            for (int k = 0; k < 4; k++) v[k] = (T)med;
-            Store4<T>::store(out, e0, n, v);
-            continue;
+        if (full) {
+            decode_lane<T, BS, LV, true>(L, e0, n, lb, lane, lseg, out, exact);
+        } else {
+            if (e0 - (uint64_t)LV * lseg >= n) continue;   // the whole block is past the end
+            decode_lane<T, BS, LV, false>(L, e0, n, lb, lane, lseg, out, exact);
         }
-        const uint64_t rem = n - bstart;
-        const int nel = rem >= (uint64_t)BS ? BS : (int)rem;
-        const int nlead = (nel + 3) >> 2;
-        const uint32_t start = s_start[lb];
-        // the record's fixed part must lie in the staged bytes
-        const bool head_ok = start + 5u + (uint32_t)nlead <= tb;
-        const uint8_t *rec = recs + (head_ok ? start : 0u);
-        const int rl = head_ok ? rec[0] : 1;
-        const bool rl_ok = head_ok && rl >= 1 && rl <= 8 * W;
-        const int rb = rl_ok ? (rl + 7) >> 3 : 1;
-        const int shift = (8 - (rl & 7)) & 7;
-        const int lo = W - rb;
-        const float med = be_f32(rec + 1);
-        const uint32_t code = (rl_ok && lseg < nlead) ? rec[5 + lseg] : 0u;
-
-        int lk[4];
-        uint32_t tot = 0;
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const int ld = (code >> (6 - 2 * k)) & 3;
-            lk[k] = ld < rb ? ld : rb;
-            if (e0 + k < n) tot += (uint32_t)(rb - lk[k]);
-        }
-        uint32_t inc = tot;
-#pragma unroll
-        for (int d = 1; d < SEG; d <<= 1) {
-            const uint32_t o = __shfl_up(inc, d, SEG);
-            if (lseg >= d) inc += o;
-        }
-        const uint32_t rsize = 5u + (uint32_t)nlead + __shfl(inc, SEG - 1, SEG);
-        const bool ok = rl_ok && start + rsize <= tb;
-        if (lseg == 0) {
-            if (ok) s_size[lb] = (uint16_t)rsize;
-            else s_err = 1;
-        }
-        const uint8_t *src = rec + 5 + nlead + (ok ? inc - tot : 0u);
-
-        const U wmask = ~(U)0 >> shift;
-        U e[4];
-        uint32_t carry = 0;
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            U ek = 0;
-            if (ok && e0 + k < n) {
-                const int m = rb - lk[k];
-                for (int j = 0; j < m; j++) ek |= (U)src[j] << (8 * (lo + j));
-                if constexpr (W == 8) {
-                    if (!exact && rb == 5 && lk[k] == 0)   // szxd_double.cc:244 reads q[3]
-                        ek = (ek & ~((U)0xFF << 56)) | ((U)src[3] << 56);
-                }
-                src += m;
-                ek &= wmask;
-#pragma unroll
-                for (int q = 0; q < 3; q++) {
-                    const int pos = W - 3 + q;
-                    if (pos >= lo && pos < W - lk[k]) {
-                        carry = (carry & ~(0xFFu << (8 * q))) |
-                                ((uint32_t)((ek >> (8 * pos)) & 0xFF) << (8 * q));
-                        carry |= 1u << (24 + q);
-                    }
-                }
-            }
-            e[k] = ek;
-        }
-        uint32_t cinc = carry;
-#pragma unroll
-        for (int d = 1; d < SEG; d <<= 1) {
-            const uint32_t o = __shfl_up(cinc, d, SEG);
-            if (lseg >= d) cinc = lw_combine(o, cinc);
-        }
-        uint32_t cin = __shfl_up(cinc, 1, SEG);
-        if (lseg == 0) cin = 0;
-        U prev = ((U)(cin & 0xFF) << (8 * (W - 3))) | ((U)((cin >> 8) & 0xFF) << (8 * (W - 2))) |
-                 ((U)((cin >> 16) & 0xFF) << (8 * (W - 1)));
-        const T mT = (T)med;
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const U cm = lk[k] == 0 ? (U)0 : (~(U)0 << (8 * (W - lk[k])));
-            const U t = e[k] | (prev & cm);
-            v[k] = from_bits<T>(t << shift) + mT;
-            prev = t;
-        }
-        Store4<T>::store(out, e0, n, v);
     }
     __syncthreads();
 
@@ -351,9 +393,11 @@ static void launch_decode(const uint8_t *s, uint64_t len, uint64_t n, const Deco
                           const uint64_t *index, int32_t *status, void *out, int exact,
                           uint64_t ntiles, hipStream_t st)
 {
-    constexpr int SLABS = (sizeof(T) == 4 ? TILE_VALUES_F32 : TILE_VALUES_F64) / 1024;
+    // 8 values per lane where a block spans at least 8 values (fp64 tiles of
+    // 2048 values take one slab of 256 x 8), else 4
+    constexpr int LV = BS >= 8 ? 8 : 4;
     const uint16_t *bidx = reinterpret_cast<const uint16_t *>(index + 2 * (ntiles + 1));
-    hipLaunchKernelGGL((k_decode_tiles<T, BS, SLABS>), dim3((unsigned)ntiles), dim3(256), 0, st, s,
+    hipLaunchKernelGGL((k_decode_tiles<T, BS, LV>), dim3((unsigned)ntiles), dim3(256), 0, st, s,
                        len, n, h, index, bidx, status, static_cast<T *>(out), exact);
 }
 
