@@ -3,10 +3,12 @@
 // One launch of k_compress turns the whole input into the records of the
 // fastMode-2 stream (szx_float.cc:1059-1139, szx_double.cc:1160-1238):
 //
-//   * a workgroup (256 lanes, 4 wave64s) owns a tile of SLABS x 1024 values;
-//     each lane holds 4 consecutive values (one 128-bit load for fp32, two for
-//     fp64), so a block of BS values is an aligned group of BS/4 lanes and
-//     every per-block reduction / scan is a butterfly inside that group;
+//   * a workgroup (256 lanes, 4 wave64s) owns a tile of 8192 fp32 / 4096
+//     fp64 values (two record-index tiles: a longer tile amortises the
+//     look-back wait over more data); each lane holds LV consecutive values
+//     (8, or 4 for blocks of 4) per slab of 256 x LV values, so a block of BS
+//     values is an aligned group of BS/LV lanes and every per-block reduction
+//     / scan is a DPP or swizzle step inside that group;
 //   * block statistics (min/max -> radius, median, constant test) and the
 //     XOR leading-byte encoding are computed in registers; the records of the
 //     tile are staged in LDS;
@@ -45,6 +47,7 @@ struct CompressParams {
     uint64_t ntiles;
     uint64_t nbit;
     uint64_t nb;
+    uint64_t nidx;        // index tiles
 };
 
 template <typename T> struct SlabLoad;
@@ -296,6 +299,8 @@ __global__ __launch_bounds__(256) void k_compress(CompressParams p)
     constexpr int BPS = SLAB / BS;        // blocks per slab
     constexpr int NBLK = TILE / BS;       // blocks per tile (multiple of 8)
     constexpr int STAGE = TILE * W + NBLK * (5 + BS / 4);
+    constexpr int IT = INDEX_PER_TILE;    // index tiles per tile
+    constexpr int NBLKI = NBLK / IT;      // blocks per index tile
     constexpr int STAGE_DW = (STAGE + 32 + 15) / 16 * 4;
     static_assert(BS >= LV && BS <= 256 && (BS & (BS - 1)) == 0, "power-of-two block size");
     static_assert(NBLK % 8 == 0, "tiles own whole bitmap bytes");
@@ -398,19 +403,26 @@ __global__ __launch_bounds__(256) void k_compress(CompressParams p)
             econst = 0;
             if (tid == 0) atomicExch(p.err, 1);
         }
+        // record index: one pair per index tile (IT per compressor tile),
+        // from the tile's prefix and the scanned word at the index tile's
+        // first block
+        if (p.index && tid < IT) {
+            const uint64_t it = tile * IT + tid;
+            if (it < p.nidx) {
+                const uint32_t pk = s_pack[tid * NBLKI];
+                p.index[2 * it] = ebytes + (pk & 0xFFFFFu);
+                p.index[2 * it + 1] = econst + (pk >> 20);
+            }
+        }
         if (tid == 0) {
             s_excl[0] = ebytes;
             s_excl[1] = econst;
-            if (p.index) {
-                p.index[2 * tile] = ebytes;
-                p.index[2 * tile + 1] = econst;
-            }
             if (tile == p.ntiles - 1) {
                 p.totals[0] = ebytes + tbytes;
                 p.totals[1] = econst + tconst;
                 if (p.index) {
-                    p.index[2 * p.ntiles] = ebytes + tbytes;
-                    p.index[2 * p.ntiles + 1] = econst + tconst;
+                    p.index[2 * p.nidx] = ebytes + tbytes;
+                    p.index[2 * p.nidx + 1] = econst + tconst;
                 }
             }
         }
@@ -468,14 +480,17 @@ __global__ __launch_bounds__(256) void k_compress(CompressParams p)
             p.bitmap[gbyte] = (uint8_t)b;
         }
     }
-    // per-block entries of the record index: the record's start inside the
-    // tile's payload bytes, or the constant block's rank among the tile's
-    // constant blocks (both fit 16 bits: a tile stages under 64 KiB)
+    // per-block entries of the record index: the record's start inside its
+    // index tile's payload bytes, or the constant block's rank among the
+    // index tile's constant blocks (both fit 16 bits)
     if (p.bidx) {
         for (int lb = tid; lb < NBLK; lb += 256) {
             const uint64_t b = tile * NBLK + lb;
-            if (b < p.nb)
-                p.bidx[b] = (uint16_t)(s_state[lb] ? (s_pack[lb] & 0xFFFFFu) : (s_pack[lb] >> 20));
+            if (b < p.nb) {
+                const uint32_t base = s_pack[(lb / NBLKI) * NBLKI];
+                p.bidx[b] = (uint16_t)(s_state[lb] ? ((s_pack[lb] - base) & 0xFFFFFu)
+                                                   : ((s_pack[lb] >> 20) - (base >> 20)));
+            }
         }
     }
 }
@@ -623,8 +638,9 @@ CompressLayout compress_layout(uint64_t n, int w, int bs)
     const uint64_t rec_max = 5 + ceil_div((uint64_t)bs, 4) + (uint64_t)bs * w;
     L.payload_max = L.nb * rec_max;
     L.capacity = L.region + L.payload_max + 16;
-    L.ntiles = ceil_div(n, (uint64_t)tile_values(w));
+    L.ntiles = ceil_div(n, compress_tile_values(w));
     if (L.ntiles == 0) L.ntiles = 1;
+    L.nidx = index_tiles(n, w);
     // workspace: tile words | chunk words | misc (totals, eb) | range parts | bitmap | cmed
     uint64_t off = 0;
     L.ws_tiles = off;   off += L.ntiles * sizeof(unsigned long long);
@@ -642,7 +658,7 @@ static void launch_main(const CompressParams &p, uint64_t ntiles, hipStream_t st
 {
     // 8 values per lane where a block spans at least 8 values, else 4
     constexpr int LV = BS >= 8 ? 8 : 4;
-    constexpr int TILE = sizeof(T) == 4 ? TILE_VALUES_F32 : TILE_VALUES_F64;
+    constexpr int TILE = INDEX_PER_TILE * (sizeof(T) == 4 ? TILE_VALUES_F32 : TILE_VALUES_F64);
     constexpr int SLABS = TILE / (256 * LV);
     hipLaunchKernelGGL((k_compress<T, BS, LV, SLABS>), dim3((unsigned)ntiles), dim3(256), 0, st, p);
 }
@@ -725,7 +741,8 @@ int compress_launch(const void *d_in, uint64_t n, int w, int rel_mode, float abs
     p.totals = reinterpret_cast<uint64_t *>(misc + 16);
     p.err = reinterpret_cast<int *>(misc + 48);
     p.index = d_index;
-    p.bidx = d_index ? reinterpret_cast<uint16_t *>(d_index + 2 * (L.ntiles + 1)) : nullptr;
+    p.bidx = d_index ? reinterpret_cast<uint16_t *>(d_index + 2 * (L.nidx + 1)) : nullptr;
+    p.nidx = L.nidx;
     p.nb = L.nb;
     p.ntiles = L.ntiles;
     p.nbit = L.nbit;

@@ -12,6 +12,10 @@ namespace szxamd {
 constexpr int TILE_VALUES_F32 = 4096;
 constexpr int TILE_VALUES_F64 = 2048;
 inline uint64_t tile_values(int w) { return w == 4 ? TILE_VALUES_F32 : TILE_VALUES_F64; }
+// The compressor encodes INDEX_PER_TILE index tiles per workgroup (longer
+// tiles amortise its look-back wait); its record index stays per index tile.
+constexpr int INDEX_PER_TILE = 2;
+inline uint64_t compress_tile_values(int w) { return INDEX_PER_TILE * tile_values(w); }
 // Record index: per tile {payload offset of its first record, constant
 // blocks before it} plus a closing pair (u64), then per block a u16 -- the
 // record's start inside its tile's payload bytes (non-constant) or its rank
@@ -41,7 +45,8 @@ struct CompressLayout {
     uint64_t region;      // bytes reserved in d_out in front of the payload
     uint64_t payload_max; // largest possible payload
     uint64_t capacity;    // bytes d_out must hold
-    uint64_t ntiles;
+    uint64_t ntiles;      // compressor tiles
+    uint64_t nidx;        // index tiles
     uint64_t ws_tiles, ws_chunks, ws_misc, ws_parts, ws_bitmap, ws_cmed, ws_bytes;
 };
 
