@@ -131,7 +131,7 @@ size_t szx_amd_compress_capacity(size_t n, int dataType, int blockSize);
 /* Workspace bytes for szx_amd_compress_dev. */
 size_t szx_amd_compress_workspace(size_t n, int dataType, int blockSize);
 
-/* uint64_t words of a record index for n values: per compressor tile (4096
+/* uint64_t words of a record index for n values: per index tile (4096
  * fp32 / 2048 fp64 values) the payload offset of the tile's first record and
  * the number of constant blocks before the tile, plus a closing pair; then
  * per block a uint16_t -- the record's start inside its tile's payload bytes,
@@ -264,6 +264,74 @@ int szx_amd_mlp_grad_dev(const float *d_params, const float *d_X, const float *d
 /* One bias-corrected Adam step (step >= 1) on the FP32 parameters. */
 int szx_amd_mlp_adam_dev(float *d_params, const float *d_grad, float *d_m, float *d_v, float lr,
                          float beta1, float beta2, float eps, int step, void *hipStream);
+
+/* ---- Batched launch -------------------------------------------------------
+ * No counterpart in the reference, which compresses one array per call
+ * (SZx_fast_compress_args, szx.cc:223-262).  Many fields of uneven sizes,
+ * fp32 and fp64 mixed, each with its own ABS bound or REL ratio, go through
+ * one persistent launch per direction.  Each field's stream is byte-identical
+ * to the stream SZx_fast_compress_args writes for that field alone, and is
+ * placed inside the field's d_out as szx_amd_compress_dev places it. */
+typedef struct szx_amd_field {
+    const void *d_in;       /* n values, 16-byte aligned */
+    size_t n;               /* >= 1 */
+    int dataType;           /* SZx_FLOAT or SZx_DOUBLE */
+    int errBoundMode;       /* ABS or REL */
+    float absErrBound;      /* ABS */
+    float relBoundRatio;    /* REL */
+    void *d_out;            /* szx_amd_compress_capacity(n, dataType, blockSize) bytes */
+    void *d_dec;            /* n values (16-byte aligned) for szx_amd_batch_decompress, or
+                               NULL (for every field) when the batch is never decompressed */
+} szx_amd_field;
+
+typedef struct szx_amd_batch szx_amd_batch;   /* host-side plan */
+
+#define SZX_AMD_BATCH_PERSISTENT 0   /* one persistent launch over every field */
+#define SZX_AMD_BATCH_PER_FIELD 1    /* the single-field launches, field by field
+                                        (for comparison) */
+
+/* Device workspace bytes of a batch (0 for invalid fields). */
+size_t szx_amd_batch_workspace(const szx_amd_field *fields, int nfields, int blockSize);
+
+/* Lay out d_work (256-byte aligned, szx_amd_batch_workspace bytes) and upload
+ * the field tables; synchronizes `stream`.  The plan keeps the pointers: the
+ * fields' buffers and d_work must outlive it.  NULL on error. */
+szx_amd_batch *szx_amd_batch_create(const szx_amd_field *fields, int nfields, int blockSize,
+                                    void *d_work, size_t workBytes, void *hipStream);
+void szx_amd_batch_destroy(szx_amd_batch *batch);
+
+/* Compress every field.  PERSISTENT only enqueues kernels: nothing is
+ * allocated, copied or waited for, so the call can be captured in a hipGraph
+ * and replayed.  Result words of field i land at szx_amd_batch_results_dev()
+ * + 4 i, with the meaning of szx_amd_compress_dev's d_result. */
+int szx_amd_batch_compress(szx_amd_batch *batch, int mode, void *hipStream);
+
+/* Device arrays of the plan: nfields x 4 result words, nfields decode
+ * statuses (int32, as szx_amd_decompress_dev's d_status), and field i's
+ * record index (szx_amd_index_words words). */
+uint64_t *szx_amd_batch_results_dev(szx_amd_batch *batch);
+int32_t *szx_amd_batch_status_dev(szx_amd_batch *batch);
+uint64_t *szx_amd_batch_index_dev(szx_amd_batch *batch, int field);
+
+/* Copy the result words to h_results (nfields x 4, or NULL) and keep them in
+ * the plan for PER_FIELD decompression; synchronizes `stream`. */
+int szx_amd_batch_results(szx_amd_batch *batch, uint64_t *h_results, void *hipStream);
+
+/* Decompress every field's stream (found through its result words) into its
+ * d_dec.  flags as szx_amd_decompress_dev.  PERSISTENT only enqueues kernels;
+ * PER_FIELD needs szx_amd_batch_results() after the compress. */
+int szx_amd_batch_decompress(szx_amd_batch *batch, int flags, int mode, void *hipStream);
+
+/* Tile shape of the persistent compress: values per lane (4 or 8; 8 needs
+ * blockSize >= 8) and index tiles per workgroup tile (1 or 2, i.e. 4096 or
+ * 8192 fp32 values).  szx_amd_batch_tune times every shape on this batch
+ * (reps timed passes each, after one untimed pass; synchronizes), keeps the
+ * fastest and reports it with its mean milliseconds per compress.  The
+ * environment variable SZX_AMD_BATCH_SHAPE="lanes,tiles" sets the initial
+ * shape of new plans. */
+int szx_amd_batch_set_shape(szx_amd_batch *batch, int valuesPerLane, int tilesPerGroup);
+int szx_amd_batch_tune(szx_amd_batch *batch, int reps, int *valuesPerLane, int *tilesPerGroup,
+                       float *ms, void *hipStream);
 
 /* Kernel timing for benchmarks: after szx_amd_timing_begin(), this thread's
  * launches bracket their main kernels with HIP events on the launch stream;

@@ -45,13 +45,25 @@ EXPORTS = (
     "szx_amd_rate_stats_workspace", "szx_amd_rate_stats_dev", "szx_amd_rate_inputs_dev",
     "szx_amd_mlp_forward_dev", "szx_amd_mlp_grad_workspace", "szx_amd_mlp_grad_dev",
     "szx_amd_mlp_adam_dev", "szx_amd_rate_octave_workspace", "szx_amd_rate_octave_dev",
+    "szx_amd_batch_workspace", "szx_amd_batch_create", "szx_amd_batch_destroy",
+    "szx_amd_batch_compress", "szx_amd_batch_results_dev", "szx_amd_batch_status_dev",
+    "szx_amd_batch_index_dev", "szx_amd_batch_results", "szx_amd_batch_decompress",
+    "szx_amd_batch_set_shape", "szx_amd_batch_tune",
 )
+BATCH_PERSISTENT, BATCH_PER_FIELD = 0, 1
 OCTAVE_BINS = 512
 RATE_BISECT, RATE_LEARNED = 0, 1
 RATE_MODEL_PARAMS = 5825
 RATE_MODEL_INPUTS = 24
 RATE_CHUNK = 524288
 RATE_MODEL_PATH = os.path.join(HERE, "rate_model_fp32.bin")
+
+
+class Field(C.Structure):
+    """szx_amd_field: one field of a batched launch (device pointers)."""
+    _fields_ = [("d_in", C.c_void_p), ("n", C.c_size_t), ("dataType", C.c_int),
+                ("errBoundMode", C.c_int), ("absErrBound", C.c_float),
+                ("relBoundRatio", C.c_float), ("d_out", C.c_void_p), ("d_dec", C.c_void_p)]
 
 
 class RateInfo(C.Structure):
@@ -145,6 +157,29 @@ def lib() -> C.CDLL:
     L.szx_amd_mlp_grad_dev.argtypes = [vp, vp, vp, sz, vp, vp, vp, sz, vp]
     L.szx_amd_mlp_adam_dev.restype = i32
     L.szx_amd_mlp_adam_dev.argtypes = [vp, vp, vp, vp, f32, f32, f32, f32, i32, vp]
+    fp = C.POINTER(Field)
+    L.szx_amd_batch_workspace.restype = sz
+    L.szx_amd_batch_workspace.argtypes = [fp, i32, i32]
+    L.szx_amd_batch_create.restype = vp
+    L.szx_amd_batch_create.argtypes = [fp, i32, i32, vp, sz, vp]
+    L.szx_amd_batch_destroy.restype = None
+    L.szx_amd_batch_destroy.argtypes = [vp]
+    L.szx_amd_batch_compress.restype = i32
+    L.szx_amd_batch_compress.argtypes = [vp, i32, vp]
+    L.szx_amd_batch_results_dev.restype = vp
+    L.szx_amd_batch_results_dev.argtypes = [vp]
+    L.szx_amd_batch_status_dev.restype = vp
+    L.szx_amd_batch_status_dev.argtypes = [vp]
+    L.szx_amd_batch_index_dev.restype = vp
+    L.szx_amd_batch_index_dev.argtypes = [vp, i32]
+    L.szx_amd_batch_results.restype = i32
+    L.szx_amd_batch_results.argtypes = [vp, vp, vp]
+    L.szx_amd_batch_decompress.restype = i32
+    L.szx_amd_batch_decompress.argtypes = [vp, i32, i32, vp]
+    L.szx_amd_batch_set_shape.restype = i32
+    L.szx_amd_batch_set_shape.argtypes = [vp, i32, i32]
+    L.szx_amd_batch_tune.restype = i32
+    L.szx_amd_batch_tune.argtypes = [vp, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(f32), vp]
     _lib = L
     return L
 
@@ -347,6 +382,95 @@ class DeviceDecompressor:
 
     def check(self) -> int:
         return int(self.status.item())
+
+
+class BatchCodec:
+    """Many device fields compressed / decompressed by one persistent launch
+    each way (szx_amd_batch_*).  fields: sequence of (tensor, errBoundMode,
+    bound), where bound is the ABS bound or the REL ratio.  Output buffers,
+    decode buffers and the workspace are allocated here once; compress() and
+    decompress() only enqueue kernels (capturable in a CUDA/HIP graph)."""
+
+    def __init__(self, fields, blockSize: int = 128, decode: bool = True, device="cuda"):
+        torch = _torch()
+        L = lib()
+        self.bs = blockSize
+        self.inputs = [f[0] for f in fields]
+        nf = len(fields)
+        self.nf = nf
+        arr = (Field * nf)()
+        self.outs, self.decs = [], []
+        for i, (x, mode, bound) in enumerate(fields):
+            code = _dtype_code(np.float32 if x.dtype == torch.float32 else np.float64)
+            cap = L.szx_amd_compress_capacity(x.numel(), code, blockSize)
+            if not cap:
+                raise SZxError(f"unsupported parameters (blockSize={blockSize})")
+            out = torch.empty(cap, dtype=torch.uint8, device=device)
+            dec = torch.empty_like(x) if decode else None
+            self.outs.append(out)
+            self.decs.append(dec)
+            arr[i] = Field(x.data_ptr(), x.numel(), code, mode,
+                           bound if mode == ABS else 0.0, bound if mode == REL else 0.0,
+                           out.data_ptr(), dec.data_ptr() if dec is not None else None)
+        self._fields = arr
+        self.wsbytes = L.szx_amd_batch_workspace(arr, nf, blockSize)
+        if not self.wsbytes:
+            raise SZxError(last_error())
+        self.ws = torch.empty(self.wsbytes, dtype=torch.uint8, device=device)
+        self.handle = L.szx_amd_batch_create(arr, nf, blockSize, self.ws.data_ptr(), self.wsbytes,
+                                             _stream_handle(None))
+        if not self.handle:
+            raise SZxError(last_error())
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib().szx_amd_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def compress(self, mode: int = BATCH_PERSISTENT, stream=None) -> None:
+        _check(lib().szx_amd_batch_compress(self.handle, mode, _stream_handle(stream)))
+
+    def results(self, stream=None) -> np.ndarray:
+        """(nf, 4) result words -- stream offset, size, bound bits, error; syncs."""
+        out = np.zeros((self.nf, 4), np.uint64)
+        _check(lib().szx_amd_batch_results(self.handle, out.ctypes.data, _stream_handle(stream)))
+        return out
+
+    def decompress(self, flags: int = DECODE_REFERENCE, mode: int = BATCH_PERSISTENT,
+                   stream=None) -> None:
+        _check(lib().szx_amd_batch_decompress(self.handle, flags, mode, _stream_handle(stream)))
+
+    def status(self):
+        """Device int32 tensor view of the nf decode statuses."""
+        torch = _torch()
+        st = torch.empty(self.nf, dtype=torch.int32, device=self.ws.device)
+        # copy out of the workspace (the plan's status array lives there)
+        base = lib().szx_amd_batch_status_dev(self.handle) - self.ws.data_ptr()
+        st.copy_(self.ws[base:base + 4 * self.nf].view(torch.int32))
+        return st
+
+    def stream(self, i: int, res=None):
+        """Device uint8 view of field i's stream (res: results() rows)."""
+        r = self.results() if res is None else res
+        off, size = int(r[i, 0]), int(r[i, 1])
+        return self.outs[i][off:off + size]
+
+    def set_shape(self, valuesPerLane: int, tilesPerGroup: int) -> None:
+        _check(lib().szx_amd_batch_set_shape(self.handle, valuesPerLane, tilesPerGroup))
+
+    def tune(self, reps: int = 3, stream=None):
+        """Time every tile shape on this batch and keep the fastest:
+        returns (valuesPerLane, tilesPerGroup, ms per compress)."""
+        lv, ipt, ms = C.c_int(0), C.c_int(0), C.c_float(0.0)
+        _check(lib().szx_amd_batch_tune(self.handle, reps, C.byref(lv), C.byref(ipt), C.byref(ms),
+                                        _stream_handle(stream)))
+        return lv.value, ipt.value, ms.value
 
 
 class DeviceFixedRateBound:

@@ -286,6 +286,17 @@ void *host_decompress(const unsigned char *bytes, size_t len, size_t n, int w, i
 }  // namespace
 
 namespace szxamd {
+void api_error(const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+}
+void api_clear_error() { g_err.clear(); }
+
 void timing_mark_begin(int kind, hipStream_t st)
 {
     TimingState &t = g_tim;

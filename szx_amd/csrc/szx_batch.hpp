@@ -1,0 +1,102 @@
+// szx_batch.hpp -- parameter blocks of the compressor (one field) and of the
+// batched launch (many fields), shared by the kernels and the host planner.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace szxamd {
+
+struct ChunkState;   // look-back chunk words (szx_common.hpp)
+
+// One field's compressor parameters (device pointers).
+struct CompressParams {
+    const void *x;
+    uint64_t n;
+    float eb;             // absolute bound (ABS)
+    const float *eb_dev;  // REL: bound computed on the device, else nullptr
+    uint8_t *payload;     // first record byte
+    uint8_t *bitmap;      // scratch bitmap, nbit bytes
+    uint32_t *cmed;       // scratch constant medians, big-endian words
+    unsigned long long *tiles;   // per-tile look-back words
+    ChunkState *chunks;          // per-64-tile look-back words
+    uint64_t *totals;     // [0] payload bytes, [1] constant blocks
+    uint64_t *index;      // optional record index: per tile {payload offset, constants before}
+    uint16_t *bidx;       // its per-block part (start in tile, or constant rank), or nullptr
+    int *err;             // set when a look-back wait overran its bound
+    uint64_t ntiles;
+    uint64_t nbit;
+    uint64_t nb;
+    uint64_t nidx;        // index tiles
+};
+
+// One field's header / bitmap / median assembly (device pointers).
+struct AssembleParams {
+    uint8_t *out;
+    uint64_t region;      // bytes reserved in front of the payload
+    const uint8_t *bitmap;
+    const uint32_t *cmed;
+    const uint64_t *totals;
+    uint64_t nbit;
+    int bs;
+    const float *eb_dev;
+    float eb;
+    uint64_t *result;     // [0] stream offset, [1] stream size, [2] bound bits, [3] error
+    const int *err;
+};
+
+// Values per REL range item of the batched range pass.
+constexpr uint64_t RANGE_ITEM = 1ull << 20;
+// Words per block of the batched clear; bytes per thread of the batched
+// header/bitmap/median assembly.
+constexpr uint64_t ZERO_ITEM = 1024;
+constexpr uint64_t ASSEMBLE_BYTES_PER_THREAD = 8;
+constexpr uint64_t ASSEMBLE_MAX_BLOCKS = 512;
+
+// One field of a batched launch.  The look-back state is sized for the
+// smaller tiles (one index tile per compressor tile); ntiles[k] is the
+// compressor tile count with k + 1 index tiles per tile.
+struct BatchField {
+    CompressParams p;
+    AssembleParams a;
+    ChunkState *chunks;
+    uint64_t ntiles[2];
+    uint64_t zwords;   // words cleared before a compress, from p.tiles (look-back words, totals, error)
+    float relv;   // REL ratio (rel != 0)
+    int rel;
+    int w;        // 4 or 8
+};
+
+// Batched kernels (szx_compress.hip), all reading device copies of the
+// field table.  tbase: first global tile of each field (nf + 1 entries) for
+// the chosen tile shape; rbase: first range item of each field (nf + 1).
+void batch_range_launch(const BatchField *bf, const uint64_t *rbase, int nf, uint64_t nitems, double *part,
+                        hipStream_t st);
+int batch_compress_launch(int bs, int lv, int ipt, const BatchField *bf, const uint64_t *tbase, int nf,
+                          uint64_t total, hipStream_t st);
+// zbase / abase: first block of each field in the flattened clear / assembly
+// grids (nf + 1 entries; the last is the grid size).
+void batch_zero_launch(const BatchField *bf, const uint64_t *zbase, int nf, uint64_t nitems, hipStream_t st);
+void batch_assemble_launch(const BatchField *bf, const uint64_t *abase, int nf, uint64_t nitems, hipStream_t st);
+
+struct DecodeHeader;
+
+// One field of a batched decode (the stream sits at out + result[0]).
+struct BatchDecodeField {
+    const uint8_t *out;
+    const uint64_t *result;
+    uint64_t n;
+    const uint64_t *index;   // record index written by the batched compress
+    void *dst;
+    int32_t *status;
+    DecodeHeader *h;         // workspace
+    uint64_t ntidx;          // index tiles
+    int w;
+};
+
+// Batched decode (szx_decompress.hip): header checks, then every index tile
+// of every field.  tbase: first global index tile of each field (nf + 1).
+int batch_decode_launch(int bs, const BatchDecodeField *f, const uint64_t *tbase, int nf, uint64_t total,
+                        int exact, hipStream_t st);
+
+}  // namespace szxamd

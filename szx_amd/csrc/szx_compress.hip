@@ -25,30 +25,12 @@
 // buffer, which is returned with the size.  This keeps HBM traffic at one read
 // of the input plus one write of the stream.
 #include "szx_common.hpp"
+#include "szx_batch.hpp"
 #include "szx_internal.hpp"
 #include "szx_wave.hpp"
 
 namespace szxamd {
 
-struct CompressParams {
-    const void *x;
-    uint64_t n;
-    float eb;             // absolute bound (ABS)
-    const float *eb_dev;  // REL: bound computed on the device, else nullptr
-    uint8_t *payload;     // first record byte
-    uint8_t *bitmap;      // scratch bitmap, nbit bytes
-    uint32_t *cmed;       // scratch constant medians, big-endian words
-    unsigned long long *tiles;   // per-tile look-back words
-    ChunkState *chunks;          // per-64-tile look-back words
-    uint64_t *totals;     // [0] payload bytes, [1] constant blocks
-    uint64_t *index;      // optional record index: per tile {payload offset, constants before}
-    uint16_t *bidx;       // its per-block part (start in tile, or constant rank), or nullptr
-    int *err;             // set when a look-back wait overran its bound
-    uint64_t ntiles;
-    uint64_t nbit;
-    uint64_t nb;
-    uint64_t nidx;        // index tiles
-};
 
 template <typename T> struct SlabLoad;
 template <> struct SlabLoad<float> {
@@ -287,37 +269,60 @@ __device__ inline void stage_bytes(uint8_t *stage, uint32_t pos, U t, int m)
         if (j < m) stage[pos + j] = (uint8_t)(t >> (8 * j));
 }
 
-template <typename T, int BS, int LV, int SLABS>
-__global__ __launch_bounds__(256) void k_compress(CompressParams p)
+// Shape of a compressor tile: T, block size, values per lane, slabs.
+template <typename T, int BS, int LV, int SLABS> struct TileShape {
+    static constexpr int W = Word<T>::W;
+    static constexpr int SLAB = 256 * LV;           // values per slab
+    static constexpr int TILE = SLAB * SLABS;       // values per tile
+    static constexpr int NBLK = TILE / BS;          // blocks per tile (multiple of 8)
+    static constexpr int STAGE = TILE * W + NBLK * (5 + BS / 4);   // largest record bytes
+    static constexpr int STAGE_DW = (STAGE + 32 + 15) / 16 * 4;
+};
+
+// The LDS one tile needs (a kernel declares it, or a union of several).
+template <typename T, int BS, int LV, int SLABS> struct CompressLds {
+    using S = TileShape<T, BS, LV, SLABS>;
+    __attribute__((aligned(16))) uint32_t stage32[S::STAGE_DW];   // staged records
+    uint32_t pack[S::NBLK];       // record bytes | constant<<20 -> exclusive prefix
+    float med[S::NBLK];           // medians (constant blocks)
+    uint8_t state[S::NBLK];       // 1 = non-constant (bitmap bit)
+    unsigned long long excl[2];   // the tile's exclusive (bytes, constants)
+    uint32_t total;               // the tile's (bytes | constants << 20)
+};
+
+// Encode tile `tile` of the field described by p (all 256 lanes call it).
+template <typename T, int BS, int LV, int SLABS, int IPT = INDEX_PER_TILE>
+__device__ inline void compress_tile(const CompressParams &p, uint64_t tile,
+                                     CompressLds<T, BS, LV, SLABS> &lds)
 {
     using U = typename Word<T>::U;
+    using S = TileShape<T, BS, LV, SLABS>;
     constexpr int W = Word<T>::W;
     constexpr int SEG = BS / LV;          // lanes per block
     constexpr int NLB = LV / 4;           // lead-code bytes per lane
-    constexpr int SLAB = 256 * LV;        // values per slab
-    constexpr int TILE = SLAB * SLABS;
+    constexpr int SLAB = S::SLAB;
+    constexpr int TILE = S::TILE;
     constexpr int BPS = SLAB / BS;        // blocks per slab
-    constexpr int NBLK = TILE / BS;       // blocks per tile (multiple of 8)
-    constexpr int STAGE = TILE * W + NBLK * (5 + BS / 4);
-    constexpr int IT = INDEX_PER_TILE;    // index tiles per tile
+    constexpr int NBLK = S::NBLK;
+    constexpr int STAGE = S::STAGE;
+    constexpr int IT = IPT;               // index tiles per tile
     constexpr int NBLKI = NBLK / IT;      // blocks per index tile
-    constexpr int STAGE_DW = (STAGE + 32 + 15) / 16 * 4;
     static_assert(BS >= LV && BS <= 256 && (BS & (BS - 1)) == 0, "power-of-two block size");
     static_assert(NBLK % 8 == 0, "tiles own whole bitmap bytes");
+    static_assert(NBLKI % 8 == 0, "index tiles own whole bitmap bytes");
     static_assert(STAGE < (1 << 16), "record starts fit the 16-bit block index");
 
-    __shared__ uint32_t s_pack[NBLK];     // record bytes | constant<<20 -> exclusive prefix
-    __shared__ uint8_t s_state[NBLK];     // 1 = non-constant (bitmap bit)
-    __shared__ float s_med[NBLK];         // medians (constant blocks)
-    __shared__ __attribute__((aligned(16))) uint32_t s_stage32[STAGE_DW];
-    __shared__ unsigned long long s_excl[2];
-    __shared__ uint32_t s_total;
+    uint32_t *s_pack = lds.pack;
+    uint8_t *s_state = lds.state;
+    float *s_med = lds.med;
+    uint32_t *s_stage32 = lds.stage32;
+    unsigned long long *s_excl = lds.excl;
+    uint32_t &s_total = lds.total;
     uint8_t *s_stage = reinterpret_cast<uint8_t *>(s_stage32);
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int lseg = tid & (SEG - 1);
-    const uint64_t tile = blockIdx.x;
     const float eb = p.eb_dev ? *p.eb_dev : p.eb;
     const int reqexp = req_expo(eb);
     const uint64_t n = p.n;
@@ -495,6 +500,13 @@ __global__ __launch_bounds__(256) void k_compress(CompressParams p)
     }
 }
 
+template <typename T, int BS, int LV, int SLABS>
+__global__ __launch_bounds__(256) void k_compress(CompressParams p)
+{
+    __shared__ CompressLds<T, BS, LV, SLABS> lds;
+    compress_tile<T, BS, LV, SLABS>(p, blockIdx.x, lds);
+}
+
 // Zero the look-back state and totals (a kernel, not a memset, so the
 // whole sequence can be captured in a HIP graph).
 __global__ void k_zero_u64(unsigned long long *p, uint64_t nwords)
@@ -505,19 +517,6 @@ __global__ void k_zero_u64(unsigned long long *p, uint64_t nwords)
 }
 
 // Header, bitmap and constant medians in front of the payload.
-struct AssembleParams {
-    uint8_t *out;
-    uint64_t region;      // bytes reserved in front of the payload
-    const uint8_t *bitmap;
-    const uint32_t *cmed;
-    const uint64_t *totals;
-    uint64_t nbit;
-    int bs;
-    const float *eb_dev;
-    float eb;
-    uint64_t *result;     // [0] stream offset, [1] stream size, [2] bound bits, [3] error
-    const int *err;
-};
 
 __global__ void k_assemble(AssembleParams a)
 {
@@ -624,6 +623,159 @@ __global__ __launch_bounds__(256) void k_range_final(const T *x, const T *part, 
 }
 
 // ---------------------------------------------------------------------------
+// Batched compress: many fields (fp32 and fp64 mixed, any sizes) in one
+// persistent launch.  Every field keeps its own look-back state; the
+// compressor tiles of all fields form one global list (field by field), which
+// the workgroups walk round-robin.  A tile waits only on earlier tiles of its
+// own field, which have lower global numbers, so with every workgroup
+// resident the walk always progresses.
+// ---------------------------------------------------------------------------
+
+// The field owning flattened item `item`: the last f with base[f] <= item
+// (base has nf + 1 entries; fields with no items are skipped over).
+__device__ inline int field_of(const uint64_t *base, int nf, uint64_t item)
+{
+    int lo = 0, hi = nf - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (base[mid] <= item) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// REL range over the batch: items of RANGE_ITEM values, field by field.
+template <typename T>
+__device__ inline void range_item(const T *x, uint64_t a, uint64_t b, double *slot)
+{
+    T mn = pos_inf<T>(), mx = -pos_inf<T>();
+    for (uint64_t i = a + threadIdx.x; i < b; i += blockDim.x) {
+        const T v = x[i];
+        mn = v < mn ? v : mn;
+        mx = v > mx ? v : mx;
+    }
+    for (int m = 1; m < 64; m <<= 1) {
+        T o = __shfl_xor(mn, m, 64);
+        mn = o < mn ? o : mn;
+        o = __shfl_xor(mx, m, 64);
+        mx = o > mx ? o : mx;
+    }
+    __shared__ T sm[2][4];
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        sm[0][w] = mn;
+        sm[1][w] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < 4; k++) {
+            mn = sm[0][k] < mn ? sm[0][k] : mn;
+            mx = sm[1][k] > mx ? sm[1][k] : mx;
+        }
+        slot[0] = (double)mn;   // exact for fp32 and fp64
+        slot[1] = (double)mx;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_batch_range(const BatchField *bf, const uint64_t *rbase, int nf,
+                                                     double *part)
+{
+    const uint64_t item = blockIdx.x;
+    const int fi = field_of(rbase, nf, item);
+    const BatchField &f = bf[fi];
+    const uint64_t k = item - rbase[fi];
+    const uint64_t a = k * RANGE_ITEM, b = a + RANGE_ITEM < f.p.n ? a + RANGE_ITEM : f.p.n;
+    if (f.w == 4) range_item(static_cast<const float *>(f.p.x), a, b, part + 2 * item);
+    else range_item(static_cast<const double *>(f.p.x), a, b, part + 2 * item);
+}
+
+// Fold a REL field's items into its bound (the reference's rounding).
+__global__ void k_batch_range_final(BatchField *bf, const uint64_t *rbase, int nf, const double *part)
+{
+    const int fi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (fi >= nf) return;
+    const BatchField &f = bf[fi];
+    if (!f.rel) return;
+    double mn = __builtin_inf(), mx = -__builtin_inf();
+    for (uint64_t i = rbase[fi]; i < rbase[fi + 1]; i++) {
+        mn = part[2 * i] < mn ? part[2 * i] : mn;
+        mx = part[2 * i + 1] > mx ? part[2 * i + 1] : mx;
+    }
+    float *eb = const_cast<float *>(f.p.eb_dev);
+    if (f.w == 4) {
+        const float first = static_cast<const float *>(f.p.x)[0];
+        float lo = (float)mn, hi = (float)mx;
+        if (first != first) lo = hi = first;
+        *eb = (float)((hi - lo) * f.relv);                // fl32(range * rel)
+    } else {
+        const double first = static_cast<const double *>(f.p.x)[0];
+        double lo = mn, hi = mx;
+        if (first != first) lo = hi = first;
+        *eb = (float)((hi - lo) * (double)f.relv);        // (float)(range * (double)rel)
+    }
+}
+
+template <int BS, int LV, int IPT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_compress_batch(const BatchField *bf, const uint64_t *tbase, int nf,
+                                                        uint64_t total)
+{
+    constexpr int SF = IPT * TILE_VALUES_F32 / (256 * LV);
+    constexpr int SD = IPT * TILE_VALUES_F64 / (256 * LV);
+    __shared__ union Lds {
+        CompressLds<float, BS, LV, SF> f;
+        CompressLds<double, BS, LV, SD> d;
+    } lds;
+    int f = 0;
+    for (uint64_t g = blockIdx.x; g < total; g += gridDim.x) {
+        while (f + 1 < nf && tbase[f + 1] <= g) f++;   // fields in tile order: f only grows
+        CompressParams p = bf[f].p;
+        p.ntiles = bf[f].ntiles[IPT - 1];
+        p.chunks = bf[f].chunks;
+        if (bf[f].w == 4) compress_tile<float, BS, LV, SF, IPT>(p, g - tbase[f], lds.f);
+        else compress_tile<double, BS, LV, SD, IPT>(p, g - tbase[f], lds.d);
+        __syncthreads();   // the tile's LDS is reused
+    }
+}
+
+// Clear every field's look-back words, totals and error flag (one block per
+// ZERO_ITEM words, flattened over the fields).
+__global__ __launch_bounds__(256) void k_zero_batch(const BatchField *bf, const uint64_t *zbase, int nf)
+{
+    const int f = field_of(zbase, nf, blockIdx.x);
+    const uint64_t a = (blockIdx.x - zbase[f]) * ZERO_ITEM;
+    const uint64_t b = a + ZERO_ITEM < bf[f].zwords ? a + ZERO_ITEM : bf[f].zwords;
+    unsigned long long *z = bf[f].p.tiles;
+    for (uint64_t i = a + threadIdx.x; i < b; i += blockDim.x) z[i] = 0;
+}
+
+// Header, bitmap and medians of every field; abase gives each field's
+// first block of this flattened grid.
+__global__ __launch_bounds__(256) void k_assemble_batch(const BatchField *bf, const uint64_t *abase, int nf)
+{
+    const int f = field_of(abase, nf, blockIdx.x);
+    const AssembleParams &a = bf[f].a;
+    const uint64_t nconst = a.totals[1];
+    const uint64_t pbytes = a.totals[0];
+    const uint64_t front = HEADER_BYTES + a.nbit + 4 * nconst;
+    const uint64_t start = a.region - front;
+    uint8_t *s = a.out + start;
+    const uint64_t gid = (blockIdx.x - abase[f]) * (uint64_t)blockDim.x + threadIdx.x;
+    const uint64_t nthr = (abase[f + 1] - abase[f]) * blockDim.x;
+    if (gid == 0) {
+        s[0] = 1; s[1] = 1; s[2] = 1; s[3] = 0;        // szx_float.cc:1091-1094
+        s[4] = (uint8_t)a.bs;                           // (u8)blockSize, :1095
+        for (int k = 0; k < 8; k++) s[5 + k] = (uint8_t)(nconst >> (56 - 8 * k));
+        a.result[0] = start;
+        a.result[1] = front + pbytes;
+        a.result[2] = f32_bits(a.eb_dev ? *a.eb_dev : a.eb);
+        a.result[3] = (uint64_t)*a.err;
+    }
+    for (uint64_t i = gid; i < a.nbit; i += nthr) s[HEADER_BYTES + i] = a.bitmap[i];
+    const uint8_t *cm = reinterpret_cast<const uint8_t *>(a.cmed);
+    for (uint64_t i = gid; i < 4 * nconst; i += nthr) s[HEADER_BYTES + a.nbit + i] = cm[i];
+}
+
+// ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
 
@@ -641,10 +793,13 @@ CompressLayout compress_layout(uint64_t n, int w, int bs)
     L.ntiles = ceil_div(n, compress_tile_values(w));
     if (L.ntiles == 0) L.ntiles = 1;
     L.nidx = index_tiles(n, w);
-    // workspace: tile words | chunk words | misc (totals, eb) | range parts | bitmap | cmed
+    // workspace: tile words | chunk words | misc (totals, eb) | range parts | bitmap | cmed.
+    // The look-back words are sized for one index tile per compressor tile,
+    // the smallest tile any launch shape uses (the batched launch shares this
+    // layout).
     uint64_t off = 0;
-    L.ws_tiles = off;   off += L.ntiles * sizeof(unsigned long long);
-    L.ws_chunks = off;  off += ceil_div(L.ntiles, LB_CHUNK) * sizeof(ChunkState);
+    L.ws_tiles = off;   off += L.nidx * sizeof(unsigned long long);
+    L.ws_chunks = off;  off += ceil_div(L.nidx, LB_CHUNK) * sizeof(ChunkState);
     L.ws_misc = off;    off += 64 * sizeof(uint64_t);   // totals at +128 B, eb at +256 B, error at +384 B
     L.ws_parts = off;   off += 2 * kRangeParts * sizeof(double);
     L.ws_bitmap = off;  off += (L.nbit + 15) & ~(uint64_t)15;
@@ -769,6 +924,79 @@ int compress_launch(const void *d_in, uint64_t n, int w, int rel_mode, float abs
     if (blocks > 2048) blocks = 2048;
     hipLaunchKernelGGL(k_assemble, dim3((unsigned)blocks), dim3(256), 0, st, a);
     return 0;
+}
+
+// ---------------------------------------------------------------------------
+// batched launchers
+// ---------------------------------------------------------------------------
+
+void batch_range_launch(const BatchField *bf, const uint64_t *rbase, int nf, uint64_t nitems, double *part,
+                        hipStream_t st)
+{
+    if (nitems) hipLaunchKernelGGL(k_batch_range, dim3((unsigned)nitems), dim3(256), 0, st, bf, rbase, nf, part);
+    hipLaunchKernelGGL(k_batch_range_final, dim3((nf + 255) / 256), dim3(256), 0, st,
+                       const_cast<BatchField *>(bf), rbase, nf, part);
+}
+
+// Persistent grid: every workgroup resident at once (the round-robin walk
+// relies on it), at most one per tile.
+template <int BS, int LV, int IPT>
+static void launch_batch(const BatchField *bf, const uint64_t *tbase, int nf, uint64_t total, hipStream_t st)
+{
+    auto kern = k_compress_batch<BS, LV, IPT>;
+    static int resident = 0;   // per instantiation; the device is fixed for a process
+    if (!resident) {
+        int dev = 0, per_cu = 0, cus = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0);
+        resident = per_cu > 0 && cus > 0 ? per_cu * cus : 256;
+    }
+    const uint64_t grid = total < (uint64_t)resident ? total : (uint64_t)resident;
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(256), 0, st, bf, tbase, nf, total);
+}
+
+template <int BS>
+static int batch_shape(int lv, int ipt, const BatchField *bf, const uint64_t *tbase, int nf, uint64_t total,
+                       hipStream_t st)
+{
+    if (lv == 8 && BS >= 8) {
+        if (ipt == 2) launch_batch<BS, (BS >= 8 ? 8 : 4), 2>(bf, tbase, nf, total, st);
+        else launch_batch<BS, (BS >= 8 ? 8 : 4), 1>(bf, tbase, nf, total, st);
+        return 0;
+    }
+    if (lv == 4) {
+        if (ipt == 2) launch_batch<BS, 4, 2>(bf, tbase, nf, total, st);
+        else launch_batch<BS, 4, 1>(bf, tbase, nf, total, st);
+        return 0;
+    }
+    return -1;
+}
+
+int batch_compress_launch(int bs, int lv, int ipt, const BatchField *bf, const uint64_t *tbase, int nf,
+                          uint64_t total, hipStream_t st)
+{
+    if (ipt != 1 && ipt != 2) return -1;
+    switch (bs) {
+    case 4: return batch_shape<4>(lv, ipt, bf, tbase, nf, total, st);
+    case 8: return batch_shape<8>(lv, ipt, bf, tbase, nf, total, st);
+    case 16: return batch_shape<16>(lv, ipt, bf, tbase, nf, total, st);
+    case 32: return batch_shape<32>(lv, ipt, bf, tbase, nf, total, st);
+    case 64: return batch_shape<64>(lv, ipt, bf, tbase, nf, total, st);
+    case 128: return batch_shape<128>(lv, ipt, bf, tbase, nf, total, st);
+    case 256: return batch_shape<256>(lv, ipt, bf, tbase, nf, total, st);
+    default: return -1;
+    }
+}
+
+void batch_zero_launch(const BatchField *bf, const uint64_t *zbase, int nf, uint64_t nitems, hipStream_t st)
+{
+    if (nitems) hipLaunchKernelGGL(k_zero_batch, dim3((unsigned)nitems), dim3(256), 0, st, bf, zbase, nf);
+}
+
+void batch_assemble_launch(const BatchField *bf, const uint64_t *abase, int nf, uint64_t nitems, hipStream_t st)
+{
+    if (nitems) hipLaunchKernelGGL(k_assemble_batch, dim3((unsigned)nitems), dim3(256), 0, st, bf, abase, nf);
 }
 
 }  // namespace szxamd

@@ -27,6 +27,7 @@
 // (at most 3) copyable byte positions, and an exclusive segmented scan with
 // the last-writer-wins operator gives every lane the bytes it inherits.
 #include "szx_common.hpp"
+#include "szx_batch.hpp"
 #include "szx_index.hpp"
 #include "szx_internal.hpp"
 #include "szx_wave.hpp"
@@ -35,10 +36,9 @@ namespace szxamd {
 
 // Header check: block size, counts and the bitmap/median region must fit.
 
-__global__ void k_check_header(const uint8_t *s, uint64_t len, uint64_t n, int bs_expect,
-                               DecodeHeader *h, int32_t *status)
+__device__ inline void check_header(const uint8_t *s, uint64_t len, uint64_t n, int bs_expect,
+                                    DecodeHeader *h, int32_t *status)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
     int32_t rc = 0;
     uint64_t bs = 0, nb = 0, nbit = 0, nconst = 0;
     if (len < HEADER_BYTES) {
@@ -54,7 +54,14 @@ __global__ void k_check_header(const uint8_t *s, uint64_t len, uint64_t n, int b
     }
     h->bs = bs; h->nb = nb; h->nbit = nbit; h->nconst = nconst;
     h->q0 = HEADER_BYTES + nbit + 4 * nconst;
+    h->rc = rc;
     *status = rc;
+}
+
+__global__ void k_check_header(const uint8_t *s, uint64_t len, uint64_t n, int bs_expect,
+                               DecodeHeader *h, int32_t *status)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) check_header(s, len, n, bs_expect, h, status);
 }
 
 // last-writer-wins combine of the packed copyable-byte record:
@@ -266,37 +273,73 @@ __device__ inline void decode_lane(const TileLds &L, uint64_t e0, uint64_t n, in
     }
 }
 
-// Decode one tile (the compressor's tile: TILE values, NBLK blocks); each
+// Everything one tile decode needs (one stream; the batch keeps one per field).
+struct DecodeParams {
+    const uint8_t *s;           // stream
+    uint64_t len, n;            // stream bytes, values
+    const DecodeHeader *h;      // checked header
+    const uint64_t *index;      // record index: tile pairs
+    const uint16_t *bidx;       // record index: per-block entries
+    int32_t *status;
+    void *out;
+    int exact;
+};
+
+template <typename T, int BS, int LV> struct DecodeShape {
+    static constexpr int W = Word<T>::W;
+    static constexpr int TILE = sizeof(T) == 4 ? TILE_VALUES_F32 : TILE_VALUES_F64;
+    static constexpr int NBLK = TILE / BS;
+    static constexpr int STAGE = TILE * W + NBLK * (5 + BS / 4);
+};
+
+template <typename T, int BS, int LV> struct DecodeLds {
+    using S = DecodeShape<T, BS, LV>;
+    __attribute__((aligned(16))) uint8_t st[S::STAGE + 64];
+    __attribute__((aligned(16))) uint8_t med[4 * S::NBLK + 48];
+    uint16_t start[S::NBLK];   // record start, or constant rank within the tile
+    uint16_t size[S::NBLK];    // decoded record size (0: constant block)
+    uint8_t state[S::NBLK];
+    int err;
+};
+
+// Decode index tile `tile` of one stream (TILE values, NBLK blocks); each
 // lane decodes LV consecutive values per slab of 256 * LV values.
 template <typename T, int BS, int LV>
-__global__ __launch_bounds__(256) void k_decode_tiles(const uint8_t *s, uint64_t len, uint64_t n,
-                                                     const DecodeHeader *h, const uint64_t *index,
-                                                     const uint16_t *bidx, int32_t *status, T *out,
-                                                     int exact)
+__device__ inline void decode_tile(const DecodeParams &p, uint64_t tile, DecodeLds<T, BS, LV> &lds)
 {
-    constexpr int W = Word<T>::W;
+    using S = DecodeShape<T, BS, LV>;
     constexpr int SEG = BS / LV;
-    constexpr int TILE = sizeof(T) == 4 ? TILE_VALUES_F32 : TILE_VALUES_F64;
+    constexpr int TILE = S::TILE;
     constexpr int SLAB = 256 * LV;
     constexpr int SLABS = TILE / SLAB;
     constexpr int BPS = SLAB / BS;
-    constexpr int NBLK = TILE / BS;
-    constexpr int STAGE = TILE * W + NBLK * (5 + BS / 4);
+    constexpr int NBLK = S::NBLK;
+    constexpr int STAGE = S::STAGE;
     constexpr int PER_LANE = (NBLK + 63) / 64;   // blocks per lane in the check scan
     static_assert(SLABS >= 1 && TILE % SLAB == 0, "slabs tile the tile");
 
-    __shared__ __attribute__((aligned(16))) uint8_t st[STAGE + 64];
-    __shared__ __attribute__((aligned(16))) uint8_t s_med[4 * NBLK + 48];
-    __shared__ uint16_t s_start[NBLK];   // record start, or constant rank within the tile
-    __shared__ uint16_t s_size[NBLK];    // decoded record size (0: constant block)
-    __shared__ uint8_t s_state[NBLK];
-    __shared__ int s_err;
+    uint8_t *st = lds.st;
+    uint8_t *s_med = lds.med;
+    uint16_t *s_start = lds.start;
+    uint16_t *s_size = lds.size;
+    uint8_t *s_state = lds.state;
+    int &s_err = lds.err;
+    const uint8_t *s = p.s;
+    const uint64_t len = p.len, n = p.n;
+    const DecodeHeader *h = p.h;
+    const uint64_t *index = p.index;
+    const uint16_t *bidx = p.bidx;
+    int32_t *status = p.status;
+    T *out = static_cast<T *>(p.out);
+    const int exact = p.exact;
 
-    if (*status != 0) return;
+    // Gate on the header check, which an earlier kernel wrote, so every
+    // thread of the workgroup takes the same branch (the status word itself
+    // may change while this kernel runs: other tiles report into it).
+    if (h->rc != 0) return;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int lseg = tid % SEG;
-    const uint64_t tile = blockIdx.x;
     const uint64_t nb = h->nb, q0 = h->q0, nconst = h->nconst, nbit = h->nbit;
     const uint64_t gb0 = tile * NBLK;
     const uint64_t off = q0 + index[2 * tile];
@@ -374,6 +417,84 @@ __global__ __launch_bounds__(256) void k_decode_tiles(const uint8_t *s, uint64_t
     if (s_err && tid == 0) atomicMin(status, -21);
 }
 
+template <typename T, int BS, int LV>
+__global__ __launch_bounds__(256) void k_decode_tiles(DecodeParams p)
+{
+    __shared__ DecodeLds<T, BS, LV> lds;
+    decode_tile<T, BS, LV>(p, blockIdx.x, lds);
+}
+
+// ---------------------------------------------------------------------------
+// Batched decode: every field of a batched compress, one persistent launch.
+// The stream of a field is found on the device from the compressor's result
+// words, so a compress and its decode can be enqueued (or captured) together.
+// ---------------------------------------------------------------------------
+
+__global__ void k_decode_header_batch(const BatchDecodeField *f, int nf, int bs)
+{
+    const int fi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (fi >= nf) return;
+    const BatchDecodeField &d = f[fi];
+    const uint8_t *s = d.out + d.result[0];
+    check_header(s, d.result[1], d.n, bs, d.h, d.status);
+}
+
+template <int BS, int LV>
+__global__ __launch_bounds__(256) void k_decode_batch(const BatchDecodeField *f, const uint64_t *tbase, int nf,
+                                                      uint64_t total, int exact)
+{
+    __shared__ union Lds {
+        DecodeLds<float, BS, LV> f;
+        DecodeLds<double, BS, LV> d;
+    } lds;
+    int fi = 0;
+    for (uint64_t g = blockIdx.x; g < total; g += gridDim.x) {
+        while (fi + 1 < nf && tbase[fi + 1] <= g) fi++;
+        const BatchDecodeField &d = f[fi];
+        const DecodeParams p{d.out + d.result[0], d.result[1], d.n, d.h, d.index,
+                             reinterpret_cast<const uint16_t *>(d.index + 2 * (d.ntidx + 1)), d.status, d.dst,
+                             exact};
+        if (d.w == 4) decode_tile<float, BS, LV>(p, g - tbase[fi], lds.f);
+        else decode_tile<double, BS, LV>(p, g - tbase[fi], lds.d);
+        __syncthreads();
+    }
+}
+
+template <int BS>
+static void launch_decode_batch(const BatchDecodeField *f, const uint64_t *tbase, int nf, uint64_t total,
+                                int exact, hipStream_t st)
+{
+    constexpr int LV = BS >= 8 ? 8 : 4;
+    auto kern = k_decode_batch<BS, LV>;
+    static int resident = 0;
+    if (!resident) {
+        int dev = 0, per_cu = 0, cus = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0);
+        resident = per_cu > 0 && cus > 0 ? per_cu * cus : 256;
+    }
+    const uint64_t grid = total < (uint64_t)resident ? total : (uint64_t)resident;
+    if (grid) hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(256), 0, st, f, tbase, nf, total, exact);
+}
+
+int batch_decode_launch(int bs, const BatchDecodeField *f, const uint64_t *tbase, int nf, uint64_t total,
+                        int exact, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_decode_header_batch, dim3((nf + 255) / 256), dim3(256), 0, st, f, nf, bs);
+    switch (bs) {
+    case 4: launch_decode_batch<4>(f, tbase, nf, total, exact, st); break;
+    case 8: launch_decode_batch<8>(f, tbase, nf, total, exact, st); break;
+    case 16: launch_decode_batch<16>(f, tbase, nf, total, exact, st); break;
+    case 32: launch_decode_batch<32>(f, tbase, nf, total, exact, st); break;
+    case 64: launch_decode_batch<64>(f, tbase, nf, total, exact, st); break;
+    case 128: launch_decode_batch<128>(f, tbase, nf, total, exact, st); break;
+    case 256: launch_decode_batch<256>(f, tbase, nf, total, exact, st); break;
+    default: return -1;
+    }
+    return 0;
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -396,9 +517,10 @@ static void launch_decode(const uint8_t *s, uint64_t len, uint64_t n, const Deco
     // 8 values per lane where a block spans at least 8 values (fp64 tiles of
     // 2048 values take one slab of 256 x 8), else 4
     constexpr int LV = BS >= 8 ? 8 : 4;
-    const uint16_t *bidx = reinterpret_cast<const uint16_t *>(index + 2 * (ntiles + 1));
-    hipLaunchKernelGGL((k_decode_tiles<T, BS, LV>), dim3((unsigned)ntiles), dim3(256), 0, st, s,
-                       len, n, h, index, bidx, status, static_cast<T *>(out), exact);
+    const DecodeParams p{s, len, n, h, index,
+                         reinterpret_cast<const uint16_t *>(index + 2 * (ntiles + 1)), status, out,
+                         exact};
+    hipLaunchKernelGGL((k_decode_tiles<T, BS, LV>), dim3((unsigned)ntiles), dim3(256), 0, st, p);
 }
 
 template <typename T>
@@ -424,7 +546,7 @@ int decompress_launch(const uint8_t *d_stream, uint64_t stream_bytes, uint64_t n
                       int exact, int32_t *d_status, hipStream_t st)
 {
     const DecompressLayout L = decompress_layout(n, w, bs, stream_bytes);
-    if (ws_bytes < L.ws_bytes) return -1;
+    if (ws_bytes < (d_index ? L.ws_index : L.ws_bytes)) return -1;   // with an index only the header is kept
     DecodeHeader *h = reinterpret_cast<DecodeHeader *>(d_ws + L.ws_hdr);
     const uint64_t tv = tile_values(w);
     const uint64_t ntiles = ceil_div(n, tv);

@@ -35,6 +35,10 @@ inline uint64_t index_words(uint64_t n, int w, int bs)
 // each main kernel (compress, decode, index build) with a pair of HIP events
 // on the launch stream.
 enum KernelKind { KK_COMPRESS = 0, KK_DECODE = 1, KK_INDEX = 2, KK_COUNT = 3 };
+// Error text of the calling thread (szx_amd_last_error).
+void api_error(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
+void api_clear_error();
+
 void timing_mark_begin(int kind, hipStream_t st);
 void timing_mark_end(int kind, hipStream_t st);
 
@@ -68,6 +72,7 @@ void range_launch(const void *d_in, uint64_t n, int w, uint8_t *d_ws, void *d_mi
 // bytes, constant blocks, first payload byte.
 struct DecodeHeader {
     uint64_t bs, nb, nbit, nconst, q0;
+    int64_t rc;   // the header check's own status (0 = well-formed)
 };
 
 // Device memory layout of one decompress call.

@@ -40,3 +40,28 @@ def test_compute_data_length_without_gpu():
     assert L.SZx_computeDataLength(0, 0, 100, 500, 500) == 25_000_000
     assert L.SZx_computeDataLength(0, 0, 0, 0, 7) == 7
     assert L.SZx_computeDataLength(0, 0, 0, 0, 0) == 0
+
+
+def test_batch_workspace_validates_fields_without_gpu():
+    """szx_amd_batch_workspace is host arithmetic: it rejects bad fields with
+    0 and sizes good ones (every field's single-field workspace and record
+    index, plus the tables) -- no device call is made."""
+    L = szx_amd.lib()
+    F = szx_amd.Field
+
+    def ws(fields, bs=128):
+        arr = (F * len(fields))(*fields)
+        return L.szx_amd_batch_workspace(arr, len(fields), bs)
+
+    good = [F(16, 1000, szx_amd.SZx_FLOAT, szx_amd.REL, 0.0, 1e-3, 16, 0),
+            F(16, 70000, szx_amd.SZx_DOUBLE, szx_amd.ABS, 1e-4, 0.0, 16, 0)]
+    total = ws(good)
+    assert total > 0
+    single = sum(L.szx_amd_compress_workspace(f.n, f.dataType, 128) +
+                 8 * L.szx_amd_index_words(f.n, f.dataType, 128) for f in good)
+    assert total >= single
+    assert ws(good, bs=100) == 0
+    assert ws([F(16, 0, szx_amd.SZx_FLOAT, szx_amd.ABS, 1.0, 0.0, 16, 0)]) == 0
+    assert ws([F(16, 10, 5, szx_amd.ABS, 1.0, 0.0, 16, 0)]) == 0
+    assert ws([F(16, 10, szx_amd.SZx_FLOAT, szx_amd.FXR, 1.0, 0.0, 16, 0)]) == 0
+    assert L.szx_amd_batch_workspace(None, 0, 128) == 0
