@@ -168,6 +168,161 @@ def fixed_rate_eval(n_fields, dev, tol=0.02, targets=(1.0, 2.0, 4.0, 8.0)):
     return out
 
 
+def _extents(r, n, nd):
+    """Random 1-D/2-D/3-D extents whose product is close to n (at most n)."""
+    import numpy as np
+    if nd == 1 or n < 8:
+        return (n,)
+    if nd == 2:
+        a = int(min(n, max(1, round(n ** 0.5 * 2 ** r.uniform(-1, 1)))))
+        return (a, max(1, n // a))
+    a = int(min(n, max(1, round(n ** (1 / 3) * 2 ** r.uniform(-1, 1)))))
+    rest = max(1, n // a)
+    b = int(min(rest, max(1, round(rest ** 0.5 * 2 ** r.uniform(-0.5, 0.5)))))
+    return (a, b, max(1, rest // b))
+
+
+def batched_eval(n_fields, dev, reps, single_gbs):
+    """Many fields in one persistent launch per direction, captured in a HIP
+    graph, against one launch sequence per field on the same buffers.
+
+    Fields: sizes log-uniform in [64 KiB, 256 MiB], 60% fp32 / 40% fp64,
+    random 1-D/2-D/3-D extents, each a 32-wave sinusoid field (k <= 16,
+    |k|^-2, noise 1e-3 std, a zero box of 25%) with its own seed, compressed at
+    its own REL ratio (log-uniform in [1e-5, 1e-2]), blockSize 128."""
+    import numpy as np
+    import torch
+
+    import szx_amd as Z
+    from szx_amd.fields import sinusoid_field_torch
+
+    r = np.random.default_rng(4)
+    fields, total, n32, n64 = [], 0, 0, 0
+    t0 = time.perf_counter()
+    for i in range(n_fields):
+        w = 4 if r.random() < 0.6 else 8
+        nb = float(np.exp(r.uniform(np.log(64 * 2 ** 10), np.log(256 * 2 ** 20))))
+        shape = _extents(r, max(1, int(nb) // w), int(r.integers(1, 4)))
+        rel = float(np.float32(np.exp(r.uniform(np.log(1e-5), np.log(1e-2)))))
+        x = sinusoid_field_torch(shape, seed=1000 + i, n_waves=32, kmax=16, slope=2.0, noise=1e-3,
+                                 n_boxes=1, box_frac=0.25, box_value="zero",
+                                 dtype="float32" if w == 4 else "float64", device=dev).reshape(-1)
+        fields.append((x, Z.REL, rel))
+        total += x.numel() * w
+        n32 += w == 4
+        n64 += w == 8
+        if (i + 1) % 100 == 0:
+            log(f"  generated {i + 1} fields, {total / 2**30:.2f} GiB")
+    torch.cuda.synchronize()
+    gen_s = time.perf_counter() - t0
+    log(f"batch: {n_fields} fields ({n32} fp32, {n64} fp64), {total / 2**30:.2f} GiB, "
+        f"generated in {gen_s:.1f} s")
+
+    bc = Z.BatchCodec(fields, 128)
+    shapes = {}
+    for lv, ipt in ((8, 2), (8, 1), (4, 2), (4, 1)):
+        bc.set_shape(lv, ipt)
+        bc.compress()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        bc.compress()
+        torch.cuda.synchronize()
+        shapes[f"{lv}x{ipt}"] = (time.perf_counter() - t) * 1e3
+    lv, ipt, tune_ms = bc.tune(reps=2)
+    log(f"tuner: {lv} values per lane, {ipt} index tiles per tile, {tune_ms:.3f} ms")
+
+    def timed(fn, k):
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(k):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / k
+
+    bc.compress()
+    bc.decompress()
+    torch.cuda.synchronize()
+    Z.timing_begin()
+    bc.compress()
+    bc.decompress()
+    kt = Z.timing_end()
+    gc, gd = torch.cuda.CUDAGraph(), torch.cuda.CUDAGraph()
+    with torch.cuda.graph(gc):
+        bc.compress()
+    with torch.cuda.graph(gd):
+        bc.decompress()
+    ms_c = timed(gc.replay, reps)
+    res_b = bc.results()
+    assert not res_b[:, 3].any(), "a bounded wait between workgroups ran out"
+    ms_d = timed(gd.replay, reps)
+    st = bc.status().cpu().numpy()
+    assert not st.any(), f"decode status {st[st != 0][:4]}"
+    comp_bytes = int(res_b[:, 1].sum())
+
+    # a sample of streams, to compare with the per-field launches' output
+    sample = r.choice(n_fields, size=min(n_fields, 64), replace=False)
+    kept = {int(i): bc.stream(int(i), res_b).clone() for i in sample}
+
+    # every field within its bound (exact decode: for fp64 the reference
+    # decoder misreads one byte of some records, which DECODE_REFERENCE keeps)
+    bc.decompress(flags=Z.DECODE_EXACT)
+    # A constant block decodes to its median rounded to fp32 (the stream's
+    # format), so |x - x'| may pass eb by that rounding: about one fp32 ulp of
+    # max(|median|, block range).  The same bytes come out of the oracle.
+    ebs = np.array(res_b[:, 2], np.uint32).view(np.float32)
+    worst, over, within_ulp = 0.0, 0, True
+    for i, (x, _, _) in enumerate(fields):
+        d = bc.decs[i]
+        e = (d.double() - x.double()).abs()
+        eb = float(ebs[i])
+        m = float(e.max())
+        worst = max(worst, m / eb if eb > 0 else (0.0 if m == 0 else float("inf")))
+        if m > eb:
+            over += 1
+            a = torch.clamp(d.double().abs(), min=2 * eb).float()
+            ulp = (torch.nextafter(a, torch.full_like(a, float("inf"))) - a).double()
+            within_ulp &= bool((e <= eb + ulp).all())
+    log(f"max |x - x'| / eb over the batch: {worst:.7f} ({over} fields above eb, "
+        f"all within one fp32 ulp of the median: {within_ulp})")
+
+    ms_pc = timed(lambda: bc.compress(mode=Z.BATCH_PER_FIELD), reps)
+    res_p = bc.results()
+    same = bool((res_p == res_b).all()) and all(torch.equal(bc.stream(i, res_p), s) for i, s in kept.items())
+    ms_pd = timed(lambda: bc.decompress(mode=Z.BATCH_PER_FIELD), reps)
+    st = bc.status().cpu().numpy()
+    assert not st.any(), "per-field decode status"
+    gbs = lambda ms: total / (ms * 1e-3) / 1e9
+    out = {
+        "fields": n_fields, "fp32_fields": n32, "fp64_fields": n64, "input_bytes": total,
+        "compressed_bytes": comp_bytes, "compression_ratio": total / comp_bytes,
+        "block_size": 128, "rel_range": [1e-5, 1e-2], "graph_captured": True,
+        "tuned_shape": {"values_per_lane": lv, "index_tiles_per_tile": ipt, "ms": tune_ms},
+        "shape_ms_single_pass": shapes,
+        "compress_ms": ms_c, "compress_gbs": gbs(ms_c),
+        "decompress_ms": ms_d, "decompress_gbs": gbs(ms_d),
+        "compress_kernel_ms": kt["compress"][0], "decode_kernel_ms": kt["decode"][0],
+        "per_field_compress_ms": ms_pc, "per_field_compress_gbs": gbs(ms_pc),
+        "per_field_decompress_ms": ms_pd, "per_field_decompress_gbs": gbs(ms_pd),
+        "speedup_compress": ms_pc / ms_c, "speedup_decompress": ms_pd / ms_d,
+        "single_field_compress_gbs": single_gbs,
+        "single_over_batched_compress": single_gbs / gbs(ms_c) if single_gbs else None,
+        "per_field_streams_identical": same,
+        "bound_check": {"max_err_over_eb": worst, "fields_above_eb": over,
+                        "all_within_eb_plus_fp32_ulp_of_median": within_ulp,
+                        "decode": "exact (fp64: DECODE_EXACT)",
+                        "note": "constant blocks decode to the fp32-rounded block median, "
+                                "as the reference's streams do"},
+        "generation_s": gen_s,
+    }
+    bc.close()
+    del fields, bc, kept
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -180,6 +335,9 @@ def main():
     ap.add_argument("--no-noindex", action="store_true", help="skip the index-free decode")
     ap.add_argument("--fxr-fields", type=int, default=64,
                     help="fixed-rate evaluation fields (0 skips it)")
+    ap.add_argument("--batch-fields", type=int, default=1000,
+                    help="fields of the batched-launch evaluation (0 skips it)")
+    ap.add_argument("--batch-reps", type=int, default=5)
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run 3 plain elementwise copies of the input (y = x * 1, "
                          "N*4 bytes read and written) to calibrate HBM byte counters")
@@ -330,6 +488,15 @@ def main():
         fixed = fixed_rate_eval(args.fxr_fields, dev)
         log(f"fixed rate {fixed}")
 
+    batched = None
+    if args.batch_fields > 0:
+        log(f"batched-launch evaluation on {args.batch_fields} fields")
+        del view, comp, dec
+        torch.cuda.empty_cache()
+        batched = batched_eval(args.batch_fields, dev, args.batch_reps,
+                               nbytes / (ms_rel * 1e-3) / 1e9)
+        log(f"batched {batched}")
+
     alg_c = nbytes + csize
     roof = {"bound": "hbm", "kernel": "k_compress", "achieved": alg_c / (k_ms * 1e-3) / 1e9,
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": measured_traffic(shape, args.rel),
@@ -367,6 +534,7 @@ def main():
         "eb": eb,
         "parity": parity,
         "fixed_rate": fixed,
+        "batched": batched,
         "host_api_pcie_inclusive": host_api,
     }
     print(json.dumps(line), flush=True)

@@ -648,12 +648,30 @@ __device__ inline int field_of(const uint64_t *base, int nf, uint64_t item)
 template <typename T>
 __device__ inline void range_item(const T *x, uint64_t a, uint64_t b, double *slot)
 {
+    // a is a multiple of RANGE_ITEM and x 16-byte aligned, so every quad of
+    // values starts on a 16-byte boundary; RU quads per thread are in flight
+    constexpr int RU = 4;
     T mn = pos_inf<T>(), mx = -pos_inf<T>();
-    for (uint64_t i = a + threadIdx.x; i < b; i += blockDim.x) {
-        const T v = x[i];
-        mn = v < mn ? v : mn;
-        mx = v > mx ? v : mx;
+    const uint64_t sweep = 4ull * blockDim.x;
+    uint64_t i = a + 4ull * threadIdx.x;
+    for (; i + (RU - 1) * sweep + 4 <= b; i += RU * sweep) {
+        T v[RU][4];
+#pragma unroll
+        for (int u = 0; u < RU; u++) SlabLoad<T>::load4(x, i + u * sweep, b, v[u]);
+#pragma unroll
+        for (int u = 0; u < RU; u++)
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                mn = v[u][k] < mn ? v[u][k] : mn;
+                mx = v[u][k] > mx ? v[u][k] : mx;
+            }
     }
+    for (; i < b; i += sweep)
+        for (int k = 0; k < 4 && i + k < b; k++) {
+            const T v = x[i + k];
+            mn = v < mn ? v : mn;
+            mx = v > mx ? v : mx;
+        }
     for (int m = 1; m < 64; m <<= 1) {
         T o = __shfl_xor(mn, m, 64);
         mn = o < mn ? o : mn;
