@@ -183,7 +183,7 @@ def _extents(r, n, nd):
 
 
 def batched_eval(n_fields, dev, reps, single_gbs):
-    """Many fields in one persistent launch per direction, captured in a HIP
+    """Many fields in one launch sequence per direction, captured in a HIP
     graph, against one launch sequence per field on the same buffers.
 
     Fields: sizes log-uniform in [64 KiB, 256 MiB], 60% fp32 / 40% fp64,

@@ -269,7 +269,8 @@ int szx_amd_mlp_adam_dev(float *d_params, const float *d_grad, float *d_m, float
  * No counterpart in the reference, which compresses one array per call
  * (SZx_fast_compress_args, szx.cc:223-262).  Many fields of uneven sizes,
  * fp32 and fp64 mixed, each with its own ABS bound or REL ratio, go through
- * one persistent launch per direction.  Each field's stream is byte-identical
+ * one launch sequence per direction (a few kernels covering every field).
+ * Each field's stream is byte-identical
  * to the stream SZx_fast_compress_args writes for that field alone, and is
  * placed inside the field's d_out as szx_amd_compress_dev places it. */
 typedef struct szx_amd_field {
@@ -286,7 +287,7 @@ typedef struct szx_amd_field {
 
 typedef struct szx_amd_batch szx_amd_batch;   /* host-side plan */
 
-#define SZX_AMD_BATCH_PERSISTENT 0   /* one persistent launch over every field */
+#define SZX_AMD_BATCH_FUSED 0        /* every field in one launch sequence */
 #define SZX_AMD_BATCH_PER_FIELD 1    /* the single-field launches, field by field
                                         (for comparison) */
 
@@ -300,7 +301,7 @@ szx_amd_batch *szx_amd_batch_create(const szx_amd_field *fields, int nfields, in
                                     void *d_work, size_t workBytes, void *hipStream);
 void szx_amd_batch_destroy(szx_amd_batch *batch);
 
-/* Compress every field.  PERSISTENT only enqueues kernels: nothing is
+/* Compress every field.  FUSED only enqueues kernels: nothing is
  * allocated, copied or waited for, so the call can be captured in a hipGraph
  * and replayed.  Result words of field i land at szx_amd_batch_results_dev()
  * + 4 i, with the meaning of szx_amd_compress_dev's d_result. */
@@ -318,11 +319,11 @@ uint64_t *szx_amd_batch_index_dev(szx_amd_batch *batch, int field);
 int szx_amd_batch_results(szx_amd_batch *batch, uint64_t *h_results, void *hipStream);
 
 /* Decompress every field's stream (found through its result words) into its
- * d_dec.  flags as szx_amd_decompress_dev.  PERSISTENT only enqueues kernels;
+ * d_dec.  flags as szx_amd_decompress_dev.  FUSED only enqueues kernels;
  * PER_FIELD needs szx_amd_batch_results() after the compress. */
 int szx_amd_batch_decompress(szx_amd_batch *batch, int flags, int mode, void *hipStream);
 
-/* Tile shape of the persistent compress: values per lane (4 or 8; 8 needs
+/* Tile shape of the fused compress: values per lane (4 or 8; 8 needs
  * blockSize >= 8) and index tiles per workgroup tile (1 or 2, i.e. 4096 or
  * 8192 fp32 values).  szx_amd_batch_tune times every shape on this batch
  * (reps timed passes each, after one untimed pass; synchronizes), keeps the

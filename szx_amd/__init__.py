@@ -50,7 +50,7 @@ EXPORTS = (
     "szx_amd_batch_index_dev", "szx_amd_batch_results", "szx_amd_batch_decompress",
     "szx_amd_batch_set_shape", "szx_amd_batch_tune",
 )
-BATCH_PERSISTENT, BATCH_PER_FIELD = 0, 1
+BATCH_FUSED, BATCH_PER_FIELD = 0, 1
 OCTAVE_BINS = 512
 RATE_BISECT, RATE_LEARNED = 0, 1
 RATE_MODEL_PARAMS = 5825
@@ -385,7 +385,7 @@ class DeviceDecompressor:
 
 
 class BatchCodec:
-    """Many device fields compressed / decompressed by one persistent launch
+    """Many device fields compressed / decompressed by one launch sequence
     each way (szx_amd_batch_*).  fields: sequence of (tensor, errBoundMode,
     bound), where bound is the ABS bound or the REL ratio.  Output buffers,
     decode buffers and the workspace are allocated here once; compress() and
@@ -433,7 +433,7 @@ class BatchCodec:
         except Exception:
             pass
 
-    def compress(self, mode: int = BATCH_PERSISTENT, stream=None) -> None:
+    def compress(self, mode: int = BATCH_FUSED, stream=None) -> None:
         _check(lib().szx_amd_batch_compress(self.handle, mode, _stream_handle(stream)))
 
     def results(self, stream=None) -> np.ndarray:
@@ -442,7 +442,7 @@ class BatchCodec:
         _check(lib().szx_amd_batch_results(self.handle, out.ctypes.data, _stream_handle(stream)))
         return out
 
-    def decompress(self, flags: int = DECODE_REFERENCE, mode: int = BATCH_PERSISTENT,
+    def decompress(self, flags: int = DECODE_REFERENCE, mode: int = BATCH_FUSED,
                    stream=None) -> None:
         _check(lib().szx_amd_batch_decompress(self.handle, flags, mode, _stream_handle(stream)))
 

@@ -9,6 +9,8 @@ Flags that matter for exactness:
   -Xclang -target-feature -Xclang -packed-fp32-ops
                                      keep the compiler from emitting packed
                                      fp32 instructions
+  -mllvm -amdgpu-atomic-optimizer-strategy=None
+                                     atomics stay as written (see DEVICE)
 """
 from __future__ import annotations
 
@@ -32,7 +34,11 @@ HEADERS = ["szx_common.hpp", "szx_internal.hpp", "szx_wave.hpp", "szx_fxr.hpp",
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",
           "-Wno-unused-result"]
 DEVICE = [f"--offload-arch={ARCH}", "-fno-gpu-flush-denormals-to-zero",
-          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+          # keep single-lane atomics single-lane: the compressor's look-back
+          # issues one and reads its result much later, which the combining
+          # rewrite (a wave-wide broadcast of the result) would force at once
+          "-mllvm", "-amdgpu-atomic-optimizer-strategy=None"]
 
 
 def build(verbose: bool = False) -> str:

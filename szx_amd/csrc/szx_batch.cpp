@@ -1,5 +1,5 @@
 // szx_batch.cpp -- the batched launch: many fields of mixed size and type
-// compressed (and decompressed) by one persistent launch each way.
+// compressed (and decompressed) by one launch sequence each way.
 //
 // The planner lays the caller's workspace out once (field tables, per-field
 // look-back words, bitmaps and medians, record indexes, result words), uploads
@@ -12,7 +12,7 @@
 //   k_zero_batch      clear look-back words, totals, error flags (flattened)
 //   k_batch_range     REL fields only: min/max of 2^20-value items
 //   k_batch_range_final   per REL field: the bound, rounded as the reference
-//   k_compress_batch  persistent grid walking the global tile list
+//   k_compress_batch  one workgroup per tile of the global tile list
 //   k_assemble_batch  header, bitmap, constant medians (flattened)
 // and of a batched decompress:
 //   k_decode_header_batch, k_decode_batch (persistent grid over every field's
@@ -169,7 +169,7 @@ int enqueue_compress(szx_amd_batch *b, int mode, hipStream_t st)
         }
         return hip_ok(hipGetLastError(), "launch") ? SZx_SCES : SZx_NSCS;
     }
-    if (mode != SZX_AMD_BATCH_PERSISTENT) { api_error("unknown batch mode %d", mode); return SZx_NSCS; }
+    if (mode != SZX_AMD_BATCH_FUSED) { api_error("unknown batch mode %d", mode); return SZx_NSCS; }
     batch_zero_launch(b->d_bf(), b->d_base(4), b->nf, Y.zitems, st);
     if (Y.nitems) batch_range_launch(b->d_bf(), b->d_base(2), b->nf, Y.nitems,
                                      reinterpret_cast<double *>(b->work + Y.part), st);
@@ -252,6 +252,7 @@ szx_amd_batch *szx_amd_batch_create(const szx_amd_field *fields, int nfields, in
         p.cmed = reinterpret_cast<uint32_t *>(ws + s.L.ws_cmed);
         p.tiles = reinterpret_cast<unsigned long long *>(ws + s.L.ws_tiles);
         p.chunks = reinterpret_cast<ChunkState *>(ws + s.L.ws_chunks);
+        p.cacc = reinterpret_cast<unsigned long long *>(ws + s.L.ws_cacc);
         p.totals = reinterpret_cast<uint64_t *>(misc + 16);
         p.err = reinterpret_cast<int *>(misc + 48);
         p.index = b->d_index(i);
@@ -369,7 +370,7 @@ int szx_amd_batch_decompress(szx_amd_batch *b, int flags, int mode, void *hipStr
         }
         return hip_ok(hipGetLastError(), "launch") ? SZx_SCES : SZx_NSCS;
     }
-    if (mode != SZX_AMD_BATCH_PERSISTENT) { api_error("unknown batch mode %d", mode); return SZx_NSCS; }
+    if (mode != SZX_AMD_BATCH_FUSED) { api_error("unknown batch mode %d", mode); return SZx_NSCS; }
     timing_mark_begin(KK_DECODE, st);
     const int rc = batch_decode_launch(b->bs, b->d_df(), b->d_base(3), b->nf, b->Y.dtotal, exact, st);
     timing_mark_end(KK_DECODE, st);
@@ -409,9 +410,9 @@ int szx_amd_batch_tune(szx_amd_batch *b, int reps, int *valuesPerLane, int *tile
         b->lv = sh[0];
         b->ipt = sh[1];
         // one untimed pass (code object load, caches), then reps timed passes
-        if ((rc = enqueue_compress(b, SZX_AMD_BATCH_PERSISTENT, st)) != SZx_SCES) break;
+        if ((rc = enqueue_compress(b, SZX_AMD_BATCH_FUSED, st)) != SZx_SCES) break;
         if (!hip_ok(hipEventRecord(e0, st), "event")) { rc = SZx_NSCS; break; }
-        for (int r = 0; r < reps && rc == SZx_SCES; r++) rc = enqueue_compress(b, SZX_AMD_BATCH_PERSISTENT, st);
+        for (int r = 0; r < reps && rc == SZx_SCES; r++) rc = enqueue_compress(b, SZX_AMD_BATCH_FUSED, st);
         float t = 0.0f;
         if (rc != SZx_SCES || !hip_ok(hipEventRecord(e1, st), "event") ||
             !hip_ok(hipEventSynchronize(e1), "tune") || !hip_ok(hipEventElapsedTime(&t, e0, e1), "event")) {

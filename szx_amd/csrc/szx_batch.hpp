@@ -20,6 +20,7 @@ struct CompressParams {
     uint32_t *cmed;       // scratch constant medians, big-endian words
     unsigned long long *tiles;   // per-tile look-back words
     ChunkState *chunks;          // per-64-tile look-back words
+    unsigned long long *cacc;    // per-64-tile aggregate accumulators
     uint64_t *totals;     // [0] payload bytes, [1] constant blocks
     uint64_t *index;      // optional record index: per tile {payload offset, constants before}
     uint16_t *bidx;       // its per-block part (start in tile, or constant rank), or nullptr

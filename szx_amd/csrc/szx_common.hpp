@@ -113,22 +113,25 @@ struct ChunkState;
 __device__ inline void publish_pair(ChunkState *t, unsigned long long flag, unsigned long long a,
                                     unsigned long long b);
 
-// Two-level decoupled look-back, called by the 64 lanes of one wave.
+// Two-level decoupled look-back (compressor tiles over one field).
 //
-// Tiles are grouped in chunks of 64.  A tile publishes its aggregate in one
-// 64-bit word -- flag | constants (22 bits) << 40 | bytes (40 bits); both fit
-// because a chunk holds 64 tiles -- and scans only the tiles before it in its
-// own chunk (one window, no chain).  The chunk's last tile publishes the
-// chunk total; every tile then scans the chunk totals before its chunk, again
-// one window of 64 in the common case.  Inclusive prefixes therefore never
-// have to travel through long chains of tiles, which matters because every
-// poll is a round trip across the XCDs (~1 us).
+// Tiles are grouped in chunks of LB_CHUNK.  As soon as a tile knows its
+// aggregate (record bytes, constant blocks) it publishes it in one 64-bit
+// word -- flag | constants << 40 | bytes -- and adds it, with a count, to its
+// chunk's accumulator; the tile that completes the count publishes the chunk
+// aggregate.  A tile's exclusive prefix is the sum of the aggregates before
+// it in its chunk (one window of at most 63 words) plus its chunk's prefix,
+// found by scanning the chunk words backwards to the nearest inclusive value;
+// the first tile of each chunk then publishes that prefix as the previous
+// chunk's inclusive value.
 //
-// Tiles must start in index order (the hardware dispatches a grid's
-// workgroups in increasing order), so every tile waited on has been
-// dispatched and the loops end.  The waits are still bounded (LB_SPIN_LIMIT
-// polls, a second or more): on overrun the look-back reports failure and the
-// caller flags the call as failed instead of hanging.
+// No tile waits for another tile's look-back, only for aggregates, which
+// every tile publishes before its own look-back, so a caller can issue the
+// first reads (lookback_poll) right after publishing, do other work, and only
+// then finish (lookback_finish): the round trips to the other XCDs overlap
+// that work.  A tile waits only for lower-numbered tiles, which start first
+// (in-order dispatch), so every wait ends; the waits are bounded all the same
+// (LB_SPIN_LIMIT polls) and report failure instead of hanging.
 struct ChunkState {
     unsigned long long w[2];   // flag | bytes, flag | constants
 };
@@ -143,17 +146,16 @@ __device__ inline void publish_pair(ChunkState *t, unsigned long long flag, unsi
 constexpr int LB_CHUNK = 64;
 constexpr unsigned long long T_BYTES = (1ull << 40) - 1;
 
-// Window accept/sum step over NL lanes' words; returns true when accepted.
-// `stop` is the lowest lane (nearest predecessor) holding an inclusive prefix
-// or 64 if none.
-__device__ inline bool window_ready(bool ready, bool isinc, int lane, int &stop)
+// Window accept/sum step over the 64 lanes' words; returns true when
+// accepted.  `stop` is the lowest lane (nearest predecessor) holding an
+// inclusive value, or 64 if none.
+__device__ inline bool window_ready(bool ready, bool isinc, int &stop)
 {
     const uint64_t nr = __ballot(!ready);
     const uint64_t inc = __ballot(ready && isinc);
     const uint64_t low = inc & (~inc + 1);
     const uint64_t upto = inc ? (low | (low - 1)) : ~0ull;
     stop = inc ? __builtin_ctzll(inc) : 64;
-    (void)lane;
     return (nr & upto) == 0;
 }
 
@@ -178,75 +180,117 @@ __device__ inline unsigned long long wave_sum(unsigned long long v)
     return v;
 }
 
-__device__ inline bool lookback_2level(unsigned long long *tiles, ChunkState *chunks, uint64_t tile,
-                                       uint64_t ntiles, unsigned long long agg_bytes,
-                                       unsigned long long agg_cnt, unsigned long long &ex_bytes,
+struct LookbackRefs {
+    unsigned long long *tiles;   // per tile: flag | constants << 40 | bytes
+    ChunkState *chunks;          // per chunk: aggregate or inclusive prefix
+    unsigned long long *cacc;    // per chunk: bytes | constants << 32 | tiles << 56
+    uint64_t ntiles;
+};
+
+// Publish a tile's aggregate (one lane) and add it to its chunk's
+// accumulator.  Returns the accumulator's previous value, which the caller
+// hands to lookback_chunk_done once convenient: waiting for the atomic's
+// round trip right away would stall the workgroup.  Chunk sums fit their
+// fields: a chunk holds 64 tiles of at most 2^16 record bytes and 2^11
+// constant blocks each.
+__device__ inline unsigned long long lookback_publish(const LookbackRefs &L, uint64_t tile, uint32_t bytes,
+                                                      uint32_t consts)
+{
+    store_relaxed(&L.tiles[tile], FLAG_AGG | ((unsigned long long)consts << 40) | bytes);
+    const unsigned long long mine =
+        (unsigned long long)bytes | ((unsigned long long)consts << 32) | (1ull << 56);
+    return atomicAdd(&L.cacc[tile / LB_CHUNK], mine);
+}
+
+// The tile that completes its chunk's count publishes the chunk aggregate
+// (one lane; `old` from lookback_publish for the same tile and aggregate).
+__device__ inline void lookback_chunk_done(const LookbackRefs &L, uint64_t tile, unsigned long long old,
+                                           uint32_t bytes, uint32_t consts)
+{
+    const uint64_t c = tile / LB_CHUNK;
+    const uint64_t left = L.ntiles - c * LB_CHUNK;
+    const uint64_t count = left < (uint64_t)LB_CHUNK ? left : (uint64_t)LB_CHUNK;
+    if ((old >> 56) + 1 == count) {
+        const unsigned long long sum =
+            old + ((unsigned long long)bytes | ((unsigned long long)consts << 32));
+        publish_pair(&L.chunks[c], FLAG_AGG, sum & 0xFFFFFFFFull, (sum >> 32) & 0xFFFFFFull);
+    }
+}
+
+// The words a tile's look-back reads first, loaded by the 64 lanes of a wave:
+// lane l holds tile (chunk start + l) for l below the tile's place in its
+// chunk, and, for l < LB_WIN, chunk (c - 1 - l) (an inclusive zero before
+// chunk 0).
+struct LookbackPoll {
+    unsigned long long t, ca, cb;
+};
+
+// Chunk words are read LB_WIN at a time (the nearest inclusive value is
+// normally one or two chunks back); lanes past the window hold zeros that
+// neither stop nor add to the scan.
+constexpr int LB_WIN = 16;
+
+__device__ inline void lookback_chunk_words(const LookbackRefs &L, int64_t j, LookbackPoll &q)
+{
+    const bool in = __lane_id() < LB_WIN;
+    q.ca = !in ? FLAG_AGG : j >= 0 ? load_relaxed(&L.chunks[j].w[0]) : FLAG_INC;
+    q.cb = !in ? FLAG_AGG : j >= 0 ? load_relaxed(&L.chunks[j].w[1]) : FLAG_INC;
+}
+
+__device__ inline LookbackPoll lookback_poll(const LookbackRefs &L, uint64_t tile)
+{
+    const int lane = __lane_id();
+    const uint64_t c = tile / LB_CHUNK;
+    const int li = (int)(tile % LB_CHUNK);
+    LookbackPoll q;
+    q.t = lane < li ? load_relaxed(&L.tiles[c * LB_CHUNK + lane]) : FLAG_AGG;
+    lookback_chunk_words(L, (int64_t)c - 1 - lane, q);
+    return q;
+}
+
+// Finish a tile's look-back from its polled words (all 64 lanes of a wave;
+// every lane returns the same prefix).  `publish`: this wave records the
+// chunk prefix of a chunk-leading tile.  False if a wait overran its bound.
+__device__ inline bool lookback_finish(const LookbackRefs &L, uint64_t tile, LookbackPoll q,
+                                       bool publish, unsigned long long &ex_bytes,
                                        unsigned long long &ex_cnt)
 {
     const int lane = __lane_id();
-    const uint64_t chunk = tile / LB_CHUNK;
+    const uint64_t c = tile / LB_CHUNK;
     const int li = (int)(tile % LB_CHUNK);
-    const uint64_t c0 = chunk * LB_CHUNK;
+    const uint64_t c0 = c * LB_CHUNK;
 
-    // 1. within the chunk
-    unsigned long long in_b = 0, in_c = 0;
-    if (li == 0) {
-        if (lane == 0) store_relaxed(&tiles[tile], FLAG_INC | (agg_cnt << 40) | agg_bytes);
-    } else {
-        if (lane == 0) store_relaxed(&tiles[tile], FLAG_AGG | (agg_cnt << 40) | agg_bytes);
-        int round = 0;
-        while (true) {
-            // lane l looks at tile (tile - 1 - l); lanes beyond the chunk start see
-            // a virtual inclusive prefix of zero
-            const int j = li - 1 - lane;
-            const unsigned long long w = j >= 0 ? load_relaxed(&tiles[c0 + j]) : FLAG_INC;
-            const unsigned long long f = w & FLAG_MASK;
-            int stop;
-            if (!window_ready(f != 0, f == FLAG_INC, lane, stop)) {
-                if (!poll_backoff(round)) return false;
-                continue;
-            }
-            const unsigned long long v = lane <= stop ? (w & VAL_MASK) : 0ull;
-            in_b = wave_sum(v & T_BYTES);
-            in_c = wave_sum(v >> 40);
-            break;
-        }
-        if (lane == 0)
-            store_relaxed(&tiles[tile], FLAG_INC | ((in_c + agg_cnt) << 40) | (in_b + agg_bytes));
+    // 1. the aggregates before the tile in its chunk
+    int round = 0;
+    while (true) {
+        const bool ready = (q.t & FLAG_MASK) != 0;
+        if (__ballot(!ready) == 0) break;
+        if (!poll_backoff(round)) return false;
+        if (!ready) q.t = load_relaxed(&L.tiles[c0 + lane]);
     }
-    const bool last_in_chunk = li == LB_CHUNK - 1 || tile == ntiles - 1;
+    const unsigned long long v = lane < li ? (q.t & VAL_MASK) : 0ull;
+    const unsigned long long in_b = wave_sum(v & T_BYTES);
+    const unsigned long long in_c = wave_sum(v >> 40);
 
-    // 2. across chunks
+    // 2. the chunk's prefix
     unsigned long long cx_b = 0, cx_c = 0;
-    if (chunk == 0) {
-        if (last_in_chunk && lane == 0)
-            publish_pair(&chunks[0], FLAG_INC, in_b + agg_bytes, in_c + agg_cnt);
-    } else {
-        if (last_in_chunk && lane == 0)
-            publish_pair(&chunks[chunk], FLAG_AGG, in_b + agg_bytes, in_c + agg_cnt);
-        int64_t base = (int64_t)chunk - 1;
-        int round = 0;
-        while (true) {
-            const int64_t j = base - lane;
-            unsigned long long wa = FLAG_INC, wb = FLAG_INC;
-            if (j >= 0) {
-                wa = load_relaxed(&chunks[j].w[0]);
-                wb = load_relaxed(&chunks[j].w[1]);
-            }
-            const unsigned long long fa = wa & FLAG_MASK, fb = wb & FLAG_MASK;
-            int stop;
-            if (!window_ready(fa != 0 && fa == fb, fa == FLAG_INC && fb == FLAG_INC, lane, stop)) {
-                if (!poll_backoff(round)) return false;
-                continue;
-            }
-            cx_b += wave_sum(lane <= stop ? (wa & VAL_MASK) : 0ull);
-            cx_c += wave_sum(lane <= stop ? (wb & VAL_MASK) : 0ull);
-            if (stop < 64) break;
-            base -= 64;
+    int64_t base = (int64_t)c - 1;
+    round = 0;
+    while (base >= 0) {
+        const unsigned long long fa = q.ca & FLAG_MASK, fb = q.cb & FLAG_MASK;
+        int stop;
+        if (!window_ready(fa != 0 && fa == fb, fa == FLAG_INC && fb == FLAG_INC, stop)) {
+            if (!poll_backoff(round)) return false;
+            lookback_chunk_words(L, base - lane, q);
+            continue;
         }
-        if (last_in_chunk && lane == 0)
-            publish_pair(&chunks[chunk], FLAG_INC, cx_b + in_b + agg_bytes, cx_c + in_c + agg_cnt);
+        cx_b += wave_sum(lane <= stop ? (q.ca & VAL_MASK) : 0ull);
+        cx_c += wave_sum(lane <= stop ? (q.cb & VAL_MASK) : 0ull);
+        if (stop < 64) break;
+        base -= LB_WIN;
+        lookback_chunk_words(L, base - lane, q);
     }
+    if (publish && li == 0 && c > 0 && lane == 0) publish_pair(&L.chunks[c - 1], FLAG_INC, cx_b, cx_c);
     ex_bytes = cx_b + in_b;
     ex_cnt = cx_c + in_c;
     return true;

@@ -13,8 +13,10 @@
 //     XOR leading-byte encoding are computed in registers; the records of the
 //     tile are staged in LDS;
 //   * the tile's record bytes and constant-block count are published through
-//     a two-level decoupled look-back (tiles within 64-tile chunks, then
-//     chunk totals; the tile order is the launch order, blockIdx.x);
+//     a two-level decoupled look-back (tiles within 64-tile chunks, whose
+//     aggregates build up in one atomic accumulator per chunk; the tile order
+//     is the launch order, blockIdx.x), whose reads are issued before the
+//     records are staged so that staging overlaps their round trip;
 //   * records are written at payload + (bytes of all earlier records).
 //
 // The payload start does not depend on the number of constant blocks: the
@@ -90,14 +92,16 @@ __device__ inline uint32_t wave0_exclusive_scan(uint32_t *a, int lane)
 }
 
 // What a lane keeps of one slab between encoding and staging.
+//   mid   = per value the word with its mid bytes as the low bytes
 //   meta  = mid-byte offset in the record (12) | rb (4) << 12 |
 //           non-constant << 16 | reqLength (7) << 17
 //   code  = the lane's 2-bit lead codes (LV/4 bytes) | lead bytes of the
 //           block (7) << 16
-//   mbits = LV 4-bit mid-byte counts
+//   mb    = per value 8 x (mid bytes), one byte each
 template <typename T, int LV> struct SlabState {
-    typename Word<T>::U s[LV];
-    uint32_t meta, code, mbits;
+    typename Word<T>::U mid[LV];
+    uint64_t mb;
+    uint32_t meta, code;
     float hmed;
 };
 
@@ -108,15 +112,15 @@ template <> __device__ inline float vmax<float>(float a, float b) { return __bui
 template <> __device__ inline double vmin<double>(double a, double b) { return __builtin_fmin(a, b); }
 template <> __device__ inline double vmax<double>(double a, double b) { return __builtin_fmax(a, b); }
 
-// Leading bytes (capped at 3) that s shares with the previous word: the
-// reference tests xr >> 8(W-3), >> 8(W-2), >> 8(W-1) against zero
+// 8 x the leading bytes (capped at 3) that s shares with the previous word:
+// the reference tests xr >> 8(W-3), >> 8(W-2), >> 8(W-1) against zero
 // (szx_float.cc:96-101, szx_double.cc:80-85), i.e. min(clz(xr) / 8, 3) on the
 // top 32 bits of the word.
-template <typename U> __device__ inline int lead_bytes(U xr)
+template <typename U> __device__ inline int lead8(U xr)
 {
     const uint32_t top = sizeof(U) == 4 ? (uint32_t)xr : (uint32_t)((uint64_t)xr >> 32);
     const int lz = __builtin_clzg(top, 32);
-    return (lz >> 3) < 3 ? (lz >> 3) : 3;
+    return (lz < 24 ? lz : 24) & ~7;
 }
 
 // Order-preserving integer key of a float: signed comparison of keys is the
@@ -177,36 +181,11 @@ __device__ __forceinline__ void encode_slab(const T (&v)[LV], uint64_t e0, uint6
         for (int k = 0; k < LV; k++) valid[k] = e0 + k < n;
     }
 
+    // fmin/fmax ignore NaNs (quiet or signalling: the compiler quiets its
+    // operands first), as the reference's comparisons do; a NaN first value
+    // makes both NaN (minmax_float).
     T mn, mx;
-    if constexpr (W == 4) {
-        // integer keys reduce with DPP-fused v_min_i32 / v_max_i32; a block
-        // holding a NaN shows up as a key beyond +-inf and takes the exact
-        // float path (the whole wave, rarely)
-        int32_t kmn = 0x7FFFFFFF, kmx = (int32_t)0x80000000;
-#pragma unroll
-        for (int k = 0; k < LV; k++) {
-            const int32_t key = fkey(f32_bits(v[k]));
-            if (FULL || valid[k]) {
-                kmn = key < kmn ? key : kmn;
-                kmx = key > kmx ? key : kmx;
-            }
-        }
-        kmn = (int32_t)seg_reduce<SEG>((uint32_t)kmn, [](uint32_t a, uint32_t b) {
-            return (uint32_t)((int32_t)a < (int32_t)b ? (int32_t)a : (int32_t)b);
-        });
-        kmx = (int32_t)seg_reduce<SEG>((uint32_t)kmx, [](uint32_t a, uint32_t b) {
-            return (uint32_t)((int32_t)a > (int32_t)b ? (int32_t)a : (int32_t)b);
-        });
-        const bool has_nan = exists && (kmn < FKEY_NEG_INF || kmx > FKEY_POS_INF);
-        if (__builtin_expect(__ballot(has_nan) != 0, 0)) {
-            minmax_float<T, LV, SEG, FULL>(v, valid, mn, mx);
-        } else {
-            mn = bits_f32((uint32_t)fkey((uint32_t)kmn));
-            mx = bits_f32((uint32_t)fkey((uint32_t)kmx));
-        }
-    } else {
-        minmax_float<T, LV, SEG, FULL>(v, valid, mn, mx);
-    }
+    minmax_float<T, LV, SEG, FULL>(v, valid, mn, mx);
 
     // radius, median, constant test (szx_float.cc:335-347,
     // szx_double.cc:990-1003): median and radius are kept as float.
@@ -225,26 +204,37 @@ __device__ __forceinline__ void encode_slab(const T (&v)[LV], uint64_t e0, uint6
     // (szx_float.cc:60,90 vs szx_double.cc:38,78).
     const T msub = W == 4 ? (T)hmed : (T)med_f;
 
+    U w[LV];
 #pragma unroll
-    for (int k = 0; k < LV; k++) st.s[k] = to_bits<T>(v[k] - msub) >> shift;
-    U prev = seg_prev<SEG>(st.s[LV - 1], lane);
+    for (int k = 0; k < LV; k++) w[k] = to_bits<T>(v[k] - msub) >> shift;
+    U prev = seg_prev<SEG>(w[LV - 1], lane);
 
-    uint32_t code = 0, mbits = 0, tot = 0;
+    // Per value: lead bytes ld (shared with the previous word), mid bytes
+    // m = rb - ld (>= 0), and the word shifted so that its mid bytes
+    // [lo, lo + m), lo = W - rb, are its low bytes (only those are staged).
+    const int rb8 = 8 * rb, lo8 = 8 * W - rb8;
+    uint32_t codex = 0, tot8 = 0;
+    uint64_t mb = 0;
 #pragma unroll
     for (int k = 0; k < LV; k++) {
-        const int ld = lead_bytes<U>(st.s[k] ^ prev);
-        int m = rb - ld;
-        m = m > 0 ? m : 0;
-        uint32_t ldv = (uint32_t)ld;
+        const int ld8 = lead8<U>(w[k] ^ prev);
+        int m8 = rb8 - ld8;
+        m8 = m8 > 0 ? m8 : 0;
+        U mid = w[k] >> lo8;
+        uint32_t ldv = (uint32_t)ld8;
         if constexpr (!FULL) {
-            if (!valid[k]) { m = 0; ldv = 0; }
+            if (!valid[k]) { m8 = 0; ldv = 0; mid = 0; }
         }
         // element k -> byte k/4 of the lane's lead codes, bits 7-6 first
-        code |= ldv << (8 * (k >> 2) + 6 - 2 * (k & 3));
-        mbits |= (uint32_t)m << (4 * k);
-        tot += (uint32_t)m;
-        prev = st.s[k];
+        // (8 ld here; the sum is divided by 8 below)
+        codex |= ldv << (8 * (k >> 2) + 6 - 2 * (k & 3));
+        mb |= (uint64_t)(uint32_t)m8 << (8 * k);
+        tot8 += (uint32_t)m8;
+        st.mid[k] = mid;
+        prev = w[k];
     }
+    const uint32_t code = codex >> 3;
+    const uint32_t tot = tot8 >> 3;
     const uint32_t inc = seg_scan_incl<SEG>(tot, lane);
     const uint32_t midtot = seg_last<SEG>(inc);
     const uint32_t nlead = (uint32_t)(nel + 3) >> 2;
@@ -252,21 +242,22 @@ __device__ __forceinline__ void encode_slab(const T (&v)[LV], uint64_t e0, uint6
     st.meta = (inc - tot) | ((uint32_t)rb << 12) | ((nonconst ? 1u : 0u) << 16) |
               ((uint32_t)len << 17);
     st.code = code | (nlead << 16);
-    st.mbits = mbits;
+    st.mb = mb;
     st.hmed = hmed;
     recb = nonconst ? 5u + nlead + midtot : 0u;
     isconst = exists && !nonconst ? 1u : 0u;
     med_out = med_f;
 }
 
-// Write the m low bytes of t to LDS at byte `pos` (every payload byte of a
-// tile is written exactly once, so plain byte stores need no zeroing).
+// Write the m low bytes of t to LDS at p (every payload byte of a tile is
+// written exactly once, by the lane that owns it, so the stage needs no
+// clearing).
 template <typename U>
-__device__ inline void stage_bytes(uint8_t *stage, uint32_t pos, U t, int m)
+__device__ inline void stage_bytes(uint8_t *p, U t, int m)
 {
 #pragma unroll
     for (int j = 0; j < (int)sizeof(U); j++)
-        if (j < m) stage[pos + j] = (uint8_t)(t >> (8 * j));
+        if (j < m) p[j] = (uint8_t)(t >> (8 * j));
 }
 
 // Shape of a compressor tile: T, block size, values per lane, slabs.
@@ -279,46 +270,58 @@ template <typename T, int BS, int LV, int SLABS> struct TileShape {
     static constexpr int STAGE_DW = (STAGE + 32 + 15) / 16 * 4;
 };
 
-// The LDS one tile needs (a kernel declares it, or a union of several).
-template <typename T, int BS, int LV, int SLABS> struct CompressLds {
+// A tile's block words (scan, medians, bitmap bits, total) and its staged
+// records.
+template <typename T, int BS, int LV, int SLABS> struct TileMeta {
     using S = TileShape<T, BS, LV, SLABS>;
-    __attribute__((aligned(16))) uint32_t stage32[S::STAGE_DW];   // staged records
     uint32_t pack[S::NBLK];       // record bytes | constant<<20 -> exclusive prefix
     float med[S::NBLK];           // medians (constant blocks)
     uint8_t state[S::NBLK];       // 1 = non-constant (bitmap bit)
-    unsigned long long excl[2];   // the tile's exclusive (bytes, constants)
     uint32_t total;               // the tile's (bytes | constants << 20)
 };
+template <typename T, int BS, int LV, int SLABS> struct Stage {
+    __attribute__((aligned(16))) uint32_t w[TileShape<T, BS, LV, SLABS>::STAGE_DW];   // staged records
+};
+template <typename T, int BS, int LV, int SLABS> struct CompressLds {
+    Stage<T, BS, LV, SLABS> stage;
+    TileMeta<T, BS, LV, SLABS> meta;
+    unsigned long long excl[2];   // the tile's exclusive (bytes, constants)
+};
 
-// Encode tile `tile` of the field described by p (all 256 lanes call it).
-template <typename T, int BS, int LV, int SLABS, int IPT = INDEX_PER_TILE>
-__device__ inline void compress_tile(const CompressParams &p, uint64_t tile,
-                                     CompressLds<T, BS, LV, SLABS> &lds)
+// Issue every load of a tile (values past n read as zero and are masked).
+template <typename T, int LV, int SLABS>
+__device__ __forceinline__ void load_tile(const CompressParams &p, uint64_t tile, T (&v)[SLABS][LV])
 {
-    using U = typename Word<T>::U;
+    constexpr int SLAB = 256 * LV;
+    const T *x = static_cast<const T *>(p.x);
+    const uint64_t t0 = tile * (uint64_t)(SLAB * SLABS);
+#pragma unroll
+    for (int u = 0; u < SLABS; u++)
+#pragma unroll
+        for (int q = 0; q < LV; q += 4)
+            SlabLoad<T>::load4(x, t0 + (uint64_t)u * SLAB + (uint64_t)LV * threadIdx.x + q, p.n, &v[u][q]);
+}
+
+// Encode a tile whose values are in v (all 256 lanes): block statistics and
+// records in registers (st), the exclusive scan of record sizes over the
+// tile's blocks (M), and the aggregate published for the look-back.  `pend`
+// (lane 0 of wave 0) receives the chunk accumulator's previous value for
+// lookback_chunk_done.
+template <typename T, int BS, int LV, int SLABS, int IPT>
+__device__ __forceinline__ void encode_scan(const CompressParams &p, const LookbackRefs &L, uint64_t tile,
+                                            const T (&v)[SLABS][LV], SlabState<T, LV> (&st)[SLABS],
+                                            TileMeta<T, BS, LV, SLABS> &M, unsigned long long &pend)
+{
     using S = TileShape<T, BS, LV, SLABS>;
-    constexpr int W = Word<T>::W;
     constexpr int SEG = BS / LV;          // lanes per block
-    constexpr int NLB = LV / 4;           // lead-code bytes per lane
     constexpr int SLAB = S::SLAB;
     constexpr int TILE = S::TILE;
     constexpr int BPS = SLAB / BS;        // blocks per slab
     constexpr int NBLK = S::NBLK;
-    constexpr int STAGE = S::STAGE;
-    constexpr int IT = IPT;               // index tiles per tile
-    constexpr int NBLKI = NBLK / IT;      // blocks per index tile
     static_assert(BS >= LV && BS <= 256 && (BS & (BS - 1)) == 0, "power-of-two block size");
     static_assert(NBLK % 8 == 0, "tiles own whole bitmap bytes");
-    static_assert(NBLKI % 8 == 0, "index tiles own whole bitmap bytes");
-    static_assert(STAGE < (1 << 16), "record starts fit the 16-bit block index");
-
-    uint32_t *s_pack = lds.pack;
-    uint8_t *s_state = lds.state;
-    float *s_med = lds.med;
-    uint32_t *s_stage32 = lds.stage32;
-    unsigned long long *s_excl = lds.excl;
-    uint32_t &s_total = lds.total;
-    uint8_t *s_stage = reinterpret_cast<uint8_t *>(s_stage32);
+    static_assert((NBLK / IPT) % 8 == 0, "index tiles own whole bitmap bytes");
+    static_assert(S::STAGE < (1 << 16), "record starts fit the 16-bit block index");
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -326,19 +329,9 @@ __device__ inline void compress_tile(const CompressParams &p, uint64_t tile,
     const float eb = p.eb_dev ? *p.eb_dev : p.eb;
     const int reqexp = req_expo(eb);
     const uint64_t n = p.n;
-    const T *x = static_cast<const T *>(p.x);
     const uint64_t t0 = tile * TILE;
     const bool full = t0 + TILE <= n;
 
-    // issue every load of the tile before any arithmetic
-    T v[SLABS][LV];
-#pragma unroll
-    for (int u = 0; u < SLABS; u++)
-#pragma unroll
-        for (int q = 0; q < LV; q += 4)
-            SlabLoad<T>::load4(x, t0 + (uint64_t)u * SLAB + (uint64_t)LV * tid + q, n, &v[u][q]);
-
-    SlabState<T, LV> st[SLABS];
 #pragma unroll
     for (int u = 0; u < SLABS; u++) {
         const uint64_t e0 = t0 + (uint64_t)u * SLAB + (uint64_t)LV * tid;
@@ -350,29 +343,44 @@ __device__ inline void compress_tile(const CompressParams &p, uint64_t tile,
             encode_slab<T, BS, LV, false>(v[u], e0, n, lane, lseg, reqexp, eb, st[u], recb, isconst, med);
         if (lseg == 0) {
             const int lb = u * BPS + tid / SEG;
-            s_pack[lb] = recb | (isconst << 20);
-            s_state[lb] = recb ? 1 : 0;
-            s_med[lb] = med;
+            M.pack[lb] = recb | (isconst << 20);
+            M.state[lb] = recb ? 1 : 0;
+            M.med[lb] = med;
         }
     }
     __syncthreads();
 
     if (tid < 64) {
-        const uint32_t tot = wave0_exclusive_scan<NBLK>(s_pack, lane);
-        if (lane == 0) s_total = tot;
+        const uint32_t tot = wave0_exclusive_scan<NBLK>(M.pack, lane);
+        if (lane == 0) {
+            M.total = tot;
+            pend = lookback_publish(L, tile, tot & 0xFFFFFu, tot >> 20);
+        }
     }
     __syncthreads();
+}
 
-    // Stage the tile's records in LDS (tile-relative byte offsets; each byte
-    // is written once, by the lane that owns it).
+// Stage an encoded tile's records in LDS (tile-relative byte offsets; each
+// byte is written once, by the lane that owns it).  M must hold the tile's
+// scan (encode_scan, then a barrier).
+template <typename T, int BS, int LV, int SLABS>
+__device__ __forceinline__ void stage_records(const SlabState<T, LV> (&st)[SLABS],
+                                              const TileMeta<T, BS, LV, SLABS> &M, Stage<T, BS, LV, SLABS> &G)
+{
+    using U = typename Word<T>::U;
+    constexpr int SEG = BS / LV;
+    constexpr int NLB = LV / 4;           // lead-code bytes per lane
+    constexpr int BPS = 256 * LV / BS;    // blocks per slab
+    uint8_t *s_stage = reinterpret_cast<uint8_t *>(G.w);
+    const int tid = threadIdx.x;
+    const int lseg = tid & (SEG - 1);
 #pragma unroll
     for (int u = 0; u < SLABS; u++) {
         const uint32_t meta = st[u].meta;
         if (!((meta >> 16) & 1)) continue;
         const int lb = u * BPS + tid / SEG;
-        const uint32_t base = s_pack[lb] & 0xFFFFFu;
+        const uint32_t base = M.pack[lb] & 0xFFFFFu;
         const uint32_t nlead = st[u].code >> 16;
-        const int rb = (meta >> 12) & 15;
         if (lseg == 0) {   // reqLength byte + big-endian median
             const uint32_t hb = bswap32(f32_bits(st[u].hmed));
             s_stage[base] = (uint8_t)(meta >> 17);
@@ -385,61 +393,52 @@ __device__ inline void compress_tile(const CompressParams &p, uint64_t tile,
         for (int b = 0; b < NLB; b++)
             if ((uint32_t)(lseg * NLB + b) < nlead)
                 s_stage[base + 5 + lseg * NLB + b] = (uint8_t)(st[u].code >> (8 * b));
-        uint32_t pos = base + 5 + nlead + (meta & 0xFFFu);
-        const int lo = W - rb;
+        uint8_t *q = s_stage + base + 5 + nlead + (meta & 0xFFFu);
 #pragma unroll
         for (int k = 0; k < LV; k++) {
-            const int m = (st[u].mbits >> (4 * k)) & 15;
-            stage_bytes<U>(s_stage, pos, st[u].s[k] >> (8 * lo), m);
-            pos += m;
+            const int m = (int)(st[u].mb >> (8 * k + 3)) & 31;
+            stage_bytes<U>(q, st[u].mid[k], m);
+            q += m;
+        }
+    }
+}
+
+// Write out a staged tile once its exclusive prefix (ebytes, econst) is known
+// (all 256 lanes; B's staging must be complete): the record index entries,
+// the payload (16-byte stores inside, bytes at the edges, whose other bytes
+// belong to the neighbouring tiles), constant medians and bitmap bytes.
+template <typename T, int BS, int LV, int SLABS, int IPT>
+__device__ __forceinline__ void copy_out(const CompressParams &p, uint64_t tile, const TileMeta<T, BS, LV, SLABS> &B,
+                                         const Stage<T, BS, LV, SLABS> &G, uint64_t ebytes, uint64_t econst)
+{
+    using S = TileShape<T, BS, LV, SLABS>;
+    constexpr int NBLK = S::NBLK;
+    constexpr int IT = IPT;               // index tiles per tile
+    constexpr int NBLKI = NBLK / IT;      // blocks per index tile
+    const int tid = threadIdx.x;
+    const uint32_t tw = B.total;
+    const uint32_t tbytes = tw & 0xFFFFFu;
+    const uint8_t *s_stage = reinterpret_cast<const uint8_t *>(G.w);
+
+    // record index: one pair per index tile, from the tile's prefix and the
+    // scanned word at the index tile's first block
+    if (p.index && tid < IT) {
+        const uint64_t it = tile * IT + tid;
+        if (it < p.nidx) {
+            const uint32_t pk = B.pack[tid * NBLKI];
+            p.index[2 * it] = ebytes + (pk & 0xFFFFFu);
+            p.index[2 * it + 1] = econst + (pk >> 20);
+        }
+    }
+    if (tid == 0 && tile == p.ntiles - 1) {
+        p.totals[0] = ebytes + tbytes;
+        p.totals[1] = econst + (tw >> 20);
+        if (p.index) {
+            p.index[2 * p.nidx] = ebytes + tbytes;
+            p.index[2 * p.nidx + 1] = econst + (tw >> 20);
         }
     }
 
-    // Decoupled look-back over (record bytes, constant blocks).
-    __syncthreads();
-    {
-        const uint32_t tw = s_total;
-        const unsigned long long tbytes = tw & 0xFFFFFu;
-        const unsigned long long tconst = tw >> 20;
-        unsigned long long ebytes = 0, econst = 0;
-        if (tid < 64 &&
-            !lookback_2level(p.tiles, p.chunks, tile, p.ntiles, tbytes, tconst, ebytes, econst)) {
-            ebytes = 0;   // keeps every write in bounds; the call reports failure
-            econst = 0;
-            if (tid == 0) atomicExch(p.err, 1);
-        }
-        // record index: one pair per index tile (IT per compressor tile),
-        // from the tile's prefix and the scanned word at the index tile's
-        // first block
-        if (p.index && tid < IT) {
-            const uint64_t it = tile * IT + tid;
-            if (it < p.nidx) {
-                const uint32_t pk = s_pack[tid * NBLKI];
-                p.index[2 * it] = ebytes + (pk & 0xFFFFFu);
-                p.index[2 * it + 1] = econst + (pk >> 20);
-            }
-        }
-        if (tid == 0) {
-            s_excl[0] = ebytes;
-            s_excl[1] = econst;
-            if (tile == p.ntiles - 1) {
-                p.totals[0] = ebytes + tbytes;
-                p.totals[1] = econst + tconst;
-                if (p.index) {
-                    p.index[2 * p.nidx] = ebytes + tbytes;
-                    p.index[2 * p.nidx + 1] = econst + tconst;
-                }
-            }
-        }
-    }
-    __syncthreads();
-
-    const uint64_t ebytes = s_excl[0];
-    const uint64_t econst = s_excl[1];
-    const uint32_t tbytes = s_total & 0xFFFFFu;
-
-    // Copy the staged records out: 16-byte stores inside, bytes at the edges
-    // (the neighbouring tiles own the other bytes of an edge chunk).
     {
         uint8_t *dst = p.payload + ebytes;
         const uint32_t head = (uint32_t)((16 - ((uintptr_t)dst & 15)) & 15);
@@ -449,7 +448,7 @@ __device__ inline void compress_tile(const CompressParams &p, uint64_t tile,
         const int r = h & 3;
         for (uint32_t c = tid; c < nchunk; c += 256) {
             const uint32_t o = h + 16 * c;             // stage offset of the chunk
-            const uint32_t *sw = s_stage32 + (o >> 2);
+            const uint32_t *sw = G.w + (o >> 2);
             const uint32_t a0 = sw[0], a1 = sw[1], a2 = sw[2], a3 = sw[3];
             const uint32_t a4 = r ? sw[4] : 0u;
             uint4 val;
@@ -471,17 +470,17 @@ __device__ inline void compress_tile(const CompressParams &p, uint64_t tile,
     // block sits in the high bits of its scanned word, so a block is
     // constant exactly when the next block's count is one higher.
     for (int lb = tid; lb < NBLK; lb += 256) {
-        const uint32_t cexcl = s_pack[lb] >> 20;
-        const uint32_t cnext = lb + 1 < NBLK ? (s_pack[lb + 1] >> 20) : (s_total >> 20);
+        const uint32_t cexcl = B.pack[lb] >> 20;
+        const uint32_t cnext = lb + 1 < NBLK ? (B.pack[lb + 1] >> 20) : (tw >> 20);
         if (cnext != cexcl)
-            p.cmed[econst + cexcl] = bswap32(f32_bits(s_med[lb]));
+            p.cmed[econst + cexcl] = bswap32(f32_bits(B.med[lb]));
     }
     for (int i = tid; i < NBLK / 8; i += 256) {
         const uint64_t gbyte = tile * (NBLK / 8) + i;
         if (gbyte < p.nbit) {
             uint32_t b = 0;
 #pragma unroll
-            for (int k = 0; k < 8; k++) b |= (uint32_t)s_state[8 * i + k] << (7 - k);
+            for (int k = 0; k < 8; k++) b |= (uint32_t)B.state[8 * i + k] << (7 - k);
             p.bitmap[gbyte] = (uint8_t)b;
         }
     }
@@ -492,19 +491,62 @@ __device__ inline void compress_tile(const CompressParams &p, uint64_t tile,
         for (int lb = tid; lb < NBLK; lb += 256) {
             const uint64_t b = tile * NBLK + lb;
             if (b < p.nb) {
-                const uint32_t base = s_pack[(lb / NBLKI) * NBLKI];
-                p.bidx[b] = (uint16_t)(s_state[lb] ? ((s_pack[lb] - base) & 0xFFFFFu)
-                                                   : ((s_pack[lb] >> 20) - (base >> 20)));
+                const uint32_t base = B.pack[(lb / NBLKI) * NBLKI];
+                p.bidx[b] = (uint16_t)(B.state[lb] ? ((B.pack[lb] - base) & 0xFFFFFu)
+                                                   : ((B.pack[lb] >> 20) - (base >> 20)));
             }
         }
     }
 }
 
-template <typename T, int BS, int LV, int SLABS>
+// Finish a tile's look-back from its polled words (wave 0) and leave its
+// exclusive prefix in ex[0..1] for the workgroup (read after a barrier).  On a
+// wait that overran its bound the call is flagged failed and the prefix is
+// zero, which keeps every write in bounds.
+__device__ __forceinline__ void tile_prefix(const CompressParams &p, const LookbackRefs &L, uint64_t tile,
+                                            const LookbackPoll &q, unsigned long long *ex)
+{
+    if (threadIdx.x < 64) {
+        unsigned long long eb = 0, ec = 0;
+        if (!lookback_finish(L, tile, q, true, eb, ec)) {
+            eb = 0;
+            ec = 0;
+            if (threadIdx.x == 0) atomicExch(p.err, 1);
+        }
+        if (threadIdx.x == 0) {
+            ex[0] = eb;
+            ex[1] = ec;
+        }
+    }
+}
+
+// One tile per workgroup.  The tile publishes its aggregate right after
+// its scan and issues its look-back reads at once; staging the records then
+// overlaps their round trip.  Every wave issues the same few look-back reads
+// (no wave's wait for its own loads has to include them); wave 0 consumes
+// them.
+template <typename T, int BS, int LV, int SLABS, int IPT>
+__device__ __forceinline__ void compress_tile(const CompressParams &p, uint64_t tile, CompressLds<T, BS, LV, SLABS> &lds)
+{
+    const LookbackRefs L{p.tiles, p.chunks, p.cacc, p.ntiles};
+    T v[SLABS][LV];
+    unsigned long long pend = 0;
+    load_tile<T, LV, SLABS>(p, tile, v);
+    SlabState<T, LV> st[SLABS];
+    encode_scan<T, BS, LV, SLABS, IPT>(p, L, tile, v, st, lds.meta, pend);
+    const LookbackPoll q = lookback_poll(L, tile);
+    stage_records<T, BS, LV, SLABS>(st, lds.meta, lds.stage);
+    if (threadIdx.x == 0) lookback_chunk_done(L, tile, pend, lds.meta.total & 0xFFFFFu, lds.meta.total >> 20);
+    tile_prefix(p, L, tile, q, lds.excl);
+    __syncthreads();
+    copy_out<T, BS, LV, SLABS, IPT>(p, tile, lds.meta, lds.stage, lds.excl[0], lds.excl[1]);
+}
+
+template <typename T, int BS, int LV, int SLABS, int IPT>
 __global__ __launch_bounds__(256) void k_compress(CompressParams p)
 {
     __shared__ CompressLds<T, BS, LV, SLABS> lds;
-    compress_tile<T, BS, LV, SLABS>(p, blockIdx.x, lds);
+    compress_tile<T, BS, LV, SLABS, IPT>(p, blockIdx.x, lds);
 }
 
 // Zero the look-back state and totals (a kernel, not a memset, so the
@@ -624,11 +666,10 @@ __global__ __launch_bounds__(256) void k_range_final(const T *x, const T *part, 
 
 // ---------------------------------------------------------------------------
 // Batched compress: many fields (fp32 and fp64 mixed, any sizes) in one
-// persistent launch.  Every field keeps its own look-back state; the
-// compressor tiles of all fields form one global list (field by field), which
-// the workgroups walk round-robin.  A tile waits only on earlier tiles of its
-// own field, which have lower global numbers, so with every workgroup
-// resident the walk always progresses.
+// launch.  Every field keeps its own look-back state; the compressor tiles of
+// all fields form one global list (field by field), one workgroup per tile.
+// A tile waits only on earlier tiles of its own field, which have lower
+// workgroup numbers and so have started before it.
 // ---------------------------------------------------------------------------
 
 // The field owning flattened item `item`: the last f with base[f] <= item
@@ -733,9 +774,11 @@ __global__ void k_batch_range_final(BatchField *bf, const uint64_t *rbase, int n
     }
 }
 
+// Many fields: one workgroup per tile of the batch's global tile list
+// (fields in order, tbase[f] = first global tile of field f).  Workgroups
+// start in index order, so every tile a look-back waits for has started.
 template <int BS, int LV, int IPT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_compress_batch(const BatchField *bf, const uint64_t *tbase, int nf,
-                                                        uint64_t total)
+__global__ __launch_bounds__(256) void k_compress_batch(const BatchField *bf, const uint64_t *tbase, int nf)
 {
     constexpr int SF = IPT * TILE_VALUES_F32 / (256 * LV);
     constexpr int SD = IPT * TILE_VALUES_F64 / (256 * LV);
@@ -743,16 +786,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
         CompressLds<float, BS, LV, SF> f;
         CompressLds<double, BS, LV, SD> d;
     } lds;
-    int f = 0;
-    for (uint64_t g = blockIdx.x; g < total; g += gridDim.x) {
-        while (f + 1 < nf && tbase[f + 1] <= g) f++;   // fields in tile order: f only grows
-        CompressParams p = bf[f].p;
-        p.ntiles = bf[f].ntiles[IPT - 1];
-        p.chunks = bf[f].chunks;
-        if (bf[f].w == 4) compress_tile<float, BS, LV, SF, IPT>(p, g - tbase[f], lds.f);
-        else compress_tile<double, BS, LV, SD, IPT>(p, g - tbase[f], lds.d);
-        __syncthreads();   // the tile's LDS is reused
-    }
+    const uint64_t g = blockIdx.x;
+    const int f = field_of(tbase, nf, g);
+    CompressParams p = bf[f].p;
+    p.ntiles = bf[f].ntiles[IPT - 1];
+    if (bf[f].w == 4) compress_tile<float, BS, LV, SF, IPT>(p, g - tbase[f], lds.f);
+    else compress_tile<double, BS, LV, SD, IPT>(p, g - tbase[f], lds.d);
 }
 
 // Clear every field's look-back words, totals and error flag (one block per
@@ -811,13 +850,15 @@ CompressLayout compress_layout(uint64_t n, int w, int bs)
     L.ntiles = ceil_div(n, compress_tile_values(w));
     if (L.ntiles == 0) L.ntiles = 1;
     L.nidx = index_tiles(n, w);
-    // workspace: tile words | chunk words | misc (totals, eb) | range parts | bitmap | cmed.
+    // workspace: tile words | chunk words | chunk accumulators | misc (totals,
+    // eb) | range parts | bitmap | cmed.
     // The look-back words are sized for one index tile per compressor tile,
     // the smallest tile any launch shape uses (the batched launch shares this
     // layout).
     uint64_t off = 0;
     L.ws_tiles = off;   off += L.nidx * sizeof(unsigned long long);
     L.ws_chunks = off;  off += ceil_div(L.nidx, LB_CHUNK) * sizeof(ChunkState);
+    L.ws_cacc = off;    off += ceil_div(L.nidx, LB_CHUNK) * sizeof(unsigned long long);
     L.ws_misc = off;    off += 64 * sizeof(uint64_t);   // totals at +128 B, eb at +256 B, error at +384 B
     L.ws_parts = off;   off += 2 * kRangeParts * sizeof(double);
     L.ws_bitmap = off;  off += (L.nbit + 15) & ~(uint64_t)15;
@@ -833,7 +874,7 @@ static void launch_main(const CompressParams &p, uint64_t ntiles, hipStream_t st
     constexpr int LV = BS >= 8 ? 8 : 4;
     constexpr int TILE = INDEX_PER_TILE * (sizeof(T) == 4 ? TILE_VALUES_F32 : TILE_VALUES_F64);
     constexpr int SLABS = TILE / (256 * LV);
-    hipLaunchKernelGGL((k_compress<T, BS, LV, SLABS>), dim3((unsigned)ntiles), dim3(256), 0, st, p);
+    hipLaunchKernelGGL((k_compress<T, BS, LV, SLABS, INDEX_PER_TILE>), dim3((unsigned)ntiles), dim3(256), 0, st, p);
 }
 
 template <typename T>
@@ -911,6 +952,7 @@ int compress_launch(const void *d_in, uint64_t n, int w, int rel_mode, float abs
     p.cmed = reinterpret_cast<uint32_t *>(d_ws + L.ws_cmed);
     p.tiles = reinterpret_cast<unsigned long long *>(d_ws + L.ws_tiles);
     p.chunks = reinterpret_cast<ChunkState *>(d_ws + L.ws_chunks);
+    p.cacc = reinterpret_cast<unsigned long long *>(d_ws + L.ws_cacc);
     p.totals = reinterpret_cast<uint64_t *>(misc + 16);
     p.err = reinterpret_cast<int *>(misc + 48);
     p.index = d_index;
@@ -961,17 +1003,7 @@ void batch_range_launch(const BatchField *bf, const uint64_t *rbase, int nf, uin
 template <int BS, int LV, int IPT>
 static void launch_batch(const BatchField *bf, const uint64_t *tbase, int nf, uint64_t total, hipStream_t st)
 {
-    auto kern = k_compress_batch<BS, LV, IPT>;
-    static int resident = 0;   // per instantiation; the device is fixed for a process
-    if (!resident) {
-        int dev = 0, per_cu = 0, cus = 0;
-        (void)hipGetDevice(&dev);
-        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0);
-        resident = per_cu > 0 && cus > 0 ? per_cu * cus : 256;
-    }
-    const uint64_t grid = total < (uint64_t)resident ? total : (uint64_t)resident;
-    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(256), 0, st, bf, tbase, nf, total);
+    if (total) hipLaunchKernelGGL((k_compress_batch<BS, LV, IPT>), dim3((unsigned)total), dim3(256), 0, st, bf, tbase, nf);
 }
 
 template <int BS>

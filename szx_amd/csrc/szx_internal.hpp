@@ -51,7 +51,7 @@ struct CompressLayout {
     uint64_t capacity;    // bytes d_out must hold
     uint64_t ntiles;      // compressor tiles
     uint64_t nidx;        // index tiles
-    uint64_t ws_tiles, ws_chunks, ws_misc, ws_parts, ws_bitmap, ws_cmed, ws_bytes;
+    uint64_t ws_tiles, ws_chunks, ws_cacc, ws_misc, ws_parts, ws_bitmap, ws_cmed, ws_bytes;
 };
 
 CompressLayout compress_layout(uint64_t n, int w, int bs);

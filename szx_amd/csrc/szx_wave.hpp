@@ -54,6 +54,24 @@ template <> struct Pieces<uint64_t> {
     }
 };
 
+// DPP move with an undefined old value: every lane of the permutations used
+// with it (quad_perm, row_mirror, row_half_mirror) reads a valid source, and
+// the compiler can then fold the move into the ALU op that consumes it.
+template <int CTRL>
+__device__ inline uint32_t dppu32(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xF, 0xF, true);
+}
+
+template <int CTRL, typename V>
+__device__ inline V dppu(V v)
+{
+    V r = v;
+#pragma unroll
+    for (int i = 0; i < Pieces<V>::N; i++) r = Pieces<V>::put(r, i, dppu32<CTRL>(Pieces<V>::get(v, i)));
+    return r;
+}
+
 template <int CTRL, int ROWMASK = 0xF, typename V>
 __device__ inline V dppv(V old, V v)
 {
@@ -80,10 +98,10 @@ __device__ inline V swzv(V v)
 template <int D, typename V>
 __device__ inline V partner(V v)
 {
-    if constexpr (D == 1) return dppv<DPP_QUAD_XOR1>(v, v);
-    else if constexpr (D == 2) return dppv<DPP_QUAD_XOR2>(v, v);
-    else if constexpr (D == 4) return dppv<DPP_ROW_HALF_MIRROR>(v, v);
-    else if constexpr (D == 8) return dppv<DPP_ROW_MIRROR>(v, v);
+    if constexpr (D == 1) return dppu<DPP_QUAD_XOR1>(v);
+    else if constexpr (D == 2) return dppu<DPP_QUAD_XOR2>(v);
+    else if constexpr (D == 4) return dppu<DPP_ROW_HALF_MIRROR>(v);
+    else if constexpr (D == 8) return dppu<DPP_ROW_MIRROR>(v);
     else if constexpr (D == 16) return swzv<SWZ(0x1F, 0, 0x10)>(v);
     else return __shfl_xor(v, 32, 64);
 }
@@ -105,11 +123,14 @@ __device__ inline V seg_reduce(V v, Op op)
 template <int SEG>
 __device__ inline uint32_t seg_scan_incl(uint32_t v, int lane)
 {
+    // row_shr reads 0 where the source lane is outside the 16-lane row, so a
+    // segment spanning whole rows needs no lane predicate
     const int ls = lane & (SEG - 1);
-    if constexpr (SEG >= 2) { uint32_t o = dpp32<DPP_ROW_SHR(1)>(0u, v); if (ls >= 1) v += o; }
-    if constexpr (SEG >= 4) { uint32_t o = dpp32<DPP_ROW_SHR(2)>(0u, v); if (ls >= 2) v += o; }
-    if constexpr (SEG >= 8) { uint32_t o = dpp32<DPP_ROW_SHR(4)>(0u, v); if (ls >= 4) v += o; }
-    if constexpr (SEG >= 16) { uint32_t o = dpp32<DPP_ROW_SHR(8)>(0u, v); if (ls >= 8) v += o; }
+    constexpr bool ROWS = SEG >= 16;
+    if constexpr (SEG >= 2) { uint32_t o = dpp32<DPP_ROW_SHR(1)>(0u, v); if (ROWS || ls >= 1) v += o; }
+    if constexpr (SEG >= 4) { uint32_t o = dpp32<DPP_ROW_SHR(2)>(0u, v); if (ROWS || ls >= 2) v += o; }
+    if constexpr (SEG >= 8) { uint32_t o = dpp32<DPP_ROW_SHR(4)>(0u, v); if (ROWS || ls >= 4) v += o; }
+    if constexpr (SEG >= 16) { uint32_t o = dpp32<DPP_ROW_SHR(8)>(0u, v); v += o; }
     if constexpr (SEG >= 32) v += dpp32<DPP_ROW_BCAST15, 0xA>(0u, v);
     if constexpr (SEG >= 64) v += dpp32<DPP_ROW_BCAST31, 0xC>(0u, v);
     return v;
