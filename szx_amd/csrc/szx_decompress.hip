@@ -64,17 +64,12 @@ __global__ void k_check_header(const uint8_t *s, uint64_t len, uint64_t n, int b
     if (threadIdx.x == 0 && blockIdx.x == 0) check_header(s, len, n, bs_expect, h, status);
 }
 
-// last-writer-wins combine of the packed copyable-byte record:
-// bytes 0..2 = values of positions W-3, W-2, W-1; bits 24..26 = written flags.
-__device__ inline uint32_t lw_combine(uint32_t left, uint32_t right)
-{
-    const uint32_t hm = (right >> 24) & 7u;
-    const uint32_t m = ((hm & 1u) ? 0xFFu : 0u) | ((hm & 2u) ? 0xFF00u : 0u) | ((hm & 4u) ? 0xFF0000u : 0u);
-    return ((right & m) | (left & ~m & 0xFFFFFFu)) | ((left | right) & 0x7000000u);
-}
-
 template <typename T> struct Store4;
 template <> struct Store4<float> {
+    __device__ static void store_full(float *o, uint64_t e0, const float v[4])
+    {
+        *reinterpret_cast<float4 *>(o + e0) = make_float4(v[0], v[1], v[2], v[3]);
+    }
     __device__ static void store(float *o, uint64_t e0, uint64_t n, const float v[4])
     {
         if (e0 + 4 <= n) {
@@ -86,6 +81,11 @@ template <> struct Store4<float> {
     }
 };
 template <> struct Store4<double> {
+    __device__ static void store_full(double *o, uint64_t e0, const double v[4])
+    {
+        *reinterpret_cast<double2 *>(o + e0) = make_double2(v[0], v[1]);
+        *reinterpret_cast<double2 *>(o + e0 + 2) = make_double2(v[2], v[3]);
+    }
     __device__ static void store(double *o, uint64_t e0, uint64_t n, const double v[4])
     {
         if (e0 + 4 <= n) {
@@ -161,7 +161,6 @@ __device__ inline void decode_lane(const TileLds &L, uint64_t e0, uint64_t n, in
     constexpr int W = Word<T>::W;
     constexpr int SEG = BS / LV;
     constexpr int NLB = LV / 4;          // lead-code bytes per lane
-    constexpr int NW = W == 4 ? 2 : 3;   // LDS words per gathered value
     T v[LV];
     if (!L.state[lb]) {
         const uint32_t rank = L.start[lb];
@@ -193,15 +192,14 @@ __device__ inline void decode_lane(const TileLds &L, uint64_t e0, uint64_t n, in
 #pragma unroll
         for (int b = 0; b < NLB; b++)
             code[b] = (rl_ok && (lseg * NLB + b) < nlead) ? rec[5 + lseg * NLB + b] : 0u;
-        int lk[LV];
-        uint32_t mk[LV], ofs[LV], tot = 0;
+        // lead bytes lk = min(code, rb) and mid bytes m = rb - lk per value
+        // (past the array end: no bytes)
+        uint32_t lk[LV], mk[LV], tot = 0;
 #pragma unroll
         for (int k = 0; k < LV; k++) {
-            const int ld = (code[k >> 2] >> (6 - 2 * (k & 3))) & 3;
-            lk[k] = ld < rb ? ld : rb;
-            const bool valid = FULL || e0 + k < n;
-            mk[k] = valid ? (uint32_t)(rb - lk[k]) : 0u;
-            ofs[k] = tot;
+            const uint32_t ld = (code[k >> 2] >> (6 - 2 * (k & 3))) & 3u;
+            lk[k] = ld < (uint32_t)rb ? ld : (uint32_t)rb;
+            mk[k] = (FULL || e0 + k < n) ? (uint32_t)rb - lk[k] : 0u;
             tot += mk[k];
         }
         const uint32_t inc = seg_scan_incl<SEG>(tot, lane);
@@ -211,61 +209,59 @@ __device__ inline void decode_lane(const TileLds &L, uint64_t e0, uint64_t n, in
             if (ok) L.size[lb] = (uint16_t)rsize;
             else *L.err = 1;
         }
-        // gather: value k's mk bytes start at run0 + ofs[k]; 2 (fp32) or 3
-        // (fp64) independent aligned word reads and a byte funnel each
-        const uint32_t run0 = (uint32_t)(rec - L.st) + 5u + (uint32_t)nlead + (ok ? inc - tot : 0u);
+        // Value k's m bytes start at stage byte p (p runs through the lane's
+        // bytes; a bad record reads from byte 0 on and is flagged above).  They
+        // land on word bytes [W - rb, W - lk): the gathered word shifted up by
+        // lo = W - rb bytes supplies bytes [0, W - lk) (zeros below lo), value
+        // k-1 the top lk bytes -- one bit-field insert.  Bits the final shift
+        // drops are never masked.  Pass 1 runs with a zero carry-in: t0[k] is
+        // value k's word if nothing came from the previous lane; inh[k] marks
+        // the top bytes value k still inherits from before the lane.
+        // (two or three aligned LDS words and a byte funnel per value:
+        // unaligned LDS reads are slow)
         const uint32_t *stw = reinterpret_cast<const uint32_t *>(L.st);
-        uint32_t dw[LV][NW];
-#pragma unroll
-        for (int k = 0; k < LV; k++) {
-            const uint32_t wi = (run0 + (ok ? ofs[k] : 0u)) >> 2;
-#pragma unroll
-            for (int i = 0; i < NW; i++) dw[k][i] = stw[wi + i];
-        }
-        // pass 1 with a zero carry-in: t0[k] is value k's word if nothing came
-        // from the previous lane; inh[k] marks the top bytes value k still
-        // inherits from before the lane (copied through values 0..k)
-        const U wmask = ~(U)0 >> shift;
+        uint32_t p = ok ? (uint32_t)(rec - L.st) + 5u + (uint32_t)nlead + (inc - tot) : 0u;
+        const int lo8 = 8 * lo;
         U t0[LV], inh[LV];
         U tp = 0, ih = ~(U)0;
 #pragma unroll
         for (int k = 0; k < LV; k++) {
-            const uint32_t sh = (run0 + (ok ? ofs[k] : 0u)) & 3u;
+            const uint32_t *w = stw + (p >> 2);
+            const uint32_t sh = p & 3u;
             U xk;
-            if constexpr (W == 4)
-                xk = __builtin_amdgcn_alignbyte(dw[k][1], dw[k][0], sh);
-            else
-                xk = (U)__builtin_amdgcn_alignbyte(dw[k][1], dw[k][0], sh) |
-                     ((U)__builtin_amdgcn_alignbyte(dw[k][2], dw[k][1], sh) << 32);
-            const uint32_t m = ok ? mk[k] : 0u;
-            // the m stream bytes land on byte positions [W - rb, W - lk)
-            U ek = m ? (U)((xk << (8 * (W - m))) >> (8 * lk[k])) : (U)0;
-            if constexpr (W == 8) {
-                if (!exact && rb == 5 && lk[k] == 0 && m)   // szxd_double.cc:244 reads q[3]
-                    ek = (ek & ~((U)0xFF << 56)) | (((xk >> 24) & 0xFF) << 56);
+            if constexpr (W == 4) {
+                xk = __builtin_amdgcn_alignbyte(w[1], w[0], sh);
+            } else {
+                const uint32_t d0 = w[0], d1 = w[1], d2 = w[2];
+                xk = (U)__builtin_amdgcn_alignbyte(d1, d0, sh) | ((U)__builtin_amdgcn_alignbyte(d2, d1, sh) << 32);
             }
-            ek &= wmask;
-            const U cm = lk[k] ? (~(U)0 << (8 * (W - lk[k]))) : (U)0;   // bytes copied from k-1
-            tp = ek | (tp & cm);
-            ih &= cm;
+            U xs = xk << lo8;
+            if constexpr (W == 8) {
+                if (!exact && rb == 5 && lk[k] == 0 && mk[k])   // szxd_double.cc:244 reads q[3]
+                    xs = (xs & ~((U)0xFF << 56)) | (((xk >> 24) & 0xFF) << 56);
+            }
+            const U take = ~(U)0 >> (lk[k] << 3);   // bytes below W - lk
+            tp = (xs & take) | (tp & ~take);
+            ih &= ~take;
             t0[k] = tp;
             inh[k] = ih;
+            p += mk[k];   // at most 8 W bytes past the start, inside the stage
         }
-        // the lane's record of the copyable bytes W-3..W-1: values and
-        // written flags (a byte is written if no longer inherited)
-        const uint32_t ih3 = (uint32_t)(ih >> (8 * (W - 3))) & 0xFFFFFFu;
-        const uint32_t wf = (ih3 & 0xFFu ? 0u : 1u) | (ih3 & 0xFF00u ? 0u : 2u) | (ih3 & 0xFF0000u ? 0u : 4u);
-        const uint32_t carry = ((uint32_t)(tp >> (8 * (W - 3))) & 0xFFFFFFu & ~ih3) | (wf << 24);
-        const uint32_t cinc = seg_scan_incl_op<SEG>(carry, lane, 0u, lw_combine);
-        const uint32_t cin = seg_prev<SEG>(cinc, lane);
-        const U cw = (U)(cin & 0xFFFFFFu) << (8 * (W - 3));
+        // the lane's copyable bytes W-3..W-1: the ones it wrote (no longer
+        // inherited) and their values; a last-writer scan over the segment
+        // then gives every lane what it inherits from the lanes before it
+        const uint32_t wm = ~(uint32_t)(ih >> (8 * (W - 3))) & 0xFFFFFFu;
+        uint32_t cd = (uint32_t)(tp >> (8 * (W - 3))) & wm, cm = wm;
+        seg_scan_last_writer<SEG>(cd, cm, lane);
+        const uint32_t cin = seg_prev<SEG>(cd, lane);
+        const U cw = (U)cin << (8 * (W - 3));
         const T mT = (T)med;
 #pragma unroll
         for (int k = 0; k < LV; k++) v[k] = from_bits<T>((t0[k] | (cw & inh[k])) << shift) + mT;
     }
     if (FULL) {
 #pragma unroll
-        for (int k = 0; k < LV; k += 4) Store4<T>::store(out, e0 + k, n, v + k);
+        for (int k = 0; k < LV; k += 4) Store4<T>::store_full(out, e0 + k, v + k);
     } else {
 #pragma unroll
         for (int k = 0; k < LV; k++)

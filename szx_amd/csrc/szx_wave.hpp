@@ -136,24 +136,33 @@ __device__ inline uint32_t seg_scan_incl(uint32_t v, int lane)
     return v;
 }
 
-// Segmented inclusive scan with an arbitrary associative op (identity id).
-template <int SEG, typename Op>
-__device__ inline uint32_t seg_scan_incl_op(uint32_t v, int lane, uint32_t id, Op op)
+// Segmented inclusive "last writer" scan over byte lanes: d holds data
+// bytes, m marks (0xFF) the bytes this lane wrote (d is zero elsewhere).  A
+// lane ends with, per byte, the data of the nearest lane at or before it in
+// its segment that wrote the byte, and m the union of the marks.  Within a
+// 16-lane row a row_shr source outside the row reads (0, 0), which changes
+// nothing, so whole-row segments need no lane predicate.
+template <int SEG>
+__device__ inline void seg_scan_last_writer(uint32_t &d, uint32_t &m, int lane)
 {
     const int ls = lane & (SEG - 1);
-    if constexpr (SEG >= 2) { uint32_t o = dpp32<DPP_ROW_SHR(1)>(id, v); if (ls >= 1) v = op(o, v); }
-    if constexpr (SEG >= 4) { uint32_t o = dpp32<DPP_ROW_SHR(2)>(id, v); if (ls >= 2) v = op(o, v); }
-    if constexpr (SEG >= 8) { uint32_t o = dpp32<DPP_ROW_SHR(4)>(id, v); if (ls >= 4) v = op(o, v); }
-    if constexpr (SEG >= 16) { uint32_t o = dpp32<DPP_ROW_SHR(8)>(id, v); if (ls >= 8) v = op(o, v); }
-    if constexpr (SEG >= 32) {
-        uint32_t o = dpp32<DPP_ROW_BCAST15, 0xA>(id, v);
-        if (ls >= 16) v = op(o, v);
+    constexpr bool ROWS = SEG >= 16;
+#define SZX_LW_STEP(CTRL, DIST, RM)                                     \
+    {                                                                   \
+        const uint32_t od = dpp32<CTRL, RM>(0u, d);                     \
+        const uint32_t om = dpp32<CTRL, RM>(0u, m);                     \
+        if (ROWS || ls >= (DIST)) {                                     \
+            d = (d & m) | (od & ~m);                                    \
+            m |= om;                                                    \
+        }                                                               \
     }
-    if constexpr (SEG >= 64) {
-        uint32_t o = dpp32<DPP_ROW_BCAST31, 0xC>(id, v);
-        if (ls >= 32) v = op(o, v);
-    }
-    return v;
+    if constexpr (SEG >= 2) SZX_LW_STEP(DPP_ROW_SHR(1), 1, 0xF)
+    if constexpr (SEG >= 4) SZX_LW_STEP(DPP_ROW_SHR(2), 2, 0xF)
+    if constexpr (SEG >= 8) SZX_LW_STEP(DPP_ROW_SHR(4), 4, 0xF)
+    if constexpr (SEG >= 16) SZX_LW_STEP(DPP_ROW_SHR(8), 8, 0xF)
+    if constexpr (SEG >= 32) SZX_LW_STEP(DPP_ROW_BCAST15, 16, 0xA)
+    if constexpr (SEG >= 64) SZX_LW_STEP(DPP_ROW_BCAST31, 32, 0xC)
+#undef SZX_LW_STEP
 }
 
 // Value held by lane-1 (0 in the first lane of each segment).
