@@ -18,7 +18,11 @@ reference's own CPU codec timed on this host (compiled from its sources into
 oracle/_ref; the oracle itself if that is absent), and the fixed-rate solve
 on 64 fp32 256^3 fields at 1/2/4/8 bits per value: learned proposal vs
 codec-driven bisection (codec passes, wall time, rate error), with the
-reference's sampled-model bound judged by its real stream beside them.
+reference's sampled-model bound judged by its real stream beside them; an fp64
+8192x8192 Gaussian random field at REL 1e-4/1e-6/1e-8 and blockSize
+64/128/256 (throughput and full-size parity per configuration); and 1000
+mixed fp32/fp64 fields compressed and decompressed by the batched launch,
+against one launch sequence per field.
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--rel 1e-3]
 """
@@ -165,6 +169,74 @@ def fixed_rate_eval(n_fields, dev, tol=0.02, targets=(1.0, 2.0, 4.0, 8.0)):
     out["learned_hits_where_bisection_hits"] = float(np.mean(
         [a for a, b in zip(res["learned"]["hit"], res["bisection"]["hit"]) if b] or [0]))
     out["same_hit_outcome"] = float(np.mean(both))
+    return out
+
+
+def fp64_eval(dev, steps, parity=True):
+    """fp64 8192x8192 Gaussian random field (P(k) ~ k^-3, times exp(3u), 10%
+    of the entries 1e-300), REL 1e-4 / 1e-6 / 1e-8 at blockSize 64 / 128 /
+    256: compress (REL: range pass + codec + assembly) and decompress with the
+    record index, per configuration; with parity, the stream and the decode
+    are compared with the oracle's at full size."""
+    import numpy as np
+    import torch
+
+    import szx_amd as Z
+    from szx_amd.fields import grf_field_torch
+
+    x = grf_field_torch((8192, 8192), seed=3, device=dev).reshape(-1).contiguous()
+    n = x.numel()
+    nbytes = n * 8
+    xh = x.cpu().numpy() if parity else None
+    O = None
+    if parity:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+    out = []
+    for bs in (64, 128, 256):
+        comp = Z.DeviceCompressor(n, np.float64, bs, device=dev)
+        dec = Z.DeviceDecompressor(n, np.float64, bs, comp.capacity, device=dev)
+        for rel in (1e-4, 1e-6, 1e-8):
+            step = lambda: comp(x, Z.REL, relBoundRatio=rel)
+            step()
+            view, off, csize, eb = comp.stream_view()
+            dstep = lambda: dec(view, csize, Z.DECODE_REFERENCE, index=comp.index)
+            dstep()
+            torch.cuda.synchronize()
+            Z.timing_begin()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                step()
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            for _ in range(steps):
+                dstep()
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            kt = Z.timing_end()
+            assert dec.check() == 0, "decoder rejected the stream"
+            kc = kt["compress"][0] / max(1, kt["compress"][1])
+            kd = kt["decode"][0] / max(1, kt["decode"][1])
+            rec = {"block_size": bs, "rel": rel, "eb": eb, "compression_ratio": nbytes / csize,
+                   "compress_gbs": nbytes * steps / (t1 - t0) / 1e9,
+                   "decompress_gbs": nbytes * steps / (t2 - t1) / 1e9,
+                   "compress_kernel_ms": kc, "decode_kernel_ms": kd,
+                   "compress_kernel_pct_hbm": (nbytes + csize) / (kc * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                   "decode_kernel_pct_hbm": (nbytes + csize) / (kd * 1e-3) / 1e9 / HBM_PEAK_GBS}
+            if parity:
+                s_gpu = view.cpu().numpy().tobytes()
+                eb_o = O.rel_eb(xh, rel)
+                rec["stream_identical"] = bool(np.float32(eb_o) == np.float32(eb)) and \
+                    s_gpu == O.compress(xh, eb_o, bs)
+                if rel == 1e-6:
+                    d_ora = O.decompress(s_gpu, n, np.float64, ref_quirk=True)
+                    rec["decode_identical"] = bool(np.array_equal(dec.out.cpu().numpy().view(np.uint64),
+                                                                  d_ora.view(np.uint64)))
+            log(f"fp64 bs {bs} rel {rel:g}: {rec}")
+            out.append(rec)
+        del comp, dec
+    del x
+    torch.cuda.empty_cache()
     return out
 
 
@@ -335,6 +407,8 @@ def main():
     ap.add_argument("--no-noindex", action="store_true", help="skip the index-free decode")
     ap.add_argument("--fxr-fields", type=int, default=64,
                     help="fixed-rate evaluation fields (0 skips it)")
+    ap.add_argument("--no-fp64", action="store_true",
+                    help="skip the fp64 8192^2 field (REL 1e-4/1e-6/1e-8 x blockSize 64/128/256)")
     ap.add_argument("--batch-fields", type=int, default=1000,
                     help="fields of the batched-launch evaluation (0 skips it)")
     ap.add_argument("--batch-reps", type=int, default=5)
@@ -488,6 +562,11 @@ def main():
         fixed = fixed_rate_eval(args.fxr_fields, dev)
         log(f"fixed rate {fixed}")
 
+    fp64 = None
+    if not args.no_fp64:
+        log("fp64 8192^2 field, 9 configurations")
+        fp64 = fp64_eval(dev, max(2, args.steps // 4), parity=not args.no_parity)
+
     batched = None
     if args.batch_fields > 0:
         log(f"batched-launch evaluation on {args.batch_fields} fields")
@@ -534,6 +613,7 @@ def main():
         "eb": eb,
         "parity": parity,
         "fixed_rate": fixed,
+        "fp64_grf_8192sq": fp64,
         "batched": batched,
         "host_api_pcie_inclusive": host_api,
     }

@@ -11,6 +11,8 @@
   complex white noise) times exp(3u) for a second such field u, with 10% of
   the entries set to 1e-300.
 
+``grf_field_torch`` builds the same kind of field in device memory.
+
 Every generator takes a seed and is deterministic.  The numpy versions serve
 tests and the CPU baseline; ``sinusoid_field_torch`` builds the same kind of
 field directly in device memory for the large single-GPU workloads (CPU and GPU
@@ -132,3 +134,33 @@ def grf_field(shape, seed, slope=3.0, dyn_exp=3.0, tiny_frac=0.10, tiny=1e-300,
     mask = rng.random(shape) < tiny_frac
     f[mask] = tiny
     return f.astype(dtype)
+
+
+def grf_field_torch(shape, seed, slope=3.0, dyn_exp=3.0, tiny_frac=0.10, tiny=1e-300,
+                    dtype="float64", device="cuda"):
+    """Device-resident version of grf_field (same construction, torch FFTs)."""
+    import torch
+    gen = torch.Generator(device=device)
+    gen.manual_seed(int(seed))
+    f64 = torch.float64
+    ky = torch.fft.fftfreq(shape[0], dtype=f64, device=device).reshape(-1, 1)
+    kx = torch.fft.fftfreq(shape[1], dtype=f64, device=device).reshape(1, -1)
+    kk = torch.sqrt(kx * kx + ky * ky)
+    kk[0, 0] = float("inf")
+    amp = kk ** (-slope / 2.0)
+    del kk
+
+    def one():
+        noise = torch.complex(torch.randn(shape, dtype=f64, device=device, generator=gen),
+                              torch.randn(shape, dtype=f64, device=device, generator=gen))
+        g = torch.fft.ifft2(noise * amp).real
+        del noise
+        return (g - g.mean()) / g.std()
+
+    g = one()
+    u = 0.5 * one()
+    f = g * torch.exp(dyn_exp * u)
+    del g, u
+    mask = torch.rand(shape, dtype=f64, device=device, generator=gen) < tiny_frac
+    f[mask] = tiny
+    return f.to(torch.float64 if dtype == "float64" else torch.float32)
