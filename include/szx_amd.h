@@ -131,18 +131,23 @@ size_t szx_amd_compress_capacity(size_t n, int dataType, int blockSize);
 /* Workspace bytes for szx_amd_compress_dev. */
 size_t szx_amd_compress_workspace(size_t n, int dataType, int blockSize);
 
-/* uint64_t words of a record index for n values: per index tile (4096
- * fp32 / 2048 fp64 values) the payload offset of the tile's first record and
- * the number of constant blocks before the tile, plus a closing pair; then
- * per block a uint16_t -- the record's start inside its tile's payload bytes,
- * or a constant block's rank among its tile's constant blocks.  The index
- * lives outside the stream (the stream stays byte-identical); handing it to
- * szx_amd_decompress_dev lets every tile decode at once without walking the
- * records. */
+/* uint64_t words of a record index for n values.  For blockSize a power of
+ * two 4..256: per index tile (4096 fp32 / 2048 fp64 values) the payload
+ * offset of the tile's first record and the number of constant blocks before
+ * the tile, plus a closing pair; then per block a uint16_t -- the record's
+ * start inside its tile's payload bytes, or a constant block's rank among its
+ * tile's constant blocks.  For every other blockSize 1..255: per block the
+ * payload offset of its record (of the next record, for a constant block)
+ * and the constant blocks before it, plus a closing pair {payload bytes,
+ * constant blocks}.  The index lives outside the stream (the stream stays
+ * byte-identical); handing it to szx_amd_decompress_dev lets every block
+ * decode at once without walking the records. */
 size_t szx_amd_index_words(size_t n, int dataType, int blockSize);
 
 /* Compress n device values.  errBoundMode ABS or REL (REL's bound is computed
- * on the device with the reference's float rounding).  On completion
+ * on the device with the reference's float rounding).  blockSize 1..256: the
+ * powers of two 4..256 run on the tiled kernels, every other size on per-block
+ * kernels (same stream bytes, lower throughput).  On completion
  * d_result[0] = byte offset of the stream inside d_out, d_result[1] = stream
  * size, d_result[2] = the absolute error bound used (as float bits),
  * d_result[3] = 0, or 1 if the call failed (a bounded wait between

@@ -51,7 +51,7 @@ int width_of(int dataType)
     return 0;
 }
 
-bool bs_supported(int bs) { return bs >= 4 && bs <= 256 && (bs & (bs - 1)) == 0; }
+bool bs_supported(int bs) { return bs_valid(bs); }
 
 // Growable device scratch kept for the life of the thread (one per role).
 struct DevBuf {
@@ -188,7 +188,7 @@ unsigned char *host_compress(const void *data, size_t n, int w, int rel_mode, fl
         return nullptr;
     }
     if (!bs_supported(bs)) {
-        set_err("blockSize %d is not supported (powers of two 4..256)", bs);
+        set_err("blockSize %d is not supported (1..256)", bs);
         return nullptr;
     }
     HostCtx &c = g_ctx;
@@ -247,7 +247,7 @@ void *host_decompress(const unsigned char *bytes, size_t len, size_t n, int w, i
     }
     const int bs = bytes[4] ? bytes[4] : 256;
     if (!bs_supported(bs)) {
-        set_err("blockSize %d is not supported (powers of two 4..256)", bs);
+        set_err("blockSize %d is not supported (1..256)", bs);
         return nullptr;
     }
     HostCtx &c = g_ctx;
@@ -665,7 +665,7 @@ int szx_amd_fxr_bound_dev(const void *d_in, size_t n, int dataType, int blockSiz
 size_t szx_amd_fixed_rate_workspace(size_t n, int dataType, int blockSize)
 {
     const int w = width_of(dataType);
-    if (!w || !bs_supported(blockSize)) return 0;
+    if (!w || !bs_tiled(blockSize)) return 0;
     return fixed_rate_workspace(n, w, blockSize);
 }
 
@@ -677,7 +677,7 @@ int szx_amd_fixed_rate_dev(const void *d_in, size_t n, int dataType, int blockSi
 {
     g_err.clear();
     const int w = width_of(dataType);
-    if (!w || !d_in || n == 0 || !d_out || !d_result || !bs_supported(blockSize) ||
+    if (!w || !d_in || n == 0 || !d_out || !d_result || !bs_tiled(blockSize) ||
         !(targetBitsPerValue > 0) || !(tolerance >= 0) ||
         (method != SZX_AMD_RATE_BISECT && method != SZX_AMD_RATE_LEARNED) ||
         (method == SZX_AMD_RATE_LEARNED && !d_modelParams)) {

@@ -123,12 +123,6 @@ template <typename U> __device__ inline int lead8(U xr)
     return (lz < 24 ? lz : 24) & ~7;
 }
 
-// Order-preserving integer key of a float: signed comparison of keys is the
-// float order (with -0 < +0; NaNs land beyond +-inf).  Its own inverse.
-__device__ inline int32_t fkey(uint32_t b) { return (int32_t)(b ^ ((uint32_t)((int32_t)b >> 31) >> 1)); }
-constexpr int32_t FKEY_NEG_INF = (int32_t)0x807FFFFFu;   // fkey(-inf)
-constexpr int32_t FKEY_POS_INF = 0x7F800000;             // fkey(+inf)
-
 // Exact block min/max with the reference's NaN behaviour: NaNs never win a
 // comparison (szx_float.cc:327-333), so fmin/fmax over the values, except that
 // a NaN first value makes both NaN.
@@ -863,6 +857,12 @@ CompressLayout compress_layout(uint64_t n, int w, int bs)
     L.ws_parts = off;   off += 2 * kRangeParts * sizeof(double);
     L.ws_bitmap = off;  off += (L.nbit + 15) & ~(uint64_t)15;
     L.ws_cmed = off;    off += (4 * L.nb + 15) & ~(uint64_t)15;
+    if (!bs_tiled(bs)) {
+        off = (off + 255) & ~(uint64_t)255;
+        L.ws_gsize = off;   off += (4 * L.nb + 255) & ~(uint64_t)255;
+        L.ws_gparts = off;  off += 16 * generic_parts(L.nb);
+        L.ws_gidx = off;    off += 8 * index_words(n, w, bs);
+    }
     L.ws_bytes = off;
     return L;
 }
@@ -961,9 +961,30 @@ int compress_launch(const void *d_in, uint64_t n, int w, int rel_mode, float abs
     p.nb = L.nb;
     p.ntiles = L.ntiles;
     p.nbit = L.nbit;
+    int rc = 0;
     timing_mark_begin(KK_COMPRESS, st);
-    const int rc = w == 4 ? dispatch_bs<float>(bs, p, L.ntiles, st)
-                          : dispatch_bs<double>(bs, p, L.ntiles, st);
+    if (bs_tiled(bs)) {
+        rc = w == 4 ? dispatch_bs<float>(bs, p, L.ntiles, st) : dispatch_bs<double>(bs, p, L.ntiles, st);
+    } else if (bs_valid(bs)) {
+        GenericCompress g{};
+        g.x = d_in;
+        g.n = n;
+        g.nb = L.nb;
+        g.bs = bs;
+        g.eb = abs_eb;
+        g.eb_dev = eb_dev;
+        g.payload = p.payload;
+        g.bitmap = p.bitmap;
+        g.cmed = p.cmed;
+        g.totals = p.totals;
+        g.size = reinterpret_cast<uint32_t *>(d_ws + L.ws_gsize);
+        g.idx = d_index ? d_index : reinterpret_cast<uint64_t *>(d_ws + L.ws_gidx);
+        g.parts = reinterpret_cast<uint64_t *>(d_ws + L.ws_gparts);
+        g.nparts = generic_parts(L.nb);
+        generic_compress_launch(g, w, st);
+    } else {
+        rc = -1;
+    }
     timing_mark_end(KK_COMPRESS, st);
     if (rc) return rc;
     AssembleParams a{};

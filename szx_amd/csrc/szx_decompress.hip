@@ -541,6 +541,7 @@ int decompress_launch(const uint8_t *d_stream, uint64_t stream_bytes, uint64_t n
                       void *d_out, uint8_t *d_ws, uint64_t ws_bytes, const uint64_t *d_index,
                       int exact, int32_t *d_status, hipStream_t st)
 {
+    if (!bs_valid(bs)) return -1;
     const DecompressLayout L = decompress_layout(n, w, bs, stream_bytes);
     if (ws_bytes < (d_index ? L.ws_index : L.ws_bytes)) return -1;   // with an index only the header is kept
     DecodeHeader *h = reinterpret_cast<DecodeHeader *>(d_ws + L.ws_hdr);
@@ -551,12 +552,20 @@ int decompress_launch(const uint8_t *d_stream, uint64_t stream_bytes, uint64_t n
     const uint64_t *index = d_index;
     if (!index) {
         uint64_t *built = reinterpret_cast<uint64_t *>(d_ws + L.ws_index);
-        uint16_t *bbuilt = reinterpret_cast<uint16_t *>(built + 2 * (index_tiles(n, w) + 1));
+        uint16_t *bbuilt =
+            bs_tiled(bs) ? reinterpret_cast<uint16_t *>(built + 2 * (index_tiles(n, w) + 1)) : nullptr;
         timing_mark_begin(KK_INDEX, st);
         index_build_launch(d_stream, stream_bytes, n, w, bs, h, built, bbuilt, d_ws + L.ws_build,
                            d_status, st);
         timing_mark_end(KK_INDEX, st);
         index = built;
+    }
+    if (!bs_tiled(bs)) {
+        const GenericDecode g{d_stream, stream_bytes, n, h, index, d_status, d_out, exact};
+        timing_mark_begin(KK_DECODE, st);
+        generic_decode_launch(g, w, ceil_div(n, (uint64_t)bs), st);
+        timing_mark_end(KK_DECODE, st);
+        return 0;
     }
     if (ntiles == 0) return 0;
     timing_mark_begin(KK_DECODE, st);

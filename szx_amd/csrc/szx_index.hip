@@ -3,10 +3,13 @@
 // this library's compressor writes).
 //
 // The payload is a chain: a record starting at byte p has size
-//   size(p) = 5 + NL + sum over its NL lead-code bytes of (rb - min(code, rb))
-// with rb = ceil(reqLength / 8), reqLength = byte p, NL = bs / 4 (every record
-// but the stream's last belongs to a full block).  Following the chain from
-// byte 0 is inherently serial, so it is cut into chunks of IX_C payload bytes:
+//   size(p) = 5 + NL + rb * bs - sum over the codes of its NL lead-code bytes
+//             of min(code, rb)
+// with rb = ceil(reqLength / 8), reqLength = byte p, NL = ceil(bs / 4) (every
+// record but the stream's last belongs to a full block; the unused codes of
+// a partly filled last lead byte are zero, szx_TypeManager.cc:246-252, and
+// subtract nothing).  Following the chain from byte 0 is inherently serial,
+// so it is cut into chunks of IX_C payload bytes:
 //
 //   A  k_ix_map     per chunk: size(p) for every byte p, then pointer doubling
 //                   in LDS gives, for every possible entry offset e (the first
@@ -26,7 +29,9 @@
 //   E  k_ix_scatter record starts in record order.
 //   F  k_ix_tail    the last record's true size (its block may be partial) and
 //                   the payload end.
-//   G  k_ix_tiles   the tile pairs and the per-block entries of the index.
+//   G  k_ix_tiles   the tile pairs and the per-block entries of the index;
+//      k_ix_pairs   for the block sizes the tiled kernels do not take, the
+//                   per-block pairs instead (tiles of one block).
 //
 // Every read stays inside [stream, stream + len); a chain that leaves the
 // stream or meets an impossible reqLength is marked invalid.  A stream whose
@@ -100,7 +105,7 @@ __device__ inline uint64_t lead_counts(uint32_t c)
 // an exclusive prefix sum over the staged bytes (pre, IX_PRE entries), so a
 // step costs two reads.  avail: payload bytes present from the chunk start.
 __device__ inline void ix_steps(const uint32_t *bw, uint32_t base, uint64_t *pre, uint16_t *J,
-                                uint32_t dom, uint32_t avail, int nl, int w)
+                                uint32_t dom, uint32_t avail, int nl, int bs, int w)
 {
     const uint8_t *b = reinterpret_cast<const uint8_t *>(bw) + base;
     __shared__ uint64_t s_part[256];
@@ -148,7 +153,7 @@ __device__ inline void ix_steps(const uint32_t *bw, uint32_t base, uint64_t *pre
             const uint32_t rb = (rl[i] + 7) >> 3;
             const uint32_t sh = rb >= 3 ? 32 : (rb == 2 ? 16 : 0);
             const uint32_t sub = (uint32_t)((hi[i] - lo[i]) >> sh) & 0xFFFFu;
-            j = (uint16_t)(p + 5 + (uint32_t)nl + 4 * rb * (uint32_t)nl - sub);
+            j = (uint16_t)(p + 5 + (uint32_t)nl + rb * (uint32_t)bs - sub);
         }
         J[p] = j;
     }
@@ -180,7 +185,7 @@ __global__ __launch_bounds__(256) void k_ix_map(IxParams x)
     const uint32_t avail = (uint32_t)(P - g0 < IX_C + IX_PAD ? P - g0 : IX_C + IX_PAD);
     const uint32_t base = ix_stage(bw, x.s + h.q0 + g0, avail, IX_WORDS, x.s, x.s + x.len);
     const uint32_t dom = IX_C + x.rmax;
-    ix_steps(bw, base, u.pre, J0, dom, avail, x.nl, x.w);
+    ix_steps(bw, base, u.pre, J0, dom, avail, x.nl, x.bs, x.w);
     __syncthreads();   // the prefix sums die here: their space holds the second table
     if (threadIdx.x == 0) {
         s_min = 0xFFFFFFFFu;
@@ -278,7 +283,7 @@ __global__ __launch_bounds__(256) void k_ix_walk(IxParams x)
     const uint32_t avail = (uint32_t)(P - g0 < IX_C + IX_PAD ? P - g0 : IX_C + IX_PAD);
     const uint32_t base = ix_stage(bw, x.s + h.q0 + g0, avail, IX_WORDS, x.s, x.s + x.len);
     const uint32_t dom = IX_C + x.rmax;
-    ix_steps(bw, base, u.pre, J0, dom, avail, x.nl, x.w);
+    ix_steps(bw, base, u.pre, J0, dom, avail, x.nl, x.bs, x.w);
     __syncthreads();
     const uint32_t e0 = x.entry[c];
     for (uint32_t p = threadIdx.x; p < dom; p += blockDim.x) {
@@ -501,6 +506,26 @@ __global__ __launch_bounds__(256) void k_ix_tiles(IxParams x)
     }
 }
 
+// G (per-block index): {record offset, constant blocks before} for every
+// block, then the closing pair.  tbase holds the non-constant blocks before
+// each block (tiles of one block).
+__global__ void k_ix_pairs(IxParams x)
+{
+    const uint64_t b = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (*x.status != 0) return;
+    const DecodeHeader &h = *x.h;
+    if (b > h.nb) return;
+    if (b == h.nb) {
+        x.index[2 * b] = *x.pend;
+        x.index[2 * b + 1] = h.nconst;
+        return;
+    }
+    const uint64_t K = h.nb - h.nconst;
+    const uint64_t ord = x.tbase[b];
+    x.index[2 * b] = ord < K ? x.rec[ord] : *x.pend;
+    x.index[2 * b + 1] = b - ord;
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -511,10 +536,10 @@ IxLayout ix_layout(uint64_t n, int w, int bs, uint64_t len)
     const uint64_t nb = (n + bs - 1) / bs;
     L.nch = (len + IX_C - 1) / IX_C;
     if (!L.nch) L.nch = 1;
-    L.nl = bs / 4;
+    L.nl = (bs + 3) / 4;
     L.rmax = 5 + (uint32_t)L.nl + (uint32_t)bs * (uint32_t)w;
     L.maxr = IX_C / (5 + (uint32_t)L.nl) + 2;
-    L.ntiles = index_tiles(n, w);
+    L.ntiles = bs_tiled(bs) ? index_tiles(n, w) : (nb ? nb : 1);
     auto al = [](uint64_t v) { return (v + 255) & ~255ull; };
     uint64_t off = 0;
     L.o_exits = off;  off += al(2 * L.nch);
@@ -548,7 +573,7 @@ void index_build_launch(const uint8_t *s, uint64_t len, uint64_t n, int w, int b
     x.maxr = L.maxr;
     x.nch = L.nch;
     x.ntiles = L.ntiles;
-    x.tile_blocks = tile_values(w) / (uint64_t)bs;
+    x.tile_blocks = bs_tiled(bs) ? tile_values(w) / (uint64_t)bs : 1;
     // doubling rounds: a chunk holds at most IX_C / (5 + NL) records
     int rounds = 1;
     while ((1u << rounds) < IX_C / (5 + (uint32_t)L.nl) + 2) rounds++;
@@ -580,7 +605,10 @@ void index_build_launch(const uint8_t *s, uint64_t len, uint64_t n, int w, int b
     hipLaunchKernelGGL(k_ix_tile_counts, dim3((unsigned)((L.ntiles + 255) / 256)), dim3(256), 0, st, x);
     scan_u32(x.tcnt, L.ntiles, x.tbase, bsum, scratch, st);
     hipLaunchKernelGGL(k_ix_tail, dim3(1), dim3(64), 0, st, x);
-    hipLaunchKernelGGL(k_ix_tiles, dim3((unsigned)L.ntiles), dim3(256), 0, st, x);
+    if (bs_tiled(bs))
+        hipLaunchKernelGGL(k_ix_tiles, dim3((unsigned)L.ntiles), dim3(256), 0, st, x);
+    else
+        hipLaunchKernelGGL(k_ix_pairs, dim3((unsigned)((L.ntiles + 256) / 256)), dim3(256), 0, st, x);
 }
 
 }  // namespace szxamd

@@ -25,9 +25,17 @@ inline uint64_t index_tiles(uint64_t n, int w)
     const uint64_t t = (n + tile_values(w) - 1) / tile_values(w);
     return t ? t : 1;
 }
+// Block sizes: the tiled kernels take the powers of two 4..256; every other
+// size 1..255 runs on the per-block kernels (szx_generic.hip).
+inline bool bs_valid(int bs) { return bs >= 1 && bs <= 256; }
+inline bool bs_tiled(int bs) { return bs >= 4 && bs <= 256 && (bs & (bs - 1)) == 0; }
+// For the other block sizes the record index is per block: {payload offset
+// of its record (or of the next record, for a constant block), constant
+// blocks before it}, plus the closing pair {payload bytes, constant blocks}.
 inline uint64_t index_words(uint64_t n, int w, int bs)
 {
     const uint64_t nb = (n + bs - 1) / bs;
+    if (!bs_tiled(bs)) return 2 * (nb + 1);
     return 2 * (index_tiles(n, w) + 1) + (nb + 3) / 4;
 }
 
@@ -52,6 +60,8 @@ struct CompressLayout {
     uint64_t ntiles;      // compressor tiles
     uint64_t nidx;        // index tiles
     uint64_t ws_tiles, ws_chunks, ws_cacc, ws_misc, ws_parts, ws_bitmap, ws_cmed, ws_bytes;
+    // per-block kernels only: record sizes, scan chunk sums, per-block pairs
+    uint64_t ws_gsize, ws_gparts, ws_gidx;
 };
 
 CompressLayout compress_layout(uint64_t n, int w, int bs);
@@ -81,6 +91,37 @@ struct DecompressLayout {
 };
 
 DecompressLayout decompress_layout(uint64_t n, int w, int bs, uint64_t stream_bytes);
+
+// Per-block kernels for the block sizes the tiled kernels do not take
+// (szx_generic.hip).
+struct GenericCompress {
+    const void *x;
+    uint64_t n, nb;
+    int bs;
+    float eb;
+    const float *eb_dev;  // REL: bound computed on the device
+    uint8_t *payload;     // records
+    uint8_t *bitmap;
+    uint32_t *cmed;       // constant medians, big-endian, by rank
+    uint64_t *totals;     // {payload bytes, constant blocks}
+    uint32_t *size;       // per block: record bytes, 0 = constant
+    uint64_t *idx;        // per-block pairs and the closing pair
+    uint64_t *parts;      // 2 words per scan chunk
+    uint64_t nparts;
+};
+uint64_t generic_parts(uint64_t nb);
+void generic_compress_launch(const GenericCompress &g, int w, hipStream_t st);
+
+struct GenericDecode {
+    const uint8_t *s;
+    uint64_t len, n;
+    const DecodeHeader *h;
+    const uint64_t *idx;  // per-block pairs and the closing pair
+    int32_t *status;
+    void *out;
+    int exact;
+};
+void generic_decode_launch(const GenericDecode &p, int w, uint64_t nb, hipStream_t st);
 
 // Enqueue a whole decompress on `st`; d_status receives 0 or a negative code.
 // d_index (optional) is the record index the compressor wrote; without it the

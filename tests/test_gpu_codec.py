@@ -18,6 +18,8 @@ pytestmark = pytest.mark.gpu
 
 GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz")))
 POW2 = (4, 8, 16, 32, 64, 128, 256)
+# block sizes outside the tiled kernels (per-block kernels)
+GENERIC = (1, 2, 3, 5, 7, 12, 33, 100, 200, 255)
 
 
 def say(*a):
@@ -39,8 +41,6 @@ def test_golden_stream_and_decode(path, device):
     g = load(path)
     x, bs, eb = g["x"], int(g["bs"]), float(g["eb"])
     name = os.path.basename(path)
-    if bs not in POW2:
-        pytest.skip("block sizes that are not powers of two use the generic path")
     say("case", name, x.dtype, x.size, bs, eb)
     if "sin_rel" in name:   # produced by SZx_fast_compress_args(REL)
         s = Z.SZx_fast_compress_args(Z.SZx_WITH_BLOCK_FAST_CMPR, x, Z.REL,
@@ -236,3 +236,64 @@ def test_index_free_rejects_damage(device):
     st = torch.from_numpy(np.frombuffer(bytes(s), np.uint8).copy()).to(device)
     dec(st, len(s), Z.DECODE_REFERENCE)
     assert dec.check() != 0
+
+
+@pytest.mark.parametrize("bs", GENERIC)
+def test_generic_block_sizes(bs, device):
+    """Block sizes the tiled kernels do not take: stream bytes, compressor
+    index, index-free decode and the fp64 exact mode against the oracle."""
+    import torch
+    r = np.random.default_rng(7000 + bs)
+    for dtype in (np.float32, np.float64):
+        for n in (1, bs + 1, int(r.integers(3000, 70000))):
+            x = _field(r, n, dtype, int(r.integers(0, 5)))
+            if n > 2 * bs:
+                x[: n // 3] = 0.25            # constant blocks
+            eb = float(np.float32((float(np.ptp(x)) + 1e-30) * 10.0 ** r.uniform(-9, -2)))
+            say("generic", bs, dtype.__name__, n, eb)
+            want_s = O.compress(x, eb, bs)
+            comp = Z.DeviceCompressor(n, dtype, bs)
+            comp(torch.from_numpy(x).to(device), Z.ABS, eb)
+            view, off, size, _ = comp.stream_view()
+            s = view.cpu().numpy().tobytes()
+            assert s == want_s
+            want = O.decompress(s, n, dtype, ref_quirk=True)
+            dec = Z.DeviceDecompressor(n, dtype, bs, size + 64)
+            dec(view, size, Z.DECODE_REFERENCE, index=comp.index)
+            assert dec.check() == 0
+            assert bits_equal(dec.out.cpu().numpy(), want)
+            st = torch.from_numpy(np.frombuffer(s, np.uint8).copy()).to(device)
+            dec(st, size, Z.DECODE_REFERENCE)                 # index rebuilt from the stream
+            assert dec.check() == 0
+            assert bits_equal(dec.out.cpu().numpy(), want)
+            if dtype == np.float64:
+                dec(view, size, Z.DECODE_EXACT, index=comp.index)
+                assert dec.check() == 0
+                assert bits_equal(dec.out.cpu().numpy(), O.decompress(s, n, dtype, ref_quirk=False))
+            # host API
+            assert Z.compress_blocked(x, eb, bs) == want_s
+            assert bits_equal(Z.decompress_blocked(want_s, n, dtype), want)
+
+
+def test_generic_rel_and_damage(device):
+    import torch
+    x = (np.sin(np.arange(100003) * 0.003) * 3).astype(np.float32)
+    bs = 100
+    comp = Z.DeviceCompressor(x.size, np.float32, bs)
+    comp(torch.from_numpy(x).to(device), Z.REL, 0.0, 1e-4)
+    view, off, size, eb = comp.stream_view()
+    s = view.cpu().numpy().tobytes()
+    say("generic REL", eb)
+    assert eb == O.rel_eb(x, 1e-4)
+    assert s == O.compress(x, eb, bs)
+    dec = Z.DeviceDecompressor(x.size, np.float32, bs, size)
+    say("generic truncated")
+    dec(view[: size - 50], size - 50, Z.DECODE_REFERENCE)
+    assert dec.check() != 0
+    say("generic damaged index")
+    bad = comp.index.clone()
+    bad[2 * 50] += 1
+    dec(view, size, Z.DECODE_REFERENCE, index=bad)
+    assert dec.check() != 0
+    dec(view, size, Z.DECODE_REFERENCE, index=comp.index)
+    assert dec.check() == 0
