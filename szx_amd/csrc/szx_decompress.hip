@@ -563,7 +563,7 @@ int decompress_launch(const uint8_t *d_stream, uint64_t stream_bytes, uint64_t n
     if (!bs_tiled(bs)) {
         const GenericDecode g{d_stream, stream_bytes, n, h, index, d_status, d_out, exact};
         timing_mark_begin(KK_DECODE, st);
-        generic_decode_launch(g, w, ceil_div(n, (uint64_t)bs), st);
+        generic_decode_launch(g, w, bs, ceil_div(n, (uint64_t)bs), st);
         timing_mark_end(KK_DECODE, st);
         return 0;
     }

@@ -4,9 +4,11 @@
 // The tiled kernels (szx_compress.hip, szx_decompress.hip) map a block onto an
 // aligned group of lanes and need a power-of-two block size of at least 4.
 // The reference's blocked API takes any block size that fits its one-byte
-// header field (szx_float.cc:1095), so every other size 1..255 runs here: one
-// lane per block, walking the block's values in order exactly as the
-// reference's per-block loops do (szx_float.cc:314-371 and 49-252 for
+// header field (szx_float.cc:1095), so every other size 1..255 runs here.
+// A block is a segment of S lanes (S = the smallest power of two with
+// 4 S >= bs), each lane holding 4 consecutive values; segment shuffles give
+// the block statistics, the byte offsets and the cross-lane carries of the
+// reference's per-block loops (szx_float.cc:314-371 and 49-252 for
 // compression, szxd_float.cc:34-244 for decompression; fp64 in szx_double.cc
 // and szxd_double.cc).  The streams are the same bytes.
 //
@@ -23,7 +25,7 @@
 // The per-block pairs are the record index for these block sizes; the
 // decoder's index builder (szx_index.hip) writes the same pairs.
 //
-// Decompression: k_gen_decode, one lane per block, checks every record
+// Decompression: k_gen_decode, one segment per block, checks every record
 // against its pair and the next one.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -49,34 +51,81 @@ __device__ inline bool gen_nonconst(const uint8_t *bitmap, uint64_t b)
     return (bitmap[b >> 3] >> (7 - (b & 7))) & 1;
 }
 
-template <typename T, bool WRITE>
+template <typename T> __device__ inline T gmin(T a, T b);
+template <typename T> __device__ inline T gmax(T a, T b);
+template <> __device__ inline float gmin<float>(float a, float b) { return __builtin_fminf(a, b); }
+template <> __device__ inline float gmax<float>(float a, float b) { return __builtin_fmaxf(a, b); }
+template <> __device__ inline double gmin<double>(double a, double b) { return __builtin_fmin(a, b); }
+template <> __device__ inline double gmax<double>(double a, double b) { return __builtin_fmax(a, b); }
+
+// Inclusive sum over the S lanes of a segment.
+template <int S> __device__ inline uint32_t seg_incl_sum(uint32_t v, int l)
+{
+#pragma unroll
+    for (int d = 1; d < S; d <<= 1) {
+        const uint32_t o = __shfl_up(v, d, S);
+        if (l >= d) v += o;
+    }
+    return v;
+}
+
+// One block per segment of S lanes, 4 consecutive values per lane
+// (S = the smallest power of two with 4 S >= bs).  WRITE = false: record
+// size per block (0 = constant); WRITE = true: the record at its offset in
+// the payload, or the constant median at its rank.
+template <typename T, int S, bool WRITE>
 __global__ __launch_bounds__(256) void k_gen_encode(GenericCompress g)
 {
     using U = typename Word<T>::U;
     constexpr int W = Word<T>::W;
-    const uint64_t b = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    if (b >= g.nb) return;
+    const int l = threadIdx.x & (S - 1);
+    const uint64_t b = blockIdx.x * (uint64_t)(256 / S) + threadIdx.x / S;
+    if (b >= g.nb) return;   // whole segments
     const float eb = g.eb_dev ? *g.eb_dev : g.eb;
     const T *x = static_cast<const T *>(g.x) + b * (uint64_t)g.bs;
     const uint64_t rest = g.n - b * (uint64_t)g.bs;
     const int m = rest < (uint64_t)g.bs ? (int)rest : g.bs;
-
-    // block statistics with the reference's compare order (szx_float.cc:327-333,
-    // szx_double.cc:985-991): NaNs never replace, a NaN first value stays, and
-    // the first of two equal-comparing zeros is kept
-    T lo = x[0], hi = x[0];
-    for (int j = 1; j < m; j++) {
-        const T v = x[j];
-        if (lo > v) lo = v;
-        else if (hi < v) hi = v;
+    T v[4];
+    bool val[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        val[k] = 4 * l + k < m;
+        v[k] = val[k] ? x[4 * l + k] : (T)0;
+    }
+    // block statistics (szx_float.cc:327-333, szx_double.cc:985-991): NaNs
+    // never win a comparison, so fmin / fmax over the values, except that a
+    // NaN first value makes both NaN.  Which of two zeros is kept cannot change
+    // the radius or the median.
+    T lo = v[0], hi = v[0];
+#pragma unroll
+    for (int k = 1; k < 4; k++)
+        if (val[k]) {
+            lo = gmin(lo, v[k]);
+            hi = gmax(hi, v[k]);
+        }
+    if (4 * l >= m) {   // lanes past the block's end
+        lo = __builtin_huge_valf();
+        hi = -__builtin_huge_valf();
+    }
+#pragma unroll
+    for (int d = 1; d < S; d <<= 1) {
+        lo = gmin(lo, __shfl_xor(lo, d, S));
+        hi = gmax(hi, __shfl_xor(hi, d, S));
+    }
+    const T first = __shfl(v[0], 0, S);
+    if (first != first) {
+        lo = first;
+        hi = first;
     }
     const T rad = (hi - lo) / (T)2;
     const T med = lo + rad;
     const float med_f = (float)med, rad_f = (float)rad;
     const bool constant = W == 4 ? (rad <= (T)eb) : ((double)rad <= (double)eb);
-    if (constant) {
-        if (WRITE) g.cmed[g.idx[2 * b + 1]] = bswap32(f32_bits(med_f));
-        else g.size[b] = 0;
+    if (constant) {   // segment-uniform
+        if (l == 0) {
+            if (WRITE) g.cmed[g.idx[2 * b + 1]] = bswap32(f32_bits(med_f));
+            else g.size[b] = 0;
+        }
         return;
     }
     int clamped;
@@ -87,10 +136,29 @@ __global__ __launch_bounds__(256) void k_gen_encode(GenericCompress g)
     // fp32 encodes against the stored median; fp64 against the median before
     // the clamp (szx_float.cc:60,90; szx_double.cc:38,78)
     const T msub = W == 4 ? (T)hmed : (T)med_f;
+    U sw[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) sw[k] = to_bits<T>(v[k] - msub) >> shift;
+    // the previous lane's last word (every lane before the last valid one is full)
+    U prev = __shfl_up(sw[3], 1, S);
+    if (l == 0) prev = 0;
+    int ld[4];
+    uint32_t lsum = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        ld[k] = gen_lead<U>(sw[k] ^ prev);
+        prev = sw[k];
+        if (val[k] && ld[k] < rb) lsum += rb - ld[k];
+    }
+    const uint32_t incl = seg_incl_sum<S>(lsum, l);
+    const uint32_t total = __shfl(incl, S - 1, S);
     const int nlead = (m + 3) >> 2;
-
-    uint8_t *rec = WRITE ? g.payload + g.idx[2 * b] : nullptr;
-    if (WRITE) {
+    if (!WRITE) {
+        if (l == 0) g.size[b] = 5 + (uint32_t)nlead + total;
+        return;
+    }
+    uint8_t *rec = g.payload + g.idx[2 * b];
+    if (l == 0) {
         const uint32_t hb = f32_bits(hmed);
         rec[0] = (uint8_t)len;
         rec[1] = (uint8_t)(hb >> 24);
@@ -98,25 +166,18 @@ __global__ __launch_bounds__(256) void k_gen_encode(GenericCompress g)
         rec[3] = (uint8_t)(hb >> 8);
         rec[4] = (uint8_t)hb;
     }
-    uint32_t pos = 5 + (uint32_t)nlead;
-    uint32_t code = 0;
-    U prev = 0;
-    for (int i = 0; i < m; i++) {
-        const U s = to_bits<T>(x[i] - msub) >> shift;
-        const int ld = gen_lead<U>(s ^ prev);
-        prev = s;
-        if (WRITE) {
-            code |= (uint32_t)ld << (6 - 2 * (i & 3));
-            if ((i & 3) == 3 || i == m - 1) {
-                rec[5 + (i >> 2)] = (uint8_t)code;
-                code = 0;
-            }
-            for (int k = W - rb; k < W - ld; k++) rec[pos++] = (uint8_t)(s >> (8 * k));
-        } else if (ld < rb) {
-            pos += rb - ld;
-        }
+    if (4 * l < m) {
+        uint32_t code = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (val[k]) code |= (uint32_t)ld[k] << (6 - 2 * k);
+        rec[5 + l] = (uint8_t)code;   // szx_TypeManager.cc:198-273
     }
-    if (!WRITE) g.size[b] = pos;
+    uint8_t *q = rec + 5 + nlead + (incl - lsum);
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        if (val[k])
+            for (int t = W - rb; t < W - ld[k]; t++) *q++ = (uint8_t)(sw[k] >> (8 * t));
 }
 
 // ---- exclusive scan of (record bytes, constant blocks) over the blocks ----
@@ -242,99 +303,202 @@ __global__ __launch_bounds__(256) void k_gen_bitmap(GenericCompress g)
 
 // ---- decompression ----
 
-template <typename T>
+// Carry of the top three bytes of the reconstructed words across the lanes
+// of a segment: bytes 0..2 hold the bytes W-1, W-2, W-3 of the last word that
+// wrote them, bits 24..26 say which were written.  combine(a, b) = b's
+// written bytes over a's.
+__device__ inline uint32_t carry_combine(uint32_t a, uint32_t b)
+{
+    const uint32_t f = b >> 24;
+    const uint32_t keep = ((f & 1u) ? 0xFFu : 0u) | ((f & 2u) ? 0xFF00u : 0u) | ((f & 4u) ? 0xFF0000u : 0u);
+    return ((b & keep) | (a & ~keep & 0xFFFFFFu)) | ((a | b) & 0x7000000u);
+}
+
+// One block per segment of S lanes, 4 consecutive values per lane.  The
+// record's lead codes give every value's stream byte count; a segment scan
+// places each lane's bytes, and a second scan carries the inherited top
+// bytes (szxd_float.cc:78-199: a word takes its first min(lead, rb) bytes
+// from the previous word) from lane to lane.
+template <typename T, int S>
 __global__ __launch_bounds__(256) void k_gen_decode(GenericDecode p)
 {
     using U = typename Word<T>::U;
     constexpr int W = Word<T>::W;
     const DecodeHeader *h = p.h;
     if (h->rc != 0) return;
-    const uint64_t b = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    const int l = threadIdx.x & (S - 1);
+    const uint64_t b = blockIdx.x * (uint64_t)(256 / S) + threadIdx.x / S;
     const uint64_t nb = h->nb, bs = h->bs, q0 = h->q0;
-    if (b >= nb) return;
+    if (b >= nb) return;   // whole segments
     const uint64_t rest = p.n - b * bs;
     const int m = rest < bs ? (int)rest : (int)bs;
     T *out = static_cast<T *>(p.out) + b * bs;
+    bool val[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) val[k] = 4 * l + k < m;
     const uint64_t off = p.idx[2 * b], rank = p.idx[2 * b + 1];
     const uint64_t noff = p.idx[2 * b + 2], nrank = p.idx[2 * b + 3];
     if (!gen_nonconst(p.s + HEADER_BYTES, b)) {
         // constant block: its median, widened for fp64 (szxd_float.cc:49-56)
         if (rank >= h->nconst || nrank != rank + 1 || noff != off) {
-            atomicMin(p.status, -21);
+            if (l == 0) atomicMin(p.status, -21);
             return;
         }
         const uint8_t *c = p.s + HEADER_BYTES + h->nbit + 4 * rank;
-        const float med = bits_f32(((uint32_t)c[0] << 24) | ((uint32_t)c[1] << 16) | ((uint32_t)c[2] << 8) |
-                                   (uint32_t)c[3]);
-        for (int j = 0; j < m; j++) out[j] = (T)med;
+        const T med = (T)bits_f32(((uint32_t)c[0] << 24) | ((uint32_t)c[1] << 16) | ((uint32_t)c[2] << 8) |
+                                  (uint32_t)c[3]);
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (val[k]) out[4 * l + k] = med;
         return;
     }
     const int nlead = (m + 3) >> 2;
     if (nrank != rank || noff < off || noff - off < 5u + (uint64_t)nlead || noff > p.len - q0) {
-        atomicMin(p.status, -21);
+        if (l == 0) atomicMin(p.status, -21);
         return;
     }
     const uint8_t *rec = p.s + q0 + off;
-    const uint8_t *end = p.s + q0 + noff;
     const int rl = rec[0];
     if (rl < 1 || rl > 8 * W) {
-        atomicMin(p.status, -21);
+        if (l == 0) atomicMin(p.status, -21);
         return;
     }
-    // szxd_float.cc:78-199: lead bytes from the previous word, the rest from
-    // the stream, masked to the low 8W - shift bits, shifted back, median added
     const int rb = (rl + 7) >> 3;
     const int shift = (-rl) & 7;
     const T med = (T)bits_f32(((uint32_t)rec[1] << 24) | ((uint32_t)rec[2] << 16) | ((uint32_t)rec[3] << 8) |
                               (uint32_t)rec[4]);
-    const uint8_t *q = rec + 5 + nlead;
+    const uint32_t code = 4 * l < m ? rec[5 + l] : 0u;
+    int lk[4];
+    uint32_t lsum = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int ld = (int)(code >> (6 - 2 * k)) & 3;
+        lk[k] = ld < rb ? ld : rb;
+        if (val[k]) lsum += rb - lk[k];
+    }
+    const uint32_t incl = seg_incl_sum<S>(lsum, l);
+    const uint32_t total = __shfl(incl, S - 1, S);
+    if (5u + (uint64_t)nlead + total != noff - off) {   // segment-uniform
+        if (l == 0) atomicMin(p.status, -21);
+        return;
+    }
+    // the lane's words from the stream alone (inherited bytes still zero),
+    // masked to the low 8W - shift bits
     const U mask = ~(U)0 >> shift;
-    U prev = 0;
-    for (int i = 0; i < m; i++) {
-        const int ld = (rec[5 + (i >> 2)] >> (6 - 2 * (i & 3))) & 3;
-        const int l = ld < rb ? ld : rb;
-        if (q + (rb - l) > end) {
-            atomicMin(p.status, -21);
-            return;
+    const uint8_t *q = rec + 5 + nlead + (incl - lsum);
+    U wv[4];
+    uint32_t mine = 0;   // this lane's carry contribution
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        U cur = 0;
+        if (val[k]) {
+            for (int t = W - rb; t < W - lk[k]; t++) {
+                int src = t - (W - rb);
+                if (W == 8 && !p.exact && rb == 5 && lk[k] == 0 && t == 7) src = 3;   // szxd_double.cc:240-245
+                cur |= (U)q[src] << (8 * t);
+            }
+            q += rb - lk[k];
+            cur &= mask;
+            // bytes W-1-i (i < 3) this word writes: those below W - lk
+            uint32_t wr = 0;
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+                if (lk[k] <= i) wr |= ((uint32_t)(cur >> (8 * (W - 1 - i))) & 0xFFu) << (8 * i) | (1u << (24 + i));
+            mine = carry_combine(mine, wr);
         }
-        U cur = l ? (prev & (~(U)0 << (8 * (W - l)))) : (U)0;
-        for (int k = W - rb; k < W - l; k++) {
-            int src = k - (W - rb);
-            if (W == 8 && !p.exact && rb == 5 && l == 0 && k == 7) src = 3;   // szxd_double.cc:240-245
-            cur |= (U)q[src] << (8 * k);
-        }
-        q += rb - l;
-        cur &= mask;
-        out[i] = from_bits<T>(cur << shift) + med;
+        wv[k] = cur;
+    }
+    uint32_t acc = mine;
+#pragma unroll
+    for (int d = 1; d < S; d <<= 1) {
+        const uint32_t o = __shfl_up(acc, d, S);
+        if (l >= d) acc = carry_combine(o, acc);
+    }
+    uint32_t cin = __shfl_up(acc, 1, S);
+    if (l == 0) cin = 0;   // the first word of a block follows a zero word
+    U prev = ((U)(cin & 0xFFu) << (8 * (W - 1))) | ((U)((cin >> 8) & 0xFFu) << (8 * (W - 2))) |
+             ((U)((cin >> 16) & 0xFFu) << (8 * (W - 3)));
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const U top = lk[k] ? ~(U)0 << (8 * (W - lk[k])) : (U)0;
+        const U cur = wv[k] | (prev & top);
+        if (val[k]) out[4 * l + k] = from_bits<T>(cur << shift) + med;
         prev = cur;
     }
-    if (q != end) atomicMin(p.status, -21);
 }
 
 // ---- host side ----
 
 uint64_t generic_parts(uint64_t nb) { return ceil_div(nb ? nb : 1, (uint64_t)GEN_CH); }
 
+// Lanes per block: the smallest power of two S with 4 S >= bs.
+static int seg_lanes(int bs)
+{
+    int s = 1;
+    while (4 * s < bs) s <<= 1;
+    return s;
+}
+
+template <typename T, int S>
+static void encode_pass(const GenericCompress &g, bool write, hipStream_t st)
+{
+    const unsigned gb = (unsigned)ceil_div(g.nb * S, 256);
+    if (write) hipLaunchKernelGGL((k_gen_encode<T, S, true>), dim3(gb), dim3(256), 0, st, g);
+    else hipLaunchKernelGGL((k_gen_encode<T, S, false>), dim3(gb), dim3(256), 0, st, g);
+}
+
+template <typename T>
+static void encode_dispatch(const GenericCompress &g, bool write, hipStream_t st)
+{
+    switch (seg_lanes(g.bs)) {
+    case 1: encode_pass<T, 1>(g, write, st); break;
+    case 2: encode_pass<T, 2>(g, write, st); break;
+    case 4: encode_pass<T, 4>(g, write, st); break;
+    case 8: encode_pass<T, 8>(g, write, st); break;
+    case 16: encode_pass<T, 16>(g, write, st); break;
+    case 32: encode_pass<T, 32>(g, write, st); break;
+    default: encode_pass<T, 64>(g, write, st); break;
+    }
+}
+
 void generic_compress_launch(const GenericCompress &g, int w, hipStream_t st)
 {
-    const unsigned gb = (unsigned)ceil_div(g.nb, 256);
     const unsigned np = (unsigned)g.nparts;
-    if (w == 4) hipLaunchKernelGGL((k_gen_encode<float, false>), dim3(gb), dim3(256), 0, st, g);
-    else hipLaunchKernelGGL((k_gen_encode<double, false>), dim3(gb), dim3(256), 0, st, g);
+    if (w == 4) encode_dispatch<float>(g, false, st);
+    else encode_dispatch<double>(g, false, st);
     hipLaunchKernelGGL(k_gen_part, dim3(np), dim3(256), 0, st, g);
     hipLaunchKernelGGL(k_gen_top, dim3(1), dim3(256), 0, st, g);
     hipLaunchKernelGGL(k_gen_apply, dim3(np), dim3(256), 0, st, g);
-    if (w == 4) hipLaunchKernelGGL((k_gen_encode<float, true>), dim3(gb), dim3(256), 0, st, g);
-    else hipLaunchKernelGGL((k_gen_encode<double, true>), dim3(gb), dim3(256), 0, st, g);
+    if (w == 4) encode_dispatch<float>(g, true, st);
+    else encode_dispatch<double>(g, true, st);
     hipLaunchKernelGGL(k_gen_bitmap, dim3((unsigned)ceil_div(ceil_div(g.nb, 8), 256)), dim3(256), 0, st, g);
 }
 
-void generic_decode_launch(const GenericDecode &p, int w, uint64_t nb, hipStream_t st)
+template <typename T, int S>
+static void decode_pass(const GenericDecode &p, uint64_t nb, hipStream_t st)
 {
-    const unsigned gb = (unsigned)ceil_div(nb, 256);
-    if (!gb) return;
-    if (w == 4) hipLaunchKernelGGL(k_gen_decode<float>, dim3(gb), dim3(256), 0, st, p);
-    else hipLaunchKernelGGL(k_gen_decode<double>, dim3(gb), dim3(256), 0, st, p);
+    hipLaunchKernelGGL((k_gen_decode<T, S>), dim3((unsigned)ceil_div(nb * S, 256)), dim3(256), 0, st, p);
+}
+
+template <typename T>
+static void decode_dispatch(const GenericDecode &p, int bs, uint64_t nb, hipStream_t st)
+{
+    switch (seg_lanes(bs)) {
+    case 1: decode_pass<T, 1>(p, nb, st); break;
+    case 2: decode_pass<T, 2>(p, nb, st); break;
+    case 4: decode_pass<T, 4>(p, nb, st); break;
+    case 8: decode_pass<T, 8>(p, nb, st); break;
+    case 16: decode_pass<T, 16>(p, nb, st); break;
+    case 32: decode_pass<T, 32>(p, nb, st); break;
+    default: decode_pass<T, 64>(p, nb, st); break;
+    }
+}
+
+void generic_decode_launch(const GenericDecode &p, int w, int bs, uint64_t nb, hipStream_t st)
+{
+    if (!nb) return;
+    if (w == 4) decode_dispatch<float>(p, bs, nb, st);
+    else decode_dispatch<double>(p, bs, nb, st);
 }
 
 }  // namespace szxamd

@@ -121,7 +121,7 @@ struct GenericDecode {
     void *out;
     int exact;
 };
-void generic_decode_launch(const GenericDecode &p, int w, uint64_t nb, hipStream_t st);
+void generic_decode_launch(const GenericDecode &p, int w, int bs, uint64_t nb, hipStream_t st);
 
 // Enqueue a whole decompress on `st`; d_status receives 0 or a negative code.
 // d_index (optional) is the record index the compressor wrote; without it the
