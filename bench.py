@@ -20,9 +20,11 @@ on 64 fp32 256^3 fields at 1/2/4/8 bits per value: learned proposal vs
 codec-driven bisection (codec passes, wall time, rate error), with the
 reference's sampled-model bound judged by its real stream beside them; an fp64
 8192x8192 Gaussian random field at REL 1e-4/1e-6/1e-8 and blockSize
-64/128/256 (throughput and full-size parity per configuration); and 1000
-mixed fp32/fp64 fields compressed and decompressed by the batched launch,
-against one launch sequence per field.
+64/128/256 (throughput and full-size parity per configuration); the 512^3
+field at block sizes outside the tiled kernels (default 100, 255, 7: the
+per-block segment kernels, throughput and full-size parity); and 1000 mixed
+fp32/fp64 fields compressed and decompressed by the batched launch, against
+one launch sequence per field.
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--rel 1e-3]
 """
@@ -240,6 +242,65 @@ def fp64_eval(dev, steps, parity=True):
     return out
 
 
+def generic_eval(x, xh, eb, dev, steps, sizes, parity=True):
+    """Block sizes outside the tiled kernels (per-block segment kernels) on
+    the same fp32 field at the same ABS bound: compress and decompress (with
+    the record index) throughput, kernel-sequence time, and with parity the
+    stream and the decode compared with the oracle's at full size."""
+    import numpy as np
+    import torch
+
+    import szx_amd as Z
+
+    n = x.numel()
+    nbytes = n * 4
+    O = None
+    if parity:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+    out = []
+    for bs in sizes:
+        comp = Z.DeviceCompressor(n, np.float32, bs, device=dev)
+        comp(x, Z.ABS, absErrBound=eb)
+        view, off, csize, _ = comp.stream_view()
+        dec = Z.DeviceDecompressor(n, np.float32, bs, comp.capacity, device=dev)
+        dstep = lambda: dec(view, csize, Z.DECODE_REFERENCE, index=comp.index)
+        dstep()
+        torch.cuda.synchronize()
+        Z.timing_begin()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            comp(x, Z.ABS, absErrBound=eb)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            dstep()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        kt = Z.timing_end()
+        assert dec.check() == 0, "decoder rejected the stream"
+        kc = kt["compress"][0] / max(1, kt["compress"][1])
+        kd = kt["decode"][0] / max(1, kt["decode"][1])
+        rec = {"block_size": bs, "compression_ratio": nbytes / csize,
+               "compress_gbs": nbytes * steps / (t1 - t0) / 1e9,
+               "decompress_gbs": nbytes * steps / (t2 - t1) / 1e9,
+               "compress_kernels_ms": kc, "decode_kernel_ms": kd,
+               "compress_kernels_pct_hbm": (nbytes + csize) / (kc * 1e-3) / 1e9 / HBM_PEAK_GBS,
+               "decode_kernel_pct_hbm": (nbytes + csize) / (kd * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        if parity:
+            s_gpu = view.cpu().numpy().tobytes()
+            s_ora = O.compress(xh, eb, bs)
+            rec["stream_identical"] = s_gpu == s_ora
+            d_ora = O.decompress(s_ora, n, np.float32)
+            rec["decode_identical"] = bool(np.array_equal(dec.out.cpu().numpy().view(np.uint32),
+                                                          d_ora.view(np.uint32)))
+        log(f"generic bs {bs}: {rec}")
+        out.append(rec)
+        del comp, dec, view
+    torch.cuda.empty_cache()
+    return out
+
+
 def _extents(r, n, nd):
     """Random 1-D/2-D/3-D extents whose product is close to n (at most n)."""
     import numpy as np
@@ -412,6 +473,9 @@ def main():
     ap.add_argument("--batch-fields", type=int, default=1000,
                     help="fields of the batched-launch evaluation (0 skips it)")
     ap.add_argument("--batch-reps", type=int, default=5)
+    ap.add_argument("--generic-bs", default="100,255,7",
+                    help="block sizes outside the tiled kernels to time on the same field "
+                         "('' to skip)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run 3 plain elementwise copies of the input (y = x * 1, "
                          "N*4 bytes read and written) to calibrate HBM byte counters")
@@ -567,6 +631,13 @@ def main():
         log("fp64 8192^2 field, 9 configurations")
         fp64 = fp64_eval(dev, max(2, args.steps // 4), parity=not args.no_parity)
 
+    generic = None
+    gsizes = [int(v) for v in args.generic_bs.split(",") if v.strip()]
+    if gsizes:
+        log(f"block sizes {gsizes} (per-block segment kernels)")
+        generic = generic_eval(x, xh, eb, dev, max(2, args.steps // 4), gsizes,
+                               parity=not args.no_parity and xh is not None)
+
     batched = None
     if args.batch_fields > 0:
         log(f"batched-launch evaluation on {args.batch_fields} fields")
@@ -615,6 +686,7 @@ def main():
         "fixed_rate": fixed,
         "fp64_grf_8192sq": fp64,
         "batched": batched,
+        "generic_block_sizes": generic,
         "host_api_pcie_inclusive": host_api,
     }
     print(json.dumps(line), flush=True)

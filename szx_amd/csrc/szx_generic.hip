@@ -69,28 +69,91 @@ template <int S> __device__ inline uint32_t seg_incl_sum(uint32_t v, int l)
     return v;
 }
 
-// One block per segment of S lanes, 4 consecutive values per lane
-// (S = the smallest power of two with 4 S >= bs).  WRITE = false: record
-// size per block (0 = constant); WRITE = true: the record at its offset in
-// the payload, or the constant median at its rank.
-template <typename T, int S, bool WRITE>
-__global__ __launch_bounds__(256) void k_gen_encode(GenericCompress g)
+// Payload bytes the blocks of one workgroup (256 / S of them) can span:
+// records of at most 5 + S + 4 S W bytes (bs <= 4 S), plus a 4-byte alignment
+// head; LDS words with 3 words of slack for the decoder's 8-byte windows.
+template <int S, int W> struct Span {
+    static constexpr int BYTES = (256 / S) * (5 + S + 4 * S * W) + 4;
+    static constexpr int WORDS = (BYTES + 3) / 4 + 3;
+};
+
+// Values of one workgroup's blocks: at most 256 / S * bs <= 1024, staged
+// through LDS in aligned 16-byte chunks (the value array is 16-byte aligned),
+// plus one chunk of alignment slack at each end.
+template <typename T> struct Vals {
+    static constexpr int CE = 16 / (int)sizeof(T);   // values per chunk
+    static constexpr int N = 1024 + 2 * CE;
+};
+
+// Load values [e0, e1) of x (n values) into xs, chunk-aligned: xs[i] holds
+// x[c0 * CE + i], c0 = e0 / CE.  Chunks past the array end go value by value.
+template <typename T>
+__device__ inline void stage_values(const T *x, uint64_t n, uint64_t e0, uint64_t e1, T *xs)
+{
+    constexpr int CE = Vals<T>::CE;
+    const uint64_t c0 = e0 / CE, c1 = (e1 + CE - 1) / CE;
+    for (uint64_t c = c0 + threadIdx.x; c < c1; c += 256) {
+        T *d = xs + (c - c0) * CE;
+        if ((c + 1) * CE <= n) {
+            *reinterpret_cast<uint4 *>(d) = *reinterpret_cast<const uint4 *>(x + c * CE);
+        } else {
+            for (int j = 0; j < CE; j++)
+                if (c * CE + j < n) d[j] = x[c * CE + j];
+        }
+    }
+}
+
+// Store values [e0, e1) from xs (same layout) to x: chunks wholly inside the
+// range as 16-byte stores, the edge chunks (shared with the neighbouring
+// workgroups) value by value.
+template <typename T>
+__device__ inline void unstage_values(T *x, uint64_t e0, uint64_t e1, const T *xs)
+{
+    constexpr int CE = Vals<T>::CE;
+    const uint64_t c0 = e0 / CE, c1 = (e1 + CE - 1) / CE;
+    for (uint64_t c = c0 + threadIdx.x; c < c1; c += 256) {
+        const T *s = xs + (c - c0) * CE;
+        if (c * CE >= e0 && (c + 1) * CE <= e1) {
+            *reinterpret_cast<uint4 *>(x + c * CE) = *reinterpret_cast<const uint4 *>(s);
+        } else {
+            for (int j = 0; j < CE; j++) {
+                const uint64_t e = c * CE + j;
+                if (e >= e0 && e < e1) x[e] = s[j];
+            }
+        }
+    }
+}
+
+// One block's encode state, per lane of its segment.
+template <typename T> struct GenBlock {
+    using U = typename Word<T>::U;
+    bool constant;
+    int m, len, rb;
+    float med_f, hmed;
+    bool val[4];
+    U sw[4];
+    int ld[4];
+    uint32_t lsum, incl, total;
+};
+
+// One block per segment of S lanes, 4 consecutive values per lane (S = the
+// smallest power of two with 4 S >= bs): statistics, reqLength, shifted
+// words, lead codes and the lane's byte offset in the block's mid bytes.
+// x = the block's values (staged in LDS).
+template <typename T, int S>
+__device__ inline void gen_block(const GenericCompress &g, uint64_t b, int l, const T *x, GenBlock<T> &k_)
 {
     using U = typename Word<T>::U;
     constexpr int W = Word<T>::W;
-    const int l = threadIdx.x & (S - 1);
-    const uint64_t b = blockIdx.x * (uint64_t)(256 / S) + threadIdx.x / S;
-    if (b >= g.nb) return;   // whole segments
     const float eb = g.eb_dev ? *g.eb_dev : g.eb;
-    const T *x = static_cast<const T *>(g.x) + b * (uint64_t)g.bs;
     const uint64_t rest = g.n - b * (uint64_t)g.bs;
     const int m = rest < (uint64_t)g.bs ? (int)rest : g.bs;
+    k_.m = m;
     T v[4];
-    bool val[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-        val[k] = 4 * l + k < m;
-        v[k] = val[k] ? x[4 * l + k] : (T)0;
+        k_.val[k] = 4 * l + k < m;
+        v[k] = k_.val[k] ? x[4 * l + k] : (T)0;
     }
     // block statistics (szx_float.cc:327-333, szx_double.cc:985-991): NaNs
     // never win a comparison, so fmin / fmax over the values, except that a
@@ -99,7 +162,7 @@ __global__ __launch_bounds__(256) void k_gen_encode(GenericCompress g)
     T lo = v[0], hi = v[0];
 #pragma unroll
     for (int k = 1; k < 4; k++)
-        if (val[k]) {
+        if (k_.val[k]) {
             lo = gmin(lo, v[k]);
             hi = gmax(hi, v[k]);
         }
@@ -119,65 +182,102 @@ __global__ __launch_bounds__(256) void k_gen_encode(GenericCompress g)
     }
     const T rad = (hi - lo) / (T)2;
     const T med = lo + rad;
-    const float med_f = (float)med, rad_f = (float)rad;
-    const bool constant = W == 4 ? (rad <= (T)eb) : ((double)rad <= (double)eb);
-    if (constant) {   // segment-uniform
-        if (l == 0) {
-            if (WRITE) g.cmed[g.idx[2 * b + 1]] = bswap32(f32_bits(med_f));
-            else g.size[b] = 0;
-        }
-        return;
-    }
+    k_.med_f = (float)med;
+    const float rad_f = (float)rad;
+    k_.constant = W == 4 ? (rad <= (T)eb) : ((double)rad <= (double)eb);
+    if (k_.constant) return;   // segment-uniform
     int clamped;
-    const int len = req_length<T>(rad_expo(rad_f), req_expo(eb), clamped);
-    const int rb = (len + 7) >> 3;
-    const int shift = (-len) & 7;
-    const float hmed = clamped ? 0.0f : med_f;
+    k_.len = req_length<T>(rad_expo(rad_f), req_expo(eb), clamped);
+    k_.rb = (k_.len + 7) >> 3;
+    const int shift = (-k_.len) & 7;
+    k_.hmed = clamped ? 0.0f : k_.med_f;
     // fp32 encodes against the stored median; fp64 against the median before
     // the clamp (szx_float.cc:60,90; szx_double.cc:38,78)
-    const T msub = W == 4 ? (T)hmed : (T)med_f;
-    U sw[4];
+    const T msub = W == 4 ? (T)k_.hmed : (T)k_.med_f;
 #pragma unroll
-    for (int k = 0; k < 4; k++) sw[k] = to_bits<T>(v[k] - msub) >> shift;
+    for (int k = 0; k < 4; k++) k_.sw[k] = to_bits<T>(v[k] - msub) >> shift;
     // the previous lane's last word (every lane before the last valid one is full)
-    U prev = __shfl_up(sw[3], 1, S);
+    U prev = __shfl_up(k_.sw[3], 1, S);
     if (l == 0) prev = 0;
-    int ld[4];
-    uint32_t lsum = 0;
+    k_.lsum = 0;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-        ld[k] = gen_lead<U>(sw[k] ^ prev);
-        prev = sw[k];
-        if (val[k] && ld[k] < rb) lsum += rb - ld[k];
+        k_.ld[k] = gen_lead<U>(k_.sw[k] ^ prev);
+        prev = k_.sw[k];
+        if (k_.val[k] && k_.ld[k] < k_.rb) k_.lsum += k_.rb - k_.ld[k];
     }
-    const uint32_t incl = seg_incl_sum<S>(lsum, l);
-    const uint32_t total = __shfl(incl, S - 1, S);
-    const int nlead = (m + 3) >> 2;
+    k_.incl = seg_incl_sum<S>(k_.lsum, l);
+    k_.total = __shfl(k_.incl, S - 1, S);
+}
+
+// WRITE = false: record size per block (0 = constant).  WRITE = true: the
+// workgroup's records are built in LDS (they are contiguous in the payload)
+// and written out with aligned word stores; constant medians go to their rank.
+template <typename T, int S, bool WRITE>
+__global__ __launch_bounds__(256) void k_gen_encode(GenericCompress g)
+{
+    constexpr int W = Word<T>::W;
+    constexpr int BPW = 256 / S;
+    __shared__ __align__(16) T xs[Vals<T>::N];
+    const int l = threadIdx.x & (S - 1);
+    const uint64_t b0 = blockIdx.x * (uint64_t)BPW;
+    const uint64_t b = b0 + threadIdx.x / S;
+    const bool live = b < g.nb;   // whole segments
+    const uint64_t bend = b0 + BPW < g.nb ? b0 + BPW : g.nb;
+    const uint64_t e0 = b0 * (uint64_t)g.bs, e1 = bend * (uint64_t)g.bs < g.n ? bend * (uint64_t)g.bs : g.n;
+    stage_values<T>(static_cast<const T *>(g.x), g.n, e0, e1, xs);
+    __syncthreads();
+    GenBlock<T> k_;
+    if (live) gen_block<T, S>(g, b, l, xs + (b * (uint64_t)g.bs - e0 / Vals<T>::CE * Vals<T>::CE), k_);
     if (!WRITE) {
-        if (l == 0) g.size[b] = 5 + (uint32_t)nlead + total;
+        if (live && l == 0) g.size[b] = k_.constant ? 0u : 5u + (uint32_t)((k_.m + 3) >> 2) + k_.total;
         return;
     }
-    uint8_t *rec = g.payload + g.idx[2 * b];
-    if (l == 0) {
-        const uint32_t hb = f32_bits(hmed);
-        rec[0] = (uint8_t)len;
-        rec[1] = (uint8_t)(hb >> 24);
-        rec[2] = (uint8_t)(hb >> 16);
-        rec[3] = (uint8_t)(hb >> 8);
-        rec[4] = (uint8_t)hb;
-    }
-    if (4 * l < m) {
-        uint32_t code = 0;
+    __shared__ uint32_t stage[WRITE ? Span<S, W>::WORDS : 1];
+    uint8_t *lds = reinterpret_cast<uint8_t *>(stage);
+    const uint64_t P0 = g.idx[2 * b0], P1 = g.idx[2 * bend];
+    uint8_t *dst0 = g.payload + P0;
+    const uint32_t head = (uint32_t)((uintptr_t)dst0 & 3);
+    if (live && k_.constant && l == 0) g.cmed[g.idx[2 * b + 1]] = bswap32(f32_bits(k_.med_f));
+    if (live && !k_.constant) {
+        const int nlead = (k_.m + 3) >> 2;
+        uint8_t *rec = lds + head + (uint32_t)(g.idx[2 * b] - P0);
+        if (l == 0) {
+            const uint32_t hb = f32_bits(k_.hmed);
+            rec[0] = (uint8_t)k_.len;
+            rec[1] = (uint8_t)(hb >> 24);
+            rec[2] = (uint8_t)(hb >> 16);
+            rec[3] = (uint8_t)(hb >> 8);
+            rec[4] = (uint8_t)hb;
+        }
+        if (4 * l < k_.m) {
+            uint32_t code = 0;
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (k_.val[k]) code |= (uint32_t)k_.ld[k] << (6 - 2 * k);
+            rec[5 + l] = (uint8_t)code;   // szx_TypeManager.cc:198-273
+        }
+        uint8_t *q = rec + 5 + nlead + (k_.incl - k_.lsum);
 #pragma unroll
         for (int k = 0; k < 4; k++)
-            if (val[k]) code |= (uint32_t)ld[k] << (6 - 2 * k);
-        rec[5 + l] = (uint8_t)code;   // szx_TypeManager.cc:198-273
+            if (k_.val[k])
+                for (int t = W - k_.rb; t < W - k_.ld[k]; t++) *q++ = (uint8_t)(k_.sw[k] >> (8 * t));
     }
-    uint8_t *q = rec + 5 + nlead + (incl - lsum);
-#pragma unroll
-    for (int k = 0; k < 4; k++)
-        if (val[k])
-            for (int t = W - rb; t < W - ld[k]; t++) *q++ = (uint8_t)(sw[k] >> (8 * t));
+    __syncthreads();
+    // copy out: words wholly inside [P0, P1) as words, the two edge words
+    // (shared with the neighbouring workgroups' spans) byte by byte
+    const uint32_t span = (uint32_t)(P1 - P0);
+    const uint32_t nw = (head + span + 3) / 4;
+    uint32_t *dw = reinterpret_cast<uint32_t *>(dst0 - head);
+    for (uint32_t i = threadIdx.x; i < nw; i += 256) {
+        const uint32_t lo = 4 * i, hi = lo + 4;
+        if (lo >= head && hi <= head + span) {
+            dw[i] = stage[i];
+        } else {
+            uint8_t *db = reinterpret_cast<uint8_t *>(dw);
+            for (uint32_t j = lo < head ? head : lo; j < hi && j < head + span; j++) db[j] = lds[j];
+        }
+    }
 }
 
 // ---- exclusive scan of (record bytes, constant blocks) over the blocks ----
@@ -314,25 +414,40 @@ __device__ inline uint32_t carry_combine(uint32_t a, uint32_t b)
     return ((b & keep) | (a & ~keep & 0xFFFFFFu)) | ((a | b) & 0x7000000u);
 }
 
+// Up to 8 staged bytes starting at LDS byte pos, little-endian.
+__device__ inline uint64_t lds_window8(const uint32_t *w, uint32_t pos)
+{
+    const uint32_t i = pos >> 2, sh = pos & 3;
+    const uint32_t a = w[i], b = w[i + 1], c = w[i + 2];
+    return ((uint64_t)__builtin_amdgcn_alignbyte(c, b, sh) << 32) | __builtin_amdgcn_alignbyte(b, a, sh);
+}
+__device__ inline uint32_t lds_window4(const uint32_t *w, uint32_t pos)
+{
+    const uint32_t i = pos >> 2;
+    return __builtin_amdgcn_alignbyte(w[i + 1], w[i], pos & 3);
+}
+
 // One block per segment of S lanes, 4 consecutive values per lane.  The
-// record's lead codes give every value's stream byte count; a segment scan
-// places each lane's bytes, and a second scan carries the inherited top
-// bytes (szxd_float.cc:78-199: a word takes its first min(lead, rb) bytes
-// from the previous word) from lane to lane.
+// workgroup's records (contiguous in the payload) are staged in LDS with
+// aligned word loads, and its values leave through LDS with aligned 16-byte
+// stores.  The record's lead codes give every value's stream byte count; a
+// segment scan places each lane's bytes, and a second scan carries the
+// inherited top bytes (szxd_float.cc:78-199: a word takes its first
+// min(lead, rb) bytes from the previous word) from lane to lane.  Decodes
+// block b into out (its values in LDS); a block that fails a check sets the
+// status and leaves its values unwritten.
 template <typename T, int S>
-__global__ __launch_bounds__(256) void k_gen_decode(GenericDecode p)
+__device__ inline void decode_segment(const GenericDecode &p, uint64_t b, uint64_t bend, uint64_t P0,
+                                      uint64_t P1, uint32_t head, const uint32_t *stage, T *out)
 {
     using U = typename Word<T>::U;
     constexpr int W = Word<T>::W;
     const DecodeHeader *h = p.h;
-    if (h->rc != 0) return;
+    const uint64_t bs = h->bs;
     const int l = threadIdx.x & (S - 1);
-    const uint64_t b = blockIdx.x * (uint64_t)(256 / S) + threadIdx.x / S;
-    const uint64_t nb = h->nb, bs = h->bs, q0 = h->q0;
-    if (b >= nb) return;   // whole segments
+    if (b >= bend) return;   // whole segments
     const uint64_t rest = p.n - b * bs;
     const int m = rest < bs ? (int)rest : (int)bs;
-    T *out = static_cast<T *>(p.out) + b * bs;
     bool val[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) val[k] = 4 * l + k < m;
@@ -353,11 +468,13 @@ __global__ __launch_bounds__(256) void k_gen_decode(GenericDecode p)
         return;
     }
     const int nlead = (m + 3) >> 2;
-    if (nrank != rank || noff < off || noff - off < 5u + (uint64_t)nlead || noff > p.len - q0) {
+    if (nrank != rank || off < P0 || noff > P1 || noff < off || noff - off < 5u + (uint64_t)nlead) {
         if (l == 0) atomicMin(p.status, -21);
         return;
     }
-    const uint8_t *rec = p.s + q0 + off;
+    const uint8_t *lds = reinterpret_cast<const uint8_t *>(stage);
+    const uint32_t r0 = head + (uint32_t)(off - P0);   // record start in LDS
+    const uint8_t *rec = lds + r0;
     const int rl = rec[0];
     if (rl < 1 || rl > 8 * W) {
         if (l == 0) atomicMin(p.status, -21);
@@ -385,20 +502,22 @@ __global__ __launch_bounds__(256) void k_gen_decode(GenericDecode p)
     // the lane's words from the stream alone (inherited bytes still zero),
     // masked to the low 8W - shift bits
     const U mask = ~(U)0 >> shift;
-    const uint8_t *q = rec + 5 + nlead + (incl - lsum);
+    uint32_t pos = r0 + 5 + (uint32_t)nlead + (incl - lsum);
     U wv[4];
     uint32_t mine = 0;   // this lane's carry contribution
 #pragma unroll
     for (int k = 0; k < 4; k++) {
+        const int cnt = rb - lk[k];
         U cur = 0;
+        if (val[k] && cnt > 0) {
+            const U raw = W == 4 ? (U)lds_window4(stage, pos) : (U)lds_window8(stage, pos);
+            cur = (raw & (~(U)0 >> (8 * (W - cnt)))) << (8 * (W - rb));
+            if (W == 8 && !p.exact && rb == 5 && lk[k] == 0)   // szxd_double.cc:240-245: byte 7 <- 4th byte
+                cur = (cur & (~(U)0 >> 8)) | (((raw >> 24) & (U)0xFF) << (8 * (W - 1)));
+            pos += cnt;
+        }
+        cur &= mask;
         if (val[k]) {
-            for (int t = W - rb; t < W - lk[k]; t++) {
-                int src = t - (W - rb);
-                if (W == 8 && !p.exact && rb == 5 && lk[k] == 0 && t == 7) src = 3;   // szxd_double.cc:240-245
-                cur |= (U)q[src] << (8 * t);
-            }
-            q += rb - lk[k];
-            cur &= mask;
             // bytes W-1-i (i < 3) this word writes: those below W - lk
             uint32_t wr = 0;
 #pragma unroll
@@ -425,6 +544,52 @@ __global__ __launch_bounds__(256) void k_gen_decode(GenericDecode p)
         if (val[k]) out[4 * l + k] = from_bits<T>(cur << shift) + med;
         prev = cur;
     }
+}
+
+template <typename T, int S>
+__global__ __launch_bounds__(256) void k_gen_decode(GenericDecode p)
+{
+    constexpr int W = Word<T>::W;
+    constexpr int BPW = 256 / S;
+    __shared__ uint32_t stage[Span<S, W>::WORDS];
+    __shared__ __align__(16) T xs[Vals<T>::N];
+    const DecodeHeader *h = p.h;
+    if (h->rc != 0) return;
+    const uint64_t nb = h->nb, bs = h->bs, q0 = h->q0;
+    const uint64_t b0 = blockIdx.x * (uint64_t)BPW;
+    if (b0 >= nb) return;
+    const uint64_t bend = b0 + BPW < nb ? b0 + BPW : nb;
+    const uint64_t P0 = p.idx[2 * b0], P1 = p.idx[2 * bend];
+    const uint64_t rmax = 5 + (bs + 3) / 4 + bs * W;
+    if (P1 < P0 || P1 - P0 > (bend - b0) * rmax || P1 > p.len - q0) {   // workgroup-uniform
+        if (threadIdx.x == 0) atomicMin(p.status, -21);
+        return;
+    }
+    // stage payload bytes [P0, P1): aligned words inside the stream, the last
+    // partial word byte by byte (q0 >= 13, so the first word starts inside)
+    const uint8_t *src = p.s + q0 + P0;
+    const uint8_t *end = p.s + p.len;
+    const uint32_t head = (uint32_t)((uintptr_t)src & 3);
+    const uint32_t *sw = reinterpret_cast<const uint32_t *>(src - head);
+    const uint32_t span = (uint32_t)(P1 - P0);
+    const uint32_t nw = (head + span + 3) / 4;
+    for (uint32_t i = threadIdx.x; i < nw; i += 256) {
+        const uint8_t *a = reinterpret_cast<const uint8_t *>(sw + i);
+        if (a + 4 <= end) {
+            stage[i] = sw[i];
+        } else {
+            uint32_t v = 0;
+            for (int j = 0; j < 4; j++)
+                if (a + j < end) v |= (uint32_t)a[j] << (8 * j);
+            stage[i] = v;
+        }
+    }
+    __syncthreads();
+    const uint64_t e0 = b0 * bs, e1 = bend * bs < p.n ? bend * bs : p.n;
+    decode_segment<T, S>(p, b0 + threadIdx.x / S, bend, P0, P1, head, stage,
+                         xs + (b0 + threadIdx.x / S) * bs - e0 / Vals<T>::CE * Vals<T>::CE);
+    __syncthreads();
+    unstage_values<T>(static_cast<T *>(p.out), e0, e1, xs);
 }
 
 // ---- host side ----
