@@ -254,6 +254,11 @@ __device__ inline void stage_bytes(uint8_t *p, U t, int m)
         if (j < m) p[j] = (uint8_t)(t >> (8 * j));
 }
 
+// Record bytes a tile stages in LDS.  A tile whose records are larger (less
+// than about 1.6x compression) writes them straight to global memory once its
+// prefix is known; the smaller stage keeps more workgroups resident per CU.
+constexpr int STAGE_CAP_BYTES = 20480;
+
 // Shape of a compressor tile: T, block size, values per lane, slabs.
 template <typename T, int BS, int LV, int SLABS> struct TileShape {
     static constexpr int W = Word<T>::W;
@@ -261,7 +266,8 @@ template <typename T, int BS, int LV, int SLABS> struct TileShape {
     static constexpr int TILE = SLAB * SLABS;       // values per tile
     static constexpr int NBLK = TILE / BS;          // blocks per tile (multiple of 8)
     static constexpr int STAGE = TILE * W + NBLK * (5 + BS / 4);   // largest record bytes
-    static constexpr int STAGE_DW = (STAGE + 32 + 15) / 16 * 4;
+    static constexpr int STAGE_CAP = STAGE < STAGE_CAP_BYTES ? STAGE : STAGE_CAP_BYTES;
+    static constexpr int STAGE_DW = (STAGE_CAP + 32 + 15) / 16 * 4;
 };
 
 // A tile's block words (scan, medians, bitmap bits, total) and its staged
@@ -296,27 +302,18 @@ __device__ __forceinline__ void load_tile(const CompressParams &p, uint64_t tile
             SlabLoad<T>::load4(x, t0 + (uint64_t)u * SLAB + (uint64_t)LV * threadIdx.x + q, p.n, &v[u][q]);
 }
 
-// Encode a tile whose values are in v (all 256 lanes): block statistics and
-// records in registers (st), the exclusive scan of record sizes over the
-// tile's blocks (M), and the aggregate published for the look-back.  `pend`
-// (lane 0 of wave 0) receives the chunk accumulator's previous value for
-// lookback_chunk_done.
-template <typename T, int BS, int LV, int SLABS, int IPT>
-__device__ __forceinline__ void encode_scan(const CompressParams &p, const LookbackRefs &L, uint64_t tile,
-                                            const T (&v)[SLABS][LV], SlabState<T, LV> (&st)[SLABS],
-                                            TileMeta<T, BS, LV, SLABS> &M, unsigned long long &pend)
+// Block statistics and records of a tile whose values are in v, in
+// registers (st); with META, each block's record bytes, constant flag and
+// median go to M for the scan.
+template <typename T, int BS, int LV, int SLABS, bool META>
+__device__ __forceinline__ void encode_slabs(const CompressParams &p, uint64_t tile, const T (&v)[SLABS][LV],
+                                             SlabState<T, LV> (&st)[SLABS], TileMeta<T, BS, LV, SLABS> &M)
 {
     using S = TileShape<T, BS, LV, SLABS>;
     constexpr int SEG = BS / LV;          // lanes per block
     constexpr int SLAB = S::SLAB;
     constexpr int TILE = S::TILE;
     constexpr int BPS = SLAB / BS;        // blocks per slab
-    constexpr int NBLK = S::NBLK;
-    static_assert(BS >= LV && BS <= 256 && (BS & (BS - 1)) == 0, "power-of-two block size");
-    static_assert(NBLK % 8 == 0, "tiles own whole bitmap bytes");
-    static_assert((NBLK / IPT) % 8 == 0, "index tiles own whole bitmap bytes");
-    static_assert(S::STAGE < (1 << 16), "record starts fit the 16-bit block index");
-
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int lseg = tid & (SEG - 1);
@@ -335,16 +332,36 @@ __device__ __forceinline__ void encode_scan(const CompressParams &p, const Lookb
             encode_slab<T, BS, LV, true>(v[u], e0, n, lane, lseg, reqexp, eb, st[u], recb, isconst, med);
         else
             encode_slab<T, BS, LV, false>(v[u], e0, n, lane, lseg, reqexp, eb, st[u], recb, isconst, med);
-        if (lseg == 0) {
+        if (META && lseg == 0) {
             const int lb = u * BPS + tid / SEG;
             M.pack[lb] = recb | (isconst << 20);
             M.state[lb] = recb ? 1 : 0;
             M.med[lb] = med;
         }
     }
+}
+
+// Encode a tile whose values are in v (all 256 lanes): block statistics and
+// records in registers (st), the exclusive scan of record sizes over the
+// tile's blocks (M), and the aggregate published for the look-back.  `pend`
+// (lane 0 of wave 0) receives the chunk accumulator's previous value for
+// lookback_chunk_done.
+template <typename T, int BS, int LV, int SLABS, int IPT>
+__device__ __forceinline__ void encode_scan(const CompressParams &p, const LookbackRefs &L, uint64_t tile,
+                                            const T (&v)[SLABS][LV], SlabState<T, LV> (&st)[SLABS],
+                                            TileMeta<T, BS, LV, SLABS> &M, unsigned long long &pend)
+{
+    using S = TileShape<T, BS, LV, SLABS>;
+    constexpr int NBLK = S::NBLK;
+    static_assert(BS >= LV && BS <= 256 && (BS & (BS - 1)) == 0, "power-of-two block size");
+    static_assert(NBLK % 8 == 0, "tiles own whole bitmap bytes");
+    static_assert((NBLK / IPT) % 8 == 0, "index tiles own whole bitmap bytes");
+    static_assert(S::STAGE < (1 << 16), "record starts fit the 16-bit block index");
+    const int lane = threadIdx.x & 63;
+    encode_slabs<T, BS, LV, SLABS, true>(p, tile, v, st, M);
     __syncthreads();
 
-    if (tid < 64) {
+    if (threadIdx.x < 64) {
         const uint32_t tot = wave0_exclusive_scan<NBLK>(M.pack, lane);
         if (lane == 0) {
             M.total = tot;
@@ -354,18 +371,18 @@ __device__ __forceinline__ void encode_scan(const CompressParams &p, const Lookb
     __syncthreads();
 }
 
-// Stage an encoded tile's records in LDS (tile-relative byte offsets; each
-// byte is written once, by the lane that owns it).  M must hold the tile's
-// scan (encode_scan, then a barrier).
+// Write an encoded tile's records at s_stage + (tile-relative byte offset):
+// the LDS stage, or the payload itself for a tile too large for the stage.
+// Each byte is written once, by the lane that owns it.  M must hold the
+// tile's scan (encode_scan, then a barrier).
 template <typename T, int BS, int LV, int SLABS>
 __device__ __forceinline__ void stage_records(const SlabState<T, LV> (&st)[SLABS],
-                                              const TileMeta<T, BS, LV, SLABS> &M, Stage<T, BS, LV, SLABS> &G)
+                                              const TileMeta<T, BS, LV, SLABS> &M, uint8_t *s_stage)
 {
     using U = typename Word<T>::U;
     constexpr int SEG = BS / LV;
     constexpr int NLB = LV / 4;           // lead-code bytes per lane
     constexpr int BPS = 256 * LV / BS;    // blocks per slab
-    uint8_t *s_stage = reinterpret_cast<uint8_t *>(G.w);
     const int tid = threadIdx.x;
     const int lseg = tid & (SEG - 1);
 #pragma unroll
@@ -397,13 +414,15 @@ __device__ __forceinline__ void stage_records(const SlabState<T, LV> (&st)[SLABS
     }
 }
 
-// Write out a staged tile once its exclusive prefix (ebytes, econst) is known
-// (all 256 lanes; B's staging must be complete): the record index entries,
-// the payload (16-byte stores inside, bytes at the edges, whose other bytes
-// belong to the neighbouring tiles), constant medians and bitmap bytes.
+// Write out a tile once its exclusive prefix (ebytes, econst) is known (all
+// 256 lanes; B's staging must be complete): the record index entries, the
+// payload when it was staged (16-byte stores inside, bytes at the edges, whose
+// other bytes belong to the neighbouring tiles), constant medians and bitmap
+// bytes.
 template <typename T, int BS, int LV, int SLABS, int IPT>
 __device__ __forceinline__ void copy_out(const CompressParams &p, uint64_t tile, const TileMeta<T, BS, LV, SLABS> &B,
-                                         const Stage<T, BS, LV, SLABS> &G, uint64_t ebytes, uint64_t econst)
+                                         const Stage<T, BS, LV, SLABS> &G, uint64_t ebytes, uint64_t econst,
+                                         bool staged)
 {
     using S = TileShape<T, BS, LV, SLABS>;
     constexpr int NBLK = S::NBLK;
@@ -433,7 +452,7 @@ __device__ __forceinline__ void copy_out(const CompressParams &p, uint64_t tile,
         }
     }
 
-    {
+    if (staged) {
         uint8_t *dst = p.payload + ebytes;
         const uint32_t head = (uint32_t)((16 - ((uintptr_t)dst & 15)) & 15);
         const uint32_t h = head < tbytes ? head : tbytes;
@@ -529,11 +548,21 @@ __device__ __forceinline__ void compress_tile(const CompressParams &p, uint64_t 
     SlabState<T, LV> st[SLABS];
     encode_scan<T, BS, LV, SLABS, IPT>(p, L, tile, v, st, lds.meta, pend);
     const LookbackPoll q = lookback_poll(L, tile);
-    stage_records<T, BS, LV, SLABS>(st, lds.meta, lds.stage);
+    const bool staged = (lds.meta.total & 0xFFFFFu) <= (uint32_t)TileShape<T, BS, LV, SLABS>::STAGE_CAP;
+    if (staged) stage_records<T, BS, LV, SLABS>(st, lds.meta, reinterpret_cast<uint8_t *>(lds.stage.w));
     if (threadIdx.x == 0) lookback_chunk_done(L, tile, pend, lds.meta.total & 0xFFFFFu, lds.meta.total >> 20);
     tile_prefix(p, L, tile, q, lds.excl);
     __syncthreads();
-    copy_out<T, BS, LV, SLABS, IPT>(p, tile, lds.meta, lds.stage, lds.excl[0], lds.excl[1]);
+    if (!staged) {
+        // records too large for the stage: encode the tile again (nothing
+        // held across the look-back) and write them at their final offsets
+        T v2[SLABS][LV];
+        load_tile<T, LV, SLABS>(p, tile, v2);
+        SlabState<T, LV> st2[SLABS];
+        encode_slabs<T, BS, LV, SLABS, false>(p, tile, v2, st2, lds.meta);
+        stage_records<T, BS, LV, SLABS>(st2, lds.meta, p.payload + lds.excl[0]);
+    }
+    copy_out<T, BS, LV, SLABS, IPT>(p, tile, lds.meta, lds.stage, lds.excl[0], lds.excl[1], staged);
 }
 
 template <typename T, int BS, int LV, int SLABS, int IPT>

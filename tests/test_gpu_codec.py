@@ -297,3 +297,25 @@ def test_generic_rel_and_damage(device):
     assert dec.check() != 0
     dec(view, size, Z.DECODE_REFERENCE, index=comp.index)
     assert dec.check() == 0
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_incompressible_tiles(dtype, device):
+    """Tiles whose records outgrow the compressor's LDS stage are written
+    straight to the payload; mixed with ordinary tiles in one field."""
+    import torch
+    r = np.random.default_rng(77)
+    n = 300_001
+    x = r.standard_normal(n).astype(dtype)
+    x[100_000:200_000] = np.sin(np.arange(100_000) * 1e-3)     # compressible middle
+    for eb in (1e-7, 1e-3):
+        say("incompressible", dtype.__name__, eb)
+        comp = Z.DeviceCompressor(n, dtype, 128)
+        comp(torch.from_numpy(x).to(device), Z.ABS, eb)
+        view, off, size, _ = comp.stream_view()
+        s = view.cpu().numpy().tobytes()
+        assert s == O.compress(x, eb, 128)
+        dec = Z.DeviceDecompressor(n, dtype, 128, size)
+        dec(view, size, Z.DECODE_REFERENCE, index=comp.index)
+        assert dec.check() == 0
+        assert bits_equal(dec.out.cpu().numpy(), O.decompress(s, n, dtype, ref_quirk=True))
