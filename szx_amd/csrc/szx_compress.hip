@@ -303,9 +303,9 @@ __device__ __forceinline__ void load_tile(const CompressParams &p, uint64_t tile
 }
 
 // Block statistics and records of a tile whose values are in v, in
-// registers (st); with META, each block's record bytes, constant flag and
-// median go to M for the scan.
-template <typename T, int BS, int LV, int SLABS, bool META>
+// registers (st); each block's record bytes, constant flag and median go to
+// M for the scan.
+template <typename T, int BS, int LV, int SLABS>
 __device__ __forceinline__ void encode_slabs(const CompressParams &p, uint64_t tile, const T (&v)[SLABS][LV],
                                              SlabState<T, LV> (&st)[SLABS], TileMeta<T, BS, LV, SLABS> &M)
 {
@@ -332,7 +332,7 @@ __device__ __forceinline__ void encode_slabs(const CompressParams &p, uint64_t t
             encode_slab<T, BS, LV, true>(v[u], e0, n, lane, lseg, reqexp, eb, st[u], recb, isconst, med);
         else
             encode_slab<T, BS, LV, false>(v[u], e0, n, lane, lseg, reqexp, eb, st[u], recb, isconst, med);
-        if (META && lseg == 0) {
+        if (lseg == 0) {
             const int lb = u * BPS + tid / SEG;
             M.pack[lb] = recb | (isconst << 20);
             M.state[lb] = recb ? 1 : 0;
@@ -358,7 +358,7 @@ __device__ __forceinline__ void encode_scan(const CompressParams &p, const Lookb
     static_assert((NBLK / IPT) % 8 == 0, "index tiles own whole bitmap bytes");
     static_assert(S::STAGE < (1 << 16), "record starts fit the 16-bit block index");
     const int lane = threadIdx.x & 63;
-    encode_slabs<T, BS, LV, SLABS, true>(p, tile, v, st, M);
+    encode_slabs<T, BS, LV, SLABS>(p, tile, v, st, M);
     __syncthreads();
 
     if (threadIdx.x < 64) {
@@ -376,8 +376,8 @@ __device__ __forceinline__ void encode_scan(const CompressParams &p, const Lookb
 // Each byte is written once, by the lane that owns it.  M must hold the
 // tile's scan (encode_scan, then a barrier).
 template <typename T, int BS, int LV, int SLABS>
-__device__ __forceinline__ void stage_records(const SlabState<T, LV> (&st)[SLABS],
-                                              const TileMeta<T, BS, LV, SLABS> &M, uint8_t *s_stage)
+__device__ __forceinline__ void stage_slab(const SlabState<T, LV> &st, int u, const TileMeta<T, BS, LV, SLABS> &M,
+                                           uint8_t *s_stage)
 {
     using U = typename Word<T>::U;
     constexpr int SEG = BS / LV;
@@ -385,33 +385,38 @@ __device__ __forceinline__ void stage_records(const SlabState<T, LV> (&st)[SLABS
     constexpr int BPS = 256 * LV / BS;    // blocks per slab
     const int tid = threadIdx.x;
     const int lseg = tid & (SEG - 1);
-#pragma unroll
-    for (int u = 0; u < SLABS; u++) {
-        const uint32_t meta = st[u].meta;
-        if (!((meta >> 16) & 1)) continue;
-        const int lb = u * BPS + tid / SEG;
-        const uint32_t base = M.pack[lb] & 0xFFFFFu;
-        const uint32_t nlead = st[u].code >> 16;
-        if (lseg == 0) {   // reqLength byte + big-endian median
-            const uint32_t hb = bswap32(f32_bits(st[u].hmed));
-            s_stage[base] = (uint8_t)(meta >> 17);
-            s_stage[base + 1] = (uint8_t)hb;
-            s_stage[base + 2] = (uint8_t)(hb >> 8);
-            s_stage[base + 3] = (uint8_t)(hb >> 16);
-            s_stage[base + 4] = (uint8_t)(hb >> 24);
-        }
-#pragma unroll
-        for (int b = 0; b < NLB; b++)
-            if ((uint32_t)(lseg * NLB + b) < nlead)
-                s_stage[base + 5 + lseg * NLB + b] = (uint8_t)(st[u].code >> (8 * b));
-        uint8_t *q = s_stage + base + 5 + nlead + (meta & 0xFFFu);
-#pragma unroll
-        for (int k = 0; k < LV; k++) {
-            const int m = (int)(st[u].mb >> (8 * k + 3)) & 31;
-            stage_bytes<U>(q, st[u].mid[k], m);
-            q += m;
-        }
+    const uint32_t meta = st.meta;
+    if (!((meta >> 16) & 1)) return;
+    const int lb = u * BPS + tid / SEG;
+    const uint32_t base = M.pack[lb] & 0xFFFFFu;
+    const uint32_t nlead = st.code >> 16;
+    if (lseg == 0) {   // reqLength byte + big-endian median
+        const uint32_t hb = bswap32(f32_bits(st.hmed));
+        s_stage[base] = (uint8_t)(meta >> 17);
+        s_stage[base + 1] = (uint8_t)hb;
+        s_stage[base + 2] = (uint8_t)(hb >> 8);
+        s_stage[base + 3] = (uint8_t)(hb >> 16);
+        s_stage[base + 4] = (uint8_t)(hb >> 24);
     }
+#pragma unroll
+    for (int b = 0; b < NLB; b++)
+        if ((uint32_t)(lseg * NLB + b) < nlead)
+            s_stage[base + 5 + lseg * NLB + b] = (uint8_t)(st.code >> (8 * b));
+    uint8_t *q = s_stage + base + 5 + nlead + (meta & 0xFFFu);
+#pragma unroll
+    for (int k = 0; k < LV; k++) {
+        const int m = (int)(st.mb >> (8 * k + 3)) & 31;
+        stage_bytes<U>(q, st.mid[k], m);
+        q += m;
+    }
+}
+
+template <typename T, int BS, int LV, int SLABS>
+__device__ __forceinline__ void stage_records(const SlabState<T, LV> (&st)[SLABS],
+                                              const TileMeta<T, BS, LV, SLABS> &M, uint8_t *s_stage)
+{
+#pragma unroll
+    for (int u = 0; u < SLABS; u++) stage_slab<T, BS, LV, SLABS>(st[u], u, M, s_stage);
 }
 
 // Write out a tile once its exclusive prefix (ebytes, econst) is known (all
@@ -554,13 +559,25 @@ __device__ __forceinline__ void compress_tile(const CompressParams &p, uint64_t 
     tile_prefix(p, L, tile, q, lds.excl);
     __syncthreads();
     if (!staged) {
-        // records too large for the stage: encode the tile again (nothing
-        // held across the look-back) and write them at their final offsets
-        T v2[SLABS][LV];
-        load_tile<T, LV, SLABS>(p, tile, v2);
-        SlabState<T, LV> st2[SLABS];
-        encode_slabs<T, BS, LV, SLABS, false>(p, tile, v2, st2, lds.meta);
-        stage_records<T, BS, LV, SLABS>(st2, lds.meta, p.payload + lds.excl[0]);
+        // records too large for the stage: encode the tile again, one slab at
+        // a time (nothing held across the look-back), and write them at their
+        // final offsets
+        constexpr int SLAB = 256 * LV;
+        const uint64_t t0 = tile * (uint64_t)(SLAB * SLABS);
+        const float eb = p.eb_dev ? *p.eb_dev : p.eb;
+        const int reqexp = req_expo(eb);
+        const int lseg = threadIdx.x & (BS / LV - 1);
+        for (int u = 0; u < SLABS; u++) {
+            T vu[LV];
+            const uint64_t e0 = t0 + (uint64_t)u * SLAB + (uint64_t)LV * threadIdx.x;
+#pragma unroll
+            for (int q = 0; q < LV; q += 4) SlabLoad<T>::load4(static_cast<const T *>(p.x), e0 + q, p.n, &vu[q]);
+            SlabState<T, LV> su;
+            uint32_t recb, isconst;
+            float med;
+            encode_slab<T, BS, LV, false>(vu, e0, p.n, threadIdx.x & 63, lseg, reqexp, eb, su, recb, isconst, med);
+            stage_slab<T, BS, LV, SLABS>(su, u, lds.meta, p.payload + lds.excl[0]);
+        }
     }
     copy_out<T, BS, LV, SLABS, IPT>(p, tile, lds.meta, lds.stage, lds.excl[0], lds.excl[1], staged);
 }
