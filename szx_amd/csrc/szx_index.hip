@@ -22,8 +22,8 @@
 //   B  k_ix_entry   entries: a chunk after a converged chunk starts at its
 //                   exit; a run of unconverged chunks is followed through their
 //                   stored exit tables by one thread.
-//   C  k_ix_walk    per chunk: walk the chain from the entry, keep the record
-//                   starts, count them.
+//   C  k_ix_walk    per chunk (one wave): walk the chain from the entry, one
+//                   step per record, keep the record starts, count them.
 //   D  scans        records before each chunk; non-constant blocks before
 //                   each compressor tile.
 //   E  k_ix_scatter record starts in record order.
@@ -153,7 +153,9 @@ __device__ inline void ix_steps(const uint32_t *bw, uint32_t base, uint64_t *pre
             const uint32_t rb = (rl[i] + 7) >> 3;
             const uint32_t sh = rb >= 3 ? 32 : (rb == 2 ? 16 : 0);
             const uint32_t sub = (uint32_t)((hi[i] - lo[i]) >> sh) & 0xFFFFu;
-            j = (uint16_t)(p + 5 + (uint32_t)nl + rb * (uint32_t)bs - sub);
+            // nonzero unused codes of a partial lead byte (a damaged stream)
+            // could subtract more than the block holds: no record there
+            if (sub <= rb * (uint32_t)bs) j = (uint16_t)(p + 5 + (uint32_t)nl + rb * (uint32_t)bs - sub);
         }
         J[p] = j;
     }
@@ -256,83 +258,73 @@ __global__ void k_ix_entry(IxParams x)
     }
 }
 
-// C: record starts inside every chunk, from its entry.  The chain from the
-// entry is marked while the step table doubles: before round r every
-// position within 2^r steps of the entry is marked, and round r marks their
-// images under J^(2^r).  Marked positions below IX_C, in increasing order,
-// are the chunk's record starts.
+// Stage bytes [src, src + cnt) into one wave's LDS words at byte offset
+// (src & 15): aligned 16-byte chunks lying wholly inside the stream [lo, hi)
+// with one load, the edge chunks byte by byte (bytes outside the stream read
+// as zero); returns the offset.  Words past the staged bytes are not cleared.
+__device__ inline uint32_t ix_stage_wave(uint32_t *bw, const uint8_t *src, uint32_t cnt, const uint8_t *lo,
+                                         const uint8_t *hi, int lane)
+{
+    uint8_t *b = reinterpret_cast<uint8_t *>(bw);
+    const uintptr_t a = (uintptr_t)src & ~(uintptr_t)15;
+    const uint32_t head = (uint32_t)((uintptr_t)src - a);
+    const uint32_t nchunk = (head + cnt + 15) / 16;
+    for (uint32_t c = lane; c < nchunk; c += 64) {
+        const uint8_t *g = reinterpret_cast<const uint8_t *>(a + 16 * (uintptr_t)c);
+        if (g >= lo && g + 16 <= hi) {
+            *reinterpret_cast<uint4 *>(b + 16 * c) = *reinterpret_cast<const uint4 *>(g);
+        } else {
+            for (int k = 0; k < 16; k++) b[16 * c + k] = (g + k >= lo && g + k < hi) ? g[k] : 0;
+        }
+    }
+    return head;
+}
+
+constexpr uint32_t IXW_WORDS = (16 + IX_C + IX_PAD) / 4 + 2;   // one wave's staged chunk
+
+// C: record starts inside every chunk, from its entry.  One wave per chunk
+// follows the chain: a record's size comes from its reqLength byte and its
+// lead-code bytes (one byte per lane, a wave sum of the same per-byte counts
+// the map kernel's prefix sums hold), so the chain costs one step per record.
+// Marks the entry and every chain position below IX_C, stopping where the
+// map kernel's step table would be invalid.
 __global__ __launch_bounds__(256) void k_ix_walk(IxParams x)
 {
-    __shared__ uint32_t bw[IX_WORDS];
-    __shared__ uint16_t J0[IX_C + IX_RMAX];
-    __shared__ union {
-        uint64_t pre[IX_PRE];
-        uint16_t j1[IX_C + IX_RMAX];
-    } u;
-    __shared__ uint8_t on[IX_C + IX_RMAX];
-    __shared__ uint32_t s_part[256];
-    const uint64_t c = blockIdx.x;
+    __shared__ uint32_t bw[4][IXW_WORDS];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint64_t c = blockIdx.x * 4ull + wv;
+    if (*x.status != 0) return;   // workgroup-uniform: written by earlier kernels only
     const DecodeHeader &h = *x.h;
-    if (*x.status != 0) return;
     const uint64_t P = x.len - h.q0;
     const uint64_t g0 = c * IX_C;
-    if (g0 >= P) {
-        if (threadIdx.x == 0) x.cnt[c] = 0;
-        return;
-    }
-    const uint32_t avail = (uint32_t)(P - g0 < IX_C + IX_PAD ? P - g0 : IX_C + IX_PAD);
-    const uint32_t base = ix_stage(bw, x.s + h.q0 + g0, avail, IX_WORDS, x.s, x.s + x.len);
-    const uint32_t dom = IX_C + x.rmax;
-    ix_steps(bw, base, u.pre, J0, dom, avail, x.nl, x.bs, x.w);
+    const bool live = c < x.nch;
+    const bool has = live && g0 < P;
+    const uint32_t avail = has ? (uint32_t)(P - g0 < IX_C + IX_PAD ? P - g0 : IX_C + IX_PAD) : 0u;
+    uint32_t base = 0;
+    if (has) base = ix_stage_wave(bw[wv], x.s + h.q0 + g0, avail, x.s, x.s + x.len, lane);
     __syncthreads();
-    const uint32_t e0 = x.entry[c];
-    for (uint32_t p = threadIdx.x; p < dom; p += blockDim.x) {
-        on[p] = (p == e0) ? 1 : 0;
-        if (p >= IX_C) u.j1[p] = (uint16_t)p;
-    }
-    __syncthreads();
-    uint16_t *ja = J0, *jb = u.j1;
-    for (int r = 0; r < x.rounds; r++) {
-        uint16_t v[IX_PER], j2[IX_PER];
-        uint8_t m[IX_PER];
-#pragma unroll
-        for (int i = 0; i < IX_PER; i++) {
-            v[i] = ja[threadIdx.x + 256 * i];
-            m[i] = on[threadIdx.x + 256 * i];
-        }
-#pragma unroll
-        for (int i = 0; i < IX_PER; i++) j2[i] = v[i] == IX_INV ? IX_INV : ja[v[i]];
-#pragma unroll
-        for (int i = 0; i < IX_PER; i++) {
-            if (m[i] && v[i] != IX_INV) on[v[i]] = 1;
-            jb[threadIdx.x + 256 * i] = j2[i];
-        }
-        __syncthreads();
-        uint16_t *t = ja;
-        ja = jb;
-        jb = t;
-    }
-    // compaction of the marked positions below IX_C (16 per thread, in order)
-    constexpr uint32_t PER = IX_C / 256;
-    const uint32_t p0 = threadIdx.x * PER;
+    if (!live) return;
+    const uint8_t *b = reinterpret_cast<const uint8_t *>(bw[wv]) + base;
+    const uint32_t nl = (uint32_t)x.nl, bsz = (uint32_t)x.bs, wmax = 8u * (uint32_t)x.w;
+    uint32_t p = has ? x.entry[c] : IX_INV;
     uint32_t k = 0;
-    for (uint32_t i = 0; i < PER; i++) k += on[p0 + i];
-    s_part[threadIdx.x] = k;
-    __syncthreads();
-    for (int d = 1; d < 256; d <<= 1) {
-        const uint32_t v = threadIdx.x >= (unsigned)d ? s_part[threadIdx.x - d] : 0;
-        __syncthreads();
-        s_part[threadIdx.x] += v;
-        __syncthreads();
-    }
-    uint32_t o = s_part[threadIdx.x] - k;
     uint16_t *pos = x.pos + c * x.maxr;
-    for (uint32_t i = 0; i < PER; i++)
-        if (on[p0 + i]) {
-            if (o < x.maxr) pos[o] = (uint16_t)(p0 + i);
-            o++;
+    if (p != IX_INV) {
+        while (p < IX_C) {
+            if (lane == 0 && k < x.maxr) pos[k] = (uint16_t)p;
+            k++;
+            const uint32_t rl = b[p];
+            if (p + 5 + nl > avail || rl == 0 || rl > wmax) break;
+            const uint32_t rb = (rl + 7) >> 3;
+            const uint32_t sh = rb >= 3 ? 32 : (rb == 2 ? 16 : 0);
+            uint32_t sub = (uint32_t)lane < nl ? (uint32_t)(lead_counts(b[p + 5 + lane]) >> sh) & 0xFFFFu : 0u;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) sub += __shfl_xor(sub, d, 64);
+            if (sub > rb * bsz) break;
+            p = p + 5 + nl + rb * bsz - sub;
         }
-    if (threadIdx.x == 255) x.cnt[c] = s_part[255] < x.maxr ? s_part[255] : x.maxr;
+    }
+    if (lane == 0) x.cnt[c] = k < x.maxr ? k : x.maxr;
 }
 
 // Exclusive scan of n u32 values in three steps: per 1024-value block an
@@ -599,7 +591,7 @@ void index_build_launch(const uint8_t *s, uint64_t len, uint64_t n, int w, int b
     const unsigned nch = (unsigned)L.nch;
     hipLaunchKernelGGL(k_ix_map, dim3(nch), dim3(256), 0, st, x);
     hipLaunchKernelGGL(k_ix_entry, dim3((nch + 255) / 256), dim3(256), 0, st, x);
-    hipLaunchKernelGGL(k_ix_walk, dim3(nch), dim3(256), 0, st, x);
+    hipLaunchKernelGGL(k_ix_walk, dim3((nch + 3) / 4), dim3(256), 0, st, x);
     scan_u32(x.cnt, L.nch, x.cbase, bsum, x.found, st);
     hipLaunchKernelGGL(k_ix_scatter, dim3(nch), dim3(256), 0, st, x);
     hipLaunchKernelGGL(k_ix_tile_counts, dim3((unsigned)((L.ntiles + 255) / 256)), dim3(256), 0, st, x);
