@@ -11,8 +11,9 @@
 // subtract nothing).  Following the chain from byte 0 is inherently serial,
 // so it is cut into chunks of IX_C payload bytes:
 //
-//   A  k_ix_map     per chunk: size(p) for every byte p, then pointer doubling
-//                   in LDS gives, for every possible entry offset e (the first
+//   A  k_ix_map     per chunk: size(p) for every byte p, then chains walked
+//                   from every candidate entry (merging chains stop where they
+//                   meet) give, for every possible entry offset e (the first
 //                   record start at or after the chunk start lies less than one
 //                   record size R into the chunk), the exit offset into the
 //                   next chunk.  Chains from different entries merge as soon
@@ -162,16 +163,29 @@ __device__ inline void ix_steps(const uint32_t *bw, uint32_t base, uint64_t *pre
     for (uint32_t p = IX_C + threadIdx.x; p < dom; p += blockDim.x) J[p] = (uint16_t)p;
 }
 
-// A: exit map of every chunk.
+// A: exit map of every chunk.  After the step table, one chain per candidate
+// entry e < rmax is walked, each position claimed by the first chain to reach
+// it (an LDS compare-and-swap): a chain that meets a claimed position stops
+// and links to its owner, whose chain continues from there exactly as its own
+// would.  Chains merge after a few records, so the walk costs about one step
+// per distinct record start; links are then resolved by pointer jumping (link
+// positions increase along a link chain, so there are no cycles).
+constexpr uint32_t IX_NONE = 0xFFFFFFFFu;
+constexpr uint32_t IX_RES_INV = 0x40000000u;
+constexpr uint32_t IX_RES_LINK = 0x80000000u;
+
 __global__ __launch_bounds__(256) void k_ix_map(IxParams x)
 {
     __shared__ uint32_t bw[IX_WORDS];
-    __shared__ uint16_t J0[IX_C + IX_RMAX];
+    __shared__ uint16_t J0[IX_C];
     __shared__ union {
         uint64_t pre[IX_PRE];
-        uint16_t j1[IX_C + IX_RMAX];
+        struct {
+            uint32_t owner[IX_C];
+            uint32_t res[IX_RMAX];
+        } w;
     } u;
-    __shared__ unsigned s_min, s_max;
+    __shared__ unsigned s_min, s_max, s_more;
     const uint64_t c = blockIdx.x;
     const DecodeHeader &h = *x.h;
     if (*x.status != 0) return;
@@ -186,39 +200,61 @@ __global__ __launch_bounds__(256) void k_ix_map(IxParams x)
     }
     const uint32_t avail = (uint32_t)(P - g0 < IX_C + IX_PAD ? P - g0 : IX_C + IX_PAD);
     const uint32_t base = ix_stage(bw, x.s + h.q0 + g0, avail, IX_WORDS, x.s, x.s + x.len);
-    const uint32_t dom = IX_C + x.rmax;
-    ix_steps(bw, base, u.pre, J0, dom, avail, x.nl, x.bs, x.w);
-    __syncthreads();   // the prefix sums die here: their space holds the second table
+    ix_steps(bw, base, u.pre, J0, IX_C, avail, x.nl, x.bs, x.w);
+    __syncthreads();   // the prefix sums die here: their space holds the claims
+    for (uint32_t p = threadIdx.x; p < IX_C; p += blockDim.x) u.w.owner[p] = IX_NONE;
     if (threadIdx.x == 0) {
         s_min = 0xFFFFFFFFu;
         s_max = 0;
     }
     __syncthreads();
-    for (uint32_t p = IX_C + threadIdx.x; p < dom; p += blockDim.x) u.j1[p] = (uint16_t)p;
+    const uint32_t rmax = x.rmax;
+    for (uint32_t e = threadIdx.x; e < rmax; e += blockDim.x) {
+        uint32_t p = e, r;
+        for (;;) {
+            if (p >= IX_C) {
+                r = p - IX_C;
+                break;
+            }
+            const uint32_t o = atomicCAS(&u.w.owner[p], IX_NONE, e);
+            if (o != IX_NONE) {
+                r = IX_RES_LINK | o;
+                break;
+            }
+            const uint16_t j = J0[p];
+            if (j == IX_INV) {
+                r = IX_RES_INV;
+                break;
+            }
+            p = j;
+        }
+        u.w.res[e] = r;
+    }
     __syncthreads();
-    uint16_t *ja = J0, *jb = u.j1;   // current table, next table
-    for (int r = 0; r < x.rounds; r++) {
-        uint16_t v[IX_PER];
-#pragma unroll
-        for (int i = 0; i < IX_PER; i++) v[i] = ja[threadIdx.x + 256 * i];
-#pragma unroll
-        for (int i = 0; i < IX_PER; i++) v[i] = v[i] == IX_INV ? IX_INV : ja[v[i]];
-#pragma unroll
-        for (int i = 0; i < IX_PER; i++) jb[threadIdx.x + 256 * i] = v[i];
+    for (int round = 0; round < 16; round++) {   // link depth < 2^16
+        if (threadIdx.x == 0) s_more = 0;
         __syncthreads();
-        uint16_t *t = ja;
-        ja = jb;
-        jb = t;
+        for (uint32_t e = threadIdx.x; e < rmax; e += blockDim.x) {
+            const uint32_t r = u.w.res[e];
+            if (r & IX_RES_LINK) {
+                const uint32_t t = u.w.res[r & 0xFFFFu];
+                u.w.res[e] = t;
+                if (t & IX_RES_LINK) s_more = 1;
+            }
+        }
+        __syncthreads();
+        const bool more = s_more != 0;
+        __syncthreads();   // every lane has read the flag before it is cleared
+        if (!more) break;
     }
     // entries e < rmax: exit offset into the next chunk (wave min / max, then
     // one LDS atomic per wave)
     unsigned mn = 0xFFFFFFFFu, mx = 0;
-    for (uint32_t e = threadIdx.x; e < x.rmax; e += blockDim.x) {
-        const uint16_t j = ja[e];
-        if (j != IX_INV && j >= IX_C) {
-            const unsigned v = (unsigned)(j - IX_C);
-            mn = v < mn ? v : mn;
-            mx = v > mx ? v : mx;
+    for (uint32_t e = threadIdx.x; e < rmax; e += blockDim.x) {
+        const uint32_t r = u.w.res[e];
+        if (!(r & (IX_RES_INV | IX_RES_LINK))) {
+            mn = r < mn ? r : mn;
+            mx = r > mx ? r : mx;
         }
     }
 #pragma unroll
@@ -238,9 +274,9 @@ __global__ __launch_bounds__(256) void k_ix_map(IxParams x)
         x.exits[c] = s_min == 0xFFFFFFFFu ? IX_INV : (uint16_t)s_min;
     }
     if (!conv)
-        for (uint32_t e = threadIdx.x; e < x.rmax; e += blockDim.x) {
-            const uint16_t j = ja[e];
-            x.tables[c * x.rmax + e] = (j != IX_INV && j >= IX_C) ? (uint16_t)(j - IX_C) : IX_INV;
+        for (uint32_t e = threadIdx.x; e < rmax; e += blockDim.x) {
+            const uint32_t r = u.w.res[e];
+            x.tables[c * rmax + e] = (r & (IX_RES_INV | IX_RES_LINK)) ? IX_INV : (uint16_t)r;
         }
 }
 
@@ -566,10 +602,6 @@ void index_build_launch(const uint8_t *s, uint64_t len, uint64_t n, int w, int b
     x.nch = L.nch;
     x.ntiles = L.ntiles;
     x.tile_blocks = bs_tiled(bs) ? tile_values(w) / (uint64_t)bs : 1;
-    // doubling rounds: a chunk holds at most IX_C / (5 + NL) records
-    int rounds = 1;
-    while ((1u << rounds) < IX_C / (5 + (uint32_t)L.nl) + 2) rounds++;
-    x.rounds = rounds + 1;
     x.h = h;
     x.status = status;
     x.exits = reinterpret_cast<uint16_t *>(ws + L.o_exits);

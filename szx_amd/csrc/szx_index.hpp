@@ -14,7 +14,7 @@ constexpr uint32_t IX_RMAX = 2117;    // largest record: 5 + 64 + 256 * 8
 struct IxParams {
     const uint8_t *s;
     uint64_t len, n;
-    int bs, w, nl, rounds;
+    int bs, w, nl;
     uint32_t rmax, maxr;
     uint64_t nch, ntiles, tile_blocks;
     const DecodeHeader *h;
