@@ -161,7 +161,7 @@ int szx_amd_compress_dev(const void *d_in, size_t n, int dataType, int errBoundM
                          size_t outCapacity, void *d_work, size_t workBytes, uint64_t *d_result,
                          uint64_t *d_index, void *hipStream);
 
-/* Workspace bytes for szx_amd_decompress_dev. */
+/* Workspace bytes for szx_amd_decompress_dev (0 for an unsupported blockSize). */
 size_t szx_amd_decompress_workspace(size_t n, int dataType, int blockSize, size_t streamBytes);
 
 /* Decode flags. */

@@ -609,7 +609,7 @@ int szx_amd_compress_dev(const void *d_in, size_t n, int dataType, int errBoundM
 size_t szx_amd_decompress_workspace(size_t n, int dataType, int blockSize, size_t streamBytes)
 {
     const int w = width_of(dataType);
-    if (!w) return 0;
+    if (!w || !bs_supported(blockSize)) return 0;
     return decompress_layout(n, w, blockSize, streamBytes).ws_bytes;
 }
 

@@ -65,3 +65,29 @@ def test_batch_workspace_validates_fields_without_gpu():
     assert ws([F(16, 10, 5, szx_amd.ABS, 1.0, 0.0, 16, 0)]) == 0
     assert ws([F(16, 10, szx_amd.SZx_FLOAT, szx_amd.FXR, 1.0, 0.0, 16, 0)]) == 0
     assert L.szx_amd_batch_workspace(None, 0, 128) == 0
+
+
+def test_block_size_range_without_gpu():
+    """Every blockSize 1..256 sizes its buffers (the powers of two 4..256 use
+    the tiled layout and index, the others the per-block ones); anything else
+    is refused with 0.  Host-side size queries only."""
+    L = szx_amd.lib()
+    F, D = szx_amd.SZx_FLOAT, szx_amd.SZx_DOUBLE
+    n = 100_003
+    for bs in (1, 2, 3, 5, 7, 100, 128, 255, 256):
+        for dt in (F, D):
+            assert L.szx_amd_compress_capacity(n, dt, bs) > 0
+            assert L.szx_amd_compress_workspace(n, dt, bs) > 0
+            assert L.szx_amd_decompress_workspace(n, dt, bs, 1 << 20) > 0
+    nb = -(-n // 100)
+    assert L.szx_amd_index_words(n, F, 100) == 2 * (nb + 1)
+    nb128, tiles = -(-n // 128), -(-n // 4096)
+    assert L.szx_amd_index_words(n, F, 128) == 2 * (tiles + 1) + (nb128 + 3) // 4
+    for bs in (0, 257, -1):
+        assert L.szx_amd_compress_capacity(n, F, bs) == 0
+        assert L.szx_amd_compress_workspace(n, F, bs) == 0
+        assert L.szx_amd_decompress_workspace(n, F, bs, 1 << 20) == 0
+        assert L.szx_amd_index_words(n, F, bs) == 0
+    # the fixed-rate solver takes the tiled block sizes only
+    assert L.szx_amd_fixed_rate_workspace(n, F, 100) == 0
+    assert L.szx_amd_fixed_rate_workspace(n, F, 128) > 0
