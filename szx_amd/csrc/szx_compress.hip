@@ -375,7 +375,7 @@ __device__ __forceinline__ void encode_scan(const CompressParams &p, const Lookb
 // the LDS stage, or the payload itself for a tile too large for the stage.
 // Each byte is written once, by the lane that owns it.  M must hold the
 // tile's scan (encode_scan, then a barrier).
-template <typename T, int BS, int LV, int SLABS>
+template <typename T, int BS, int LV, int SLABS, bool LDS>
 __device__ __forceinline__ void stage_slab(const SlabState<T, LV> &st, int u, const TileMeta<T, BS, LV, SLABS> &M,
                                            uint8_t *s_stage)
 {
@@ -402,12 +402,37 @@ __device__ __forceinline__ void stage_slab(const SlabState<T, LV> &st, int u, co
     for (int b = 0; b < NLB; b++)
         if ((uint32_t)(lseg * NLB + b) < nlead)
             s_stage[base + 5 + lseg * NLB + b] = (uint8_t)(st.code >> (8 * b));
-    uint8_t *q = s_stage + base + 5 + nlead + (meta & 0xFFFu);
+    if constexpr (LDS && sizeof(U) == 4) {
+        // Mid bytes packed into aligned words: each value's m bytes are
+        // appended to a 64-bit accumulator whose low word is OR-ed into the
+        // zeroed stage after every value and dropped once complete.  OR is
+        // idempotent and every stage byte has one writer, so the words a run
+        // shares with its neighbours need no care, and nothing branches.
+        const uint32_t q = base + 5 + nlead + (meta & 0xFFFu);
+        uint32_t *sw = reinterpret_cast<uint32_t *>(s_stage);
+        uint32_t wq = q >> 2, pend = q & 3;   // pending bytes: placeholders first
+        uint64_t acc = 0;
 #pragma unroll
-    for (int k = 0; k < LV; k++) {
-        const int m = (int)(st.mb >> (8 * k + 3)) & 31;
-        stage_bytes<U>(q, st.mid[k], m);
-        q += m;
+        for (int k = 0; k < LV; k++) {
+            const uint32_t m = (st.mb >> (8 * k + 3)) & 31u;
+            const uint32_t x = (uint32_t)st.mid[k] & (uint32_t)(0xFFFFFFFFull >> (32 - 8 * m));
+            acc |= (uint64_t)x << (8 * pend);
+            pend += m;
+            __hip_atomic_fetch_or(&sw[wq], (uint32_t)acc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            const bool adv = pend >= 4;
+            acc = adv ? acc >> 32 : acc;
+            wq += adv ? 1u : 0u;
+            pend -= adv ? 4u : 0u;
+        }
+        __hip_atomic_fetch_or(&sw[wq], (uint32_t)acc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    } else {
+        uint8_t *q = s_stage + base + 5 + nlead + (meta & 0xFFFu);
+#pragma unroll
+        for (int k = 0; k < LV; k++) {
+            const int m = (int)(st.mb >> (8 * k + 3)) & 31;
+            stage_bytes<U>(q, st.mid[k], m);
+            q += m;
+        }
     }
 }
 
@@ -416,7 +441,7 @@ __device__ __forceinline__ void stage_records(const SlabState<T, LV> (&st)[SLABS
                                               const TileMeta<T, BS, LV, SLABS> &M, uint8_t *s_stage)
 {
 #pragma unroll
-    for (int u = 0; u < SLABS; u++) stage_slab<T, BS, LV, SLABS>(st[u], u, M, s_stage);
+    for (int u = 0; u < SLABS; u++) stage_slab<T, BS, LV, SLABS, true>(st[u], u, M, s_stage);
 }
 
 // Write out a tile once its exclusive prefix (ebytes, econst) is known (all
@@ -550,6 +575,10 @@ __device__ __forceinline__ void compress_tile(const CompressParams &p, uint64_t 
     T v[SLABS][LV];
     unsigned long long pend = 0;
     load_tile<T, LV, SLABS>(p, tile, v);
+    if constexpr (sizeof(T) == 4) {   // the fp32 stage is OR-ed into (stage_slab)
+        constexpr int N4 = TileShape<T, BS, LV, SLABS>::STAGE_DW / 4;
+        for (int i = threadIdx.x; i < N4; i += 256) reinterpret_cast<uint4 *>(lds.stage.w)[i] = make_uint4(0, 0, 0, 0);
+    }
     SlabState<T, LV> st[SLABS];
     encode_scan<T, BS, LV, SLABS, IPT>(p, L, tile, v, st, lds.meta, pend);
     const LookbackPoll q = lookback_poll(L, tile);
@@ -576,7 +605,7 @@ __device__ __forceinline__ void compress_tile(const CompressParams &p, uint64_t 
             uint32_t recb, isconst;
             float med;
             encode_slab<T, BS, LV, false>(vu, e0, p.n, threadIdx.x & 63, lseg, reqexp, eb, su, recb, isconst, med);
-            stage_slab<T, BS, LV, SLABS>(su, u, lds.meta, p.payload + lds.excl[0]);
+            stage_slab<T, BS, LV, SLABS, false>(su, u, lds.meta, p.payload + lds.excl[0]);
         }
     }
     copy_out<T, BS, LV, SLABS, IPT>(p, tile, lds.meta, lds.stage, lds.excl[0], lds.excl[1], staged);
