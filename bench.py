@@ -547,6 +547,15 @@ def main():
     kd_ms = kd["decode"][0] / max(1, kd["decode"][1])
     assert dec.check() == 0, "decoder rejected the stream"
 
+    # measured copy bandwidth of this GPU (device-to-device copy of the input:
+    # N*W bytes read + N*W written), beside the 8 TB/s peak
+    ycopy = torch.empty_like(x)
+    ycopy.copy_(x)
+    ms_copy = timed(lambda: ycopy.copy_(x), 10)
+    copy_gbs = 2 * nbytes / (ms_copy * 1e-3) / 1e9
+    del ycopy
+    log(f"device copy {copy_gbs:.0f} GB/s (read + write)")
+
     if args.calib_copy:
         log("calibration: 3 plain elementwise copies of the input")
         y = torch.empty_like(x)
@@ -657,6 +666,8 @@ def main():
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": measured_traffic(shape, args.rel),
             "algorithmic_bytes": alg_c, "kernel_ms": k_ms}
     roof["frac"] = roof["achieved"] / roof["peak"]
+    roof["measured_copy_gbs"] = copy_gbs
+    roof["frac_of_copy"] = roof["achieved"] / copy_gbs
     line = {
         "metric": METRIC,
         "value": nbytes / (ms_rel * 1e-3) / 1e9,
@@ -683,6 +694,7 @@ def main():
         "decompress_pct_hbm": (nbytes + csize) / (ms_dec * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "decode_kernel_ms": kd_ms,
         "decode_kernel_pct_hbm": (nbytes + csize) / (kd_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "decode_kernel_frac_of_copy": (nbytes + csize) / (kd_ms * 1e-3) / 1e9 / copy_gbs,
         "decompress_noindex_gbs": (nbytes / (ms_dec_noidx * 1e-3) / 1e9) if ms_dec_noidx else None,
         "index_build_ms": ix_ms,
         "compressed_bytes": csize,
