@@ -71,6 +71,12 @@ def lib():
             f = getattr(L, nm)
             f.restype = C.c_float
             f.argtypes = [vp, _sz, C.c_float]
+        L.szxo_compress_ra_bound.restype = _sz
+        L.szxo_compress_ra_bound.argtypes = [_sz, C.c_int, C.c_int]
+        L.szxo_compress_ra.restype = _sz
+        L.szxo_compress_ra.argtypes = [vp, _sz, C.c_float, C.c_int, C.c_int, C.c_int, vp]
+        L.szxo_decompress_ra.restype = C.c_int
+        L.szxo_decompress_ra.argtypes = [vp, _sz, _sz, C.c_int, C.c_int, vp, C.c_int]
         L.szxo_fxr_error_bound.restype = C.c_float
         L.szxo_fxr_error_bound.argtypes = [vp, C.c_int, _sz, C.c_float, C.c_float, C.c_int,
                                            C.POINTER(C.c_int)]
@@ -122,6 +128,28 @@ def decompress(stream: bytes, n: int, dtype, ref_quirk: bool = True) -> np.ndarr
         rc = lib().szxo_decompress_f32(_ptr(s), s.size, n, _ptr(out))
     else:
         rc = lib().szxo_decompress_f64(_ptr(s), s.size, n, _ptr(out), int(ref_quirk))
+    if rc != 0:
+        raise ValueError(f"malformed stream (oracle rc={rc})")
+    return out
+
+
+def compress_ra(x: np.ndarray, eb: float, bs: int = 128, fast_mode: int = 3) -> bytes:
+    """Random-access container (fastMode 3: big-endian medians; fastMode 4:
+    native medians and, for fp64, float-radius block statistics)."""
+    x = np.ascontiguousarray(x).ravel()
+    w = _width(x)
+    out = np.empty(lib().szxo_compress_ra_bound(x.size, bs, w), np.uint8)
+    n = lib().szxo_compress_ra(_ptr(x), x.size, float(np.float32(eb)), bs, w, fast_mode, _ptr(out))
+    return out[:n].tobytes()
+
+
+def decompress_ra(stream: bytes, n: int, dtype, fast_mode: int = 3,
+                  ref_quirk: bool = True) -> np.ndarray:
+    dtype = np.dtype(dtype)
+    s = np.frombuffer(stream, np.uint8)
+    out = np.empty(n, dtype)
+    rc = lib().szxo_decompress_ra(_ptr(s), s.size, n, dtype.itemsize, fast_mode, _ptr(out),
+                                  int(ref_quirk))
     if rc != 0:
         raise ValueError(f"malformed stream (oracle rc={rc})")
     return out

@@ -390,6 +390,134 @@ long szxo_record_offsets(const uint8_t *s, size_t len, size_t n, int w, uint64_t
     return k;
 }
 
+/* ---- random-access container (fastMode 3 and 4) ------------------------ */
+
+size_t szxo_compress_ra_bound(size_t n, int bs, int w)
+{
+    size_t nb = ceil_div(n, (size_t)bs);
+    return 20 + ceil_div(nb, 8) + nb * (2 + 4 + 5 + ceil_div((size_t)bs, 4) + (size_t)bs * (size_t)w);
+}
+
+/* fastMode-4 fp64 block statistics (szx_double.cc:1120-1135): the radius is
+ * rounded to float first; the median and the constant test use that float. */
+static size_t block_stats_f64_fm4(const double *x, size_t n, float eb, int bs,
+                                  uint8_t *state, float *median, float *radius)
+{
+    size_t nb = ceil_div(n, (size_t)bs), nconst = 0;
+    for (size_t b = 0; b < nb; b++) {
+        const double *v = x + b * (size_t)bs;
+        size_t m = (b + 1) * (size_t)bs <= n ? (size_t)bs : n - b * (size_t)bs;
+        double lo = v[0], hi = v[0];
+        for (size_t j = 1; j < m; j++) {
+            if (lo > v[j]) lo = v[j];
+            else if (hi < v[j]) hi = v[j];
+        }
+        float rad = (float)((hi - lo) / 2);
+        float med = (float)(lo + (double)rad);
+        state[b] = rad <= eb ? 0 : 1;
+        nconst += state[b] == 0;
+        median[b] = med;
+        radius[b] = rad;
+    }
+    return nconst;
+}
+
+size_t szxo_compress_ra(const void *x, size_t n, float eb, int bs, int w, int fast_mode,
+                        uint8_t *out)
+{
+    size_t nb = ceil_div(n, (size_t)bs);
+    size_t nbit = ceil_div(nb, 8);
+    uint8_t *state = (uint8_t *)malloc(nb ? nb : 1);
+    float *med = (float *)malloc((nb ? nb : 1) * sizeof(float));
+    float *rad = (float *)malloc((nb ? nb : 1) * sizeof(float));
+    size_t nconst;
+    if (w == 4) nconst = szxo_block_stats_f32((const float *)x, n, eb, bs, state, med, rad);
+    else if (fast_mode == 4) nconst = block_stats_f64_fm4((const double *)x, n, eb, bs, state, med, rad);
+    else nconst = szxo_block_stats_f64((const double *)x, n, eb, bs, state, med, rad);
+    size_t nrec = nb - nconst;
+    out[0] = 1; out[1] = 1; out[2] = 1; out[3] = 1;   /* szx_float.cc:1438-1441 */
+    put_be_u64(out + 4, (uint64_t)bs);                  /* sizeToBytes, :1444-1447 */
+    put_be_u64(out + 12, nconst);
+    uint8_t *osz = out + 20;                            /* native u16 sizes */
+    uint8_t *bitmap = osz + 2 * nrec;
+    memset(bitmap, 0, nbit);
+    uint8_t *cmed = bitmap + nbit;
+    uint8_t *rec = cmed + 4 * nconst;
+    size_t k = 0, r = 0;
+    for (size_t b = 0; b < nb; b++) {
+        size_t m = (b + 1) * (size_t)bs <= n ? (size_t)bs : n - b * (size_t)bs;
+        if (state[b]) {
+            bitmap[b >> 3] |= (uint8_t)(0x80 >> (b & 7));
+            size_t sz = w == 4
+                ? szxo_encode_block_f32((const float *)x + b * (size_t)bs, m, eb, med[b], rad[b], rec)
+                : szxo_encode_block_f64((const double *)x + b * (size_t)bs, m, eb, med[b], rad[b], rec);
+            osz[2 * r] = (uint8_t)sz;                   /* little-endian host */
+            osz[2 * r + 1] = (uint8_t)(sz >> 8);
+            r++;
+            rec += sz;
+        } else if (fast_mode == 4) {
+            memcpy(cmed + 4 * k++, &med[b], 4);         /* native f32, szx_float.cc:1375 */
+        } else {
+            put_be_f32(cmed + 4 * k++, med[b]);         /* floatToBytes, :1466 */
+        }
+    }
+    free(state); free(med); free(rad);
+    return (size_t)(rec - out);
+}
+
+int szxo_decompress_ra(const uint8_t *s, size_t len, size_t n, int w, int fast_mode, void *out,
+                       int ref_quirk)
+{
+    if (len < 20) return -1;
+    size_t bs = (size_t)get_be_u64(s + 4);
+    if (bs < 1 || bs > 256) return -2;
+    size_t nb = ceil_div(n, bs), nbit = ceil_div(nb, 8);
+    size_t nconst = (size_t)get_be_u64(s + 12);
+    if (nconst > nb) return -2;
+    size_t nrec = nb - nconst;
+    const uint8_t *osz = s + 20;
+    const uint8_t *bitmap = osz + 2 * nrec;
+    const uint8_t *cmed = bitmap + nbit;
+    const uint8_t *q = cmed + 4 * nconst, *end = s + len;
+    if (q > end) return -3;
+    uint64_t t[256];
+    size_t k = 0, r = 0;
+    for (size_t b = 0; b < nb; b++) {
+        size_t m = (b + 1) * bs <= n ? bs : n - b * bs;
+        if (!((bitmap[b >> 3] >> (7 - (b & 7))) & 1)) {
+            if (k >= nconst) return -4;
+            float med;
+            if (fast_mode == 4) memcpy(&med, cmed + 4 * k, 4);
+            else med = get_be_f32(cmed + 4 * k);
+            k++;
+            for (size_t j = 0; j < m; j++) {
+                if (w == 4) ((float *)out)[b * bs + j] = med;
+                else ((double *)out)[b * bs + j] = (double)med;
+            }
+            continue;
+        }
+        if (r >= nrec) return -4;
+        size_t sz = (size_t)osz[2 * r] | ((size_t)osz[2 * r + 1] << 8);
+        r++;
+        if (q + 5 + ceil_div(m, 4) > end || q + sz > end) return -5;
+        int rlen = q[0];
+        if (rlen < 1 || rlen > 8 * w) return -6;
+        int shift = (8 - rlen % 8) % 8;
+        float med = get_be_f32(q + 1);
+        if (!decode_words(q, m, w, t, ref_quirk, (size_t)(end - q))) return -7;
+        for (size_t j = 0; j < m; j++) {
+            if (w == 4) {
+                ((float *)out)[b * bs + j] = bits_f32((uint32_t)(t[j] << shift)) + med;
+            } else {
+                ((double *)out)[b * bs + j] = bits_f64(t[j] << shift) + (double)med;
+            }
+        }
+        q += sz;   /* the reference advances by the stored size (szxd_float.cc:444) */
+    }
+    if (k != nconst || r != nrec) return -8;
+    return 0;
+}
+
 /* ---- value range and REL bound (A1, A2) --------------------------------- */
 
 float szxo_value_range_f32(const float *x, size_t n)

@@ -76,6 +76,30 @@ int szxo_decompress_f64(const uint8_t *s, size_t len, size_t n, double *out,
 long szxo_record_offsets(const uint8_t *s, size_t len, size_t n, int w,
                          uint64_t *offs);
 
+/* Random-access container (fastMode 3 and 4): [1,1,1,1], big-endian u64
+ * blockSize, big-endian u64 constant-block count, a native-endian u16 size
+ * per non-constant record, the state bitmap, the constant medians
+ * (big-endian f32 for fastMode 3, native f32 for fastMode 4), then the same
+ * records as fastMode 2.
+ *   fastMode 3: szx_float.cc:1407-1489, szx_double.cc:1510-1590 (block
+ *               statistics as fastMode 2).
+ *   fastMode 4: szx_float.cc:1218-1405, szx_double.cc:1320-1507; its block
+ *               statistics (computeStateMedianRadius_*2, szx_float.cc:461-476,
+ *               szx_double.cc:1120-1135) differ for fp64: the radius is rounded
+ *               to float before the constant test (float compare) and before
+ *               the median (median = (float)(min + (double)radius_f32)).
+ *               Min/max equal the sequential ones for NaN-free data.
+ * out must hold szxo_compress_ra_bound() bytes.  Returns the stream size. */
+size_t szxo_compress_ra_bound(size_t n, int bs, int w);
+size_t szxo_compress_ra(const void *x, size_t n, float eb, int bs, int w, int fast_mode,
+                        uint8_t *out);
+
+/* Decode a random-access stream (szxd_float.cc:313-476,
+ * szxd_double.cc:518-681): records are located through the size table.
+ * Returns 0 or <0 on a malformed stream.  ref_quirk as for fastMode 2. */
+int szxo_decompress_ra(const uint8_t *s, size_t len, size_t n, int w, int fast_mode, void *out,
+                       int ref_quirk);
+
 /* Value range and REL -> ABS error bound (szx.cc:249-253, 303-308;
  * szx_float.cc:478-499, szx_double.cc:1137-1158). */
 float szxo_value_range_f32(const float *x, size_t n);

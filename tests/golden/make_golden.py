@@ -8,7 +8,8 @@ ABS and REL bounds, every reqBytes branch the data reaches, constant blocks,
 remainder blocks and block sizes 64/128/256 (256 is compressed only: the
 reference decoder reads its block-size byte 0 as 0 and divides by it).
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py        # fastMode-2 vectors
+    python tests/golden/make_golden.py ra     # random-access containers (ra*.npz)
 """
 import os
 import sys
@@ -54,9 +55,55 @@ def cases():
         yield f"f64_walk_eb1e{ebexp}", b64, dict(mode="abs", eb=eb, bs=128)
 
 
+def ra_cases():
+    """Random-access containers (fastMode 3: big-endian medians; fastMode 4:
+    native medians, fp64 float-radius statistics) through
+    SZx_fast_compress_args (blockSize 128)."""
+    x = sinusoid_field((32, 32, 33), seed=2, n_waves=64, kmax=32, slope=2.5, noise=1e-4,
+                       n_boxes=2, box_frac=0.30, box_value="mean")
+    g = grf_field((96, 100), seed=3)
+    for fm in (3, 4):
+        yield f"ra{fm}_f32_sin_rel1e-3", x, dict(mode=1, rel=1e-3, fm=fm)
+        yield f"ra{fm}_f32_sin_abs1e-5", x, dict(mode=0, eb=1e-5, fm=fm)
+        yield f"ra{fm}_f64_grf_rel1e-6", g, dict(mode=1, rel=1e-6, fm=fm)
+    # fp64 blocks whose radius exceeds the bound in double but not once rounded
+    # to float: constant under fastMode 4 only (szx_double.cc:1120-1135)
+    r = np.random.default_rng(6)
+    flip = np.repeat(r.uniform(-4, 4, 40), 128)
+    flip[::128] += 2.0 + 2.0 ** -30        # radius 1 + 2^-31 against eb 1
+    flip[64::128] += r.uniform(0, 1e-3, 40)
+    for fm in (3, 4):
+        yield f"ra{fm}_f64_radius_flip", flip, dict(mode=0, eb=1.0, fm=fm)
+
+
+def main_ra():
+    manifest = []
+    for name, x, prm in ra_cases():
+        x = np.ascontiguousarray(x).ravel()
+        eb = prm["eb"] if prm["mode"] == 0 else O.rel_eb(x, prm["rel"])
+        if len(O.compress_ra(x, eb, 128, prm["fm"])) > x.size * x.itemsize:
+            print(f"{name}: skipped (stream would exceed the reference's N*W buffer)")
+            continue
+        stream = O.ref_compress_args(x, prm["mode"], abs_eb=prm.get("eb", 0.0), rel=prm.get("rel", 0.0),
+                                     fast_mode=prm["fm"])
+        dec = O.ref_decompress(stream, x.size, x.dtype, fast_mode=prm["fm"])
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), x=x,
+                            stream=np.frombuffer(stream, np.uint8), decoded=dec,
+                            eb=np.float32(eb), bs=np.int32(128), fast_mode=np.int32(prm["fm"]),
+                            rel=np.float32(prm.get("rel", 0.0)))
+        manifest.append(f"{name}\tn={x.size}\t{x.dtype}\tfastMode={prm['fm']}\teb={eb!r}\tbytes={len(stream)}")
+        print(manifest[-1])
+    with open(os.path.join(HERE, "MANIFEST_RA.txt"), "w") as f:
+        f.write("# produced by make_golden.py ra from the compiled reference codec\n")
+        f.write("\n".join(manifest) + "\n")
+
+
 def main():
     if not O.ref_available():
         raise SystemExit("oracle/_ref/libszx_ref.so is missing: make -C oracle ref")
+    if sys.argv[1:] == ["ra"]:
+        main_ra()
+        return
     manifest = []
     for name, x, prm in cases():
         x = np.ascontiguousarray(x).ravel()

@@ -16,7 +16,8 @@ import szx_amd as Z
 
 pytestmark = pytest.mark.gpu
 
-GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz")))
+GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "f*.npz")))   # fastMode 2
+GOLDEN_RA = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "ra*.npz")))  # fastMode 3/4
 POW2 = (4, 8, 16, 32, 64, 128, 256)
 # block sizes outside the tiled kernels (per-block kernels)
 GENERIC = (1, 2, 3, 5, 7, 12, 33, 100, 200, 255)

@@ -18,7 +18,8 @@ import pytest
 
 import oracle as O
 
-GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz")))
+GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "f*.npz")))   # fastMode 2
+GOLDEN_RA = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "ra*.npz")))  # fastMode 3/4
 needs_ref = pytest.mark.skipif(not O.ref_available(), reason="compiled reference not built")
 
 
@@ -49,6 +50,60 @@ def test_rel_bound_matches_golden(path):
     # the bound the reference derived from the range.
     g = load_golden(path)
     assert np.float32(O.rel_eb(g["x"], float(g["rel"]))) == g["eb"]
+
+
+@pytest.mark.parametrize("path", GOLDEN_RA, ids=[os.path.basename(p)[:-4] for p in GOLDEN_RA])
+def test_oracle_reproduces_golden_random_access(path):
+    g = load_golden(path)
+    x, eb, fm = g["x"], float(g["eb"]), int(g["fast_mode"])
+    stream = O.compress_ra(x, eb, 128, fm)
+    assert stream == g["stream"].tobytes()
+    dec = O.decompress_ra(stream, x.size, x.dtype, fm, ref_quirk=True)
+    assert np.array_equal(dec.view(np.uint8), g["decoded"].view(np.uint8))
+
+
+def test_random_access_fastmode4_fp64_statistics_differ():
+    # a radius of 1 + 2^-31 against eb = 1: non-constant in double (fastMode
+    # 3), constant once rounded to float (fastMode 4)
+    assert len(GOLDEN_RA) >= 8
+    g3 = load_golden([p for p in GOLDEN_RA if "ra3_f64_radius_flip" in p][0])
+    g4 = load_golden([p for p in GOLDEN_RA if "ra4_f64_radius_flip" in p][0])
+    s3, s4 = g3["stream"].tobytes(), g4["stream"].tobytes()
+    assert int.from_bytes(s4[12:20], "big") > int.from_bytes(s3[12:20], "big")
+    # and fastMode 3 is fastMode 2's statistics: same records, same medians
+    x = g3["x"]
+    s2 = O.compress(x, 1.0, 128)
+    assert int.from_bytes(s2[5:13], "big") == int.from_bytes(s3[12:20], "big")
+
+
+@needs_ref
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_random_access_vs_reference_random(dtype):
+    r = np.random.default_rng(21 if dtype == np.float32 else 22)
+    compared = 0
+    for trial in range(24):
+        n = int(r.integers(1, 60000))
+        t = np.arange(n)
+        x = (np.sin(t * r.uniform(1e-4, 0.05)) * 10 ** r.uniform(-3, 3)
+             + r.standard_normal(n) * 10 ** r.uniform(-8, -2)).astype(dtype)
+        if trial % 3 == 0:
+            x[: n // 3] = 0.5
+        fm = 3 + trial % 2
+        if trial % 4 < 2:
+            mode, kw = 0, dict(abs_eb=float(np.float32(np.ptp(x) * 10 ** r.uniform(-7, -2)) + 1e-30))
+            eb = kw["abs_eb"]
+        else:
+            mode, kw = 1, dict(rel=10 ** r.uniform(-6, -2))
+            eb = O.rel_eb(x, kw["rel"])
+        a = O.compress_ra(x, eb, 128, fm)
+        if len(a) > x.size * x.itemsize:
+            continue   # the reference would overflow its N*W output buffer
+        compared += 1
+        assert a == O.ref_compress_args(x, mode, fast_mode=fm, **kw), (trial, n, fm)
+        ref_dec = O.ref_decompress(a, n, dtype, fast_mode=fm)
+        dec = O.decompress_ra(a, n, dtype, fm, ref_quirk=True)
+        assert np.array_equal(dec.view(np.uint8), ref_dec.view(np.uint8)), trial
+    assert compared > 16
 
 
 # ---- function-level pins against the compiled reference --------------------
