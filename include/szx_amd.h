@@ -56,7 +56,11 @@ extern "C" {
 size_t SZx_computeDataLength(size_t r5, size_t r4, size_t r3, size_t r2, size_t r1);
 
 /* szx.h:77-78 / szx.cc:231-348.  Compress a host array.  blockSize is 128,
- * as in the reference.  errBoundMode ABS uses absErrBound, REL uses
+ * as in the reference.  fastMode 2 writes the blocked stream; fastMode 3 and
+ * 4 the random-access container (szx_float.cc:1407-1489 and 1218-1405: a u16
+ * size per record before the bitmap; 3 stores big-endian medians, 4 native
+ * ones and rounds fp64 block radii to float before the constant test, as the
+ * reference's OpenMP compressor does).  errBoundMode ABS uses absErrBound, REL uses
  * relBoundRatio * (max - min) computed with the reference's float rounding,
  * FXR (fixed rate, any fastMode as in szx.cc:238-243) picks the bound the
  * reference's fixed-rate compressor picks for `compressionRatio` within
@@ -68,7 +72,8 @@ unsigned char *SZx_fast_compress_args(int fastMode, int dataType, void *data, si
                                       float compressionRatio, float tolerance, size_t r5,
                                       size_t r4, size_t r3, size_t r2, size_t r1);
 
-/* szx.h:80 / szx.cc:378-430.  Decompress a fastMode-2 stream into a malloc'd
+/* szx.h:80 / szx.cc:378-430.  Decompress a fastMode-2 stream, or a fastMode
+ * 3 / 4 random-access container (szxd_float.cc:313-476), into a malloc'd
  * host array of SZx_computeDataLength(r5..r1) values.  byteLength bounds
  * every read (the reference ignores it).  NULL on a malformed stream. */
 void *SZx_fast_decompress(int fastMode, int dataType, unsigned char *bytes, size_t byteLength,
@@ -179,6 +184,25 @@ int szx_amd_decompress_dev(const void *d_stream, size_t streamBytes, size_t n, i
                            int blockSize, void *d_out, void *d_work, size_t workBytes,
                            const uint64_t *d_index, int flags, int32_t *d_status,
                            void *hipStream);
+
+/* szx.cc:254-268 (the fastMode 3 / 4 branches of SZx_fast_compress_args) on
+ * device memory: the random-access container
+ * (SZx_fast_compress_args_unpredictable_blocked_randomaccess_float[_openmp],
+ * szx_float.cc:1407-1489 / 1218-1405, and the double versions).  fastMode 3
+ * or 4; blockSize a power of two 4..256; buffers, workspace, index and
+ * results as szx_amd_compress_dev. */
+int szx_amd_compress_ra_dev(int fastMode, const void *d_in, size_t n, int dataType, int errBoundMode,
+                            float absErrBound, float relBoundRatio, int blockSize, void *d_out,
+                            size_t outCapacity, void *d_work, size_t workBytes, uint64_t *d_result,
+                            uint64_t *d_index, void *hipStream);
+
+/* szx.cc:391-403 (SZx_fast_decompress, fastMode 3 / 4) on device memory:
+ * decode a random-access container (szxd_float.cc:313-476,
+ * szxd_double.cc:518-681).  Without d_index the record index comes from the
+ * container's size table.  Arguments as szx_amd_decompress_dev. */
+int szx_amd_decompress_ra_dev(int fastMode, const void *d_stream, size_t streamBytes, size_t n,
+                              int dataType, int blockSize, void *d_out, void *d_work, size_t workBytes,
+                              const uint64_t *d_index, int flags, int32_t *d_status, void *hipStream);
 
 /* Workspace bytes for szx_amd_fxr_bound_dev (0 for unsupported arguments). */
 size_t szx_amd_fxr_workspace(size_t n, int dataType, int blockSize);

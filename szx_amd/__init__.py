@@ -39,7 +39,8 @@ EXPORTS = (
     "SZx_estimateErrorBoundbasedonCR", "SZ_fast_compress_args", "SZ_fast_decompress",
     "szx_amd_last_error", "szx_amd_compress_bound", "szx_amd_compress_capacity",
     "szx_amd_compress_workspace", "szx_amd_compress_dev", "szx_amd_decompress_workspace",
-    "szx_amd_decompress_dev", "szx_amd_index_words", "szx_amd_timing_begin",
+    "szx_amd_decompress_dev", "szx_amd_compress_ra_dev", "szx_amd_decompress_ra_dev",
+    "szx_amd_index_words", "szx_amd_timing_begin",
     "szx_amd_timing_end", "szx_amd_fxr_workspace", "szx_amd_fxr_bound_dev",
     "szx_amd_fixed_rate_workspace", "szx_amd_fixed_rate_dev", "szx_amd_rate_stats_bytes",
     "szx_amd_rate_stats_workspace", "szx_amd_rate_stats_dev", "szx_amd_rate_inputs_dev",
@@ -123,6 +124,10 @@ def lib() -> C.CDLL:
     L.szx_amd_decompress_workspace.argtypes = [sz, i32, i32, sz]
     L.szx_amd_decompress_dev.restype = i32
     L.szx_amd_decompress_dev.argtypes = [vp, sz, sz, i32, i32, vp, vp, sz, vp, i32, vp, vp]
+    L.szx_amd_compress_ra_dev.restype = i32
+    L.szx_amd_compress_ra_dev.argtypes = [i32] + list(L.szx_amd_compress_dev.argtypes)
+    L.szx_amd_decompress_ra_dev.restype = i32
+    L.szx_amd_decompress_ra_dev.argtypes = [i32] + list(L.szx_amd_decompress_dev.argtypes)
     L.szx_amd_timing_begin.restype = None
     L.szx_amd_timing_begin.argtypes = []
     L.szx_amd_timing_end.restype = i32
@@ -319,11 +324,13 @@ def _stream_handle(stream):
 
 class DeviceCompressor:
     """Holds the output buffer and workspace for repeated compresses of one
-    shape/dtype/blockSize, so the timed path allocates nothing."""
+    shape/dtype/blockSize, so the timed path allocates nothing.  fastMode 2
+    writes the blocked stream, 3 / 4 the random-access container."""
 
-    def __init__(self, n: int, dtype, blockSize: int = 128, device="cuda"):
+    def __init__(self, n: int, dtype, blockSize: int = 128, device="cuda",
+                 fastMode: int = SZx_WITH_BLOCK_FAST_CMPR):
         torch = _torch()
-        self.n, self.dtype, self.bs = n, np.dtype(dtype), blockSize
+        self.n, self.dtype, self.bs, self.fast_mode = n, np.dtype(dtype), blockSize, fastMode
         code = _dtype_code(self.dtype)
         L = lib()
         self.capacity = L.szx_amd_compress_capacity(n, code, blockSize)
@@ -339,11 +346,13 @@ class DeviceCompressor:
     def __call__(self, x, errBoundMode: int = ABS, absErrBound: float = 0.0,
                  relBoundRatio: float = 0.0, stream=None) -> None:
         """Enqueue one compress of tensor x (asynchronous)."""
-        rc = lib().szx_amd_compress_dev(x.data_ptr(), self.n, _dtype_code(self.dtype),
-                                        errBoundMode, absErrBound, relBoundRatio, self.bs,
-                                        self.out.data_ptr(), self.capacity, self.ws.data_ptr(),
-                                        self.wsbytes, self.result.data_ptr(),
-                                        self.index.data_ptr(), _stream_handle(stream))
+        args = (x.data_ptr(), self.n, _dtype_code(self.dtype), errBoundMode, absErrBound,
+                relBoundRatio, self.bs, self.out.data_ptr(), self.capacity, self.ws.data_ptr(),
+                self.wsbytes, self.result.data_ptr(), self.index.data_ptr(), _stream_handle(stream))
+        if self.fast_mode == SZx_WITH_BLOCK_FAST_CMPR:
+            rc = lib().szx_amd_compress_dev(*args)
+        else:
+            rc = lib().szx_amd_compress_ra_dev(self.fast_mode, *args)
         if rc != 0:
             raise SZxError(last_error())
 
@@ -357,9 +366,10 @@ class DeviceCompressor:
 
 
 class DeviceDecompressor:
-    def __init__(self, n: int, dtype, blockSize: int, stream_capacity: int, device="cuda"):
+    def __init__(self, n: int, dtype, blockSize: int, stream_capacity: int, device="cuda",
+                 fastMode: int = SZx_WITH_BLOCK_FAST_CMPR):
         torch = _torch()
-        self.n, self.dtype, self.bs = n, np.dtype(dtype), blockSize
+        self.n, self.dtype, self.bs, self.fast_mode = n, np.dtype(dtype), blockSize, fastMode
         code = _dtype_code(self.dtype)
         self.wsbytes = lib().szx_amd_decompress_workspace(n, code, blockSize, stream_capacity)
         self.ws = torch.empty(self.wsbytes, dtype=torch.uint8, device=device)
@@ -372,11 +382,14 @@ class DeviceDecompressor:
         """Enqueue one decode; `index` is the compressor's record index
         (DeviceCompressor.index) or None to rebuild it from the stream."""
         nb = stream_tensor.numel() if nbytes is None else nbytes
-        rc = lib().szx_amd_decompress_dev(stream_tensor.data_ptr(), nb, self.n,
-                                          _dtype_code(self.dtype), self.bs, self.out.data_ptr(),
-                                          self.ws.data_ptr(), self.wsbytes,
-                                          None if index is None else index.data_ptr(), flags,
-                                          self.status.data_ptr(), _stream_handle(stream))
+        args = (stream_tensor.data_ptr(), nb, self.n, _dtype_code(self.dtype), self.bs,
+                self.out.data_ptr(), self.ws.data_ptr(), self.wsbytes,
+                None if index is None else index.data_ptr(), flags, self.status.data_ptr(),
+                _stream_handle(stream))
+        if self.fast_mode == SZx_WITH_BLOCK_FAST_CMPR:
+            rc = lib().szx_amd_decompress_dev(*args)
+        else:
+            rc = lib().szx_amd_decompress_ra_dev(self.fast_mode, *args)
         if rc != 0:
             raise SZxError(last_error())
 

@@ -180,7 +180,8 @@ struct FxrReq {
 // Host-in / host-out compress through the device path.  With fxr set the
 // bound is the reference's fixed-rate choice for (target ratio, tolerance).
 unsigned char *host_compress(const void *data, size_t n, int w, int rel_mode, float abs_eb,
-                             float rel, int bs, size_t *outSize, const FxrReq *fxr = nullptr)
+                             float rel, int bs, size_t *outSize, const FxrReq *fxr = nullptr,
+                             int fast_mode = SZx_WITH_BLOCK_FAST_CMPR)
 {
     if (outSize) *outSize = 0;
     if (!data || n == 0) {
@@ -208,7 +209,7 @@ unsigned char *host_compress(const void *data, size_t n, int w, int rel_mode, fl
         rel_mode = 0;
     }
     if (compress_launch(c.in.p, n, w, rel_mode, abs_eb, rel, bs, (uint8_t *)c.out.p,
-                        (uint8_t *)c.ws.p, (uint64_t *)c.res.p, nullptr, c.stream)) {
+                        (uint8_t *)c.ws.p, (uint64_t *)c.res.p, nullptr, c.stream, fast_mode)) {
         set_err("compress launch failed");
         return nullptr;
     }
@@ -235,19 +236,26 @@ unsigned char *host_compress(const void *data, size_t n, int w, int rel_mode, fl
     return out;
 }
 
-void *host_decompress(const unsigned char *bytes, size_t len, size_t n, int w, int exact)
+void *host_decompress(const unsigned char *bytes, size_t len, size_t n, int w, int exact,
+                      int fast_mode = SZx_WITH_BLOCK_FAST_CMPR)
 {
     if (!bytes || n == 0) {
         set_err("empty input");
         return nullptr;
     }
-    if (len < 13) {
+    const bool ra = fast_mode != SZx_WITH_BLOCK_FAST_CMPR;
+    if (len < (ra ? 20u : 13u)) {
         set_err("stream shorter than its header");
         return nullptr;
     }
-    const int bs = bytes[4] ? bytes[4] : 256;
-    if (!bs_supported(bs)) {
-        set_err("blockSize %d is not supported (1..256)", bs);
+    int bs = bytes[4] ? bytes[4] : 256;
+    if (ra) {   // big-endian u64 blockSize (szxd_float.cc:419)
+        uint64_t v = 0;
+        for (int k = 0; k < 8; k++) v = (v << 8) | bytes[4 + k];
+        bs = v >= 4 && v <= 256 && (v & (v - 1)) == 0 ? (int)v : 0;
+    }
+    if (!bs_supported(bs) || (ra && !bs_tiled(bs))) {
+        set_err("blockSize %d is not supported", bs);
         return nullptr;
     }
     HostCtx &c = g_ctx;
@@ -259,7 +267,7 @@ void *host_decompress(const unsigned char *bytes, size_t len, size_t n, int w, i
     if (!hip_ok(hipMemcpyAsync(c.in.p, bytes, len, hipMemcpyHostToDevice, c.stream), "H2D"))
         return nullptr;
     if (decompress_launch((const uint8_t *)c.in.p, len, n, w, bs, c.out.p, (uint8_t *)c.ws.p,
-                          L.ws_bytes, nullptr, exact, (int32_t *)c.res.p, c.stream)) {
+                          L.ws_bytes, nullptr, exact, (int32_t *)c.res.p, c.stream, fast_mode)) {
         set_err("decompress launch failed");
         return nullptr;
     }
@@ -372,15 +380,19 @@ unsigned char *SZx_fast_compress_args(int fastMode, int dataType, void *data, si
         const FxrReq req{compressionRatio, tolerance};
         return host_compress(data, n, w, 0, 0.0f, 0.0f, 128, outSize, &req);
     }
-    if (fastMode != SZx_WITH_BLOCK_FAST_CMPR) {
-        set_err("fastMode %d is not provided (fastMode 2 is)", fastMode);
+    // szx.cc:245-268: fastMode 2 is the blocked stream, 3 and 4 the
+    // random-access container (3: serial layout, 4: the OpenMP one)
+    if (fastMode != SZx_WITH_BLOCK_FAST_CMPR && fastMode != SZx_RANDOMACCESS_FAST_CMPR &&
+        fastMode != SZx_OPENMP_FAST_CMPR) {
+        set_err("fastMode %d is not provided (fastMode 2, 3 and 4 are)", fastMode);
         return nullptr;
     }
     if (errBoundMode != ABS && errBoundMode != REL) {
         set_err("errBoundMode %d is not provided", errBoundMode);
         return nullptr;
     }
-    return host_compress(data, n, w, errBoundMode == REL, absErrBound, relBoundRatio, 128, outSize);
+    return host_compress(data, n, w, errBoundMode == REL, absErrBound, relBoundRatio, 128, outSize,
+                         nullptr, fastMode);
 }
 
 unsigned char *SZ_fast_compress_args(int fastMode, int dataType, void *data, size_t *outSize,
@@ -401,11 +413,13 @@ void *SZx_fast_decompress(int fastMode, int dataType, unsigned char *bytes, size
         set_err("unsupported dataType %d", dataType);
         return nullptr;
     }
-    if (fastMode != SZx_WITH_BLOCK_FAST_CMPR) {
-        set_err("fastMode %d is not provided (fastMode 2 is)", fastMode);
+    // szx.cc:391-403: fastMode 3 and 4 read the random-access container
+    if (fastMode != SZx_WITH_BLOCK_FAST_CMPR && fastMode != SZx_RANDOMACCESS_FAST_CMPR &&
+        fastMode != SZx_OPENMP_FAST_CMPR) {
+        set_err("fastMode %d is not provided (fastMode 2, 3 and 4 are)", fastMode);
         return nullptr;
     }
-    return host_decompress(bytes, byteLength, data_length(r5, r4, r3, r2, r1), w, 0);
+    return host_decompress(bytes, byteLength, data_length(r5, r4, r3, r2, r1), w, 0, fastMode);
 }
 
 void *SZ_fast_decompress(int fastMode, int dataType, unsigned char *bytes, size_t byteLength,
@@ -584,15 +598,19 @@ size_t szx_amd_index_words(size_t n, int dataType, int blockSize)
     return (w && bs_supported(blockSize)) ? index_words(n, w, blockSize) : 0;
 }
 
-int szx_amd_compress_dev(const void *d_in, size_t n, int dataType, int errBoundMode,
-                         float absErrBound, float relBoundRatio, int blockSize, void *d_out,
-                         size_t outCapacity, void *d_work, size_t workBytes, uint64_t *d_result,
-                         uint64_t *d_index, void *hipStream)
+static int compress_dev(int fastMode, const void *d_in, size_t n, int dataType, int errBoundMode,
+                        float absErrBound, float relBoundRatio, int blockSize, void *d_out,
+                        size_t outCapacity, void *d_work, size_t workBytes, uint64_t *d_result,
+                        uint64_t *d_index, void *hipStream)
 {
     g_err.clear();
     const int w = width_of(dataType);
     if (!w) { set_err("unsupported dataType %d", dataType); return SZx_TERR; }
     if (!bs_supported(blockSize)) { set_err("unsupported blockSize %d", blockSize); return SZx_NSCS; }
+    if (fastMode != SZx_WITH_BLOCK_FAST_CMPR && !bs_tiled(blockSize)) {
+        set_err("the random-access container takes blockSize 4..256, powers of two");
+        return SZx_NSCS;
+    }
     if (errBoundMode != ABS && errBoundMode != REL) { set_err("bad errBoundMode"); return SZx_BERR; }
     if (!d_in || !d_out || !d_work || !d_result || n == 0) { set_err("NULL argument or n == 0"); return SZx_NSCS; }
     if (((uintptr_t)d_in & 15) != 0) { set_err("d_in must be 16-byte aligned"); return SZx_NSCS; }
@@ -601,9 +619,33 @@ int szx_amd_compress_dev(const void *d_in, size_t n, int dataType, int errBoundM
     if (workBytes < L.ws_bytes) { set_err("workBytes < %llu", (unsigned long long)L.ws_bytes); return SZx_NSCS; }
     if (compress_launch(d_in, n, w, errBoundMode == REL, absErrBound, relBoundRatio, blockSize,
                         (uint8_t *)d_out, (uint8_t *)d_work, d_result, d_index,
-                        (hipStream_t)hipStream))
+                        (hipStream_t)hipStream, fastMode))
         return SZx_NSCS;
     return hip_ok(hipGetLastError(), "launch") ? SZx_SCES : SZx_NSCS;
+}
+
+int szx_amd_compress_dev(const void *d_in, size_t n, int dataType, int errBoundMode,
+                         float absErrBound, float relBoundRatio, int blockSize, void *d_out,
+                         size_t outCapacity, void *d_work, size_t workBytes, uint64_t *d_result,
+                         uint64_t *d_index, void *hipStream)
+{
+    return compress_dev(SZx_WITH_BLOCK_FAST_CMPR, d_in, n, dataType, errBoundMode, absErrBound,
+                        relBoundRatio, blockSize, d_out, outCapacity, d_work, workBytes, d_result, d_index,
+                        hipStream);
+}
+
+int szx_amd_compress_ra_dev(int fastMode, const void *d_in, size_t n, int dataType, int errBoundMode,
+                            float absErrBound, float relBoundRatio, int blockSize, void *d_out,
+                            size_t outCapacity, void *d_work, size_t workBytes, uint64_t *d_result,
+                            uint64_t *d_index, void *hipStream)
+{
+    if (fastMode != SZx_RANDOMACCESS_FAST_CMPR && fastMode != SZx_OPENMP_FAST_CMPR) {
+        g_err.clear();
+        set_err("fastMode %d is not a random-access mode (3 or 4)", fastMode);
+        return SZx_NSCS;
+    }
+    return compress_dev(fastMode, d_in, n, dataType, errBoundMode, absErrBound, relBoundRatio, blockSize,
+                        d_out, outCapacity, d_work, workBytes, d_result, d_index, hipStream);
 }
 
 size_t szx_amd_decompress_workspace(size_t n, int dataType, int blockSize, size_t streamBytes)
@@ -613,23 +655,48 @@ size_t szx_amd_decompress_workspace(size_t n, int dataType, int blockSize, size_
     return decompress_layout(n, w, blockSize, streamBytes).ws_bytes;
 }
 
-int szx_amd_decompress_dev(const void *d_stream, size_t streamBytes, size_t n, int dataType,
-                           int blockSize, void *d_out, void *d_work, size_t workBytes,
-                           const uint64_t *d_index, int flags, int32_t *d_status, void *hipStream)
+static int decompress_dev(int fastMode, const void *d_stream, size_t streamBytes, size_t n, int dataType,
+                          int blockSize, void *d_out, void *d_work, size_t workBytes,
+                          const uint64_t *d_index, int flags, int32_t *d_status, void *hipStream)
 {
     g_err.clear();
     const int w = width_of(dataType);
     if (!w) { set_err("unsupported dataType %d", dataType); return SZx_TERR; }
     if (!bs_supported(blockSize)) { set_err("unsupported blockSize %d", blockSize); return SZx_NSCS; }
+    if (fastMode != SZx_WITH_BLOCK_FAST_CMPR && !bs_tiled(blockSize)) {
+        set_err("the random-access container takes blockSize 4..256, powers of two");
+        return SZx_NSCS;
+    }
     if (!d_stream || !d_out || !d_work || !d_status || n == 0) { set_err("NULL argument or n == 0"); return SZx_NSCS; }
     if (((uintptr_t)d_out & 15) != 0) { set_err("d_out must be 16-byte aligned"); return SZx_NSCS; }
     if (decompress_launch((const uint8_t *)d_stream, streamBytes, n, w, blockSize, d_out,
                           (uint8_t *)d_work, workBytes, d_index, flags & SZX_AMD_DECODE_EXACT,
-                          d_status, (hipStream_t)hipStream)) {
+                          d_status, (hipStream_t)hipStream, fastMode)) {
         set_err("workBytes too small or bad parameters");
         return SZx_NSCS;
     }
     return hip_ok(hipGetLastError(), "launch") ? SZx_SCES : SZx_NSCS;
+}
+
+int szx_amd_decompress_dev(const void *d_stream, size_t streamBytes, size_t n, int dataType,
+                           int blockSize, void *d_out, void *d_work, size_t workBytes,
+                           const uint64_t *d_index, int flags, int32_t *d_status, void *hipStream)
+{
+    return decompress_dev(SZx_WITH_BLOCK_FAST_CMPR, d_stream, streamBytes, n, dataType, blockSize, d_out,
+                          d_work, workBytes, d_index, flags, d_status, hipStream);
+}
+
+int szx_amd_decompress_ra_dev(int fastMode, const void *d_stream, size_t streamBytes, size_t n,
+                              int dataType, int blockSize, void *d_out, void *d_work, size_t workBytes,
+                              const uint64_t *d_index, int flags, int32_t *d_status, void *hipStream)
+{
+    if (fastMode != SZx_RANDOMACCESS_FAST_CMPR && fastMode != SZx_OPENMP_FAST_CMPR) {
+        g_err.clear();
+        set_err("fastMode %d is not a random-access mode (3 or 4)", fastMode);
+        return SZx_NSCS;
+    }
+    return decompress_dev(fastMode, d_stream, streamBytes, n, dataType, blockSize, d_out, d_work, workBytes,
+                          d_index, flags, d_status, hipStream);
 }
 
 size_t szx_amd_fxr_workspace(size_t n, int dataType, int blockSize)

@@ -29,6 +29,8 @@ struct CompressParams {
     uint64_t nbit;
     uint64_t nb;
     uint64_t nidx;        // index tiles
+    uint16_t *osz;        // random-access container: record size by non-constant rank, or nullptr
+    int fm4;              // fastMode-4 fp64 block statistics (radius rounded to float first)
 };
 
 // One field's header / bitmap / median assembly (device pointers).
@@ -44,6 +46,9 @@ struct AssembleParams {
     float eb;
     uint64_t *result;     // [0] stream offset, [1] stream size, [2] bound bits, [3] error
     const int *err;
+    int ra;               // 0: fastMode-2 header; 3 / 4: random-access container
+    uint64_t nb;          // blocks (random-access container)
+    const uint16_t *osz;  // random-access container: record sizes
 };
 
 // Values per REL range item of the batched range pass.

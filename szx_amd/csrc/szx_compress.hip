@@ -155,7 +155,7 @@ __device__ inline void minmax_float(const T (&v)[LV], const bool (&valid)[LV], T
 // block median as float.
 template <typename T, int BS, int LV, bool FULL>
 __device__ __forceinline__ void encode_slab(const T (&v)[LV], uint64_t e0, uint64_t n, int lane,
-                                            int lseg, int reqexp, float eb, SlabState<T, LV> &st,
+                                            int lseg, int reqexp, float eb, int fm4, SlabState<T, LV> &st,
                                             uint32_t &recb, uint32_t &isconst, float &med_out)
 {
     using U = typename Word<T>::U;
@@ -182,12 +182,16 @@ __device__ __forceinline__ void encode_slab(const T (&v)[LV], uint64_t e0, uint6
     minmax_float<T, LV, SEG, FULL>(v, valid, mn, mx);
 
     // radius, median, constant test (szx_float.cc:335-347,
-    // szx_double.cc:990-1003): median and radius are kept as float.
+    // szx_double.cc:990-1003): median and radius are kept as float.  The
+    // random-access OpenMP compressor's fp64 statistics round the radius to
+    // float first and use it for the median and the test
+    // (szx_double.cc:1120-1135).
     const T rad = (mx - mn) * (T)0.5;
-    const T med = mn + rad;
-    const float med_f = (float)med;
     const float rad_f = (float)rad;
-    const bool nonconst = exists && !(rad <= (T)eb);
+    const bool f4 = W == 8 && fm4;
+    const T med = mn + (f4 ? (T)rad_f : rad);
+    const float med_f = (float)med;
+    const bool nonconst = exists && !(f4 ? (rad_f <= eb) : (rad <= (T)eb));
 
     int clamped;
     const int len = req_length<T>(rad_expo(rad_f), reqexp, clamped);
@@ -329,9 +333,9 @@ __device__ __forceinline__ void encode_slabs(const CompressParams &p, uint64_t t
         uint32_t recb, isconst;
         float med;
         if (full)
-            encode_slab<T, BS, LV, true>(v[u], e0, n, lane, lseg, reqexp, eb, st[u], recb, isconst, med);
+            encode_slab<T, BS, LV, true>(v[u], e0, n, lane, lseg, reqexp, eb, p.fm4, st[u], recb, isconst, med);
         else
-            encode_slab<T, BS, LV, false>(v[u], e0, n, lane, lseg, reqexp, eb, st[u], recb, isconst, med);
+            encode_slab<T, BS, LV, false>(v[u], e0, n, lane, lseg, reqexp, eb, p.fm4, st[u], recb, isconst, med);
         if (lseg == 0) {
             const int lb = u * BPS + tid / SEG;
             M.pack[lb] = recb | (isconst << 20);
@@ -527,6 +531,18 @@ __device__ __forceinline__ void copy_out(const CompressParams &p, uint64_t tile,
             p.bitmap[gbyte] = (uint8_t)b;
         }
     }
+    // random-access container: every record's size at its rank among the
+    // non-constant blocks (szx_float.cc:1464)
+    if (p.osz) {
+        for (int lb = tid; lb < NBLK; lb += 256) {
+            const uint64_t b = tile * NBLK + lb;
+            if (b < p.nb && B.state[lb]) {
+                const uint32_t cur = B.pack[lb];
+                const uint32_t nxt = lb + 1 < NBLK ? B.pack[lb + 1] : tw;
+                p.osz[b - (econst + (cur >> 20))] = (uint16_t)((nxt - cur) & 0xFFFFFu);
+            }
+        }
+    }
     // per-block entries of the record index: the record's start inside its
     // index tile's payload bytes, or the constant block's rank among the
     // index tile's constant blocks (both fit 16 bits)
@@ -604,7 +620,8 @@ __device__ __forceinline__ void compress_tile(const CompressParams &p, uint64_t 
             SlabState<T, LV> su;
             uint32_t recb, isconst;
             float med;
-            encode_slab<T, BS, LV, false>(vu, e0, p.n, threadIdx.x & 63, lseg, reqexp, eb, su, recb, isconst, med);
+            encode_slab<T, BS, LV, false>(vu, e0, p.n, threadIdx.x & 63, lseg, reqexp, eb, p.fm4, su, recb, isconst,
+                                          med);
             stage_slab<T, BS, LV, SLABS, false>(su, u, lds.meta, p.payload + lds.excl[0]);
         }
     }
@@ -627,29 +644,52 @@ __global__ void k_zero_u64(unsigned long long *p, uint64_t nwords)
         p[i] = 0;
 }
 
-// Header, bitmap and constant medians in front of the payload.
-
-__global__ void k_assemble(AssembleParams a)
+// Header, bitmap and constant medians in front of the payload, by thread gid
+// of nthr.  fastMode 2 (szx_float.cc:1091-1115): [1,1,1,0], (u8)blockSize,
+// big-endian constant count, bitmap, big-endian medians.  Random-access
+// container (szx_float.cc:1438-1478, 1260-1375): [1,1,1,1], big-endian u64
+// blockSize and constant count, the u16 record sizes, bitmap, medians
+// (big-endian for fastMode 3, native for fastMode 4).
+__device__ inline void assemble_part(const AssembleParams &a, uint64_t gid, uint64_t nthr)
 {
     const uint64_t nconst = a.totals[1];
     const uint64_t pbytes = a.totals[0];
-    const uint64_t front = HEADER_BYTES + a.nbit + 4 * nconst;
+    const uint64_t nrec = a.ra ? a.nb - nconst : 0;
+    const uint64_t hdr = a.ra ? 20 + 2 * nrec : HEADER_BYTES;   // bitmap offset
+    const uint64_t front = hdr + a.nbit + 4 * nconst;
     const uint64_t start = a.region - front;
     uint8_t *s = a.out + start;
-    const uint64_t gid = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    const uint64_t nthr = (uint64_t)gridDim.x * blockDim.x;
     if (gid == 0) {
-        s[0] = 1; s[1] = 1; s[2] = 1; s[3] = 0;        // szx_float.cc:1091-1094
-        s[4] = (uint8_t)a.bs;                           // (u8)blockSize, :1095
-        for (int k = 0; k < 8; k++) s[5 + k] = (uint8_t)(nconst >> (56 - 8 * k));
+        if (a.ra) {
+            s[0] = 1; s[1] = 1; s[2] = 1; s[3] = 1;
+            for (int k = 0; k < 8; k++) s[4 + k] = (uint8_t)((uint64_t)a.bs >> (56 - 8 * k));
+            for (int k = 0; k < 8; k++) s[12 + k] = (uint8_t)(nconst >> (56 - 8 * k));
+        } else {
+            s[0] = 1; s[1] = 1; s[2] = 1; s[3] = 0;
+            s[4] = (uint8_t)a.bs;                           // (u8)blockSize, :1095
+            for (int k = 0; k < 8; k++) s[5 + k] = (uint8_t)(nconst >> (56 - 8 * k));
+        }
         a.result[0] = start;
         a.result[1] = front + pbytes;
         a.result[2] = f32_bits(a.eb_dev ? *a.eb_dev : a.eb);
         a.result[3] = (uint64_t)*a.err;
     }
-    for (uint64_t i = gid; i < a.nbit; i += nthr) s[HEADER_BYTES + i] = a.bitmap[i];
-    const uint8_t *cm = reinterpret_cast<const uint8_t *>(a.cmed);
-    for (uint64_t i = gid; i < 4 * nconst; i += nthr) s[HEADER_BYTES + a.nbit + i] = cm[i];
+    if (a.ra) {
+        const uint8_t *ob = reinterpret_cast<const uint8_t *>(a.osz);   // little-endian u16
+        for (uint64_t i = gid; i < 2 * nrec; i += nthr) s[20 + i] = ob[i];
+    }
+    for (uint64_t i = gid; i < a.nbit; i += nthr) s[hdr + i] = a.bitmap[i];
+    const uint8_t *cm = reinterpret_cast<const uint8_t *>(a.cmed);   // big-endian medians
+    if (a.ra == 4) {
+        for (uint64_t i = gid; i < 4 * nconst; i += nthr) s[hdr + a.nbit + i] = cm[(i & ~(uint64_t)3) | (3 - (i & 3))];
+    } else {
+        for (uint64_t i = gid; i < 4 * nconst; i += nthr) s[hdr + a.nbit + i] = cm[i];
+    }
+}
+
+__global__ void k_assemble(AssembleParams a)
+{
+    assemble_part(a, blockIdx.x * (uint64_t)blockDim.x + threadIdx.x, (uint64_t)gridDim.x * blockDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -879,26 +919,8 @@ __global__ __launch_bounds__(256) void k_zero_batch(const BatchField *bf, const 
 __global__ __launch_bounds__(256) void k_assemble_batch(const BatchField *bf, const uint64_t *abase, int nf)
 {
     const int f = field_of(abase, nf, blockIdx.x);
-    const AssembleParams &a = bf[f].a;
-    const uint64_t nconst = a.totals[1];
-    const uint64_t pbytes = a.totals[0];
-    const uint64_t front = HEADER_BYTES + a.nbit + 4 * nconst;
-    const uint64_t start = a.region - front;
-    uint8_t *s = a.out + start;
-    const uint64_t gid = (blockIdx.x - abase[f]) * (uint64_t)blockDim.x + threadIdx.x;
-    const uint64_t nthr = (abase[f + 1] - abase[f]) * blockDim.x;
-    if (gid == 0) {
-        s[0] = 1; s[1] = 1; s[2] = 1; s[3] = 0;        // szx_float.cc:1091-1094
-        s[4] = (uint8_t)a.bs;                           // (u8)blockSize, :1095
-        for (int k = 0; k < 8; k++) s[5 + k] = (uint8_t)(nconst >> (56 - 8 * k));
-        a.result[0] = start;
-        a.result[1] = front + pbytes;
-        a.result[2] = f32_bits(a.eb_dev ? *a.eb_dev : a.eb);
-        a.result[3] = (uint64_t)*a.err;
-    }
-    for (uint64_t i = gid; i < a.nbit; i += nthr) s[HEADER_BYTES + i] = a.bitmap[i];
-    const uint8_t *cm = reinterpret_cast<const uint8_t *>(a.cmed);
-    for (uint64_t i = gid; i < 4 * nconst; i += nthr) s[HEADER_BYTES + a.nbit + i] = cm[i];
+    assemble_part(bf[f].a, (blockIdx.x - abase[f]) * (uint64_t)blockDim.x + threadIdx.x,
+                  (abase[f + 1] - abase[f]) * blockDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -912,7 +934,9 @@ CompressLayout compress_layout(uint64_t n, int w, int bs)
     CompressLayout L{};
     L.nb = ceil_div(n, (uint64_t)bs);
     L.nbit = ceil_div(L.nb, 8);
-    L.region = (HEADER_BYTES + L.nbit + 4 * L.nb + 15) & ~(uint64_t)15;
+    // front region: the larger (random-access) header, the size table and the
+    // medians take at most 20 + 2 nb' + 4 nc <= 20 + 4 nb bytes with the bitmap
+    L.region = (20 + L.nbit + 4 * L.nb + 15) & ~(uint64_t)15;
     const uint64_t rec_max = 5 + ceil_div((uint64_t)bs, 4) + (uint64_t)bs * w;
     L.payload_max = L.nb * rec_max;
     L.capacity = L.region + L.payload_max + 16;
@@ -932,6 +956,7 @@ CompressLayout compress_layout(uint64_t n, int w, int bs)
     L.ws_parts = off;   off += 2 * kRangeParts * sizeof(double);
     L.ws_bitmap = off;  off += (L.nbit + 15) & ~(uint64_t)15;
     L.ws_cmed = off;    off += (4 * L.nb + 15) & ~(uint64_t)15;
+    L.ws_osz = off;     off += (2 * L.nb + 15) & ~(uint64_t)15;
     if (!bs_tiled(bs)) {
         off = (off + 255) & ~(uint64_t)255;
         L.ws_gsize = off;   off += (4 * L.nb + 255) & ~(uint64_t)255;
@@ -991,8 +1016,10 @@ void range_launch(const void *d_in, uint64_t n, int w, uint8_t *d_ws, void *d_mi
 
 int compress_launch(const void *d_in, uint64_t n, int w, int rel_mode, float abs_eb, float rel,
                     int bs, uint8_t *d_out, uint8_t *d_ws, uint64_t *d_result, uint64_t *d_index,
-                    hipStream_t st)
+                    hipStream_t st, int fast_mode)
 {
+    const bool ra = fast_mode == 3 || fast_mode == 4;
+    if (!(fast_mode == 2 || ra) || (ra && !bs_tiled(bs))) return -1;
     const CompressLayout L = compress_layout(n, w, bs);
     unsigned long long *misc = reinterpret_cast<unsigned long long *>(d_ws + L.ws_misc);
     const uint64_t zero_words = (L.ws_misc + 64 * sizeof(uint64_t)) / 8;
@@ -1036,6 +1063,8 @@ int compress_launch(const void *d_in, uint64_t n, int w, int rel_mode, float abs
     p.nb = L.nb;
     p.ntiles = L.ntiles;
     p.nbit = L.nbit;
+    p.osz = ra ? reinterpret_cast<uint16_t *>(d_ws + L.ws_osz) : nullptr;
+    p.fm4 = fast_mode == 4;
     int rc = 0;
     timing_mark_begin(KK_COMPRESS, st);
     if (bs_tiled(bs)) {
@@ -1074,6 +1103,9 @@ int compress_launch(const void *d_in, uint64_t n, int w, int rel_mode, float abs
     a.eb = abs_eb;
     a.result = d_result;
     a.err = p.err;
+    a.ra = ra ? fast_mode : 0;
+    a.nb = L.nb;
+    a.osz = p.osz;
     const uint64_t work = L.nbit > 4 * L.nb ? L.nbit : 4 * L.nb;
     uint64_t blocks = (work + 255) / 256;
     if (blocks < 1) blocks = 1;

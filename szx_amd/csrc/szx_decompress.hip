@@ -54,14 +54,55 @@ __device__ inline void check_header(const uint8_t *s, uint64_t len, uint64_t n, 
     }
     h->bs = bs; h->nb = nb; h->nbit = nbit; h->nconst = nconst;
     h->q0 = HEADER_BYTES + nbit + 4 * nconst;
+    h->qbit = HEADER_BYTES;
+    h->qmed = HEADER_BYTES + nbit;
+    h->medle = 0;
     h->rc = rc;
     *status = rc;
 }
 
-__global__ void k_check_header(const uint8_t *s, uint64_t len, uint64_t n, int bs_expect,
+// Random-access container (szxd_float.cc:313-476): [1,1,1,1], big-endian u64
+// blockSize and constant count, a u16 size per record, bitmap, medians
+// (native order for fastMode 4), records.
+__device__ inline void check_header_ra(const uint8_t *s, uint64_t len, uint64_t n, int bs_expect, int fm,
+                                       DecodeHeader *h, int32_t *status)
+{
+    int32_t rc = 0;
+    uint64_t bs = 0, nb = 0, nbit = 0, nconst = 0, qbit = 20;
+    if (len < 20) {
+        rc = -10;
+    } else {
+        for (int k = 0; k < 8; k++) bs = (bs << 8) | s[4 + k];
+        for (int k = 0; k < 8; k++) nconst = (nconst << 8) | s[12 + k];
+        if (bs != (uint64_t)bs_expect) {
+            rc = -11;
+        } else {
+            nb = ceil_div(n, bs);
+            nbit = ceil_div(nb, 8);
+            if (nconst > nb) rc = -12;
+            else {
+                qbit = 20 + 2 * (nb - nconst);
+                if (qbit + nbit + 4 * nconst > len) rc = -13;
+            }
+        }
+    }
+    if (rc) bs = (uint64_t)bs_expect;
+    h->bs = bs; h->nb = nb; h->nbit = nbit; h->nconst = nconst;
+    h->qbit = qbit;
+    h->qmed = qbit + nbit;
+    h->q0 = qbit + nbit + 4 * nconst;
+    h->medle = fm == 4;
+    h->rc = rc;
+    *status = rc;
+}
+
+__global__ void k_check_header(const uint8_t *s, uint64_t len, uint64_t n, int bs_expect, int fm,
                                DecodeHeader *h, int32_t *status)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) check_header(s, len, n, bs_expect, h, status);
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        if (fm == 3 || fm == 4) check_header_ra(s, len, n, bs_expect, fm, h, status);
+        else check_header(s, len, n, bs_expect, h, status);
+    }
 }
 
 template <typename T> struct Store4;
@@ -149,6 +190,7 @@ struct TileLds {
     uint16_t *size;             // per block: decoded record size
     const uint8_t *state;       // per block: non-constant
     int *err;
+    int medle;                  // medians in native (little-endian) order
 };
 
 // Decode the LV values of one lane: lane lseg of the BS/LV lanes holding
@@ -167,8 +209,9 @@ __device__ inline void decode_lane(const TileLds &L, uint64_t e0, uint64_t n, in
         float med = 0.0f;
         if (rank < L.nct) {
             const uint8_t *p = L.meds + 4 * rank;
-            med = bits_f32(((uint32_t)p[0] << 24) | ((uint32_t)p[1] << 16) | ((uint32_t)p[2] << 8) |
-                           (uint32_t)p[3]);
+            const uint32_t be = ((uint32_t)p[0] << 24) | ((uint32_t)p[1] << 16) | ((uint32_t)p[2] << 8) |
+                                (uint32_t)p[3];
+            med = bits_f32(L.medle ? bswap32(be) : be);
         } else if (lseg == 0) {
             *L.err = 1;
         }
@@ -336,7 +379,7 @@ __device__ inline void decode_tile(const DecodeParams &p, uint64_t tile, DecodeL
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int lseg = tid % SEG;
-    const uint64_t nb = h->nb, q0 = h->q0, nconst = h->nconst, nbit = h->nbit;
+    const uint64_t nb = h->nb, q0 = h->q0, nconst = h->nconst;
     const uint64_t gb0 = tile * NBLK;
     const uint64_t off = q0 + index[2 * tile];
     const uint64_t end = q0 + index[2 * tile + 2];
@@ -350,20 +393,20 @@ __device__ inline void decode_tile(const DecodeParams &p, uint64_t tile, DecodeL
     const uint32_t tb = (uint32_t)(end - off);
     const uint32_t nct = (uint32_t)(cend - cbase);
     const uint8_t *rsrc = s + off;
-    const uint8_t *msrc = s + HEADER_BYTES + nbit + 4 * cbase;   // inside [0, q0) <= len
+    const uint8_t *msrc = s + h->qmed + 4 * cbase;   // inside [0, q0) <= len
     const uint32_t rdel = (uint32_t)((uintptr_t)rsrc & 15), mdel = (uint32_t)((uintptr_t)msrc & 15);
     if (tid == 0) s_err = 0;
     stage_bytes(st, rsrc, tb, s, s + len, tid);
     stage_bytes(s_med, msrc, 4 * nct, s, s + len, tid);
     for (int lb = tid; lb < NBLK; lb += 256) {
         const uint64_t b = gb0 + lb;
-        s_state[lb] = b < nb ? (s[HEADER_BYTES + (b >> 3)] >> (7 - (b & 7))) & 1 : 0;
+        s_state[lb] = b < nb ? (s[h->qbit + (b >> 3)] >> (7 - (b & 7))) & 1 : 0;
         s_start[lb] = b < nb ? bidx[b] : 0;
         s_size[lb] = 0;
     }
     __syncthreads();
 
-    const TileLds L{st, rdel, s_med + mdel, nct, tb, s_start, s_size, s_state, &s_err};
+    const TileLds L{st, rdel, s_med + mdel, nct, tb, s_start, s_size, s_state, &s_err, (int)h->medle};
     const bool full = (tile + 1) * (uint64_t)TILE <= n;
 #pragma unroll
     for (int u = 0; u < SLABS; u++) {
@@ -539,15 +582,16 @@ static int dispatch_decode(int bs, const uint8_t *s, uint64_t len, uint64_t n,
 
 int decompress_launch(const uint8_t *d_stream, uint64_t stream_bytes, uint64_t n, int w, int bs,
                       void *d_out, uint8_t *d_ws, uint64_t ws_bytes, const uint64_t *d_index,
-                      int exact, int32_t *d_status, hipStream_t st)
+                      int exact, int32_t *d_status, hipStream_t st, int fast_mode)
 {
-    if (!bs_valid(bs)) return -1;
+    const bool ra = fast_mode == 3 || fast_mode == 4;
+    if (!bs_valid(bs) || !(fast_mode == 2 || ra) || (ra && !bs_tiled(bs))) return -1;
     const DecompressLayout L = decompress_layout(n, w, bs, stream_bytes);
     if (ws_bytes < (d_index ? L.ws_index : L.ws_bytes)) return -1;   // with an index only the header is kept
     DecodeHeader *h = reinterpret_cast<DecodeHeader *>(d_ws + L.ws_hdr);
     const uint64_t tv = tile_values(w);
     const uint64_t ntiles = ceil_div(n, tv);
-    hipLaunchKernelGGL(k_check_header, dim3(1), dim3(64), 0, st, d_stream, stream_bytes, n, bs, h,
+    hipLaunchKernelGGL(k_check_header, dim3(1), dim3(64), 0, st, d_stream, stream_bytes, n, bs, fast_mode, h,
                        d_status);
     const uint64_t *index = d_index;
     if (!index) {
@@ -555,8 +599,12 @@ int decompress_launch(const uint8_t *d_stream, uint64_t stream_bytes, uint64_t n
         uint16_t *bbuilt =
             bs_tiled(bs) ? reinterpret_cast<uint16_t *>(built + 2 * (index_tiles(n, w) + 1)) : nullptr;
         timing_mark_begin(KK_INDEX, st);
-        index_build_launch(d_stream, stream_bytes, n, w, bs, h, built, bbuilt, d_ws + L.ws_build,
-                           d_status, st);
+        if (ra)
+            index_from_sizes_launch(d_stream, stream_bytes, n, w, bs, h, built, bbuilt, d_ws + L.ws_build,
+                                    d_status, st);
+        else
+            index_build_launch(d_stream, stream_bytes, n, w, bs, h, built, bbuilt, d_ws + L.ws_build,
+                               d_status, st);
         timing_mark_end(KK_INDEX, st);
         index = built;
     }

@@ -453,15 +453,15 @@ __device__ inline void decode_segment(const GenericDecode &p, uint64_t b, uint64
     for (int k = 0; k < 4; k++) val[k] = 4 * l + k < m;
     const uint64_t off = p.idx[2 * b], rank = p.idx[2 * b + 1];
     const uint64_t noff = p.idx[2 * b + 2], nrank = p.idx[2 * b + 3];
-    if (!gen_nonconst(p.s + HEADER_BYTES, b)) {
+    if (!gen_nonconst(p.s + h->qbit, b)) {
         // constant block: its median, widened for fp64 (szxd_float.cc:49-56)
         if (rank >= h->nconst || nrank != rank + 1 || noff != off) {
             if (l == 0) atomicMin(p.status, -21);
             return;
         }
-        const uint8_t *c = p.s + HEADER_BYTES + h->nbit + 4 * rank;
-        const T med = (T)bits_f32(((uint32_t)c[0] << 24) | ((uint32_t)c[1] << 16) | ((uint32_t)c[2] << 8) |
-                                  (uint32_t)c[3]);
+        const uint8_t *c = p.s + h->qmed + 4 * rank;
+        const uint32_t be = ((uint32_t)c[0] << 24) | ((uint32_t)c[1] << 16) | ((uint32_t)c[2] << 8) | (uint32_t)c[3];
+        const T med = (T)bits_f32(h->medle ? bswap32(be) : be);
 #pragma unroll
         for (int k = 0; k < 4; k++)
             if (val[k]) out[4 * l + k] = med;

@@ -437,11 +437,11 @@ __global__ void k_ix_scatter(IxParams x)
 __global__ void k_ix_tile_counts(IxParams x)
 {
     const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    if (t >= x.ntiles) return;
+    if (t >= x.ntiles || *x.status != 0) return;
     const DecodeHeader &h = *x.h;
     const uint64_t b0 = t * x.tile_blocks, b1 = b0 + x.tile_blocks < h.nb ? b0 + x.tile_blocks : h.nb;
     uint32_t c = 0;
-    for (uint64_t b = b0; b < b1; b++) c += (x.s[HEADER_BYTES + (b >> 3)] >> (7 - (b & 7))) & 1;
+    for (uint64_t b = b0; b < b1; b++) c += (x.s[h.qbit + (b >> 3)] >> (7 - (b & 7))) & 1;
     x.tcnt[t] = c;
 }
 
@@ -464,7 +464,7 @@ __global__ void k_ix_tail(IxParams x)
     uint64_t b = h.nb;
     while (b > 0) {
         b--;
-        if ((x.s[HEADER_BYTES + (b >> 3)] >> (7 - (b & 7))) & 1) break;
+        if ((x.s[h.qbit + (b >> 3)] >> (7 - (b & 7))) & 1) break;
     }
     const uint64_t nel = (b + 1) * (uint64_t)x.bs <= x.n ? (uint64_t)x.bs : x.n - b * (uint64_t)x.bs;
     const uint64_t nl = (nel + 3) / 4;
@@ -507,7 +507,7 @@ __global__ __launch_bounds__(256) void k_ix_tiles(IxParams x)
         uint32_t a = 0;
         for (uint64_t i = 0; i < nbt; i++) {
             s_nc[i] = a;
-            a += (x.s[HEADER_BYTES + ((b0 + i) >> 3)] >> (7 - ((b0 + i) & 7))) & 1;
+            a += (x.s[h.qbit + ((b0 + i) >> 3)] >> (7 - ((b0 + i) & 7))) & 1;
         }
         s_nc[nbt] = a;
         const uint64_t off = ord0 < K ? x.rec[ord0] : *x.pend;
@@ -522,7 +522,7 @@ __global__ __launch_bounds__(256) void k_ix_tiles(IxParams x)
     const uint64_t off = ord0 < K ? x.rec[ord0] : *x.pend;
     for (uint64_t i = threadIdx.x; i < nbt; i += blockDim.x) {
         const uint64_t b = b0 + i;
-        const bool nc = (x.s[HEADER_BYTES + (b >> 3)] >> (7 - (b & 7))) & 1;
+        const bool nc = (x.s[h.qbit + (b >> 3)] >> (7 - (b & 7))) & 1;
         uint64_t v;
         if (nc) {
             const uint64_t ord = ord0 + s_nc[i];
@@ -554,6 +554,27 @@ __global__ void k_ix_pairs(IxParams x)
     x.index[2 * b + 1] = b - ord;
 }
 
+// Random-access container (szxd_float.cc:313-476): the record sizes are in
+// the stream (u16, native order, from byte 20), so the record starts are
+// their exclusive scan.  Sizes as u32, zero past the last record.
+__global__ void k_ra_sizes(IxParams x, uint32_t *sz, uint64_t nmax)
+{
+    const uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (k >= nmax) return;
+    const DecodeHeader &h = *x.h;
+    const uint64_t K = *x.status == 0 ? h.nb - h.nconst : 0;   // 20 + 2K <= len (header check)
+    sz[k] = k < K ? (uint32_t)x.s[20 + 2 * k] | ((uint32_t)x.s[21 + 2 * k] << 8) : 0u;
+}
+
+// The records found are the size table's; the payload must fit the stream.
+__global__ void k_ra_tail(IxParams x)
+{
+    if (threadIdx.x != 0 || *x.status != 0) return;
+    const DecodeHeader &h = *x.h;
+    *x.found = h.nb - h.nconst;
+    if (*x.pend > x.len - h.q0) *x.status = -33;
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -581,15 +602,17 @@ IxLayout ix_layout(uint64_t n, int w, int bs, uint64_t len)
     L.o_tcnt = off;   off += al(4 * L.ntiles);
     L.o_tbase = off;  off += al(8 * L.ntiles);
     L.o_misc = off;   off += 256;   // found, pend, scratch total
-    L.o_bsum = off;   off += al(8 * ((L.nch > L.ntiles ? L.nch : L.ntiles) / 1024 + 1));
+    L.o_osz = off;    off += al(4 * (nb ? nb : 1));   // random-access container sizes
+    uint64_t mx = L.nch > L.ntiles ? L.nch : L.ntiles;
+    mx = mx > nb ? mx : nb;
+    L.o_bsum = off;   off += al(8 * (mx / 1024 + 1));
     L.bytes = off;
     return L;
 }
 
-void index_build_launch(const uint8_t *s, uint64_t len, uint64_t n, int w, int bs, const DecodeHeader *h,
-                        uint64_t *index, uint16_t *bidx, uint8_t *ws, int32_t *status, hipStream_t st)
+static IxParams ix_params(const uint8_t *s, uint64_t len, uint64_t n, int w, int bs, const DecodeHeader *h,
+                          uint64_t *index, uint16_t *bidx, uint8_t *ws, int32_t *status, const IxLayout &L)
 {
-    const IxLayout L = ix_layout(n, w, bs, len);
     IxParams x{};
     x.s = s;
     x.len = len;
@@ -616,23 +639,54 @@ void index_build_launch(const uint8_t *s, uint64_t len, uint64_t n, int w, int b
     x.tbase = reinterpret_cast<uint64_t *>(ws + L.o_tbase);
     x.found = reinterpret_cast<uint64_t *>(ws + L.o_misc);
     x.pend = reinterpret_cast<uint64_t *>(ws + L.o_misc + 8);
-    uint64_t *scratch = reinterpret_cast<uint64_t *>(ws + L.o_misc + 16);
-    uint64_t *bsum = reinterpret_cast<uint64_t *>(ws + L.o_bsum);
     x.index = index;
     x.bidx = bidx;
+    return x;
+}
+
+// G: the index entries from the record starts (x.rec), the records found and
+// the payload end.
+static void index_tiles_launch(IxParams &x, const IxLayout &L, uint8_t *ws, bool tiled, bool chain_tail,
+                               hipStream_t st)
+{
+    uint64_t *scratch = reinterpret_cast<uint64_t *>(ws + L.o_misc + 16);
+    uint64_t *bsum = reinterpret_cast<uint64_t *>(ws + L.o_bsum);
+    hipLaunchKernelGGL(k_ix_tile_counts, dim3((unsigned)((L.ntiles + 255) / 256)), dim3(256), 0, st, x);
+    scan_u32(x.tcnt, L.ntiles, x.tbase, bsum, scratch, st);
+    if (chain_tail) hipLaunchKernelGGL(k_ix_tail, dim3(1), dim3(64), 0, st, x);
+    if (tiled)
+        hipLaunchKernelGGL(k_ix_tiles, dim3((unsigned)L.ntiles), dim3(256), 0, st, x);
+    else
+        hipLaunchKernelGGL(k_ix_pairs, dim3((unsigned)((L.ntiles + 256) / 256)), dim3(256), 0, st, x);
+}
+
+void index_build_launch(const uint8_t *s, uint64_t len, uint64_t n, int w, int bs, const DecodeHeader *h,
+                        uint64_t *index, uint16_t *bidx, uint8_t *ws, int32_t *status, hipStream_t st)
+{
+    const IxLayout L = ix_layout(n, w, bs, len);
+    IxParams x = ix_params(s, len, n, w, bs, h, index, bidx, ws, status, L);
+    uint64_t *bsum = reinterpret_cast<uint64_t *>(ws + L.o_bsum);
     const unsigned nch = (unsigned)L.nch;
     hipLaunchKernelGGL(k_ix_map, dim3(nch), dim3(256), 0, st, x);
     hipLaunchKernelGGL(k_ix_entry, dim3((nch + 255) / 256), dim3(256), 0, st, x);
     hipLaunchKernelGGL(k_ix_walk, dim3((nch + 3) / 4), dim3(256), 0, st, x);
     scan_u32(x.cnt, L.nch, x.cbase, bsum, x.found, st);
     hipLaunchKernelGGL(k_ix_scatter, dim3(nch), dim3(256), 0, st, x);
-    hipLaunchKernelGGL(k_ix_tile_counts, dim3((unsigned)((L.ntiles + 255) / 256)), dim3(256), 0, st, x);
-    scan_u32(x.tcnt, L.ntiles, x.tbase, bsum, scratch, st);
-    hipLaunchKernelGGL(k_ix_tail, dim3(1), dim3(64), 0, st, x);
-    if (bs_tiled(bs))
-        hipLaunchKernelGGL(k_ix_tiles, dim3((unsigned)L.ntiles), dim3(256), 0, st, x);
-    else
-        hipLaunchKernelGGL(k_ix_pairs, dim3((unsigned)((L.ntiles + 256) / 256)), dim3(256), 0, st, x);
+    index_tiles_launch(x, L, ws, bs_tiled(bs), true, st);
+}
+
+void index_from_sizes_launch(const uint8_t *s, uint64_t len, uint64_t n, int w, int bs, const DecodeHeader *h,
+                             uint64_t *index, uint16_t *bidx, uint8_t *ws, int32_t *status, hipStream_t st)
+{
+    const IxLayout L = ix_layout(n, w, bs, len);
+    IxParams x = ix_params(s, len, n, w, bs, h, index, bidx, ws, status, L);
+    uint64_t *bsum = reinterpret_cast<uint64_t *>(ws + L.o_bsum);
+    uint32_t *sz = reinterpret_cast<uint32_t *>(ws + L.o_osz);
+    const uint64_t nb = (n + bs - 1) / bs;
+    hipLaunchKernelGGL(k_ra_sizes, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st, x, sz, nb);
+    scan_u32(sz, nb, x.rec, bsum, x.pend, st);
+    hipLaunchKernelGGL(k_ra_tail, dim3(1), dim3(64), 0, st, x);
+    index_tiles_launch(x, L, ws, bs_tiled(bs), false, st);
 }
 
 }  // namespace szxamd

@@ -39,7 +39,7 @@ struct IxLayout {
     uint64_t nch, ntiles, nl;
     uint32_t rmax, maxr;
     uint64_t o_exits, o_conv, o_entry, o_cnt, o_cbase, o_tables, o_pos, o_rec, o_tcnt, o_tbase,
-        o_misc, o_bsum, bytes;
+        o_misc, o_osz, o_bsum, bytes;
 };
 
 IxLayout ix_layout(uint64_t n, int w, int bs, uint64_t len);
@@ -48,5 +48,9 @@ IxLayout ix_layout(uint64_t n, int w, int bs, uint64_t len);
 // into *h).  ws holds ix_layout(...).bytes.
 void index_build_launch(const uint8_t *s, uint64_t len, uint64_t n, int w, int bs, const DecodeHeader *h,
                         uint64_t *index, uint16_t *bidx, uint8_t *ws, int32_t *status, hipStream_t st);
+
+// The same index for a random-access container, from its record size table.
+void index_from_sizes_launch(const uint8_t *s, uint64_t len, uint64_t n, int w, int bs, const DecodeHeader *h,
+                             uint64_t *index, uint16_t *bidx, uint8_t *ws, int32_t *status, hipStream_t st);
 
 }  // namespace szxamd

@@ -62,16 +62,19 @@ struct CompressLayout {
     uint64_t ws_tiles, ws_chunks, ws_cacc, ws_misc, ws_parts, ws_bitmap, ws_cmed, ws_bytes;
     // per-block kernels only: record sizes, scan chunk sums, per-block pairs
     uint64_t ws_gsize, ws_gparts, ws_gidx;
+    uint64_t ws_osz;      // random-access container: u16 record sizes
 };
 
 CompressLayout compress_layout(uint64_t n, int w, int bs);
 
 // Enqueue a whole compress on `st`.  d_result receives
 // {stream offset in d_out, stream bytes, bound bits}; d_index (optional)
-// receives the record index.  Returns <0 for an unsupported block size.
+// receives the record index.  fast_mode 2 writes the blocked stream, 3 and 4
+// the random-access container (tiled block sizes only).  Returns <0 for an
+// unsupported block size or mode.
 int compress_launch(const void *d_in, uint64_t n, int w, int rel_mode, float abs_eb, float rel,
                     int bs, uint8_t *d_out, uint8_t *d_ws, uint64_t *d_result, uint64_t *d_index,
-                    hipStream_t st);
+                    hipStream_t st, int fast_mode = 2);
 
 // Value range (computeValueRange_*, szx_float.cc:478-499): d_minmax receives
 // {min, max} as T; d_ws holds range_workspace_bytes().
@@ -83,6 +86,8 @@ void range_launch(const void *d_in, uint64_t n, int w, uint8_t *d_ws, void *d_mi
 struct DecodeHeader {
     uint64_t bs, nb, nbit, nconst, q0;
     int64_t rc;   // the header check's own status (0 = well-formed)
+    uint64_t qbit, qmed;   // stream offsets of the bitmap and the constant medians
+    int64_t medle;         // 1: medians in native (little-endian) order
 };
 
 // Device memory layout of one decompress call.
@@ -126,8 +131,10 @@ void generic_decode_launch(const GenericDecode &p, int w, int bs, uint64_t nb, h
 // Enqueue a whole decompress on `st`; d_status receives 0 or a negative code.
 // d_index (optional) is the record index the compressor wrote; without it the
 // index is rebuilt from the stream.
+// fast_mode 2 reads the blocked stream, 3 and 4 the random-access container
+// (its size table gives the record index; tiled block sizes only).
 int decompress_launch(const uint8_t *d_stream, uint64_t stream_bytes, uint64_t n, int w, int bs,
                       void *d_out, uint8_t *d_ws, uint64_t ws_bytes, const uint64_t *d_index,
-                      int exact, int32_t *d_status, hipStream_t st);
+                      int exact, int32_t *d_status, hipStream_t st, int fast_mode = 2);
 
 }  // namespace szxamd
