@@ -22,7 +22,8 @@ reference's sampled-model bound judged by its real stream beside them; an fp64
 8192x8192 Gaussian random field at REL 1e-4/1e-6/1e-8 and blockSize
 64/128/256 (throughput and full-size parity per configuration); the 512^3
 field at block sizes outside the tiled kernels (default 100, 255, 7: the
-per-block segment kernels, throughput and full-size parity); and 1000 mixed
+per-block segment kernels, throughput and full-size parity); the random-access
+container (fastMode 3 and 4) on the same field; and 1000 mixed
 fp32/fp64 fields compressed and decompressed by the batched launch, against
 one launch sequence per field.
 
@@ -301,6 +302,67 @@ def generic_eval(x, xh, eb, dev, steps, sizes, parity=True):
     return out
 
 
+def random_access_eval(x, xh, rel, dev, steps, parity=True):
+    """The random-access container (fastMode 3: big-endian medians, fastMode
+    4: native medians) on the same fp32 field at the same REL bound: compress
+    (whole call), decompress with the record index and from the container's
+    size table, and with parity the stream and the decode against the oracle."""
+    import numpy as np
+    import torch
+
+    import szx_amd as Z
+
+    n = x.numel()
+    nbytes = n * 4
+    O = None
+    if parity:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+    out = []
+    for fm in (3, 4):
+        comp = Z.DeviceCompressor(n, np.float32, 128, device=dev, fastMode=fm)
+        comp(x, Z.REL, relBoundRatio=rel)
+        view, off, csize, eb = comp.stream_view()
+        dec = Z.DeviceDecompressor(n, np.float32, 128, comp.capacity, device=dev, fastMode=fm)
+        dec(view, csize, Z.DECODE_REFERENCE, index=comp.index)
+        dec(view, csize, Z.DECODE_REFERENCE)
+        torch.cuda.synchronize()
+        Z.timing_begin()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            comp(x, Z.REL, relBoundRatio=rel)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            dec(view, csize, Z.DECODE_REFERENCE, index=comp.index)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        for _ in range(steps):
+            dec(view, csize, Z.DECODE_REFERENCE)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        kt = Z.timing_end()
+        assert dec.check() == 0, "decoder rejected the container"
+        kc = kt["compress"][0] / max(1, kt["compress"][1])
+        rec = {"fast_mode": fm, "compression_ratio": nbytes / csize,
+               "compress_gbs": nbytes * steps / (t1 - t0) / 1e9,
+               "compress_kernel_ms": kc,
+               "decompress_gbs": nbytes * steps / (t2 - t1) / 1e9,
+               "decompress_size_table_gbs": nbytes * steps / (t3 - t2) / 1e9}
+        if parity:
+            s_gpu = view.cpu().numpy().tobytes()
+            s_ora = O.compress_ra(xh, eb, 128, fm)
+            rec["stream_identical"] = s_gpu == s_ora
+            d_ora = O.decompress_ra(s_ora, n, np.float32, fm)
+            rec["decode_identical"] = bool(np.array_equal(dec.out.cpu().numpy().view(np.uint32),
+                                                          d_ora.view(np.uint32)))
+        log(f"random access fastMode {fm}: {rec}")
+        out.append(rec)
+        del comp, dec, view
+    torch.cuda.empty_cache()
+    return out
+
+
 def _extents(r, n, nd):
     """Random 1-D/2-D/3-D extents whose product is close to n (at most n)."""
     import numpy as np
@@ -473,6 +535,8 @@ def main():
     ap.add_argument("--batch-fields", type=int, default=1000,
                     help="fields of the batched-launch evaluation (0 skips it)")
     ap.add_argument("--batch-reps", type=int, default=5)
+    ap.add_argument("--no-ra", action="store_true",
+                    help="skip the random-access container (fastMode 3/4) leg")
     ap.add_argument("--generic-bs", default="100,255,7",
                     help="block sizes outside the tiled kernels to time on the same field "
                          "('' to skip)")
@@ -645,6 +709,12 @@ def main():
         log("fp64 8192^2 field, 9 configurations")
         fp64 = fp64_eval(dev, max(2, args.steps // 4), parity=not args.no_parity)
 
+    ra = None
+    if not args.no_ra:
+        log("random-access container (fastMode 3 and 4)")
+        ra = random_access_eval(x, xh, args.rel, dev, max(2, args.steps // 4),
+                                parity=not args.no_parity and xh is not None)
+
     generic = None
     gsizes = [int(v) for v in args.generic_bs.split(",") if v.strip()]
     if gsizes:
@@ -705,6 +775,7 @@ def main():
         "fp64_grf_8192sq": fp64,
         "batched": batched,
         "generic_block_sizes": generic,
+        "random_access": ra,
         "host_api_pcie_inclusive": host_api,
     }
     print(json.dumps(line), flush=True)
