@@ -77,6 +77,12 @@ def lib():
         L.szxo_compress_ra.argtypes = [vp, _sz, C.c_float, C.c_int, C.c_int, C.c_int, vp]
         L.szxo_decompress_ra.restype = C.c_int
         L.szxo_decompress_ra.argtypes = [vp, _sz, _sz, C.c_int, C.c_int, vp, C.c_int]
+        L.szxo_compress_nb_bound.restype = _sz
+        L.szxo_compress_nb_bound.argtypes = [_sz, C.c_int]
+        L.szxo_compress_nb.restype = _sz
+        L.szxo_compress_nb.argtypes = [vp, _sz, C.c_float, C.c_int, vp]
+        L.szxo_decompress_nb.restype = C.c_int
+        L.szxo_decompress_nb.argtypes = [vp, _sz, _sz, C.c_int, vp, C.c_int]
         L.szxo_fxr_error_bound.restype = C.c_float
         L.szxo_fxr_error_bound.argtypes = [vp, C.c_int, _sz, C.c_float, C.c_float, C.c_int,
                                            C.POINTER(C.c_int)]
@@ -150,6 +156,25 @@ def decompress_ra(stream: bytes, n: int, dtype, fast_mode: int = 3,
     out = np.empty(n, dtype)
     rc = lib().szxo_decompress_ra(_ptr(s), s.size, n, dtype.itemsize, fast_mode, _ptr(out),
                                   int(ref_quirk))
+    if rc != 0:
+        raise ValueError(f"malformed stream (oracle rc={rc})")
+    return out
+
+
+def compress_nb(x: np.ndarray, eb: float) -> bytes:
+    """Non-blocked stream (fastMode 1): one record against the global median."""
+    x = np.ascontiguousarray(x).ravel()
+    w = _width(x)
+    out = np.empty(lib().szxo_compress_nb_bound(x.size, w), np.uint8)
+    n = lib().szxo_compress_nb(_ptr(x), x.size, float(np.float32(eb)), w, _ptr(out))
+    return out[:n].tobytes()
+
+
+def decompress_nb(stream: bytes, n: int, dtype, ref_quirk: bool = True) -> np.ndarray:
+    dtype = np.dtype(dtype)
+    s = np.frombuffer(stream, np.uint8)
+    out = np.empty(n, dtype)
+    rc = lib().szxo_decompress_nb(_ptr(s), s.size, n, dtype.itemsize, _ptr(out), int(ref_quirk))
     if rc != 0:
         raise ValueError(f"malformed stream (oracle rc={rc})")
     return out

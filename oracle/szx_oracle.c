@@ -518,6 +518,80 @@ int szxo_decompress_ra(const uint8_t *s, size_t len, size_t n, int w, int fast_m
     return 0;
 }
 
+/* ---- non-blocked codec (fastMode 1) -------------------------------------- */
+
+size_t szxo_compress_nb_bound(size_t n, int w)
+{
+    return 17 + ceil_div(n, 4) + n * (size_t)w;
+}
+
+size_t szxo_compress_nb(const void *x, size_t n, float eb, int w, uint8_t *out)
+{
+    float med, rad;
+    if (w == 4) {
+        const float *v = (const float *)x;
+        float lo = v[0], hi = v[0];
+        for (size_t i = 0; i < n; i++) {
+            if (lo > v[i]) lo = v[i];
+            else if (hi < v[i]) hi = v[i];
+        }
+        rad = (hi - lo) / 2;                 /* szx_float.cc:494-495 */
+        med = lo + rad;
+    } else {
+        const double *v = (const double *)x;
+        double lo = v[0], hi = v[0];
+        for (size_t i = 0; i < n; i++) {
+            if (lo > v[i]) lo = v[i];
+            else if (hi < v[i]) hi = v[i];
+        }
+        rad = (float)((hi - lo) / 2);        /* float *radius, szx_double.cc:1153-1154 */
+        med = (float)(lo + (double)rad);
+    }
+    int clamped;
+    int len = szxo_req_length(w, eb, rad, &clamped);
+    int shift = (8 - len % 8) % 8;
+    /* fp32 subtracts the median after the clamp zeroed it; fp64 the one
+     * captured before (szx_float.cc:1516, szx_double.cc:1597,1617) */
+    float hdr = clamped ? 0.0f : med;
+    uint64_t *s = (uint64_t *)malloc((n ? n : 1) * sizeof(uint64_t));
+    for (size_t i = 0; i < n; i++) {
+        if (w == 4) s[i] = f32_bits(((const float *)x)[i] - hdr) >> shift;
+        else s[i] = f64_bits(((const double *)x)[i] - (double)med) >> shift;
+    }
+    /* one record written 12 bytes in: its reqLength byte and median move to
+     * bytes 4..8, the big-endian lead-array size takes bytes 9..16 */
+    size_t rec = encode_words(s, n, w, len, hdr, out + 12);
+    free(s);
+    out[0] = 1; out[1] = 1; out[2] = 1; out[3] = 0;   /* szx_float.cc:1506-1509 */
+    memmove(out + 4, out + 12, 5);
+    put_be_u64(out + 9, ceil_div(n, 4));
+    return 12 + rec;
+}
+
+int szxo_decompress_nb(const uint8_t *s, size_t len, size_t n, int w, void *out, int ref_quirk)
+{
+    if (len < 17) return -1;
+    int rlen = s[4];
+    if (rlen < 1 || rlen > 8 * w) return -6;
+    if (get_be_u64(s + 9) != ceil_div(n, 4)) return -2;
+    if (17 + ceil_div(n, 4) > len) return -5;
+    /* the record as fastMode 2 lays it out: reqLength, median, leads, mids */
+    uint8_t *rec = (uint8_t *)malloc(len - 12);
+    memcpy(rec, s + 4, 5);
+    memcpy(rec + 5, s + 17, len - 17);
+    uint64_t *t = (uint64_t *)malloc((n ? n : 1) * sizeof(uint64_t));
+    size_t used = decode_words(rec, n, w, t, ref_quirk, len - 12);
+    int shift = (8 - rlen % 8) % 8;
+    float med = get_be_f32(s + 5);
+    for (size_t j = 0; used && j < n; j++) {
+        if (w == 4) ((float *)out)[j] = bits_f32((uint32_t)(t[j] << shift)) + med;
+        else ((double *)out)[j] = bits_f64(t[j] << shift) + (double)med;
+    }
+    free(t);
+    free(rec);
+    return used ? 0 : -7;
+}
+
 /* ---- value range and REL bound (A1, A2) --------------------------------- */
 
 float szxo_value_range_f32(const float *x, size_t n)

@@ -100,6 +100,18 @@ size_t szxo_compress_ra(const void *x, size_t n, float eb, int bs, int w, int fa
 int szxo_decompress_ra(const uint8_t *s, size_t len, size_t n, int w, int fast_mode, void *out,
                        int ref_quirk);
 
+/* Non-blocked codec (fastMode 1): one record over all n values against the
+ * global median -- [1,1,1,0], reqLength byte, big-endian f32 median,
+ * big-endian u64 lead-code bytes, lead codes, mid bytes
+ * (szx_float.cc:1492-1699, szx_double.cc:1593-2010).  The median and radius
+ * come from the value range (computeValueRange_*, szx_float.cc:478-499,
+ * szx_double.cc:1137-1158: for fp64 the radius is rounded to float before the
+ * median is formed).  Decoder: szxd_float.cc:478-662, szxd_double.cc:683-1000
+ * (ref_quirk as for fastMode 2).  out must hold szxo_compress_nb_bound(). */
+size_t szxo_compress_nb_bound(size_t n, int w);
+size_t szxo_compress_nb(const void *x, size_t n, float eb, int w, uint8_t *out);
+int szxo_decompress_nb(const uint8_t *s, size_t len, size_t n, int w, void *out, int ref_quirk);
+
 /* Value range and REL -> ABS error bound (szx.cc:249-253, 303-308;
  * szx_float.cc:478-499, szx_double.cc:1137-1158). */
 float szxo_value_range_f32(const float *x, size_t n);

@@ -10,6 +10,7 @@ reference decoder reads its block-size byte 0 as 0 and divides by it).
 
     python tests/golden/make_golden.py        # fastMode-2 vectors
     python tests/golden/make_golden.py ra     # random-access containers (ra*.npz)
+    python tests/golden/make_golden.py nb     # non-blocked streams (nb*.npz)
 """
 import os
 import sys
@@ -98,11 +99,50 @@ def main_ra():
         f.write("\n".join(manifest) + "\n")
 
 
+def nb_cases():
+    """Non-blocked streams (fastMode 1) through SZx_fast_compress_args."""
+    x = sinusoid_field((32, 32, 33), seed=2, n_waves=64, kmax=32, slope=2.5, noise=1e-4,
+                       n_boxes=2, box_frac=0.30, box_value="mean")
+    yield "nb_f32_sin_rel1e-3", x, dict(mode=1, rel=1e-3)
+    yield "nb_f32_sin_abs1e-5", x, dict(mode=0, eb=1e-5)
+    g = grf_field((96, 100), seed=3)
+    yield "nb_f64_grf_rel1e-6", g, dict(mode=1, rel=1e-6)
+    r = np.random.default_rng(7)
+    walk = np.cumsum(r.standard_normal(6000)) * 1e-2
+    for ebexp in (-5, -8, -11):
+        yield f"nb_f64_walk_eb1e{ebexp}", walk, dict(mode=0, eb=float(np.float32(10.0 ** ebexp)))
+
+
+def main_nb():
+    manifest = []
+    for name, x, prm in nb_cases():
+        x = np.ascontiguousarray(x).ravel()
+        eb = prm["eb"] if prm["mode"] == 0 else O.rel_eb(x, prm["rel"])
+        if len(O.compress_nb(x, eb)) > x.size * x.itemsize:
+            print(f"{name}: skipped (stream would exceed the reference's N*W buffer)")
+            continue
+        stream = O.ref_compress_args(x, prm["mode"], abs_eb=prm.get("eb", 0.0), rel=prm.get("rel", 0.0),
+                                     fast_mode=1)
+        dec = O.ref_decompress(stream, x.size, x.dtype, fast_mode=1)
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), x=x,
+                            stream=np.frombuffer(stream, np.uint8), decoded=dec,
+                            eb=np.float32(eb), fast_mode=np.int32(1),
+                            rel=np.float32(prm.get("rel", 0.0)))
+        manifest.append(f"{name}\tn={x.size}\t{x.dtype}\tfastMode=1\teb={eb!r}\tbytes={len(stream)}")
+        print(manifest[-1])
+    with open(os.path.join(HERE, "MANIFEST_NB.txt"), "w") as f:
+        f.write("# produced by make_golden.py nb from the compiled reference codec\n")
+        f.write("\n".join(manifest) + "\n")
+
+
 def main():
     if not O.ref_available():
         raise SystemExit("oracle/_ref/libszx_ref.so is missing: make -C oracle ref")
     if sys.argv[1:] == ["ra"]:
         main_ra()
+        return
+    if sys.argv[1:] == ["nb"]:
+        main_nb()
         return
     manifest = []
     for name, x, prm in cases():

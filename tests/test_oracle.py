@@ -20,6 +20,7 @@ import oracle as O
 
 GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "f*.npz")))   # fastMode 2
 GOLDEN_RA = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "ra*.npz")))  # fastMode 3/4
+GOLDEN_NB = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "nb*.npz")))  # fastMode 1
 needs_ref = pytest.mark.skipif(not O.ref_available(), reason="compiled reference not built")
 
 
@@ -74,6 +75,63 @@ def test_random_access_fastmode4_fp64_statistics_differ():
     x = g3["x"]
     s2 = O.compress(x, 1.0, 128)
     assert int.from_bytes(s2[5:13], "big") == int.from_bytes(s3[12:20], "big")
+
+
+@pytest.mark.parametrize("path", GOLDEN_NB, ids=[os.path.basename(p)[:-4] for p in GOLDEN_NB])
+def test_oracle_reproduces_golden_non_blocked(path):
+    g = load_golden(path)
+    x, eb = g["x"], float(g["eb"])
+    stream = O.compress_nb(x, eb)
+    assert stream == g["stream"].tobytes()
+    dec = O.decompress_nb(stream, x.size, x.dtype, ref_quirk=True)
+    assert np.array_equal(dec.view(np.uint8), g["decoded"].view(np.uint8))
+
+
+def test_non_blocked_layout_known_answer():
+    # x = [1, 3], eb = 0.5: radius 1, median 2, reqLength 10 + 0 - (-1) = 11
+    # (2 bytes, shift 5); s = bits(-1) >> 5 = 0x05FC0000, bits(1) >> 5 =
+    # 0x01FC0000, both lead 0; mid bytes 2..3 of each, ascending
+    x = np.array([1.0, 3.0], np.float32)
+    s = O.compress_nb(x, 0.5)
+    want = bytes([1, 1, 1, 0, 11]) + struct.pack(">f", 2.0) + (1).to_bytes(8, "big") \
+        + bytes([0x00, 0xFC, 0x05, 0xFC, 0x01])
+    assert s == want
+    assert np.array_equal(O.decompress_nb(s, 2, np.float32), x)
+
+
+@needs_ref
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_non_blocked_vs_reference_random(dtype):
+    r = np.random.default_rng(31 if dtype == np.float32 else 32)
+    compared = quirk = 0
+    for trial in range(30):
+        n = int(r.integers(1, 40000))
+        t = np.arange(n)
+        k = trial % 3
+        if k == 0:
+            x = np.sin(t * r.uniform(1e-4, 0.05)) * 10 ** r.uniform(-3, 3) + r.standard_normal(n) * 1e-5
+        elif k == 1:
+            x = np.cumsum(r.standard_normal(n)) * 10 ** r.uniform(-6, 2)
+        else:
+            x = r.standard_normal(n) * 10.0 ** r.uniform(-30, 0)
+        x = x.astype(dtype)
+        if trial % 2 == 0:
+            mode, kw = 0, dict(abs_eb=float(np.float32(np.ptp(x) * 10 ** r.uniform(-9, -1)) + 1e-30))
+            eb = kw["abs_eb"]
+        else:
+            mode, kw = 1, dict(rel=10 ** r.uniform(-7, -1))
+            eb = O.rel_eb(x, kw["rel"])
+        a = O.compress_nb(x, eb)
+        if len(a) > x.size * x.itemsize:
+            continue   # the reference would overflow its N*W output buffer
+        compared += 1
+        assert a == O.ref_compress_args(x, mode, fast_mode=1, **kw), (trial, n)
+        ref_dec = O.ref_decompress(a, n, dtype, fast_mode=1)
+        dec = O.decompress_nb(a, n, dtype, ref_quirk=True)
+        assert np.array_equal(dec.view(np.uint8), ref_dec.view(np.uint8)), trial
+        if dtype == np.float64:
+            quirk += not np.array_equal(dec, O.decompress_nb(a, n, dtype, ref_quirk=False))
+    assert compared > 20
 
 
 @needs_ref
