@@ -204,6 +204,27 @@ int szx_amd_decompress_ra_dev(int fastMode, const void *d_stream, size_t streamB
                               int dataType, int blockSize, void *d_out, void *d_work, size_t workBytes,
                               const uint64_t *d_index, int flags, int32_t *d_status, void *hipStream);
 
+/* Non-blocked codec (fastMode 1, szx_float.cc:1492-1699 /
+ * szx_double.cc:1593-2010 and szxd_float.cc:478-662 / szxd_double.cc:683-1000:
+ * one record over all n values against the global median) on device memory.
+ * Output capacity and workspace (shared by both directions) in bytes, 0 for
+ * unsupported arguments. */
+size_t szx_amd_nb_capacity(size_t n, int dataType);
+size_t szx_amd_nb_workspace(size_t n, int dataType);
+
+/* szx.cc:270-287 (the non-blocked branch of SZx_fast_compress_args) on the
+ * device: ABS or REL.  The stream starts at d_out; d_result receives
+ * {0, stream bytes, bound bits, 0}.  Asynchronous on hipStream. */
+int szx_amd_compress_nb_dev(const void *d_in, size_t n, int dataType, int errBoundMode, float absErrBound,
+                            float relBoundRatio, void *d_out, size_t outCapacity, void *d_work, size_t workBytes,
+                            uint64_t *d_result, void *hipStream);
+
+/* szx.cc:393-394 (SZx_fast_decompress, fastMode 1) on the device: decode a
+ * non-blocked stream of streamBytes bytes into n values; flags and d_status as
+ * for szx_amd_decompress_dev. */
+int szx_amd_decompress_nb_dev(const void *d_stream, size_t streamBytes, size_t n, int dataType, void *d_out,
+                              void *d_work, size_t workBytes, int flags, int32_t *d_status, void *hipStream);
+
 /* Workspace bytes for szx_amd_fxr_bound_dev (0 for unsupported arguments). */
 size_t szx_amd_fxr_workspace(size_t n, int dataType, int blockSize);
 

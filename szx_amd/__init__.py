@@ -40,6 +40,7 @@ EXPORTS = (
     "szx_amd_last_error", "szx_amd_compress_bound", "szx_amd_compress_capacity",
     "szx_amd_compress_workspace", "szx_amd_compress_dev", "szx_amd_decompress_workspace",
     "szx_amd_decompress_dev", "szx_amd_compress_ra_dev", "szx_amd_decompress_ra_dev",
+    "szx_amd_nb_capacity", "szx_amd_nb_workspace", "szx_amd_compress_nb_dev", "szx_amd_decompress_nb_dev",
     "szx_amd_index_words", "szx_amd_timing_begin",
     "szx_amd_timing_end", "szx_amd_fxr_workspace", "szx_amd_fxr_bound_dev",
     "szx_amd_fixed_rate_workspace", "szx_amd_fixed_rate_dev", "szx_amd_rate_stats_bytes",
@@ -124,6 +125,14 @@ def lib() -> C.CDLL:
     L.szx_amd_decompress_workspace.argtypes = [sz, i32, i32, sz]
     L.szx_amd_decompress_dev.restype = i32
     L.szx_amd_decompress_dev.argtypes = [vp, sz, sz, i32, i32, vp, vp, sz, vp, i32, vp, vp]
+    L.szx_amd_nb_capacity.restype = sz
+    L.szx_amd_nb_capacity.argtypes = [sz, i32]
+    L.szx_amd_nb_workspace.restype = sz
+    L.szx_amd_nb_workspace.argtypes = [sz, i32]
+    L.szx_amd_compress_nb_dev.restype = i32
+    L.szx_amd_compress_nb_dev.argtypes = [vp, sz, i32, i32, f32, f32, vp, sz, vp, sz, vp, vp]
+    L.szx_amd_decompress_nb_dev.restype = i32
+    L.szx_amd_decompress_nb_dev.argtypes = [vp, sz, sz, i32, vp, vp, sz, i32, vp, vp]
     L.szx_amd_compress_ra_dev.restype = i32
     L.szx_amd_compress_ra_dev.argtypes = [i32] + list(L.szx_amd_compress_dev.argtypes)
     L.szx_amd_decompress_ra_dev.restype = i32
@@ -390,6 +399,52 @@ class DeviceDecompressor:
             rc = lib().szx_amd_decompress_dev(*args)
         else:
             rc = lib().szx_amd_decompress_ra_dev(self.fast_mode, *args)
+        if rc != 0:
+            raise SZxError(last_error())
+
+    def check(self) -> int:
+        return int(self.status.item())
+
+
+class DeviceNonBlocked:
+    """fastMode 1 on device tensors: one record over the whole array
+    (szx_amd_compress_nb_dev / szx_amd_decompress_nb_dev)."""
+
+    def __init__(self, n: int, dtype, device="cuda"):
+        torch = _torch()
+        self.n, self.dtype = n, np.dtype(dtype)
+        code = _dtype_code(self.dtype)
+        L = lib()
+        self.capacity = L.szx_amd_nb_capacity(n, code)
+        self.wsbytes = L.szx_amd_nb_workspace(n, code)
+        if not self.capacity or not self.wsbytes:
+            raise SZxError("unsupported parameters")
+        self.out = torch.empty(self.capacity, dtype=torch.uint8, device=device)
+        self.ws = torch.empty(self.wsbytes, dtype=torch.uint8, device=device)
+        self.result = torch.zeros(4, dtype=torch.int64, device=device)
+        tdt = torch.float32 if self.dtype == np.float32 else torch.float64
+        self.dec = torch.empty(n, dtype=tdt, device=device)
+        self.status = torch.zeros(1, dtype=torch.int32, device=device)
+
+    def compress(self, x, errBoundMode: int = ABS, absErrBound: float = 0.0, relBoundRatio: float = 0.0,
+                 stream=None) -> None:
+        rc = lib().szx_amd_compress_nb_dev(x.data_ptr(), self.n, _dtype_code(self.dtype), errBoundMode,
+                                           absErrBound, relBoundRatio, self.out.data_ptr(), self.capacity,
+                                           self.ws.data_ptr(), self.wsbytes, self.result.data_ptr(),
+                                           _stream_handle(stream))
+        if rc != 0:
+            raise SZxError(last_error())
+
+    def stream_view(self):
+        """(device uint8 view of the stream, size, eb) -- syncs."""
+        off, size, ebbits, _ = (int(v) for v in self.result.cpu().tolist())
+        eb = float(np.array([ebbits], np.uint32).view(np.float32)[0])
+        return self.out[off:off + size], size, eb
+
+    def decompress(self, stream_tensor, nbytes: int, flags: int = DECODE_REFERENCE, stream=None) -> None:
+        rc = lib().szx_amd_decompress_nb_dev(stream_tensor.data_ptr(), nbytes, self.n, _dtype_code(self.dtype),
+                                             self.dec.data_ptr(), self.ws.data_ptr(), self.wsbytes, flags,
+                                             self.status.data_ptr(), _stream_handle(stream))
         if rc != 0:
             raise SZxError(last_error())
 

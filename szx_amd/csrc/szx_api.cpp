@@ -291,6 +291,76 @@ void *host_decompress(const unsigned char *bytes, size_t len, size_t n, int w, i
     return out;
 }
 
+// fastMode 1 (one record over the whole array) through the device path.
+unsigned char *host_compress_nb(const void *data, size_t n, int w, int rel_mode, float abs_eb, float rel,
+                                size_t *outSize)
+{
+    if (outSize) *outSize = 0;
+    if (!data || n == 0) {
+        set_err("empty input");
+        return nullptr;
+    }
+    HostCtx &c = g_ctx;
+    if (!c.ready()) return nullptr;
+    if (!c.in.reserve(n * (size_t)w) || !c.out.reserve(nb_capacity(n, w)) || !c.ws.reserve(nb_workspace(n)) ||
+        !c.res.reserve(64))
+        return nullptr;
+    if (!hip_ok(hipMemcpyAsync(c.in.p, data, n * (size_t)w, hipMemcpyHostToDevice, c.stream), "H2D"))
+        return nullptr;
+    nb_compress_launch(c.in.p, n, w, rel_mode, abs_eb, rel, (uint8_t *)c.out.p, (uint8_t *)c.ws.p,
+                       (uint64_t *)c.res.p, c.stream);
+    uint64_t res[4] = {0, 0, 0, 0};
+    if (!hip_ok(hipMemcpyAsync(res, c.res.p, sizeof res, hipMemcpyDeviceToHost, c.stream), "D2H") ||
+        !hip_ok(hipStreamSynchronize(c.stream), "compress"))
+        return nullptr;
+    unsigned char *out = (unsigned char *)malloc(res[1] ? res[1] : 1);
+    if (!out) {
+        set_err("malloc failed");
+        return nullptr;
+    }
+    if (!hip_ok(hipMemcpy(out, (uint8_t *)c.out.p + res[0], res[1], hipMemcpyDeviceToHost), "D2H stream")) {
+        free(out);
+        return nullptr;
+    }
+    if (outSize) *outSize = res[1];
+    return out;
+}
+
+void *host_decompress_nb(const unsigned char *bytes, size_t len, size_t n, int w)
+{
+    if (!bytes || n == 0) {
+        set_err("empty input");
+        return nullptr;
+    }
+    HostCtx &c = g_ctx;
+    if (!c.ready()) return nullptr;
+    if (!c.in.reserve(len ? len : 1) || !c.out.reserve(n * (size_t)w) || !c.ws.reserve(nb_workspace(n)) ||
+        !c.res.reserve(64))
+        return nullptr;
+    if (!hip_ok(hipMemcpyAsync(c.in.p, bytes, len, hipMemcpyHostToDevice, c.stream), "H2D"))
+        return nullptr;
+    nb_decompress_launch((const uint8_t *)c.in.p, len, n, w, c.out.p, (uint8_t *)c.ws.p, 0, (int32_t *)c.res.p,
+                         c.stream);
+    int32_t status = 0;
+    if (!hip_ok(hipMemcpyAsync(&status, c.res.p, 4, hipMemcpyDeviceToHost, c.stream), "D2H") ||
+        !hip_ok(hipStreamSynchronize(c.stream), "decompress"))
+        return nullptr;
+    if (status != 0) {
+        set_err("malformed stream (code %d)", status);
+        return nullptr;
+    }
+    void *out = malloc(n * (size_t)w);
+    if (!out) {
+        set_err("malloc failed");
+        return nullptr;
+    }
+    if (!hip_ok(hipMemcpy(out, c.out.p, n * (size_t)w, hipMemcpyDeviceToHost), "D2H output")) {
+        free(out);
+        return nullptr;
+    }
+    return out;
+}
+
 }  // namespace
 
 namespace szxamd {
@@ -381,16 +451,15 @@ unsigned char *SZx_fast_compress_args(int fastMode, int dataType, void *data, si
         return host_compress(data, n, w, 0, 0.0f, 0.0f, 128, outSize, &req);
     }
     // szx.cc:245-268: fastMode 2 is the blocked stream, 3 and 4 the
-    // random-access container (3: serial layout, 4: the OpenMP one)
-    if (fastMode != SZx_WITH_BLOCK_FAST_CMPR && fastMode != SZx_RANDOMACCESS_FAST_CMPR &&
-        fastMode != SZx_OPENMP_FAST_CMPR) {
-        set_err("fastMode %d is not provided (fastMode 2, 3 and 4 are)", fastMode);
-        return nullptr;
-    }
+    // random-access container (3: serial layout, 4: the OpenMP one); any other
+    // fastMode takes the non-blocked codec (szx.cc:270-287)
     if (errBoundMode != ABS && errBoundMode != REL) {
         set_err("errBoundMode %d is not provided", errBoundMode);
         return nullptr;
     }
+    if (fastMode != SZx_WITH_BLOCK_FAST_CMPR && fastMode != SZx_RANDOMACCESS_FAST_CMPR &&
+        fastMode != SZx_OPENMP_FAST_CMPR)
+        return host_compress_nb(data, n, w, errBoundMode == REL, absErrBound, relBoundRatio, outSize);
     return host_compress(data, n, w, errBoundMode == REL, absErrBound, relBoundRatio, 128, outSize,
                          nullptr, fastMode);
 }
@@ -413,10 +482,13 @@ void *SZx_fast_decompress(int fastMode, int dataType, unsigned char *bytes, size
         set_err("unsupported dataType %d", dataType);
         return nullptr;
     }
-    // szx.cc:391-403: fastMode 3 and 4 read the random-access container
+    // szx.cc:391-403: fastMode 1 reads the non-blocked stream, 3 and 4 the
+    // random-access container
+    if (fastMode == SZx_NO_BLOCK_FAST_CMPR)
+        return host_decompress_nb(bytes, byteLength, data_length(r5, r4, r3, r2, r1), w);
     if (fastMode != SZx_WITH_BLOCK_FAST_CMPR && fastMode != SZx_RANDOMACCESS_FAST_CMPR &&
         fastMode != SZx_OPENMP_FAST_CMPR) {
-        set_err("fastMode %d is not provided (fastMode 2, 3 and 4 are)", fastMode);
+        set_err("fastMode %d is not provided (fastMode 1, 2, 3 and 4 are)", fastMode);
         return nullptr;
     }
     return host_decompress(bytes, byteLength, data_length(r5, r4, r3, r2, r1), w, 0, fastMode);
@@ -697,6 +769,49 @@ int szx_amd_decompress_ra_dev(int fastMode, const void *d_stream, size_t streamB
     }
     return decompress_dev(fastMode, d_stream, streamBytes, n, dataType, blockSize, d_out, d_work, workBytes,
                           d_index, flags, d_status, hipStream);
+}
+
+size_t szx_amd_nb_capacity(size_t n, int dataType)
+{
+    const int w = width_of(dataType);
+    return w && n ? nb_capacity(n, w) : 0;
+}
+
+size_t szx_amd_nb_workspace(size_t n, int dataType)
+{
+    const int w = width_of(dataType);
+    return w && n ? nb_workspace(n) : 0;
+}
+
+int szx_amd_compress_nb_dev(const void *d_in, size_t n, int dataType, int errBoundMode, float absErrBound,
+                            float relBoundRatio, void *d_out, size_t outCapacity, void *d_work, size_t workBytes,
+                            uint64_t *d_result, void *hipStream)
+{
+    g_err.clear();
+    const int w = width_of(dataType);
+    if (!w) { set_err("unsupported dataType %d", dataType); return SZx_TERR; }
+    if (errBoundMode != ABS && errBoundMode != REL) { set_err("bad errBoundMode"); return SZx_BERR; }
+    if (!d_in || !d_out || !d_work || !d_result || n == 0) { set_err("NULL argument or n == 0"); return SZx_NSCS; }
+    if (outCapacity < nb_capacity(n, w) || workBytes < nb_workspace(n)) {
+        set_err("output buffer or workspace too small");
+        return SZx_NSCS;
+    }
+    nb_compress_launch(d_in, n, w, errBoundMode == REL, absErrBound, relBoundRatio, (uint8_t *)d_out,
+                       (uint8_t *)d_work, d_result, (hipStream_t)hipStream);
+    return hip_ok(hipGetLastError(), "launch") ? SZx_SCES : SZx_NSCS;
+}
+
+int szx_amd_decompress_nb_dev(const void *d_stream, size_t streamBytes, size_t n, int dataType, void *d_out,
+                              void *d_work, size_t workBytes, int flags, int32_t *d_status, void *hipStream)
+{
+    g_err.clear();
+    const int w = width_of(dataType);
+    if (!w) { set_err("unsupported dataType %d", dataType); return SZx_TERR; }
+    if (!d_stream || !d_out || !d_work || !d_status || n == 0) { set_err("NULL argument or n == 0"); return SZx_NSCS; }
+    if (workBytes < nb_workspace(n)) { set_err("workspace too small"); return SZx_NSCS; }
+    nb_decompress_launch((const uint8_t *)d_stream, streamBytes, n, w, d_out, (uint8_t *)d_work,
+                         flags & SZX_AMD_DECODE_EXACT, d_status, (hipStream_t)hipStream);
+    return hip_ok(hipGetLastError(), "launch") ? SZx_SCES : SZx_NSCS;
 }
 
 size_t szx_amd_fxr_workspace(size_t n, int dataType, int blockSize)

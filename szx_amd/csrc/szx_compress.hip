@@ -995,6 +995,24 @@ static int dispatch_bs(int bs, const CompressParams &p, uint64_t ntiles, hipStre
 
 uint64_t range_workspace_bytes() { return 2 * kRangeParts * sizeof(double) + 256; }
 
+void range_bound_launch(const void *d_in, uint64_t n, int w, float rel, uint8_t *d_ws, float *eb_out,
+                        void *d_minmax, hipStream_t st)
+{
+    if (w == 4) {
+        float *part = reinterpret_cast<float *>(d_ws);
+        hipLaunchKernelGGL(k_range_partial<float>, dim3(kRangeParts), dim3(256), 0, st,
+                           static_cast<const float *>(d_in), n, part);
+        hipLaunchKernelGGL(k_range_final<float>, dim3(1), dim3(256), 0, st, static_cast<const float *>(d_in),
+                           part, kRangeParts, rel, eb_out, static_cast<float *>(d_minmax));
+    } else {
+        double *part = reinterpret_cast<double *>(d_ws);
+        hipLaunchKernelGGL(k_range_partial<double>, dim3(kRangeParts), dim3(256), 0, st,
+                           static_cast<const double *>(d_in), n, part);
+        hipLaunchKernelGGL(k_range_final<double>, dim3(1), dim3(256), 0, st, static_cast<const double *>(d_in),
+                           part, kRangeParts, rel, eb_out, static_cast<double *>(d_minmax));
+    }
+}
+
 void range_launch(const void *d_in, uint64_t n, int w, uint8_t *d_ws, void *d_minmax, hipStream_t st)
 {
     if (w == 4) {

@@ -80,6 +80,18 @@ int compress_launch(const void *d_in, uint64_t n, int w, int rel_mode, float abs
 // {min, max} as T; d_ws holds range_workspace_bytes().
 uint64_t range_workspace_bytes();
 void range_launch(const void *d_in, uint64_t n, int w, uint8_t *d_ws, void *d_minmax, hipStream_t st);
+// The same pass with the REL bound fl(range * rel) in *eb_out as well.
+void range_bound_launch(const void *d_in, uint64_t n, int w, float rel, uint8_t *d_ws, float *eb_out,
+                        void *d_minmax, hipStream_t st);
+
+// Non-blocked codec (fastMode 1, szx_nonblocked.hip): one record over all
+// values against the global median.
+uint64_t nb_capacity(uint64_t n, int w);
+uint64_t nb_workspace(uint64_t n);
+void nb_compress_launch(const void *d_in, uint64_t n, int w, int rel_mode, float abs_eb, float rel, uint8_t *d_out,
+                        uint8_t *d_ws, uint64_t *d_result, hipStream_t st);
+void nb_decompress_launch(const uint8_t *d_stream, uint64_t len, uint64_t n, int w, void *d_out, uint8_t *d_ws,
+                          int exact, int32_t *d_status, hipStream_t st);
 
 // Checked header of a stream (k_check_header): block size, blocks, bitmap
 // bytes, constant blocks, first payload byte.
