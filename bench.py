@@ -23,7 +23,8 @@ reference's sampled-model bound judged by its real stream beside them; an fp64
 64/128/256 (throughput and full-size parity per configuration); the 512^3
 field at block sizes outside the tiled kernels (default 100, 255, 7: the
 per-block segment kernels, throughput and full-size parity); the random-access
-container (fastMode 3 and 4) on the same field; and 1000 mixed
+container (fastMode 3 and 4) and the non-blocked codec (fastMode 1) on the
+same field; and 1000 mixed
 fp32/fp64 fields compressed and decompressed by the batched launch, against
 one launch sequence per field.
 
@@ -363,6 +364,52 @@ def random_access_eval(x, xh, rel, dev, steps, parity=True):
     return out
 
 
+def non_blocked_eval(x, xh, rel, dev, steps, parity=True):
+    """The non-blocked codec (fastMode 1) on the same fp32 field at the same
+    REL bound: compress and decompress (whole calls), kernel time, and with
+    parity the stream and the decode against the oracle."""
+    import numpy as np
+    import torch
+
+    import szx_amd as Z
+
+    n = x.numel()
+    nbytes = n * 4
+    c = Z.DeviceNonBlocked(n, np.float32, device=dev)
+    c.compress(x, Z.REL, relBoundRatio=rel)
+    view, csize, eb = c.stream_view()
+    c.decompress(view, csize)
+    torch.cuda.synchronize()
+    Z.timing_begin()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        c.compress(x, Z.REL, relBoundRatio=rel)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        c.decompress(view, csize)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    kt = Z.timing_end()
+    assert c.check() == 0, "decoder rejected the stream"
+    rec = {"compression_ratio": nbytes / csize, "compress_gbs": nbytes * steps / (t1 - t0) / 1e9,
+           "decompress_gbs": nbytes * steps / (t2 - t1) / 1e9,
+           "compress_kernels_ms": kt["compress"][0] / max(1, kt["compress"][1]),
+           "decode_kernels_ms": kt["decode"][0] / max(1, kt["decode"][1])}
+    if parity:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        s_gpu = view.cpu().numpy().tobytes()
+        s_ora = O.compress_nb(xh, eb)
+        rec["stream_identical"] = s_gpu == s_ora
+        d_ora = O.decompress_nb(s_ora, n, np.float32)
+        rec["decode_identical"] = bool(np.array_equal(c.dec.cpu().numpy().view(np.uint32), d_ora.view(np.uint32)))
+    log(f"non-blocked fastMode 1: {rec}")
+    del c, view
+    torch.cuda.empty_cache()
+    return rec
+
+
 def _extents(r, n, nd):
     """Random 1-D/2-D/3-D extents whose product is close to n (at most n)."""
     import numpy as np
@@ -535,6 +582,8 @@ def main():
     ap.add_argument("--batch-fields", type=int, default=1000,
                     help="fields of the batched-launch evaluation (0 skips it)")
     ap.add_argument("--batch-reps", type=int, default=5)
+    ap.add_argument("--no-nb", action="store_true",
+                    help="skip the non-blocked codec (fastMode 1) leg")
     ap.add_argument("--no-ra", action="store_true",
                     help="skip the random-access container (fastMode 3/4) leg")
     ap.add_argument("--generic-bs", default="100,255,7",
@@ -715,6 +764,12 @@ def main():
         ra = random_access_eval(x, xh, args.rel, dev, max(2, args.steps // 4),
                                 parity=not args.no_parity and xh is not None)
 
+    nb = None
+    if not args.no_nb:
+        log("non-blocked codec (fastMode 1)")
+        nb = non_blocked_eval(x, xh, args.rel, dev, max(2, args.steps // 4),
+                              parity=not args.no_parity and xh is not None)
+
     generic = None
     gsizes = [int(v) for v in args.generic_bs.split(",") if v.strip()]
     if gsizes:
@@ -776,6 +831,7 @@ def main():
         "batched": batched,
         "generic_block_sizes": generic,
         "random_access": ra,
+        "non_blocked": nb,
         "host_api_pcie_inclusive": host_api,
     }
     print(json.dumps(line), flush=True)

@@ -1,14 +1,15 @@
 /*
  * szx_amd.h -- C ABI of libszx_amd.so, an MI355X (gfx950) implementation of
- * SZx's blockwise fast codec (fastMode = SZx_WITH_BLOCK_FAST_CMPR).
+ * SZx's fast codecs: the blockwise one (fastMode = SZx_WITH_BLOCK_FAST_CMPR),
+ * its random-access container (fastMode 3 / 4) and the non-blocked one
+ * (fastMode 1).
  *
  * Two layers:
  *
  *  1. Drop-in host API.  Same names, argument meaning and ownership as the
  *     reference's szx.h; a caller relinks against libszx_amd.so unchanged.
  *     Host pointers in, malloc'd host buffers out (release with free()).
- *     fastMode 2 (and 3/4, which the reference routes to the same blocked
- *     compressor family) runs on the GPU.  Failures return NULL / 0 and set
+ *     Every fastMode (1, 2, 3, 4) runs on the GPU.  Failures return NULL / 0 and set
  *     szx_amd_last_error() instead of exiting the process.
  *
  *  2. Device-resident API (szx_amd_*_dev).  Device pointers, a caller
@@ -16,9 +17,9 @@
  *     the host, and every call is asynchronous on `stream`.
  *
  * Streams are byte-identical to the reference CPU codec at the same
- * parameters (szx_float.cc:1059-1139, szx_double.cc:1160-1238) and decode
- * bit-identically to the reference decoder (szxd_float.cc:247-311,
- * szxd_double.cc:452-516).
+ * parameters (szx_float.cc:1059-1139, szx_double.cc:1160-1238; the other
+ * modes cited at their entry points) and decode bit-identically to the
+ * reference decoder (szxd_float.cc:247-311, szxd_double.cc:452-516).
  */
 #ifndef SZX_AMD_H
 #define SZX_AMD_H
