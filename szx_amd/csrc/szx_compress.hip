@@ -700,8 +700,21 @@ __global__ __launch_bounds__(256) void k_range_partial(const T *x, uint64_t n, T
 {
     T mn = pos_inf<T>(), mx = -pos_inf<T>();
     const uint64_t nq = n / 4;
-    for (uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; q < nq;
-         q += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    for (; q + 3 * stride < nq; q += 4 * stride) {
+        T v[4][4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) SlabLoad<T>::load4(x, 4 * (q + u * stride), n, v[u]);
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                mn = v[u][k] < mn ? v[u][k] : mn;
+                mx = v[u][k] > mx ? v[u][k] : mx;
+            }
+    }
+    for (; q < nq; q += stride) {
         T v[4];
         SlabLoad<T>::load4(x, 4 * q, n, v);
 #pragma unroll
