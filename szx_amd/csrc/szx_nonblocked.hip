@@ -11,15 +11,16 @@
 //
 // Every value's lead code depends only on its predecessor, so the stream is
 // built in tiles of 4096 values (16 consecutive values per lane):
-//   compress:   k_nb_setup (median, reqLength), k_nb_count (lead codes
-//               straight into the stream, mid-byte count per tile), a scan
-//               over the tiles, k_nb_write (mid bytes at their offsets),
-//               k_nb_finish (header, sizes)
+//   compress:   k_nb_setup (median, reqLength), k_nb_pass<T, false> (lead
+//               codes straight into the stream, mid-byte count per tile), a
+//               scan over the tiles, k_nb_pass<T, true> (mid bytes at their
+//               offsets), k_nb_finish (header, sizes)
 //   decompress: k_nb_header, k_nb_dcount (mid-byte count per tile from the
 //               lead codes), a scan, k_nb_carry (per tile: the last bytes it
 //               writes at the three positions a later value can inherit),
 //               k_nb_carry_scan (the bytes every tile inherits, last writer
-//               wins), k_nb_decode.
+//               wins), k_nb_decode.  The decoder stages each tile's mid bytes
+//               (contiguous in the stream) in LDS with aligned word loads.
 // A value inherits its first min(lead, rb) bytes from the previous value
 // (szxd_float.cc:78-199), possibly across many tiles; the carry scan resolves
 // that chain with the same last-writer operator the blocked decoders use.
@@ -149,8 +150,9 @@ __device__ inline typename Word<T>::U nb_word(T v, const NbParams &P)
     return to_bits<T>(v - (T)P.msub) >> P.shift;
 }
 
-// Lead codes of the lane's 16 values and their mid-byte counts; WRITE_LEADS
-// stores the lead-code bytes (4 per lane, at their fixed place).
+// Lead codes of the lane's 16 values and their mid-byte counts.  WRITE =
+// false stores the lead-code bytes (4 per lane, at their fixed place) and the
+// tile's mid-byte count; WRITE = true stores the mid bytes at their offsets.
 template <typename T, bool WRITE>
 __global__ __launch_bounds__(256) void k_nb_pass(const T *x, uint64_t n, const NbParams *Pp, uint8_t *out,
                                                  uint32_t *tsum, const uint64_t *toff)
@@ -298,8 +300,8 @@ __global__ void k_nb_check(uint64_t len, uint64_t n, NbParams *P, int32_t *statu
 // The lane's words from the stream alone (inherited bytes still zero),
 // masked to the low 8W - shift bits, and its last-writer record.
 template <typename T>
-__device__ inline uint32_t nb_words(const uint8_t *s, uint64_t n, uint64_t e0, const NbParams &P, uint64_t q,
-                                    int exact, typename Word<T>::U (&wv)[NB_PER], int (&lk)[NB_PER])
+__device__ inline uint32_t nb_words(const uint8_t *s, const uint8_t *mids, uint64_t n, uint64_t e0, const NbParams &P,
+                                    uint32_t q, int exact, typename Word<T>::U (&wv)[NB_PER], int (&lk)[NB_PER])
 {
     using U = typename Word<T>::U;
     constexpr int W = Word<T>::W;
@@ -312,9 +314,9 @@ __device__ inline uint32_t nb_words(const uint8_t *s, uint64_t n, uint64_t e0, c
             const int ld = nb_code(s, e0 + k);
             lk[k] = ld < P.rb ? ld : P.rb;
             for (int t = W - P.rb; t < W - lk[k]; t++) {
-                uint64_t src = (uint64_t)(t - (W - P.rb));
+                uint32_t src = (uint32_t)(t - (W - P.rb));
                 if (W == 8 && !exact && P.rb == 5 && lk[k] == 0 && t == 7) src = 3;   // szxd_double.cc:906
-                cur |= (U)s[q + src] << (8 * t);
+                cur |= (U)mids[q + src] << (8 * t);
             }
             q += P.rb - lk[k];
             cur &= mask;
@@ -328,11 +330,34 @@ __device__ inline uint32_t nb_words(const uint8_t *s, uint64_t n, uint64_t e0, c
     return mine;
 }
 
+// Stage the tile's mid bytes [src, src + cnt) of the stream (ending at end)
+// into LDS words at byte offset (src & 3); returns that offset.
+__device__ inline uint32_t nb_stage(uint32_t *bw, const uint8_t *src, uint32_t cnt, const uint8_t *end)
+{
+    const uint32_t head = (uint32_t)((uintptr_t)src & 3);
+    const uint32_t *sw = reinterpret_cast<const uint32_t *>(src - head);
+    const uint32_t nw = (head + cnt + 3) / 4;
+    for (uint32_t i = threadIdx.x; i < nw; i += 256) {
+        const uint8_t *a = reinterpret_cast<const uint8_t *>(sw + i);
+        if (a + 4 <= end) {
+            bw[i] = sw[i];
+        } else {
+            uint32_t v = 0;
+            for (int j = 0; j < 4; j++)
+                if (a + j < end) v |= (uint32_t)a[j] << (8 * j);
+            bw[i] = v;
+        }
+    }
+    __syncthreads();
+    return head;
+}
+
 template <typename T>
-__global__ __launch_bounds__(256) void k_nb_carry(const uint8_t *s, uint64_t n, const NbParams *Pp,
+__global__ __launch_bounds__(256) void k_nb_carry(const uint8_t *s, uint64_t len, uint64_t n, const NbParams *Pp,
                                                   const uint64_t *toff, uint32_t *tcar, int exact)
 {
     using U = typename Word<T>::U;
+    __shared__ uint32_t bw[(NB_TILE * sizeof(T)) / 4 + 2];
     const NbParams P = *Pp;
     if (!P.ok) return;
     const uint64_t e0 = blockIdx.x * (uint64_t)NB_TILE + threadIdx.x * (uint64_t)NB_PER;
@@ -344,9 +369,10 @@ __global__ __launch_bounds__(256) void k_nb_carry(const uint8_t *s, uint64_t n, 
     }
     uint32_t total;
     const uint32_t ex = nb_block_excl(cnt, &total);
+    const uint32_t head = nb_stage(bw, s + NB_HDR + (n + 3) / 4 + toff[blockIdx.x], total, s + len);
     U wv[NB_PER];
     int lk[NB_PER];
-    const uint32_t mine = nb_words<T>(s, n, e0, P, NB_HDR + (n + 3) / 4 + toff[blockIdx.x] + ex, exact, wv, lk);
+    const uint32_t mine = nb_words<T>(s, reinterpret_cast<const uint8_t *>(bw), n, e0, P, head + ex, exact, wv, lk);
     uint32_t tot;
     nb_block_excl_lw(mine, &tot);
     if (threadIdx.x == 0) tcar[blockIdx.x] = tot;
@@ -378,11 +404,12 @@ __global__ __launch_bounds__(1024) void k_nb_carry_scan(uint32_t *tcar, uint64_t
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_nb_decode(const uint8_t *s, uint64_t n, const NbParams *Pp,
+__global__ __launch_bounds__(256) void k_nb_decode(const uint8_t *s, uint64_t len, uint64_t n, const NbParams *Pp,
                                                    const uint64_t *toff, const uint32_t *tcar, T *out, int exact)
 {
     using U = typename Word<T>::U;
     constexpr int W = Word<T>::W;
+    __shared__ uint32_t bw[(NB_TILE * sizeof(T)) / 4 + 2];
     const NbParams P = *Pp;
     if (!P.ok) return;
     const uint64_t e0 = blockIdx.x * (uint64_t)NB_TILE + threadIdx.x * (uint64_t)NB_PER;
@@ -394,9 +421,10 @@ __global__ __launch_bounds__(256) void k_nb_decode(const uint8_t *s, uint64_t n,
     }
     uint32_t total;
     const uint32_t ex = nb_block_excl(cnt, &total);
+    const uint32_t head = nb_stage(bw, s + NB_HDR + (n + 3) / 4 + toff[blockIdx.x], total, s + len);
     U wv[NB_PER];
     int lk[NB_PER];
-    const uint32_t mine = nb_words<T>(s, n, e0, P, NB_HDR + (n + 3) / 4 + toff[blockIdx.x] + ex, exact, wv, lk);
+    const uint32_t mine = nb_words<T>(s, reinterpret_cast<const uint8_t *>(bw), n, e0, P, head + ex, exact, wv, lk);
     uint32_t tot;
     const uint32_t cin = nb_combine(tcar[blockIdx.x], nb_block_excl_lw(mine, &tot));
     U prev = ((U)(cin & 0xFFu) << (8 * (W - 1))) | ((U)((cin >> 8) & 0xFFu) << (8 * (W - 2))) |
@@ -492,16 +520,16 @@ void nb_decompress_launch(const uint8_t *d_stream, uint64_t len, uint64_t n, int
     hipLaunchKernelGGL(k_nb_scan, dim3(1), dim3(1024), 0, st, S.tsum, t, S.toff, S.P);
     hipLaunchKernelGGL(k_nb_check, dim3(1), dim3(64), 0, st, len, n, S.P, d_status);
     if (w == 4) {
-        hipLaunchKernelGGL(k_nb_carry<float>, dim3((unsigned)t), dim3(256), 0, st, d_stream, n, S.P, S.toff, S.tcar,
-                           exact);
+        hipLaunchKernelGGL(k_nb_carry<float>, dim3((unsigned)t), dim3(256), 0, st, d_stream, len, n, S.P, S.toff,
+                           S.tcar, exact);
         hipLaunchKernelGGL(k_nb_carry_scan, dim3(1), dim3(1024), 0, st, S.tcar, t, S.P);
-        hipLaunchKernelGGL(k_nb_decode<float>, dim3((unsigned)t), dim3(256), 0, st, d_stream, n, S.P, S.toff, S.tcar,
-                           static_cast<float *>(d_out), exact);
+        hipLaunchKernelGGL(k_nb_decode<float>, dim3((unsigned)t), dim3(256), 0, st, d_stream, len, n, S.P, S.toff,
+                           S.tcar, static_cast<float *>(d_out), exact);
     } else {
-        hipLaunchKernelGGL(k_nb_carry<double>, dim3((unsigned)t), dim3(256), 0, st, d_stream, n, S.P, S.toff, S.tcar,
-                           exact);
+        hipLaunchKernelGGL(k_nb_carry<double>, dim3((unsigned)t), dim3(256), 0, st, d_stream, len, n, S.P, S.toff,
+                           S.tcar, exact);
         hipLaunchKernelGGL(k_nb_carry_scan, dim3(1), dim3(1024), 0, st, S.tcar, t, S.P);
-        hipLaunchKernelGGL(k_nb_decode<double>, dim3((unsigned)t), dim3(256), 0, st, d_stream, n, S.P, S.toff,
+        hipLaunchKernelGGL(k_nb_decode<double>, dim3((unsigned)t), dim3(256), 0, st, d_stream, len, n, S.P, S.toff,
                            S.tcar, static_cast<double *>(d_out), exact);
     }
     timing_mark_end(KK_DECODE, st);
