@@ -76,7 +76,20 @@ def cpu_baseline(x_host, eb, sample_values):
         t4 = best(lambda: O.ref_compress_args(xs, mode=0, abs_eb=eb, fast_mode=4))
         out["serial"] = {"compress_gbs": nbytes / t / 1e9, "decompress_gbs": nbytes / td / 1e9,
                          "cores": 1}
-        out["openmp"] = {"compress_gbs": nbytes / t4 / 1e9, "cores": threads}
+        s4 = O.ref_compress_args(xs, mode=0, abs_eb=eb, fast_mode=4)
+        td4 = best(lambda: O.ref_decompress(s4, xs.size, np.float32, fast_mode=4))
+        out["openmp"] = {"compress_gbs": nbytes / t4 / 1e9, "decompress_gbs": nbytes / td4 / 1e9,
+                         "cores": threads}
+        # the other modes on the same sample, one core each
+        others = {}
+        for name, fm in (("fastmode1_serial", 1), ("fastmode3_serial", 3)):
+            tc = best(lambda: O.ref_compress_args(xs, mode=0, abs_eb=eb, fast_mode=fm))
+            sm = O.ref_compress_args(xs, mode=0, abs_eb=eb, fast_mode=fm)
+            tdm = best(lambda: O.ref_decompress(sm, xs.size, np.float32, fast_mode=fm))
+            others[name] = {"compress_gbs": nbytes / tc / 1e9, "decompress_gbs": nbytes / tdm / 1e9}
+        tb = best(lambda: O.ref_compress(xs, eb, 100))
+        others["blocksize100_serial"] = {"compress_gbs": nbytes / tb / 1e9}
+        out["other_modes"] = others
     else:
         kind = "port"
         t = best(lambda: O.compress(xs, eb, 128))
@@ -729,7 +742,8 @@ def main():
                          f"ABS eb {eb:.6g}, blockSize 128, best of 3 after 1 warm-up; "
                          + ("reference fastMode-4 OpenMP compress" if "openmp" in res
                             else "oracle serial compress"),
-               "serial_fastmode2": res["serial"]}
+               "serial_fastmode2": res["serial"], "openmp_fastmode4": res.get("openmp"),
+               "other_modes": res.get("other_modes")}
 
     host_api = None
     if xh is not None:
