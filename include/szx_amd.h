@@ -226,6 +226,51 @@ int szx_amd_compress_nb_dev(const void *d_in, size_t n, int dataType, int errBou
 int szx_amd_decompress_nb_dev(const void *d_stream, size_t streamBytes, size_t n, int dataType, void *d_out,
                               void *d_work, size_t workBytes, int flags, int32_t *d_status, void *hipStream);
 
+/* Reader of the stream the reference's CUDA codec (cuSZx) writes, fp32.
+ * cuszx_entry.h:11 / cuszx_entry.cu:129-209: decode nbEle values from
+ * cmpBytes into a malloc'd array (*newData; free with free()).  The block size
+ * comes from the stream and must be a multiple of 32 in 32..224 (the sizes the
+ * reference's kernel decodes whole).  The reference takes no stream length;
+ * it is found, as the reference's host loop finds it, from the header and the
+ * per-record counts.  On error *newData is NULL and szx_amd_last_error() says
+ * why; where the reference prints a warning and exits (a record longer than
+ * 4 blockSize bytes) this returns that error. */
+void cuSZx_fast_decompress_args_unpredictable_blocked_float(float **newData, size_t nbEle,
+                                                            unsigned char *cmpBytes);
+
+/* Writer of the same stream, fp32 (cuszx_entry.h:10 / cuszx_entry.cu:72-127
+ * with the kernel cuszx_float.cu:222-298): compress nbEle values with the
+ * absolute bound absErrBound and blockSize values a block (a multiple of 32 in
+ * 32..224; the reference's block reduction covers the whole block for 32, 64
+ * and 128, and for the other sizes this takes the whole block's range).
+ * Returns a malloc'd stream of *outSize bytes (free with free()), or NULL
+ * with szx_amd_last_error() set, also when a block would need more than 32
+ * significant bits (the reference then writes mid bytes it never filled). */
+unsigned char *cuSZx_fast_compress_args_unpredictable_blocked_float(float *oriData, size_t *outSize,
+                                                                     float absErrBound, size_t nbEle,
+                                                                     int blockSize);
+
+/* The writer on device memory: d_out holds szx_amd_cuszx_capacity() bytes and
+ * receives the stream at offset 0; d_work holds
+ * szx_amd_cuszx_compress_workspace() bytes; d_result (4 x uint64) receives
+ * {0, stream bytes, bound bits, 1 if a block needs more than 32 significant
+ * bits (no valid stream) else 0}.  Asynchronous on hipStream. */
+size_t szx_amd_cuszx_capacity(size_t nbEle, int blockSize);
+size_t szx_amd_cuszx_compress_workspace(size_t nbEle, int blockSize);
+int szx_amd_compress_cuszx_dev(const float *d_in, size_t nbEle, float absErrBound, int blockSize, void *d_out,
+                               size_t outCapacity, void *d_work, size_t workBytes, uint64_t *d_result,
+                               void *hipStream);
+
+/* The same decode on device memory: d_stream holds streamBytes bytes,
+ * d_out nbEle floats; d_work holds szx_amd_cuszx_workspace(nbEle) bytes
+ * (256-byte aligned).  *d_status is 0 after the stream completes, or a
+ * negative code for a malformed stream (-41 header, -42 record length, -43
+ * bitmap, -44 truncated, -45 a mid-byte position outside its block).
+ * Asynchronous on hipStream; allocates nothing. */
+size_t szx_amd_cuszx_workspace(size_t nbEle);
+int szx_amd_decompress_cuszx_dev(const void *d_stream, size_t streamBytes, size_t nbEle, float *d_out,
+                                 void *d_work, size_t workBytes, int32_t *d_status, void *hipStream);
+
 /* Workspace bytes for szx_amd_fxr_bound_dev (0 for unsupported arguments). */
 size_t szx_amd_fxr_workspace(size_t n, int dataType, int blockSize);
 

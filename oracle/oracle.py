@@ -83,6 +83,12 @@ def lib():
         L.szxo_compress_nb.argtypes = [vp, _sz, C.c_float, C.c_int, vp]
         L.szxo_decompress_nb.restype = C.c_int
         L.szxo_decompress_nb.argtypes = [vp, _sz, _sz, C.c_int, vp, C.c_int]
+        L.szxo_cuszx_bound.restype = _sz
+        L.szxo_cuszx_bound.argtypes = [_sz, C.c_int]
+        L.szxo_cuszx_compress.restype = C.c_long
+        L.szxo_cuszx_compress.argtypes = [vp, _sz, C.c_float, C.c_int, vp]
+        L.szxo_cuszx_decompress.restype = C.c_int
+        L.szxo_cuszx_decompress.argtypes = [vp, _sz, _sz, vp]
         L.szxo_fxr_error_bound.restype = C.c_float
         L.szxo_fxr_error_bound.argtypes = [vp, C.c_int, _sz, C.c_float, C.c_float, C.c_int,
                                            C.POINTER(C.c_int)]
@@ -177,6 +183,26 @@ def decompress_nb(stream: bytes, n: int, dtype, ref_quirk: bool = True) -> np.nd
     rc = lib().szxo_decompress_nb(_ptr(s), s.size, n, dtype.itemsize, _ptr(out), int(ref_quirk))
     if rc != 0:
         raise ValueError(f"malformed stream (oracle rc={rc})")
+    return out
+
+
+def compress_cuszx(x: np.ndarray, eb: float, bs: int = 128) -> bytes:
+    """The stream the reference's CUDA codec (cuSZx) writes for fp32 x."""
+    x = np.ascontiguousarray(x, np.float32).ravel()
+    out = np.empty(max(lib().szxo_cuszx_bound(x.size, bs), 1), np.uint8)
+    n = lib().szxo_cuszx_compress(_ptr(x), x.size, float(np.float32(eb)), bs, _ptr(out))
+    if n < 0:
+        raise ValueError(f"cuSZx stream undefined for this input (oracle rc={n})")
+    return out[:n].tobytes()
+
+
+def decompress_cuszx(stream: bytes, n: int) -> np.ndarray:
+    """What the reference's cuSZx decoder reads from a stream (fp32)."""
+    s = np.frombuffer(stream, np.uint8)
+    out = np.empty(n, np.float32)
+    rc = lib().szxo_cuszx_decompress(_ptr(s), s.size, n, _ptr(out))
+    if rc != 0:
+        raise ValueError(f"malformed cuSZx stream (oracle rc={rc})")
     return out
 
 

@@ -112,6 +112,15 @@ size_t szxo_compress_nb_bound(size_t n, int w);
 size_t szxo_compress_nb(const void *x, size_t n, float eb, int w, uint8_t *out);
 int szxo_decompress_nb(const uint8_t *s, size_t len, size_t n, int w, void *out, int ref_quirk);
 
+/* The stream of the reference's CUDA codec, cuSZx (fp32;
+ * szx_oracle_cuszx.c): compress (cuszx_float.cu:222-298 and
+ * cuszx_entry.cu:20-70; bs a multiple of 32 in 32..224; returns the size or <0) and
+ * decompress (cuszxd_float.cu:161-294 and cuszx_entry.cu:129-209; returns 0
+ * or <0 on a stream the reference has no defined result for). */
+size_t szxo_cuszx_bound(size_t n, int bs);
+long szxo_cuszx_compress(const float *x, size_t n, float eb, int bs, uint8_t *out);
+int szxo_cuszx_decompress(const uint8_t *s, size_t len, size_t n, float *out);
+
 /* Value range and REL -> ABS error bound (szx.cc:249-253, 303-308;
  * szx_float.cc:478-499, szx_double.cc:1137-1158). */
 float szxo_value_range_f32(const float *x, size_t n);

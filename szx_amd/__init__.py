@@ -51,6 +51,9 @@ EXPORTS = (
     "szx_amd_batch_compress", "szx_amd_batch_results_dev", "szx_amd_batch_status_dev",
     "szx_amd_batch_index_dev", "szx_amd_batch_results", "szx_amd_batch_decompress",
     "szx_amd_batch_set_shape", "szx_amd_batch_tune",
+    "cuSZx_fast_decompress_args_unpredictable_blocked_float", "szx_amd_cuszx_workspace",
+    "szx_amd_decompress_cuszx_dev", "cuSZx_fast_compress_args_unpredictable_blocked_float",
+    "szx_amd_cuszx_capacity", "szx_amd_cuszx_compress_workspace", "szx_amd_compress_cuszx_dev",
 )
 BATCH_FUSED, BATCH_PER_FIELD = 0, 1
 OCTAVE_BINS = 512
@@ -133,6 +136,20 @@ def lib() -> C.CDLL:
     L.szx_amd_compress_nb_dev.argtypes = [vp, sz, i32, i32, f32, f32, vp, sz, vp, sz, vp, vp]
     L.szx_amd_decompress_nb_dev.restype = i32
     L.szx_amd_decompress_nb_dev.argtypes = [vp, sz, sz, i32, vp, vp, sz, i32, vp, vp]
+    L.cuSZx_fast_decompress_args_unpredictable_blocked_float.restype = None
+    L.cuSZx_fast_decompress_args_unpredictable_blocked_float.argtypes = [C.POINTER(C.c_void_p), sz, vp]
+    L.cuSZx_fast_compress_args_unpredictable_blocked_float.restype = vp
+    L.cuSZx_fast_compress_args_unpredictable_blocked_float.argtypes = [vp, C.POINTER(sz), f32, sz, i32]
+    L.szx_amd_cuszx_capacity.restype = sz
+    L.szx_amd_cuszx_capacity.argtypes = [sz, i32]
+    L.szx_amd_cuszx_compress_workspace.restype = sz
+    L.szx_amd_cuszx_compress_workspace.argtypes = [sz, i32]
+    L.szx_amd_compress_cuszx_dev.restype = i32
+    L.szx_amd_compress_cuszx_dev.argtypes = [vp, sz, f32, i32, vp, sz, vp, sz, vp, vp]
+    L.szx_amd_cuszx_workspace.restype = sz
+    L.szx_amd_cuszx_workspace.argtypes = [sz]
+    L.szx_amd_decompress_cuszx_dev.restype = i32
+    L.szx_amd_decompress_cuszx_dev.argtypes = [vp, sz, sz, vp, vp, sz, vp, vp]
     L.szx_amd_compress_ra_dev.restype = i32
     L.szx_amd_compress_ra_dev.argtypes = [i32] + list(L.szx_amd_compress_dev.argtypes)
     L.szx_amd_decompress_ra_dev.restype = i32
@@ -292,6 +309,29 @@ def decompress_blocked(stream: bytes, n: int, dtype) -> np.ndarray:
     return np.frombuffer(_take(out.value, n * dtype.itemsize), dtype).copy()
 
 
+def cuszx_compress(data: np.ndarray, absErrBound: float, blockSize: int = 128) -> bytes:
+    """cuszx_entry.h:10 cuSZx_fast_compress_args_unpredictable_blocked_float:
+    the stream the reference's CUDA codec writes (fp32)."""
+    x = np.ascontiguousarray(data, np.float32).ravel()
+    osz = C.c_size_t(0)
+    p = lib().cuSZx_fast_compress_args_unpredictable_blocked_float(x.ctypes.data, C.byref(osz), absErrBound,
+                                                                   x.size, blockSize)
+    if not p:
+        raise SZxError(last_error())
+    return _take(p, osz.value)
+
+
+def cuszx_decompress(stream: bytes, n: int) -> np.ndarray:
+    """cuszx_entry.h:11 cuSZx_fast_decompress_args_unpredictable_blocked_float:
+    read a stream the reference's CUDA codec wrote (fp32, length-free)."""
+    buf = C.create_string_buffer(bytes(stream), len(stream))
+    out = C.c_void_p(0)
+    lib().cuSZx_fast_decompress_args_unpredictable_blocked_float(C.byref(out), n, buf)
+    if not out.value:
+        raise SZxError(last_error())
+    return np.frombuffer(_take(out.value, n * 4), np.float32).copy()
+
+
 def SZx_computeValueRange(data: np.ndarray):
     """szx.h:81: (range, radius, median) of the array."""
     x = np.ascontiguousarray(data).ravel()
@@ -445,6 +485,61 @@ class DeviceNonBlocked:
         rc = lib().szx_amd_decompress_nb_dev(stream_tensor.data_ptr(), nbytes, self.n, _dtype_code(self.dtype),
                                              self.dec.data_ptr(), self.ws.data_ptr(), self.wsbytes, flags,
                                              self.status.data_ptr(), _stream_handle(stream))
+        if rc != 0:
+            raise SZxError(last_error())
+
+    def check(self) -> int:
+        return int(self.status.item())
+
+
+class DeviceCuszxWriter:
+    """Writer of cuSZx streams on device tensors (szx_amd_compress_cuszx_dev);
+    the stream starts at out[0]."""
+
+    def __init__(self, n: int, blockSize: int = 128, device="cuda"):
+        torch = _torch()
+        L = lib()
+        self.n, self.bs = n, blockSize
+        self.capacity = L.szx_amd_cuszx_capacity(n, blockSize)
+        self.wsbytes = L.szx_amd_cuszx_compress_workspace(n, blockSize)
+        if not self.capacity or not self.wsbytes:
+            raise SZxError("unsupported parameters")
+        self.out = torch.empty(self.capacity, dtype=torch.uint8, device=device)
+        self.ws = torch.empty(self.wsbytes, dtype=torch.uint8, device=device)
+        self.result = torch.zeros(4, dtype=torch.int64, device=device)
+
+    def compress(self, x, absErrBound: float, stream=None) -> None:
+        rc = lib().szx_amd_compress_cuszx_dev(x.data_ptr(), self.n, absErrBound, self.bs, self.out.data_ptr(),
+                                              self.capacity, self.ws.data_ptr(), self.wsbytes,
+                                              self.result.data_ptr(), _stream_handle(stream))
+        if rc != 0:
+            raise SZxError(last_error())
+
+    def stream_view(self):
+        """(device uint8 view of the stream, size, refused) -- syncs."""
+        _, size, _, bad = (int(v) for v in self.result.cpu().tolist())
+        return self.out[:size], size, bool(bad)
+
+
+class DeviceCuszxReader:
+    """Reader of cuSZx (the reference's CUDA codec) streams on device tensors
+    (szx_amd_decompress_cuszx_dev)."""
+
+    def __init__(self, n: int, device="cuda"):
+        torch = _torch()
+        self.n = n
+        self.wsbytes = lib().szx_amd_cuszx_workspace(n)
+        if not self.wsbytes:
+            raise SZxError("unsupported parameters")
+        self.ws = torch.empty(self.wsbytes, dtype=torch.uint8, device=device)
+        self.dec = torch.empty(n, dtype=torch.float32, device=device)
+        self.status = torch.zeros(1, dtype=torch.int32, device=device)
+
+    def decompress(self, stream_tensor, nbytes: int, out=None, stream=None) -> None:
+        dst = self.dec if out is None else out
+        rc = lib().szx_amd_decompress_cuszx_dev(stream_tensor.data_ptr(), nbytes, self.n, dst.data_ptr(),
+                                                self.ws.data_ptr(), self.wsbytes, self.status.data_ptr(),
+                                                _stream_handle(stream))
         if rc != 0:
             raise SZxError(last_error())
 

@@ -361,6 +361,41 @@ void *host_decompress_nb(const unsigned char *bytes, size_t len, size_t n, int w
     return out;
 }
 
+// Bytes of a cuSZx stream for n values, from its header and the i16 count
+// of every record (the reference's decoder walks the same fields,
+// cuszx_entry.cu:134-172, and takes no length).
+bool cuszx_stream_length(const unsigned char *s, size_t n, size_t *len)
+{
+    const int bs = s[4];
+    if (bs == 0 || bs % 32 != 0 || bs > 224) {
+        set_err("cuSZx blockSize %d is not a multiple of 32 in 32..224", bs);
+        return false;
+    }
+    const size_t nb = n / (size_t)bs;
+    uint64_t nconst = 0;
+    for (int k = 0; k < 8; k++) nconst = (nconst << 8) | s[5 + k];
+    if (nconst > nb) {
+        set_err("cuSZx stream: %llu constant blocks of %llu", (unsigned long long)nconst,
+                (unsigned long long)nb);
+        return false;
+    }
+    const size_t nc = nb - (size_t)nconst, msize = 5 + (size_t)bs / 4;
+    const size_t q_off = 13 + (nb + 7) / 8 + 4 * (n % (size_t)bs) + 4 * (size_t)nconst;
+    size_t total = q_off + 2 * nc;
+    for (size_t j = 0; j < nc; j++) {
+        const int16_t sh = (int16_t)((uint16_t)s[q_off + 2 * j] | (uint16_t)s[q_off + 2 * j + 1] << 8);
+        const long leng = (long)sh + (long)msize;
+        if (leng < 0 || leng > 4L * bs) {   // cuszx_entry.cu:165-169
+            set_err("cuSZx stream: record %llu is %ld bytes, larger than its block",
+                    (unsigned long long)j, leng);
+            return false;
+        }
+        total += (size_t)leng;
+    }
+    *len = total;
+    return true;
+}
+
 }  // namespace
 
 namespace szxamd {
@@ -811,6 +846,148 @@ int szx_amd_decompress_nb_dev(const void *d_stream, size_t streamBytes, size_t n
     if (workBytes < nb_workspace(n)) { set_err("workspace too small"); return SZx_NSCS; }
     nb_decompress_launch((const uint8_t *)d_stream, streamBytes, n, w, d_out, (uint8_t *)d_work,
                          flags & SZX_AMD_DECODE_EXACT, d_status, (hipStream_t)hipStream);
+    return hip_ok(hipGetLastError(), "launch") ? SZx_SCES : SZx_NSCS;
+}
+
+void cuSZx_fast_decompress_args_unpredictable_blocked_float(float **newData, size_t nbEle,
+                                                            unsigned char *cmpBytes)
+{
+    g_err.clear();
+    if (!newData) {
+        set_err("NULL newData");
+        return;
+    }
+    *newData = nullptr;
+    size_t len = 0;
+    if (!cmpBytes || nbEle == 0) {
+        set_err("empty input");
+        return;
+    }
+    if (!cuszx_stream_length(cmpBytes, nbEle, &len)) return;
+    HostCtx &c = g_ctx;
+    if (!c.ready()) return;
+    if (!c.in.reserve(len) || !c.out.reserve(nbEle * 4) || !c.ws.reserve(cuszx_workspace(nbEle)) ||
+        !c.res.reserve(64))
+        return;
+    if (!hip_ok(hipMemcpyAsync(c.in.p, cmpBytes, len, hipMemcpyHostToDevice, c.stream), "H2D"))
+        return;
+    cuszx_decompress_launch((const uint8_t *)c.in.p, len, nbEle, (float *)c.out.p, (uint8_t *)c.ws.p,
+                            (int32_t *)c.res.p, c.stream);
+    int32_t status = 0;
+    if (!hip_ok(hipMemcpyAsync(&status, c.res.p, 4, hipMemcpyDeviceToHost, c.stream), "D2H") ||
+        !hip_ok(hipStreamSynchronize(c.stream), "decompress"))
+        return;
+    if (status != 0) {
+        set_err("malformed cuSZx stream (code %d)", status);
+        return;
+    }
+    float *out = (float *)malloc(nbEle * 4);
+    if (!out) {
+        set_err("malloc failed");
+        return;
+    }
+    if (!hip_ok(hipMemcpy(out, c.out.p, nbEle * 4, hipMemcpyDeviceToHost), "D2H output")) {
+        free(out);
+        return;
+    }
+    *newData = out;
+}
+
+static bool cuszx_bs_ok(int bs) { return bs >= 32 && bs <= 224 && bs % 32 == 0; }
+
+unsigned char *cuSZx_fast_compress_args_unpredictable_blocked_float(float *oriData, size_t *outSize,
+                                                                     float absErrBound, size_t nbEle,
+                                                                     int blockSize)
+{
+    g_err.clear();
+    if (outSize) *outSize = 0;
+    if (!oriData || nbEle == 0) {
+        set_err("empty input");
+        return nullptr;
+    }
+    if (!cuszx_bs_ok(blockSize)) {
+        set_err("cuSZx blockSize %d is not a multiple of 32 in 32..224", blockSize);
+        return nullptr;
+    }
+    HostCtx &c = g_ctx;
+    if (!c.ready()) return nullptr;
+    const uint64_t cap = cuszx_capacity(nbEle, blockSize);
+    if (!c.in.reserve(nbEle * 4) || !c.out.reserve(cap) || !c.ws.reserve(cuszx_compress_workspace(nbEle, blockSize)) ||
+        !c.res.reserve(64))
+        return nullptr;
+    if (!hip_ok(hipMemcpyAsync(c.in.p, oriData, nbEle * 4, hipMemcpyHostToDevice, c.stream), "H2D"))
+        return nullptr;
+    cuszx_compress_launch((const float *)c.in.p, nbEle, absErrBound, blockSize, (uint8_t *)c.out.p,
+                          (uint8_t *)c.ws.p, (uint64_t *)c.res.p, c.stream);
+    uint64_t res[4] = {0, 0, 0, 0};
+    if (!hip_ok(hipMemcpyAsync(res, c.res.p, sizeof res, hipMemcpyDeviceToHost, c.stream), "D2H") ||
+        !hip_ok(hipStreamSynchronize(c.stream), "compress"))
+        return nullptr;
+    if (res[3]) {
+        set_err("cuSZx: a block needs more than 32 significant bits for this bound");
+        return nullptr;
+    }
+    unsigned char *out = (unsigned char *)malloc(res[1]);
+    if (!out) {
+        set_err("malloc failed");
+        return nullptr;
+    }
+    if (!hip_ok(hipMemcpy(out, c.out.p, res[1], hipMemcpyDeviceToHost), "D2H stream")) {
+        free(out);
+        return nullptr;
+    }
+    if (outSize) *outSize = res[1];
+    return out;
+}
+
+size_t szx_amd_cuszx_capacity(size_t nbEle, int blockSize)
+{
+    return nbEle && cuszx_bs_ok(blockSize) ? cuszx_capacity(nbEle, blockSize) : 0;
+}
+
+size_t szx_amd_cuszx_compress_workspace(size_t nbEle, int blockSize)
+{
+    return nbEle && cuszx_bs_ok(blockSize) ? cuszx_compress_workspace(nbEle, blockSize) : 0;
+}
+
+int szx_amd_compress_cuszx_dev(const float *d_in, size_t nbEle, float absErrBound, int blockSize, void *d_out,
+                               size_t outCapacity, void *d_work, size_t workBytes, uint64_t *d_result,
+                               void *hipStream)
+{
+    g_err.clear();
+    if (!d_in || !d_out || !d_work || !d_result || nbEle == 0) {
+        set_err("NULL argument or nbEle == 0");
+        return SZx_NSCS;
+    }
+    if (!cuszx_bs_ok(blockSize)) {
+        set_err("cuSZx blockSize %d is not a multiple of 32 in 32..224", blockSize);
+        return SZx_NSCS;
+    }
+    if (outCapacity < cuszx_capacity(nbEle, blockSize) || workBytes < cuszx_compress_workspace(nbEle, blockSize)) {
+        set_err("output buffer or workspace too small");
+        return SZx_NSCS;
+    }
+    cuszx_compress_launch(d_in, nbEle, absErrBound, blockSize, (uint8_t *)d_out, (uint8_t *)d_work, d_result,
+                          (hipStream_t)hipStream);
+    return hip_ok(hipGetLastError(), "launch") ? SZx_SCES : SZx_NSCS;
+}
+
+size_t szx_amd_cuszx_workspace(size_t nbEle) { return nbEle ? cuszx_workspace(nbEle) : 0; }
+
+int szx_amd_decompress_cuszx_dev(const void *d_stream, size_t streamBytes, size_t nbEle, float *d_out,
+                                 void *d_work, size_t workBytes, int32_t *d_status, void *hipStream)
+{
+    g_err.clear();
+    if (!d_stream || !d_out || !d_work || !d_status || nbEle == 0) {
+        set_err("NULL argument or nbEle == 0");
+        return SZx_NSCS;
+    }
+    if (workBytes < cuszx_workspace(nbEle)) {
+        set_err("workspace too small");
+        return SZx_NSCS;
+    }
+    cuszx_decompress_launch((const uint8_t *)d_stream, streamBytes, nbEle, d_out, (uint8_t *)d_work, d_status,
+                            (hipStream_t)hipStream);
     return hip_ok(hipGetLastError(), "launch") ? SZx_SCES : SZx_NSCS;
 }
 

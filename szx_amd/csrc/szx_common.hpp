@@ -296,4 +296,41 @@ __device__ inline bool lookback_finish(const LookbackRefs &L, uint64_t tile, Loo
     return true;
 }
 
+// ---- segment helpers shared by the per-block decoders ----
+
+// Inclusive sum over the S lanes of a segment.
+template <int S> __device__ inline uint32_t seg_incl_sum(uint32_t v, int l)
+{
+#pragma unroll
+    for (int d = 1; d < S; d <<= 1) {
+        const uint32_t o = __shfl_up(v, d, S);
+        if (l >= d) v += o;
+    }
+    return v;
+}
+
+// Carry of the top three bytes of the reconstructed words across the lanes
+// of a segment: bytes 0..2 hold the bytes W-1, W-2, W-3 of the last word that
+// wrote them, bits 24..26 say which were written.  combine(a, b) = b's
+// written bytes over a's.
+__device__ inline uint32_t carry_combine(uint32_t a, uint32_t b)
+{
+    const uint32_t f = b >> 24;
+    const uint32_t keep = ((f & 1u) ? 0xFFu : 0u) | ((f & 2u) ? 0xFF00u : 0u) | ((f & 4u) ? 0xFF0000u : 0u);
+    return ((b & keep) | (a & ~keep & 0xFFFFFFu)) | ((a | b) & 0x7000000u);
+}
+
+// Up to 8 staged bytes starting at LDS byte pos, little-endian.
+__device__ inline uint64_t lds_window8(const uint32_t *w, uint32_t pos)
+{
+    const uint32_t i = pos >> 2, sh = pos & 3;
+    const uint32_t a = w[i], b = w[i + 1], c = w[i + 2];
+    return ((uint64_t)__builtin_amdgcn_alignbyte(c, b, sh) << 32) | __builtin_amdgcn_alignbyte(b, a, sh);
+}
+__device__ inline uint32_t lds_window4(const uint32_t *w, uint32_t pos)
+{
+    const uint32_t i = pos >> 2;
+    return __builtin_amdgcn_alignbyte(w[i + 1], w[i], pos & 3);
+}
+
 }  // namespace szxamd

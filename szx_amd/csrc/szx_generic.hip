@@ -58,17 +58,6 @@ template <> __device__ inline float gmax<float>(float a, float b) { return __bui
 template <> __device__ inline double gmin<double>(double a, double b) { return __builtin_fmin(a, b); }
 template <> __device__ inline double gmax<double>(double a, double b) { return __builtin_fmax(a, b); }
 
-// Inclusive sum over the S lanes of a segment.
-template <int S> __device__ inline uint32_t seg_incl_sum(uint32_t v, int l)
-{
-#pragma unroll
-    for (int d = 1; d < S; d <<= 1) {
-        const uint32_t o = __shfl_up(v, d, S);
-        if (l >= d) v += o;
-    }
-    return v;
-}
-
 // Payload bytes the blocks of one workgroup (256 / S of them) can span:
 // records of at most 5 + S + 4 S W bytes (bs <= 4 S), plus a 4-byte alignment
 // head; LDS words with 3 words of slack for the decoder's 8-byte windows.
@@ -403,30 +392,6 @@ __global__ __launch_bounds__(256) void k_gen_bitmap(GenericCompress g)
 
 // ---- decompression ----
 
-// Carry of the top three bytes of the reconstructed words across the lanes
-// of a segment: bytes 0..2 hold the bytes W-1, W-2, W-3 of the last word that
-// wrote them, bits 24..26 say which were written.  combine(a, b) = b's
-// written bytes over a's.
-__device__ inline uint32_t carry_combine(uint32_t a, uint32_t b)
-{
-    const uint32_t f = b >> 24;
-    const uint32_t keep = ((f & 1u) ? 0xFFu : 0u) | ((f & 2u) ? 0xFF00u : 0u) | ((f & 4u) ? 0xFF0000u : 0u);
-    return ((b & keep) | (a & ~keep & 0xFFFFFFu)) | ((a | b) & 0x7000000u);
-}
-
-// Up to 8 staged bytes starting at LDS byte pos, little-endian.
-__device__ inline uint64_t lds_window8(const uint32_t *w, uint32_t pos)
-{
-    const uint32_t i = pos >> 2, sh = pos & 3;
-    const uint32_t a = w[i], b = w[i + 1], c = w[i + 2];
-    return ((uint64_t)__builtin_amdgcn_alignbyte(c, b, sh) << 32) | __builtin_amdgcn_alignbyte(b, a, sh);
-}
-__device__ inline uint32_t lds_window4(const uint32_t *w, uint32_t pos)
-{
-    const uint32_t i = pos >> 2;
-    return __builtin_amdgcn_alignbyte(w[i + 1], w[i], pos & 3);
-}
-
 // One block per segment of S lanes, 4 consecutive values per lane.  The
 // workgroup's records (contiguous in the payload) are staged in LDS with
 // aligned word loads, and its values leave through LDS with aligned 16-byte
@@ -624,6 +589,17 @@ static void encode_dispatch(const GenericCompress &g, bool write, hipStream_t st
     case 32: encode_pass<T, 32>(g, write, st); break;
     default: encode_pass<T, 64>(g, write, st); break;
     }
+}
+
+// The scan and the bitmap alone, from g.size (callers with their own
+// per-block passes, e.g. the cuSZx writer).
+void generic_scan_launch(const GenericCompress &g, hipStream_t st)
+{
+    const unsigned np = (unsigned)g.nparts;
+    hipLaunchKernelGGL(k_gen_part, dim3(np), dim3(256), 0, st, g);
+    hipLaunchKernelGGL(k_gen_top, dim3(1), dim3(256), 0, st, g);
+    hipLaunchKernelGGL(k_gen_apply, dim3(np), dim3(256), 0, st, g);
+    hipLaunchKernelGGL(k_gen_bitmap, dim3((unsigned)ceil_div(ceil_div(g.nb, 8), 256)), dim3(256), 0, st, g);
 }
 
 void generic_compress_launch(const GenericCompress &g, int w, hipStream_t st)

@@ -86,6 +86,18 @@ void range_bound_launch(const void *d_in, uint64_t n, int w, float rel, uint8_t 
 
 // Non-blocked codec (fastMode 1, szx_nonblocked.hip): one record over all
 // values against the global median.
+// Reader of the reference's CUDA codec (cuSZx) stream, fp32 (szx_cuszx.hip).
+uint64_t cuszx_workspace(uint64_t n);
+void cuszx_decompress_launch(const uint8_t *d_stream, uint64_t len, uint64_t n, float *d_out, uint8_t *d_ws,
+                             int32_t *d_status, hipStream_t st);
+// The writer: bs a multiple of 32 in 32..224; d_out holds cuszx_capacity(n,
+// bs) bytes; d_result receives {0, stream bytes, bound bits, 1 if a block
+// needs more than 32 significant bits (no stream) else 0}.
+uint64_t cuszx_capacity(uint64_t n, int bs);
+uint64_t cuszx_compress_workspace(uint64_t n, int bs);
+void cuszx_compress_launch(const float *d_in, uint64_t n, float eb, int bs, uint8_t *d_out, uint8_t *d_ws,
+                           uint64_t *d_result, hipStream_t st);
+
 uint64_t nb_capacity(uint64_t n, int w);
 uint64_t nb_workspace(uint64_t n);
 void nb_compress_launch(const void *d_in, uint64_t n, int w, int rel_mode, float abs_eb, float rel, uint8_t *d_out,
@@ -128,6 +140,7 @@ struct GenericCompress {
 };
 uint64_t generic_parts(uint64_t nb);
 void generic_compress_launch(const GenericCompress &g, int w, hipStream_t st);
+void generic_scan_launch(const GenericCompress &g, hipStream_t st);
 
 struct GenericDecode {
     const uint8_t *s;
