@@ -423,6 +423,56 @@ def non_blocked_eval(x, xh, rel, dev, steps, parity=True):
     return rec
 
 
+def cuszx_eval(x, xh, eb, dev, steps, parity=True):
+    """The cuSZx stream (the reference's CUDA codec format, blockSize 128) on
+    the same fp32 field at the headline's absolute bound: the writer and the
+    reader (whole calls and kernel time), and with parity the stream against
+    the oracle's restatement of the reference's writer and the decode against
+    its decoder."""
+    import numpy as np
+    import torch
+
+    import szx_amd as Z
+
+    n = x.numel()
+    nbytes = n * 4
+    w = Z.DeviceCuszxWriter(n, 128, device=dev)
+    rd = Z.DeviceCuszxReader(n, device=dev)
+    w.compress(x, eb)
+    view, csize, bad = w.stream_view()
+    assert not bad, "a block needs more than 32 significant bits"
+    rd.decompress(view, csize)
+    torch.cuda.synchronize()
+    Z.timing_begin()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        w.compress(x, eb)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        rd.decompress(view, csize)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    kt = Z.timing_end()
+    assert rd.check() == 0, "reader rejected the stream"
+    rec = {"compression_ratio": nbytes / csize, "compress_gbs": nbytes * steps / (t1 - t0) / 1e9,
+           "decompress_gbs": nbytes * steps / (t2 - t1) / 1e9,
+           "compress_kernels_ms": kt["compress"][0] / max(1, kt["compress"][1]),
+           "decode_kernels_ms": kt["decode"][0] / max(1, kt["decode"][1])}
+    if parity:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        s_gpu = view.cpu().numpy().tobytes()
+        s_ora = O.compress_cuszx(xh, eb, 128)
+        rec["stream_identical"] = s_gpu == s_ora
+        d_ora = O.decompress_cuszx(s_ora, n)
+        rec["decode_identical"] = bool(np.array_equal(rd.dec.cpu().numpy().view(np.uint32), d_ora.view(np.uint32)))
+    log(f"cuSZx stream: {rec}")
+    del w, rd, view
+    torch.cuda.empty_cache()
+    return rec
+
+
 def _extents(r, n, nd):
     """Random 1-D/2-D/3-D extents whose product is close to n (at most n)."""
     import numpy as np
@@ -597,6 +647,8 @@ def main():
     ap.add_argument("--batch-reps", type=int, default=5)
     ap.add_argument("--no-nb", action="store_true",
                     help="skip the non-blocked codec (fastMode 1) leg")
+    ap.add_argument("--no-cuszx", action="store_true",
+                    help="skip the cuSZx stream (reference CUDA codec format) leg")
     ap.add_argument("--no-ra", action="store_true",
                     help="skip the random-access container (fastMode 3/4) leg")
     ap.add_argument("--generic-bs", default="100,255,7",
@@ -784,6 +836,12 @@ def main():
         nb = non_blocked_eval(x, xh, args.rel, dev, max(2, args.steps // 4),
                               parity=not args.no_parity and xh is not None)
 
+    cz = None
+    if not args.no_cuszx:
+        log("cuSZx stream (writer and reader)")
+        cz = cuszx_eval(x, xh, eb, dev, max(2, args.steps // 4),
+                        parity=not args.no_parity and xh is not None)
+
     generic = None
     gsizes = [int(v) for v in args.generic_bs.split(",") if v.strip()]
     if gsizes:
@@ -846,6 +904,7 @@ def main():
         "generic_block_sizes": generic,
         "random_access": ra,
         "non_blocked": nb,
+        "cuszx": cz,
         "host_api_pcie_inclusive": host_api,
     }
     print(json.dumps(line), flush=True)

@@ -124,19 +124,6 @@ __device__ inline uint32_t cz_ones(const uint8_t *s, const CzHeader &H, uint64_t
     return c;
 }
 
-__device__ inline bool cz_nonconst(const uint8_t *s, const CzHeader &H, uint64_t b)
-{
-    return (s[H.q_bits + (b >> 3)] >> (7 - (b & 7))) & 1;
-}
-
-// Non-constant blocks before block b (b <= nb).
-__device__ inline uint64_t cz_rank(const CzParams &p, const CzHeader &H, uint64_t b)
-{
-    const uint64_t wi = b >> 6;
-    const int within = (int)(b & 63);
-    return p.wbase[wi] + (within ? cz_ones(p.s, H, b - within, within) : 0u);
-}
-
 // Scan item i of mode 0 (ones of bitmap word i) or mode 1 (length of record
 // i: its i16 count plus the fixed part, cuszx_entry.cu:163).
 __device__ inline uint32_t cz_item(const CzParams &p, const CzHeader &H, int mode, uint64_t i)
@@ -252,23 +239,41 @@ __device__ inline uint32_t cz_slot_word(const uint32_t *stage, uint32_t r0, int 
     return leng - q >= 4 ? v : v & ((1u << (8 * (leng - q))) - 1u);
 }
 
-// One decode unit = 256 / S consecutive blocks; a grid-stride loop covers
-// the units, so the grid does not depend on the block size in the stream.
-__global__ __launch_bounds__(256) void k_cz_decode(CzParams p, int aligned)
+// Bitmap word wi (blocks 64 wi .. 64 wi + 63) with block 64 wi + k at bit
+// 63 - k; bytes past the bitmap read as zero.
+__device__ inline uint64_t cz_word(const uint8_t *s, const CzHeader &H, uint64_t wi)
 {
-    __shared__ uint32_t stage[CZ_SPAN / 4 + 4];
-    const CzHeader H = *p.h;
-    if (H.rc != 0 || *p.status != 0) return;
+    const uint64_t nbit = (H.nb + 7) / 8;
+    uint64_t w = 0;
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+        const uint64_t i = 8 * wi + t;
+        w = (w << 8) | (i < nbit ? s[H.q_bits + i] : 0u);
+    }
+    return w;
+}
+
+// Set bits among the first k (0..64) of a bitmap word.
+__device__ inline uint32_t cz_first(uint64_t w, int k) { return k ? __builtin_popcountll(w >> (64 - k)) : 0u; }
+
+// Decode units of 256 / S consecutive blocks.  256 / S divides 64, so a unit
+// lies inside one bitmap word: its blocks' states and ranks come from that
+// word and the word's scanned base.
+template <int S>
+__device__ inline void cz_decode_units(const CzParams &p, const CzHeader &H, uint32_t *stage, int aligned)
+{
+    constexpr int BPW = 256 / S;
     const int bs = H.bs, nl = bs / 4;
-    int S = 8;
-    while (S < nl) S <<= 1;
-    const int bpw = 256 / S;
-    const uint64_t nb = H.nb, units = (nb + bpw - 1) / bpw;
+    const uint64_t nb = H.nb, units = (nb + BPW - 1) / BPW;
     const int seg = threadIdx.x / S, l = threadIdx.x & (S - 1);
+    const bool act = l < nl;
     const uint8_t *lds = reinterpret_cast<const uint8_t *>(stage);
     for (uint64_t u = blockIdx.x; u < units; u += gridDim.x) {
-        const uint64_t b0 = u * bpw, bend = b0 + bpw < nb ? b0 + bpw : nb;
-        const uint64_t j0 = cz_rank(p, H, b0), j1 = cz_rank(p, H, bend);
+        const uint64_t b0 = u * BPW, bend = b0 + BPW < nb ? b0 + BPW : nb;
+        const uint64_t wi = b0 >> 6;
+        const uint64_t word = cz_word(p.s, H, wi), base = p.wbase[wi];
+        const int k0 = (int)(b0 & 63);
+        const uint64_t j0 = base + cz_first(word, k0), j1 = base + cz_first(word, k0 + (int)(bend - b0));
         const uint64_t P0 = p.roff[j0], P1 = p.roff[j1];
         // stage record bytes [P0, P1): aligned words inside the stream, a
         // partial last word byte by byte
@@ -292,19 +297,19 @@ __global__ __launch_bounds__(256) void k_cz_decode(CzParams p, int aligned)
         __syncthreads();
         const uint64_t b = b0 + seg;
         if (b >= bend) continue;
+        const int kb = k0 + seg;
+        const uint64_t rank = base + cz_first(word, kb);   // non-constant blocks before b
         float *y = p.out + b * (uint64_t)bs + 4 * l;
-        const bool act = l < nl;
         float v[4];
-        if (!cz_nonconst(p.s, H, b)) {
+        if (!((word >> (63 - kb)) & 1)) {
             // constant block: its native median (cuszx_entry.cu:157-159, 191-196)
-            const uint8_t *c = p.s + H.q_med + 4 * (b - cz_rank(p, H, b));
+            const uint8_t *c = p.s + H.q_med + 4 * (b - rank);
             const float med = bits_f32((uint32_t)c[0] | (uint32_t)c[1] << 8 | (uint32_t)c[2] << 16 |
                                        (uint32_t)c[3] << 24);
             v[0] = v[1] = v[2] = v[3] = med;
         } else {
-            const uint64_t j = cz_rank(p, H, b);
-            const uint32_t r0 = head + (uint32_t)(p.roff[j] - P0);
-            const int leng = (int)(p.roff[j + 1] - p.roff[j]);
+            const uint32_t r0 = head + (uint32_t)(p.roff[rank] - P0);
+            const int leng = (int)(p.roff[rank + 1] - p.roff[rank]);
             // cuszxd_float.cu:185-197
             const float med = bits_f32(cz_slot_word(stage, r0, 0, leng));
             const int rl = 4 < leng ? lds[r0 + 4] : 0;
@@ -312,6 +317,8 @@ __global__ __launch_bounds__(256) void k_cz_decode(CzParams p, int aligned)
             const int shift = rl % 8 ? 8 - rl % 8 : 0;
             const bool bytes_ok = rb >= 1 && rb <= 4;
             const uint32_t code = act && 5 + l < leng ? lds[r0 + 5 + l] : 0u;
+            // word bytes at or above 4 - rb (rb >= 1)
+            const uint32_t sig = rb >= 4 ? ~0u : rb >= 1 ? ~0u << (8 * (4 - rb)) : 0u;
             int lk[4];
             int lsum = 0;
 #pragma unroll
@@ -320,12 +327,8 @@ __global__ __launch_bounds__(256) void k_cz_decode(CzParams p, int aligned)
                 if (act) lsum += rb - lk[k];
             }
             // prefix sums of the mid-byte counts (cuszxd_float.cu:197-198)
-            uint32_t incl = (uint32_t)lsum;
-            for (int d = 1; d < S; d <<= 1) {
-                const uint32_t o = __shfl_up(incl, d, S);
-                if (l >= d) incl += o;
-            }
-            int at = (int)incl - lsum;
+            const int at0 = (int)seg_incl_sum<S>((uint32_t)lsum, l) - lsum;
+            int at = at0;
             uint32_t wv[4], wr[4];
             uint32_t mine = 0;
 #pragma unroll
@@ -334,7 +337,7 @@ __global__ __launch_bounds__(256) void k_cz_decode(CzParams p, int aligned)
                 at += m;
                 uint32_t cur = 0;
                 wr[k] = 0;
-                if (act && bytes_ok && m >= 1 && m <= rb) {
+                if (act && bytes_ok && m >= 1) {
                     // the value's own bytes 4-rb .. 4-rb+m-1, in that order,
                     // ending at slot byte msize + at (cuszxd_float.cu:206-267)
                     const int p0 = H.msize + at - m;
@@ -344,18 +347,19 @@ __global__ __launch_bounds__(256) void k_cz_decode(CzParams p, int aligned)
                         const uint32_t raw = cz_slot_word(stage, r0, p0, leng);
                         cur = (m == 4 ? raw : raw & ((1u << (8 * m)) - 1u)) << (8 * (4 - rb));
                     }
-                    // word bytes 3 - i the value carries itself: i in [lead, rb)
                     if (rb >= 2) {
-#pragma unroll
-                        for (int i = 0; i < 3; i++)
-                            if (i >= lk[k] && i < rb)
-                                wr[k] |= ((cur >> (8 * (3 - i))) & 0xFFu) << (8 * i) | (1u << (24 + i));
+                        // carried bytes: the value's own among word bytes 1..3
+                        const uint32_t pm = sig & (~0u >> (8 * lk[k])) & 0xFFFFFF00u;
+                        const uint32_t fl = __builtin_bswap32(pm);
+                        wr[k] = (__builtin_bswap32(cur & pm) & 0xFFFFFFu) |
+                                (((fl & 1u) | ((fl >> 7) & 2u) | ((fl >> 14) & 4u)) << 24);
                     }
                     mine = carry_combine(mine, wr[k]);
                 }
                 wv[k] = cur;
             }
             uint32_t acc = mine;
+#pragma unroll
             for (int d = 1; d < S; d <<= 1) {
                 const uint32_t o = __shfl_up(acc, d, S);
                 if (l >= d) acc = carry_combine(o, acc);
@@ -364,14 +368,11 @@ __global__ __launch_bounds__(256) void k_cz_decode(CzParams p, int aligned)
             if (l == 0) cin = 0;
 #pragma unroll
             for (int k = 0; k < 4; k++) {
-                // inherited word bytes 3 - i, i < lead, from the last value
+                // inherited word bytes: the top `lead`, from the last value
                 // that carried them; none with one significant byte
-                uint32_t w = wv[k];
-                if (rb >= 2)
-                    for (int i = 0; i < lk[k] && i < 3; i++)
-                        w = (w & ~(0xFFu << (8 * (3 - i)))) | ((cin >> (8 * i)) & 0xFFu) << (8 * (3 - i));
-                else
-                    for (int i = 0; i < lk[k] && i < 3; i++) w &= ~(0xFFu << (8 * (3 - i)));
+                const uint32_t inh = lk[k] ? ~0u << (8 * (4 - lk[k])) : 0u;
+                const uint32_t from = rb >= 2 ? __builtin_bswap32(cin) & 0xFFFFFF00u : 0u;
+                const uint32_t w = (wv[k] & ~inh) | (from & inh);
                 cin = carry_combine(cin, wr[k]);
                 v[k] = bits_f32(w << shift) + med;   // cuszxd_float.cu:290-292
             }
@@ -385,6 +386,20 @@ __global__ __launch_bounds__(256) void k_cz_decode(CzParams p, int aligned)
             }
         }
     }
+}
+
+// A grid-stride loop over the decode units, so the grid does not depend on
+// the block size in the stream; the segment width is chosen here.
+__global__ __launch_bounds__(256) void k_cz_decode(CzParams p, int aligned)
+{
+    __shared__ uint32_t stage[CZ_SPAN / 4 + 4];
+    const CzHeader H = *p.h;
+    if (H.rc != 0 || *p.status != 0) return;
+    const int nl = H.bs / 4;
+    if (nl <= 8) cz_decode_units<8>(p, H, stage, aligned);
+    else if (nl <= 16) cz_decode_units<16>(p, H, stage, aligned);
+    else if (nl <= 32) cz_decode_units<32>(p, H, stage, aligned);
+    else cz_decode_units<64>(p, H, stage, aligned);
 }
 
 // The n % bs trailing values, stored raw (cuszx_entry.cu:155).
@@ -435,9 +450,9 @@ struct CzwBlock {
     uint32_t lsum, incl, total;
 };
 
+template <int S>
 __device__ inline void czw_block(const CzwParams &p, uint64_t b, int l, CzwBlock &k)
 {
-    const int S = p.S;
     const bool act = l < p.bs / 4;
     const float *xb = p.x + b * (uint64_t)p.bs + 4 * l;
     float v[4];
@@ -445,6 +460,7 @@ __device__ inline void czw_block(const CzwParams &p, uint64_t b, int l, CzwBlock
     for (int i = 0; i < 4; i++) v[i] = act ? xb[i] : 0.0f;
     float mn = act ? fminf(fminf(v[0], v[1]), fminf(v[2], v[3])) : __builtin_huge_valf();
     float mx = act ? fmaxf(fmaxf(v[0], v[1]), fmaxf(v[2], v[3])) : -__builtin_huge_valf();
+#pragma unroll
     for (int d = 1; d < S; d <<= 1) {
         mn = fminf(mn, __shfl_xor(mn, d, S));
         mx = fmaxf(mx, __shfl_xor(mx, d, S));
@@ -474,24 +490,19 @@ __device__ inline void czw_block(const CzwParams &p, uint64_t b, int l, CzwBlock
         if (act) k.lsum += (uint32_t)(k.rb - k.ld[i]);
         prev = k.w[i];
     }
-    uint32_t incl = k.lsum;
-    for (int d = 1; d < S; d <<= 1) {
-        const uint32_t o = __shfl_up(incl, d, S);
-        if (l >= d) incl += o;
-    }
-    k.incl = incl;
-    k.total = __shfl(incl, S - 1, S);
+    k.incl = seg_incl_sum<S>(k.lsum, l);
+    k.total = __shfl(k.incl, S - 1, S);
 }
 
-template <bool WRITE>
+template <int S, bool WRITE>
 __global__ __launch_bounds__(256) void k_czw_pass(CzwParams p)
 {
-    const int S = p.S, bpw = 256 / S;
+    constexpr int BPW = 256 / S;
     const int l = threadIdx.x & (S - 1);
-    const uint64_t b = blockIdx.x * (uint64_t)bpw + threadIdx.x / S;
+    const uint64_t b = blockIdx.x * (uint64_t)BPW + threadIdx.x / S;
     if (b >= p.nb) return;   // whole segments
     CzwBlock k;
-    czw_block(p, b, l, k);
+    czw_block<S>(p, b, l, k);
     if (!WRITE) {
         if (l == 0) {
             p.size[b] = k.cst ? 0u : (uint32_t)p.msize + k.total;
@@ -619,6 +630,23 @@ static int czw_lanes(int bs)
     return S;
 }
 
+template <int S>
+static void czw_pass_s(const CzwParams &p, bool write, unsigned grid, hipStream_t st)
+{
+    if (write) hipLaunchKernelGGL((k_czw_pass<S, true>), dim3(grid), dim3(256), 0, st, p);
+    else hipLaunchKernelGGL((k_czw_pass<S, false>), dim3(grid), dim3(256), 0, st, p);
+}
+
+static void czw_pass(const CzwParams &p, bool write, unsigned grid, hipStream_t st)
+{
+    switch (p.S) {
+    case 8: czw_pass_s<8>(p, write, grid, st); break;
+    case 16: czw_pass_s<16>(p, write, grid, st); break;
+    case 32: czw_pass_s<32>(p, write, grid, st); break;
+    default: czw_pass_s<64>(p, write, grid, st); break;
+    }
+}
+
 uint64_t cuszx_capacity(uint64_t n, int bs)
 {
     const uint64_t nb = n / (uint64_t)bs;
@@ -667,7 +695,7 @@ void cuszx_compress_launch(const float *d_in, uint64_t n, float eb, int bs, uint
     timing_mark_begin(KK_COMPRESS, st);
     if (L.nb) {
         const unsigned grid = (unsigned)ceil_div(L.nb, (uint64_t)(256 / p.S));
-        hipLaunchKernelGGL(k_czw_pass<false>, dim3(grid), dim3(256), 0, st, p);
+        czw_pass(p, false, grid, st);
         GenericCompress g{};
         g.nb = L.nb;
         g.bs = bs;
@@ -678,7 +706,7 @@ void cuszx_compress_launch(const float *d_in, uint64_t n, float eb, int bs, uint
         g.totals = reinterpret_cast<uint64_t *>(d_ws + L.o_tot);
         g.bitmap = d_out + 13;
         generic_scan_launch(g, st);
-        hipLaunchKernelGGL(k_czw_pass<true>, dim3(grid), dim3(256), 0, st, p);
+        czw_pass(p, true, grid, st);
     }
     hipLaunchKernelGGL(k_czw_finish, dim3(1), dim3(256), 0, st, p);
     timing_mark_end(KK_COMPRESS, st);
