@@ -79,6 +79,26 @@ def test_known_answer_stream_and_unaligned_output(device):
         assert st == 0 and same(got, want)
 
 
+def test_stream_at_unaligned_device_addresses(device):
+    """The reader stages records with aligned word loads from wherever the
+    stream starts."""
+    import torch
+    r = np.random.default_rng(77)
+    n = 128 * 500 + 9
+    x = field(r, n, 1)
+    s = O.compress_cuszx(x, 1e-3 * float(np.ptp(x)), 128)
+    want = O.decompress_cuszx(s, n)
+    rd = Z.DeviceCuszxReader(n)
+    for off in (1, 2, 3, 5):
+        say("case stream offset", off)
+        buf = torch.zeros(len(s) + 8, dtype=torch.uint8, device="cuda")
+        buf[off:off + len(s)] = torch.frombuffer(bytearray(s), dtype=torch.uint8).cuda()
+        rd.dec.fill_(float("nan"))
+        rd.decompress(buf[off:], len(s))
+        torch.cuda.synchronize()
+        assert rd.check() == 0 and same(rd.dec.cpu().numpy(), want)
+
+
 def random_stream(r, bs, nb, rem):
     """A stream whose records are random bytes of random length in [0, 4 bs],
     with reqLength bytes drawn to reach every significant-byte count (0..5) and
