@@ -147,3 +147,24 @@ def test_host_reader_validates_before_the_device():
     for bs in (0, 16, 100, 256):
         with pytest.raises(Z.SZxError, match="multiple of 32"):
             Z.cuszx_compress(np.zeros(512, np.float32), 1e-3, bs)
+
+
+def test_device_entries_validate_arguments_without_gpu():
+    """Argument checks of the device-resident cuSZx entries run before any
+    launch (NULL buffers, short workspace or capacity, block size)."""
+    import ctypes as C
+    import szx_amd as Z
+    L = Z.lib()
+    n = 1 << 16
+    assert L.szx_amd_cuszx_workspace(0) == 0
+    assert L.szx_amd_cuszx_capacity(n, 100) == 0 and L.szx_amd_cuszx_compress_workspace(n, 16) == 0
+    cap, ws = L.szx_amd_cuszx_capacity(n, 128), L.szx_amd_cuszx_compress_workspace(n, 128)
+    assert cap > 13 + n // 128 // 8 and ws > 0
+    fake = C.c_void_p(0x1000)   # never dereferenced: every call below fails its checks first
+    assert L.szx_amd_decompress_cuszx_dev(None, 100, n, fake, fake, L.szx_amd_cuszx_workspace(n), fake, None) != 0
+    assert L.szx_amd_decompress_cuszx_dev(fake, 100, n, fake, fake, 16, fake, None) != 0
+    assert "workspace" in Z.last_error()
+    assert L.szx_amd_compress_cuszx_dev(fake, n, 1e-3, 100, fake, cap, fake, ws, fake, None) != 0
+    assert "multiple of 32" in Z.last_error()
+    assert L.szx_amd_compress_cuszx_dev(fake, n, 1e-3, 128, fake, cap - 1, fake, ws, fake, None) != 0
+    assert "too small" in Z.last_error()
