@@ -660,8 +660,9 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this codec build has no multi-GPU path "
-              "(fields shard by block ranges; see DESIGN.md).", file=sys.stderr)
+        print("bench.py measures one GPU: the codec has no multi-GPU path "
+              "(a field would shard by contiguous block ranges with one gather of "
+              "per-shard totals).", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

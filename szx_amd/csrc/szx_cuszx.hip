@@ -1,5 +1,5 @@
-// szx_cuszx.hip -- reader of the stream the reference's CUDA codec (cuSZx,
-// fp32) writes.
+// szx_cuszx.hip -- reader and writer of the stream the reference's CUDA
+// codec (cuSZx, fp32) writes (the writer is at the end of the file).
 //
 // Stream (cuszx_entry.cu:20-70):
 //   [1, 1, 1, 0, (u8)bs], big-endian u64 constant-block count, the state
