@@ -93,7 +93,7 @@ bool plan_layout(const szx_amd_field *f, int nf, int bs, Layout &Y)
         s.L = compress_layout(n, s.w, bs);
         s.nidx = s.L.nidx;
         s.ntiles[0] = s.nidx;
-        s.ntiles[1] = (n + compress_tile_values(s.w) - 1) / compress_tile_values(s.w);
+        s.ntiles[1] = (n + INDEX_PER_TILE * tile_values(s.w) - 1) / (INDEX_PER_TILE * tile_values(s.w));
         s.range_items = f[i].errBoundMode == REL ? (n + RANGE_ITEM - 1) / RANGE_ITEM : 0;
         s.zero_items = ((s.L.ws_misc + 64 * sizeof(uint64_t)) / 8 + ZERO_ITEM - 1) / ZERO_ITEM;
         const uint64_t work = s.L.nbit > 4 * s.L.nb ? s.L.nbit : 4 * s.L.nb;

@@ -3,9 +3,10 @@
 // One launch of k_compress turns the whole input into the records of the
 // fastMode-2 stream (szx_float.cc:1059-1139, szx_double.cc:1160-1238):
 //
-//   * a workgroup (256 lanes, 4 wave64s) owns a tile of 8192 fp32 / 4096
-//     fp64 values (two record-index tiles: a longer tile amortises the
-//     look-back wait over more data); each lane holds LV consecutive values
+//   * a workgroup (256 lanes, 4 wave64s) owns a tile of 16384 fp32 values
+//     for blocks of 16..128 values, else 8192 fp32 / 4096 fp64 values (four or
+//     two record-index tiles: a longer tile amortises the look-back wait over
+//     more data); each lane holds LV consecutive values
 //     (8, or 4 for blocks of 4) per slab of 256 x LV values, so a block of BS
 //     values is an aligned group of BS/LV lanes and every per-block reduction
 //     / scan is a DPP or swizzle step inside that group;
@@ -258,10 +259,14 @@ __device__ inline void stage_bytes(uint8_t *p, U t, int m)
         if (j < m) p[j] = (uint8_t)(t >> (8 * j));
 }
 
-// Record bytes a tile stages in LDS.  A tile whose records are larger (less
-// than about 1.6x compression) writes them straight to global memory once its
-// prefix is known; the smaller stage keeps more workgroups resident per CU.
+// Record bytes a tile stages in LDS.  A tile whose records are larger writes
+// them straight to global memory once its prefix is known (a second encode of
+// the tile, several times slower), so the cap trades resident workgroups per
+// CU against how compressible a tile must be to stay staged: 20 KiB for the
+// 8192 fp32 / 4096 fp64 value tiles (staged above about 1.6x compression),
+// 48 KiB for the 16384-value fp32 tiles (above about 1.33x).
 constexpr int STAGE_CAP_BYTES = 20480;
+constexpr int STAGE_CAP_WIDE_BYTES = 49152;
 
 // Shape of a compressor tile: T, block size, values per lane, slabs.
 template <typename T, int BS, int LV, int SLABS> struct TileShape {
@@ -270,7 +275,8 @@ template <typename T, int BS, int LV, int SLABS> struct TileShape {
     static constexpr int TILE = SLAB * SLABS;       // values per tile
     static constexpr int NBLK = TILE / BS;          // blocks per tile (multiple of 8)
     static constexpr int STAGE = TILE * W + NBLK * (5 + BS / 4);   // largest record bytes
-    static constexpr int STAGE_CAP = STAGE < STAGE_CAP_BYTES ? STAGE : STAGE_CAP_BYTES;
+    static constexpr int CAP = TILE >= 16384 && W == 4 ? STAGE_CAP_WIDE_BYTES : STAGE_CAP_BYTES;
+    static constexpr int STAGE_CAP = STAGE < CAP ? STAGE : CAP;
     static constexpr int STAGE_DW = (STAGE_CAP + 32 + 15) / 16 * 4;
 };
 
@@ -360,7 +366,7 @@ __device__ __forceinline__ void encode_scan(const CompressParams &p, const Lookb
     static_assert(BS >= LV && BS <= 256 && (BS & (BS - 1)) == 0, "power-of-two block size");
     static_assert(NBLK % 8 == 0, "tiles own whole bitmap bytes");
     static_assert((NBLK / IPT) % 8 == 0, "index tiles own whole bitmap bytes");
-    static_assert(S::STAGE < (1 << 16), "record starts fit the 16-bit block index");
+    static_assert(S::STAGE / IPT < (1 << 16), "record starts in an index tile fit the 16-bit block index");
     const int lane = threadIdx.x & 63;
     encode_slabs<T, BS, LV, SLABS>(p, tile, v, st, M);
     __syncthreads();
@@ -953,7 +959,7 @@ CompressLayout compress_layout(uint64_t n, int w, int bs)
     const uint64_t rec_max = 5 + ceil_div((uint64_t)bs, 4) + (uint64_t)bs * w;
     L.payload_max = L.nb * rec_max;
     L.capacity = L.region + L.payload_max + 16;
-    L.ntiles = ceil_div(n, compress_tile_values(w));
+    L.ntiles = ceil_div(n, compress_tile_values(w, bs));
     if (L.ntiles == 0) L.ntiles = 1;
     L.nidx = index_tiles(n, w);
     // workspace: tile words | chunk words | chunk accumulators | misc (totals,
@@ -985,9 +991,10 @@ static void launch_main(const CompressParams &p, uint64_t ntiles, hipStream_t st
 {
     // 8 values per lane where a block spans at least 8 values, else 4
     constexpr int LV = BS >= 8 ? 8 : 4;
-    constexpr int TILE = INDEX_PER_TILE * (sizeof(T) == 4 ? TILE_VALUES_F32 : TILE_VALUES_F64);
+    constexpr int IPT = compress_wide((int)sizeof(T), BS) ? INDEX_PER_TILE_WIDE : INDEX_PER_TILE;
+    constexpr int TILE = IPT * (sizeof(T) == 4 ? TILE_VALUES_F32 : TILE_VALUES_F64);
     constexpr int SLABS = TILE / (256 * LV);
-    hipLaunchKernelGGL((k_compress<T, BS, LV, SLABS, INDEX_PER_TILE>), dim3((unsigned)ntiles), dim3(256), 0, st, p);
+    hipLaunchKernelGGL((k_compress<T, BS, LV, SLABS, IPT>), dim3((unsigned)ntiles), dim3(256), 0, st, p);
 }
 
 template <typename T>
