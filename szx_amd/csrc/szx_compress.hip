@@ -3,10 +3,10 @@
 // One launch of k_compress turns the whole input into the records of the
 // fastMode-2 stream (szx_float.cc:1059-1139, szx_double.cc:1160-1238):
 //
-//   * a workgroup (256 lanes, 4 wave64s) owns a tile of 16384 fp32 values
-//     for blocks of 16..128 values, else 8192 fp32 / 4096 fp64 values (four or
-//     two record-index tiles: a longer tile amortises the look-back wait over
-//     more data); each lane holds LV consecutive values
+//   * a workgroup (256 lanes, 4 wave64s) owns a tile of 16384 fp32 / 8192
+//     fp64 values for blocks of 16..128 values, else 8192 fp32 / 4096 fp64
+//     values (four or two record-index tiles: a longer tile amortises the
+//     look-back wait over more data); each lane holds LV consecutive values
 //     (8, or 4 for blocks of 4) per slab of 256 x LV values, so a block of BS
 //     values is an aligned group of BS/LV lanes and every per-block reduction
 //     / scan is a DPP or swizzle step inside that group;
@@ -263,8 +263,8 @@ __device__ inline void stage_bytes(uint8_t *p, U t, int m)
 // them straight to global memory once its prefix is known (a second encode of
 // the tile, several times slower), so the cap trades resident workgroups per
 // CU against how compressible a tile must be to stay staged: 20 KiB for the
-// 8192 fp32 / 4096 fp64 value tiles (staged above about 1.6x compression),
-// 48 KiB for the 16384-value fp32 tiles (above about 1.33x).
+// 32 KiB tiles (staged above about 1.6x compression), 48 KiB for the 64 KiB
+// tiles (above about 1.33x).
 constexpr int STAGE_CAP_BYTES = 20480;
 constexpr int STAGE_CAP_WIDE_BYTES = 49152;
 
@@ -275,7 +275,7 @@ template <typename T, int BS, int LV, int SLABS> struct TileShape {
     static constexpr int TILE = SLAB * SLABS;       // values per tile
     static constexpr int NBLK = TILE / BS;          // blocks per tile (multiple of 8)
     static constexpr int STAGE = TILE * W + NBLK * (5 + BS / 4);   // largest record bytes
-    static constexpr int CAP = TILE >= 16384 && W == 4 ? STAGE_CAP_WIDE_BYTES : STAGE_CAP_BYTES;
+    static constexpr int CAP = TILE * W >= 65536 ? STAGE_CAP_WIDE_BYTES : STAGE_CAP_BYTES;
     static constexpr int STAGE_CAP = STAGE < CAP ? STAGE : CAP;
     static constexpr int STAGE_DW = (STAGE_CAP + 32 + 15) / 16 * 4;
 };

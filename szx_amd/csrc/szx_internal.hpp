@@ -13,12 +13,12 @@ constexpr int TILE_VALUES_F32 = 4096;
 constexpr int TILE_VALUES_F64 = 2048;
 inline uint64_t tile_values(int w) { return w == 4 ? TILE_VALUES_F32 : TILE_VALUES_F64; }
 // The compressor encodes several index tiles per workgroup (longer tiles
-// amortise its look-back wait): INDEX_PER_TILE_WIDE for fp32 blocks of 16..128
-// values (16384-value tiles), INDEX_PER_TILE otherwise and in the batched
-// launch; its record index stays per index tile.
+// amortise its look-back wait): INDEX_PER_TILE_WIDE for blocks of 16..128
+// values (16384 fp32 / 8192 fp64 values, 64 KiB), INDEX_PER_TILE otherwise
+// and in the batched launch; its record index stays per index tile.
 constexpr int INDEX_PER_TILE = 2;
 constexpr int INDEX_PER_TILE_WIDE = 4;
-constexpr bool compress_wide(int w, int bs) { return w == 4 && bs >= 16 && bs <= 128; }
+constexpr bool compress_wide(int w, int bs) { return w > 0 && bs >= 16 && bs <= 128; }
 inline uint64_t compress_tile_values(int w, int bs)
 {
     return (compress_wide(w, bs) ? INDEX_PER_TILE_WIDE : INDEX_PER_TILE) * tile_values(w);
