@@ -23,8 +23,8 @@ reference's sampled-model bound judged by its real stream beside them; an fp64
 64/128/256 (throughput and full-size parity per configuration); the 512^3
 field at block sizes outside the tiled kernels (default 100, 255, 7: the
 per-block segment kernels, throughput and full-size parity); the random-access
-container (fastMode 3 and 4) and the non-blocked codec (fastMode 1) on the
-same field; and 1000 mixed
+container (fastMode 3 and 4), the non-blocked codec (fastMode 1) and the
+cuSZx stream writer and reader on the same field; and 1000 mixed
 fp32/fp64 fields compressed and decompressed by the batched launch, against
 one launch sequence per field.
 
@@ -525,7 +525,7 @@ def batched_eval(n_fields, dev, reps, single_gbs):
 
     bc = Z.BatchCodec(fields, 128)
     shapes = {}
-    for lv, ipt in ((8, 2), (8, 1), (4, 2), (4, 1)):
+    for lv, ipt in ((8, 4), (8, 2), (8, 1), (4, 2), (4, 1)):
         bc.set_shape(lv, ipt)
         bc.compress()
         torch.cuda.synchronize()

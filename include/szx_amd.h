@@ -420,8 +420,9 @@ int szx_amd_batch_results(szx_amd_batch *batch, uint64_t *h_results, void *hipSt
 int szx_amd_batch_decompress(szx_amd_batch *batch, int flags, int mode, void *hipStream);
 
 /* Tile shape of the fused compress: values per lane (4 or 8; 8 needs
- * blockSize >= 8) and index tiles per workgroup tile (1 or 2, i.e. 4096 or
- * 8192 fp32 values).  szx_amd_batch_tune times every shape on this batch
+ * blockSize >= 8) and index tiles per workgroup tile (1, 2 or 4, i.e. 4096,
+ * 8192 or 16384 fp32 values; 4 needs 8 values per lane and blockSize 16..128).
+ * szx_amd_batch_tune times every shape on this batch
  * (reps timed passes each, after one untimed pass; synchronizes), keeps the
  * fastest and reports it with its mean milliseconds per compress.  The
  * environment variable SZX_AMD_BATCH_SHAPE="lanes,tiles" sets the initial

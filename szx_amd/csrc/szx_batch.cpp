@@ -54,7 +54,7 @@ struct FieldSlot {
     uint64_t index;   // record index
     uint64_t hdr;     // decode header (256 bytes)
     uint64_t nidx;    // index tiles
-    uint64_t ntiles[2];
+    uint64_t ntiles[3];
     uint64_t range_items;
     uint64_t zero_items;
     uint64_t asm_items;
@@ -63,11 +63,15 @@ struct FieldSlot {
 
 struct Layout {
     uint64_t bf, df, bases, res, status, part, fields, bytes;
-    uint64_t total[2] = {0, 0}, nitems = 0, dtotal = 0, zitems = 0, aitems = 0;
+    uint64_t total[3] = {0, 0, 0}, nitems = 0, dtotal = 0, zitems = 0, aitems = 0;
     std::vector<FieldSlot> slot;
 };
 
-constexpr int NBASE = 6;   // tbase (ipt 1), tbase (ipt 2), rbase, dbase, zbase, abase
+constexpr int NBASE = 7;   // tbase (ipt 1), tbase (ipt 2), rbase, dbase, zbase, abase, tbase (ipt 4)
+
+// Slot of a tile shape's counts: ntiles[] / total[], and the base arrays.
+int tile_slot(int ipt) { return ipt == 4 ? 2 : ipt - 1; }
+int tbase_slot(int ipt) { return ipt == 4 ? 6 : ipt - 1; }
 
 bool plan_layout(const szx_amd_field *f, int nf, int bs, Layout &Y)
 {
@@ -94,6 +98,7 @@ bool plan_layout(const szx_amd_field *f, int nf, int bs, Layout &Y)
         s.nidx = s.L.nidx;
         s.ntiles[0] = s.nidx;
         s.ntiles[1] = (n + INDEX_PER_TILE * tile_values(s.w) - 1) / (INDEX_PER_TILE * tile_values(s.w));
+        s.ntiles[2] = (n + INDEX_PER_TILE_WIDE * tile_values(s.w) - 1) / (INDEX_PER_TILE_WIDE * tile_values(s.w));
         s.range_items = f[i].errBoundMode == REL ? (n + RANGE_ITEM - 1) / RANGE_ITEM : 0;
         s.zero_items = ((s.L.ws_misc + 64 * sizeof(uint64_t)) / 8 + ZERO_ITEM - 1) / ZERO_ITEM;
         const uint64_t work = s.L.nbit > 4 * s.L.nb ? s.L.nbit : 4 * s.L.nb;
@@ -101,6 +106,7 @@ bool plan_layout(const szx_amd_field *f, int nf, int bs, Layout &Y)
         s.asm_items = blocks < 1 ? 1 : blocks > ASSEMBLE_MAX_BLOCKS ? ASSEMBLE_MAX_BLOCKS : blocks;
         Y.total[0] += s.ntiles[0];
         Y.total[1] += s.ntiles[1];
+        Y.total[2] += s.ntiles[2];
         Y.nitems += s.range_items;
         Y.dtotal += s.nidx;
         Y.zitems += s.zero_items;
@@ -150,7 +156,11 @@ bool hip_ok(hipError_t e, const char *what)
     return false;
 }
 
-bool shape_ok(int bs, int lv, int ipt) { return (ipt == 1 || ipt == 2) && (lv == 4 || (lv == 8 && bs >= 8)); }
+bool shape_ok(int bs, int lv, int ipt)
+{
+    if (ipt == 4) return lv == 8 && bs >= 16 && bs <= 128;
+    return (ipt == 1 || ipt == 2) && (lv == 4 || (lv == 8 && bs >= 8));
+}
 
 int enqueue_compress(szx_amd_batch *b, int mode, hipStream_t st)
 {
@@ -174,8 +184,8 @@ int enqueue_compress(szx_amd_batch *b, int mode, hipStream_t st)
     if (Y.nitems) batch_range_launch(b->d_bf(), b->d_base(2), b->nf, Y.nitems,
                                      reinterpret_cast<double *>(b->work + Y.part), st);
     timing_mark_begin(KK_COMPRESS, st);
-    const int rc = batch_compress_launch(b->bs, b->lv, b->ipt, b->d_bf(), b->d_base(b->ipt - 1), b->nf,
-                                         Y.total[b->ipt - 1], st);
+    const int rc = batch_compress_launch(b->bs, b->lv, b->ipt, b->d_bf(), b->d_base(tbase_slot(b->ipt)), b->nf,
+                                         Y.total[tile_slot(b->ipt)], st);
     timing_mark_end(KK_COMPRESS, st);
     if (rc) { api_error("unsupported tile shape"); return SZx_NSCS; }
     batch_assemble_launch(b->d_bf(), b->d_base(5), b->nf, Y.aitems, st);
@@ -276,6 +286,7 @@ szx_amd_batch *szx_amd_batch_create(const szx_amd_field *fields, int nfields, in
         B.chunks = p.chunks;
         B.ntiles[0] = s.ntiles[0];
         B.ntiles[1] = s.ntiles[1];
+        B.ntiles[2] = s.ntiles[2];
         B.zwords = (s.L.ws_misc + 64 * sizeof(uint64_t)) / 8;
         B.relv = f.relBoundRatio;
         B.rel = f.errBoundMode == REL;
@@ -293,7 +304,8 @@ szx_amd_batch *szx_amd_batch_create(const szx_amd_field *fields, int nfields, in
         D.ntidx = s.nidx;
         D.w = s.w;
 
-        const uint64_t cnt[NBASE] = {s.ntiles[0], s.ntiles[1], s.range_items, s.nidx, s.zero_items, s.asm_items};
+        const uint64_t cnt[NBASE] = {s.ntiles[0], s.ntiles[1], s.range_items, s.nidx,
+                                     s.zero_items, s.asm_items, s.ntiles[2]};
         for (int k = 0; k < NBASE; k++) {
             base[k * (nfields + 1) + i] = acc[k];
             acc[k] += cnt[k];
@@ -397,7 +409,7 @@ int szx_amd_batch_tune(szx_amd_batch *b, int reps, int *valuesPerLane, int *tile
     if (!b) { api_error("NULL batch"); return SZx_NSCS; }
     if (reps < 1) reps = 1;
     hipStream_t st = static_cast<hipStream_t>(hipStream);
-    const int shapes[4][2] = {{8, 2}, {8, 1}, {4, 2}, {4, 1}};
+    const int shapes[5][2] = {{8, 4}, {8, 2}, {8, 1}, {4, 2}, {4, 1}};
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (!hip_ok(hipEventCreate(&e0), "event") || !hip_ok(hipEventCreate(&e1), "event")) {
         if (e0) (void)hipEventDestroy(e0);

@@ -60,13 +60,13 @@ constexpr uint64_t ASSEMBLE_BYTES_PER_THREAD = 8;
 constexpr uint64_t ASSEMBLE_MAX_BLOCKS = 512;
 
 // One field of a batched launch.  The look-back state is sized for the
-// smaller tiles (one index tile per compressor tile); ntiles[k] is the
-// compressor tile count with k + 1 index tiles per tile.
+// smallest tiles (one index tile per compressor tile); ntiles[k] is the
+// compressor tile count with 1, 2 or 4 index tiles per tile (k = 0, 1, 2).
 struct BatchField {
     CompressParams p;
     AssembleParams a;
     ChunkState *chunks;
-    uint64_t ntiles[2];
+    uint64_t ntiles[3];
     uint64_t zwords;   // words cleared before a compress, from p.tiles (look-back words, totals, error)
     float relv;   // REL ratio (rel != 0)
     int rel;

@@ -917,7 +917,7 @@ __global__ __launch_bounds__(256) void k_compress_batch(const BatchField *bf, co
     const uint64_t g = blockIdx.x;
     const int f = field_of(tbase, nf, g);
     CompressParams p = bf[f].p;
-    p.ntiles = bf[f].ntiles[IPT - 1];
+    p.ntiles = bf[f].ntiles[IPT == 4 ? 2 : IPT - 1];
     if (bf[f].w == 4) compress_tile<float, BS, LV, SF, IPT>(p, g - tbase[f], lds.f);
     else compress_tile<double, BS, LV, SD, IPT>(p, g - tbase[f], lds.d);
 }
@@ -1177,11 +1177,18 @@ static int batch_shape(int lv, int ipt, const BatchField *bf, const uint64_t *tb
                        hipStream_t st)
 {
     if (lv == 8 && BS >= 8) {
-        if (ipt == 2) launch_batch<BS, (BS >= 8 ? 8 : 4), 2>(bf, tbase, nf, total, st);
-        else launch_batch<BS, (BS >= 8 ? 8 : 4), 1>(bf, tbase, nf, total, st);
+        if (ipt == 4) {
+            if constexpr (BS >= 16 && BS <= 128) launch_batch<BS, 8, 4>(bf, tbase, nf, total, st);
+            else return -1;
+        } else if (ipt == 2) {
+            launch_batch<BS, (BS >= 8 ? 8 : 4), 2>(bf, tbase, nf, total, st);
+        } else {
+            launch_batch<BS, (BS >= 8 ? 8 : 4), 1>(bf, tbase, nf, total, st);
+        }
         return 0;
     }
     if (lv == 4) {
+        if (ipt == 4) return -1;
         if (ipt == 2) launch_batch<BS, 4, 2>(bf, tbase, nf, total, st);
         else launch_batch<BS, 4, 1>(bf, tbase, nf, total, st);
         return 0;
@@ -1192,7 +1199,7 @@ static int batch_shape(int lv, int ipt, const BatchField *bf, const uint64_t *tb
 int batch_compress_launch(int bs, int lv, int ipt, const BatchField *bf, const uint64_t *tbase, int nf,
                           uint64_t total, hipStream_t st)
 {
-    if (ipt != 1 && ipt != 2) return -1;
+    if (ipt != 1 && ipt != 2 && ipt != 4) return -1;
     switch (bs) {
     case 4: return batch_shape<4>(lv, ipt, bf, tbase, nf, total, st);
     case 8: return batch_shape<8>(lv, ipt, bf, tbase, nf, total, st);

@@ -96,7 +96,7 @@ def batch(device):
     return bc, xs, ebs, oracle_of(xs, ebs)
 
 
-@pytest.mark.parametrize("shape", [(8, 2), (8, 1), (4, 2), (4, 1)])
+@pytest.mark.parametrize("shape", [(8, 4), (8, 2), (8, 1), (4, 2), (4, 1)])
 def test_batch_shapes_match_oracle(batch, shape):
     bc, xs, ebs, want = batch
     say("shape", shape)
@@ -142,7 +142,7 @@ def test_batch_tune_then_compress(batch):
     bc, xs, ebs, want = batch
     lv, ipt, ms = bc.tune(reps=2)
     say("tuned", lv, ipt, ms)
-    assert (lv, ipt) in {(8, 2), (8, 1), (4, 2), (4, 1)} and ms > 0
+    assert (lv, ipt) in {(8, 4), (8, 2), (8, 1), (4, 2), (4, 1)} and ms > 0
     for o in bc.outs:
         o.zero_()
     bc.compress()
