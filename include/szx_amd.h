@@ -1,13 +1,15 @@
 /*
  * szx_amd.h -- C ABI of libszx_amd.so, an MI355X (gfx950) implementation of
  * SZx's fast codecs: the blockwise one (fastMode = SZx_WITH_BLOCK_FAST_CMPR),
- * its random-access container (fastMode 3 / 4) and the non-blocked one
- * (fastMode 1).
+ * its random-access container (fastMode 3 / 4), the non-blocked one
+ * (fastMode 1), and a reader and writer of the stream the reference's CUDA
+ * codec (cuSZx) produces.
  *
  * Two layers:
  *
  *  1. Drop-in host API.  Same names, argument meaning and ownership as the
- *     reference's szx.h; a caller relinks against libszx_amd.so unchanged.
+ *     reference's szx.h (and cuszx_entry.h); a caller relinks against
+ *     libszx_amd.so unchanged.
  *     Host pointers in, malloc'd host buffers out (release with free()).
  *     Every fastMode (1, 2, 3, 4) runs on the GPU.  Failures return NULL / 0 and set
  *     szx_amd_last_error() instead of exiting the process.
