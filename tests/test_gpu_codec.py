@@ -300,9 +300,11 @@ def test_generic_rel_and_damage(device):
     assert dec.check() == 0
 
 
+@pytest.mark.parametrize("bs", [16, 64, 128, 256])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-def test_incompressible_tiles(dtype, device):
-    """Tiles whose records outgrow the compressor's LDS stage are written
+def test_incompressible_tiles(dtype, bs, device):
+    """Tiles whose records outgrow the compressor's LDS stage (48 KiB for the
+    64 KiB tiles of block sizes 16..128, 20 KiB otherwise) are written
     straight to the payload; mixed with ordinary tiles in one field."""
     import torch
     r = np.random.default_rng(77)
@@ -310,13 +312,13 @@ def test_incompressible_tiles(dtype, device):
     x = r.standard_normal(n).astype(dtype)
     x[100_000:200_000] = np.sin(np.arange(100_000) * 1e-3)     # compressible middle
     for eb in (1e-7, 1e-3):
-        say("incompressible", dtype.__name__, eb)
-        comp = Z.DeviceCompressor(n, dtype, 128)
+        say("incompressible", dtype.__name__, bs, eb)
+        comp = Z.DeviceCompressor(n, dtype, bs)
         comp(torch.from_numpy(x).to(device), Z.ABS, eb)
         view, off, size, _ = comp.stream_view()
         s = view.cpu().numpy().tobytes()
-        assert s == O.compress(x, eb, 128)
-        dec = Z.DeviceDecompressor(n, dtype, 128, size)
+        assert s == O.compress(x, eb, bs)
+        dec = Z.DeviceDecompressor(n, dtype, bs, size)
         dec(view, size, Z.DECODE_REFERENCE, index=comp.index)
         assert dec.check() == 0
         assert bits_equal(dec.out.cpu().numpy(), O.decompress(s, n, dtype, ref_quirk=True))
