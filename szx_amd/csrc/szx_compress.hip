@@ -264,12 +264,13 @@ __device__ inline void stage_bytes(uint8_t *p, U t, int m)
 // the tile, several times slower), so the cap trades resident workgroups per
 // CU against how compressible a tile must be to stay staged: 20 KiB for the
 // 32 KiB tiles (staged above about 1.6x compression), 48 KiB for the 64 KiB
-// tiles (above about 1.33x) -- 44 KiB at block size 32, whose 512 blocks'
-// scan words would otherwise push a workgroup past the LDS share of three
-// per CU (above about 1.45x).
+// tiles (above about 1.33x) -- 44 KiB for 64 KiB tiles of 512 or more blocks
+// (fp32 block sizes 16 and 32, fp64 16), whose per-block scan words would
+// otherwise push a workgroup past the LDS share of three per CU (above about
+// 1.45x).
 constexpr int STAGE_CAP_BYTES = 20480;
 constexpr int STAGE_CAP_WIDE_BYTES = 49152;
-constexpr int STAGE_CAP_WIDE_BS32_BYTES = 45056;
+constexpr int STAGE_CAP_WIDE_MANY_BLOCKS_BYTES = 45056;
 
 // Shape of a compressor tile: T, block size, values per lane, slabs.
 template <typename T, int BS, int LV, int SLABS> struct TileShape {
@@ -279,7 +280,7 @@ template <typename T, int BS, int LV, int SLABS> struct TileShape {
     static constexpr int NBLK = TILE / BS;          // blocks per tile (multiple of 8)
     static constexpr int STAGE = TILE * W + NBLK * (5 + BS / 4);   // largest record bytes
     static constexpr int CAP = TILE * W < 65536 ? STAGE_CAP_BYTES
-                               : BS == 32       ? STAGE_CAP_WIDE_BS32_BYTES
+                               : NBLK >= 512    ? STAGE_CAP_WIDE_MANY_BLOCKS_BYTES
                                                 : STAGE_CAP_WIDE_BYTES;
     static constexpr int STAGE_CAP = STAGE < CAP ? STAGE : CAP;
     static constexpr int STAGE_DW = (STAGE_CAP + 32 + 15) / 16 * 4;
