@@ -7,9 +7,10 @@
  *
  * Two layers:
  *
- *  1. Drop-in host API.  Same names, argument meaning and ownership as the
- *     reference's szx.h (and cuszx_entry.h); a caller relinks against
- *     libszx_amd.so unchanged.
+ *  1. Drop-in host API.  Every function szx.h:74-98 declares (and the two of
+ *     cuszx_entry.h), with the same names, argument meaning and ownership,
+ *     plus szx.h's byte-view unions; a caller relinks against libszx_amd.so
+ *     unchanged.
  *     Host pointers in, malloc'd host buffers out (release with free()).
  *     Every fastMode (1, 2, 3, 4) runs on the GPU.  Failures return NULL / 0 and set
  *     szx_amd_last_error() instead of exiting the process.
@@ -53,7 +54,51 @@ extern "C" {
 #define SZx_BERR -6
 #endif
 
+/* ---- byte-view unions (szx.h:39-72) -----------------------------------------
+ * Same layout as the reference's; skipped when the reference's szx.h was
+ * included first. */
+#ifndef _SZX_H
+typedef union lint16 {
+    unsigned short usvalue;
+    short svalue;
+    unsigned char byte[2];
+} lint16;
+
+typedef union lint32 {
+    int ivalue;
+    unsigned int uivalue;
+    unsigned char byte[4];
+} lint32;
+
+typedef union lint64 {
+    long lvalue;
+    unsigned long ulvalue;
+    unsigned char byte[8];
+} lint64;
+
+typedef union ldouble {
+    double value;
+    unsigned long lvalue;
+    unsigned char byte[8];
+} ldouble;
+
+typedef union lfloat {
+    float value;
+    unsigned int ivalue;
+    unsigned char byte[4];
+} lfloat;
+#endif
+
 /* ---- 1. drop-in host API ---------------------------------------------------- */
+
+/* szx.h:74 / szx.cc:41-71.  Number of dims given: r1 first, the first zero
+ * ends them (0..5). */
+int SZx_computeDimension(size_t r5, size_t r4, size_t r3, size_t r2, size_t r1);
+
+/* szx.h:76 / szx.cc:109-229.  correctedDimension[0..4] = {r1..r5} with the
+ * dims of extent 1 removed the way the reference removes them; returns 1 if
+ * a dim was removed, else 0. */
+int SZx_filterDimension(size_t r5, size_t r4, size_t r3, size_t r2, size_t r1, size_t *correctedDimension);
 
 /* szx.h:75 / szx.cc:73-101.  Product of the non-zero dims, r1 fastest. */
 size_t SZx_computeDataLength(size_t r5, size_t r4, size_t r3, size_t r2, size_t r1);
@@ -113,6 +158,50 @@ double SZx_computeValueRange(void *oriData, int dataType, size_t length, float *
 float SZx_estimateErrorBoundbasedonCR(int dataType, float targetCompressionRatio, float tolerance,
                                       void *data, float initErrorBound, int blockSize,
                                       size_t nbEle);
+
+/* szx.h:97 / szx.cc:501-512 (szx_float.cc:980-1057).  The blocked stream
+ * written into the caller's buffer outputBytes, which the caller sizes (the
+ * reference writes without a bound; szx_amd_compress_bound() is the largest
+ * stream); returns the stream size, 0 on error.  The reference returns 0
+ * for double (its call is commented out); this writes the double stream,
+ * the bytes SZx_fast_compress_args_unpredictable_blocked writes. */
+size_t SZx_fast_compress_args_unpredictable_blocked_args(void *oriData, int dataType, unsigned char *outputBytes,
+                                                         float absErrBound, size_t nbEle, int blockSize);
+
+/* szx.h:82-83 / szx.cc:441-448 (szx_float.cc:1218-1405,
+ * szx_double.cc:1320-1507).  The fastMode-4 random-access container at an
+ * explicit blockSize: powers of two 4..256 (others return NULL). */
+unsigned char *SZx_fast_compress_args_unpredictable_blocked_randomaccess_openmp(void *oriData, int dataType,
+                                                                               size_t *outSize, float absErrBound,
+                                                                               size_t nbEle, int blockSize);
+
+/* szx.h:84 / szx.cc:450-456 (szxd_float.cc:313-399, szxd_double.cc:518-605).
+ * Decode a fastMode-4 container of nbEle values into a malloc'd array; the
+ * container's length comes from its own size table. */
+void *SZx_fast_decompress_args_unpredictable_blocked_randomaccess_openmp(int dataType, size_t nbEle,
+                                                                        unsigned char *cmpBytes);
+
+/* szx.h:79 / szx.cc:353-376 (szxd_float.cc:25-32; deprecated in the
+ * reference).  Decode a non-blocked (fastMode 1) float stream and add
+ * preData[i] to each value; NULL for double, as in the reference. */
+void *SZx_fast_decompress_pred(int dataType, float *preData, unsigned char *curBytes, size_t byteLength,
+                               size_t r5, size_t r4, size_t r3, size_t r2, size_t r1);
+
+/* szx.h:91 / szx.cc:475-485 (szx_float.cc:254-312, szx_double.cc:437-495).
+ * Sample every samplingRate-th block of blockSize values: *buffer (really
+ * T**) receives a malloc'd copy of the sampled blocks, *radiusArray and
+ * *mediusArray malloc'd per-block radius and median (float); returns the
+ * value range of the sampled blocks.  Release the three arrays with free(). */
+float computeRadiusBuffer(void *oriData, int dataType, size_t nbEle, int samplingRate, int blockSize,
+                          float **radiusArray, float **mediusArray, void *buffer);
+
+/* szx.h:93-94 / szx.cc:487-499 (szx_float.cc:501-584, szx_double.cc:497-582).
+ * Modelled compression ratio at errorBound of the blocks computeRadiusBuffer
+ * sampled; *sumReqNbBytes and *sum_actual_leadNumbers receive the model's
+ * sums.  0 on error. */
+float estimateCRbasedonErrorBound_buffered(int dataType, float errorBound, void *buffer, float *medianArray,
+                                           float *radiusArray, int samplingRate, int blockSize, size_t nbEle,
+                                           size_t *sumReqNbBytes, size_t *sum_actual_leadNumbers);
 
 /* Aliases under the pre-SZx names used by older callers
  * (example/testfloat_compress_fastmode2.cu:78). */

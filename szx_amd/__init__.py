@@ -54,6 +54,11 @@ EXPORTS = (
     "cuSZx_fast_decompress_args_unpredictable_blocked_float", "szx_amd_cuszx_workspace",
     "szx_amd_decompress_cuszx_dev", "cuSZx_fast_compress_args_unpredictable_blocked_float",
     "szx_amd_cuszx_capacity", "szx_amd_cuszx_compress_workspace", "szx_amd_compress_cuszx_dev",
+    "SZx_computeDimension", "SZx_filterDimension",
+    "SZx_fast_compress_args_unpredictable_blocked_args",
+    "SZx_fast_compress_args_unpredictable_blocked_randomaccess_openmp",
+    "SZx_fast_decompress_args_unpredictable_blocked_randomaccess_openmp",
+    "SZx_fast_decompress_pred", "computeRadiusBuffer", "estimateCRbasedonErrorBound_buffered",
 )
 BATCH_FUSED, BATCH_PER_FIELD = 0, 1
 OCTAVE_BINS = 512
@@ -113,6 +118,25 @@ def lib() -> C.CDLL:
     L.SZx_computeValueRange.argtypes = [vp, i32, sz, C.POINTER(f32), C.POINTER(f32)]
     L.SZx_estimateErrorBoundbasedonCR.restype = f32
     L.SZx_estimateErrorBoundbasedonCR.argtypes = [i32, f32, f32, vp, f32, i32, sz]
+    L.SZx_computeDimension.restype = i32
+    L.SZx_computeDimension.argtypes = [sz] * 5
+    L.SZx_filterDimension.restype = i32
+    L.SZx_filterDimension.argtypes = [sz] * 5 + [C.POINTER(sz)]
+    L.SZx_fast_compress_args_unpredictable_blocked_args.restype = sz
+    L.SZx_fast_compress_args_unpredictable_blocked_args.argtypes = [vp, i32, vp, f32, sz, i32]
+    L.SZx_fast_compress_args_unpredictable_blocked_randomaccess_openmp.restype = vp
+    L.SZx_fast_compress_args_unpredictable_blocked_randomaccess_openmp.argtypes = [vp, i32, C.POINTER(sz), f32,
+                                                                                   sz, i32]
+    L.SZx_fast_decompress_args_unpredictable_blocked_randomaccess_openmp.restype = vp
+    L.SZx_fast_decompress_args_unpredictable_blocked_randomaccess_openmp.argtypes = [i32, sz, vp]
+    L.SZx_fast_decompress_pred.restype = vp
+    L.SZx_fast_decompress_pred.argtypes = [i32, vp, vp, sz, sz, sz, sz, sz, sz]
+    L.computeRadiusBuffer.restype = f32
+    L.computeRadiusBuffer.argtypes = [vp, i32, sz, i32, i32, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                      C.POINTER(C.c_void_p)]
+    L.estimateCRbasedonErrorBound_buffered.restype = f32
+    L.estimateCRbasedonErrorBound_buffered.argtypes = [i32, f32, vp, vp, vp, i32, i32, sz, C.POINTER(sz),
+                                                       C.POINTER(sz)]
     L.szx_amd_compress_bound.restype = sz
     L.szx_amd_compress_bound.argtypes = [sz, i32, i32]
     L.szx_amd_compress_capacity.restype = sz

@@ -14,6 +14,7 @@ Flags that matter for exactness:
 """
 from __future__ import annotations
 
+import concurrent.futures
 import os
 import subprocess
 import sys
@@ -45,7 +46,7 @@ def build(verbose: bool = False) -> str:
     """Compile every source and link the shared library; returns its path."""
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
-    objs = []
+    cmds, objs = [], []
     for src in SOURCES:
         path = os.path.join(CSRC, src)
         obj = os.path.join(objdir, src + ".o")
@@ -56,10 +57,18 @@ def build(verbose: bool = False) -> str:
                    "-I" + INCLUDE, "-c", path, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
+        cmds.append(cmd)
         objs.append(obj)
+    # the translation units are independent: compile them side by side
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with concurrent.futures.ThreadPoolExecutor(jobs) as pool:
+        for r in pool.map(lambda c: subprocess.run(c), cmds):
+            if r.returncode != 0:
+                raise subprocess.CalledProcessError(r.returncode, r.args)
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs]
+    # export the C ABI of include/szx_amd.h only
+    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}",
+           "-Wl,--version-script=" + os.path.join(CSRC, "exports.map"), "-o", tmp, *objs]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -179,9 +179,13 @@ struct FxrReq {
 
 // Host-in / host-out compress through the device path.  With fxr set the
 // bound is the reference's fixed-rate choice for (target ratio, tolerance).
+// With dst set the stream is copied there (the caller sized it) instead of
+// into a malloc'd buffer.  A look-back wait that overran its bound (another
+// process holding the GPU's compute units for a long time) makes the stream
+// invalid; the compress is then enqueued once more before the call fails.
 unsigned char *host_compress(const void *data, size_t n, int w, int rel_mode, float abs_eb,
                              float rel, int bs, size_t *outSize, const FxrReq *fxr = nullptr,
-                             int fast_mode = SZx_WITH_BLOCK_FAST_CMPR)
+                             int fast_mode = SZx_WITH_BLOCK_FAST_CMPR, unsigned char *dst = nullptr)
 {
     if (outSize) *outSize = 0;
     if (!data || n == 0) {
@@ -208,28 +212,31 @@ unsigned char *host_compress(const void *data, size_t n, int w, int rel_mode, fl
             return nullptr;
         rel_mode = 0;
     }
-    if (compress_launch(c.in.p, n, w, rel_mode, abs_eb, rel, bs, (uint8_t *)c.out.p,
-                        (uint8_t *)c.ws.p, (uint64_t *)c.res.p, nullptr, c.stream, fast_mode)) {
-        set_err("compress launch failed");
-        return nullptr;
-    }
     uint64_t res[4] = {0, 0, 0, 0};
-    if (!hip_ok(hipMemcpyAsync(res, c.res.p, sizeof res, hipMemcpyDeviceToHost, c.stream),
-                "hipMemcpyAsync D2H") ||
-        !hip_ok(hipStreamSynchronize(c.stream), "compress"))
-        return nullptr;
+    for (int attempt = 0; attempt < 2; attempt++) {
+        if (compress_launch(c.in.p, n, w, rel_mode, abs_eb, rel, bs, (uint8_t *)c.out.p,
+                            (uint8_t *)c.ws.p, (uint64_t *)c.res.p, nullptr, c.stream, fast_mode)) {
+            set_err("compress launch failed");
+            return nullptr;
+        }
+        if (!hip_ok(hipMemcpyAsync(res, c.res.p, sizeof res, hipMemcpyDeviceToHost, c.stream),
+                    "hipMemcpyAsync D2H") ||
+            !hip_ok(hipStreamSynchronize(c.stream), "compress"))
+            return nullptr;
+        if (!res[3]) break;
+    }
     if (res[3]) {
-        set_err("compress: a look-back wait overran its bound (device busy?)");
+        set_err("compress: a look-back wait overran its bound twice (device busy?)");
         return nullptr;
     }
-    unsigned char *out = (unsigned char *)malloc(res[1] ? res[1] : 1);
+    unsigned char *out = dst ? dst : (unsigned char *)malloc(res[1] ? res[1] : 1);
     if (!out) {
         set_err("malloc(%llu) failed", (unsigned long long)res[1]);
         return nullptr;
     }
     if (!hip_ok(hipMemcpy(out, (uint8_t *)c.out.p + res[0], res[1], hipMemcpyDeviceToHost),
                 "hipMemcpy D2H stream")) {
-        free(out);
+        if (!dst) free(out);
         return nullptr;
     }
     if (outSize) *outSize = res[1];
@@ -396,6 +403,70 @@ bool cuszx_stream_length(const unsigned char *s, size_t n, size_t *len)
     return true;
 }
 
+// Length of a fastMode-2 stream from its own structure (header, bitmap,
+// constant medians, then each record's size from its lead codes).
+size_t stream_bytes(const unsigned char *s, size_t nbEle, int w)
+{
+    const size_t bs = s[4] ? s[4] : 256;
+    const size_t nb = (nbEle + bs - 1) / bs, nbit = (nb + 7) / 8;
+    uint64_t nconst = 0;
+    for (int k = 0; k < 8; k++) nconst = (nconst << 8) | s[5 + k];
+    if (nconst > nb) {
+        set_err("corrupt stream header");
+        return 0;
+    }
+    size_t q = 13 + nbit + 4 * nconst;
+    for (size_t b = 0; b < nb; b++) {
+        if (!((s[13 + (b >> 3)] >> (7 - (b & 7))) & 1)) continue;
+        const size_t nel = (b + 1) * bs <= nbEle ? bs : nbEle - b * bs;
+        const int rl = s[q];
+        if (rl == 0 || rl > 8 * w) {
+            set_err("corrupt record at byte %zu", q);
+            return 0;
+        }
+        const int rb = (rl + 7) / 8;
+        size_t sz = 5 + (nel + 3) / 4;
+        for (size_t i = 0; i < nel; i++) {
+            const int ld = (s[q + 5 + (i >> 2)] >> (6 - 2 * (int)(i & 3))) & 3;
+            if (ld < rb) sz += (size_t)(rb - ld);
+        }
+        q += sz;
+    }
+    return q;
+}
+
+// Length of a random-access container (fastMode 3 / 4) for nbEle values from
+// its own header: [1,1,1,1], big-endian u64 blockSize and constant count,
+// the native u16 record sizes, bitmap, constant medians, records
+// (szxd_float.cc:313-366 walks the same fields).  0 on a malformed header.
+size_t ra_stream_bytes(const unsigned char *s, size_t nbEle, int w)
+{
+    uint64_t bs = 0, nconst = 0;
+    for (int k = 0; k < 8; k++) bs = (bs << 8) | s[4 + k];
+    for (int k = 0; k < 8; k++) nconst = (nconst << 8) | s[12 + k];
+    if (s[3] != 1 || bs < 1 || bs > 256) {
+        set_err("not a random-access container, or blockSize %llu unsupported", (unsigned long long)bs);
+        return 0;
+    }
+    const size_t nb = (nbEle + bs - 1) / bs, nbit = (nb + 7) / 8;
+    if (nconst > nb) {
+        set_err("corrupt container header");
+        return 0;
+    }
+    const size_t nrec = nb - nconst;
+    const size_t rec_max = 5 + (bs + 3) / 4 + (size_t)bs * w;
+    size_t q = 20 + 2 * nrec + nbit + 4 * nconst;
+    for (size_t j = 0; j < nrec; j++) {
+        const size_t sz = (size_t)s[20 + 2 * j] | (size_t)s[21 + 2 * j] << 8;
+        if (sz < 5 || sz > rec_max) {
+            set_err("corrupt record size %zu at record %zu", sz, j);
+            return 0;
+        }
+        q += sz;
+    }
+    return q;
+}
+
 }  // namespace
 
 namespace szxamd {
@@ -548,8 +619,6 @@ unsigned char *SZx_fast_compress_args_unpredictable_blocked(void *oriData, int d
     return host_compress(oriData, nbEle, w, 0, absErrBound, 0.0f, blockSize, outSize);
 }
 
-size_t szx_amd_stream_bytes(const unsigned char *s, size_t nbEle, int w);
-
 void SZx_fast_decompress_args_unpredictable_blocked(void *newData, int dataType, size_t nbEle,
                                                     unsigned char *cmpBytes)
 {
@@ -566,7 +635,7 @@ void SZx_fast_decompress_args_unpredictable_blocked(void *newData, int dataType,
         return;
     }
     // The signature carries no length: walk the stream's own structure.
-    const size_t len = szx_amd_stream_bytes(cmpBytes, nbEle, w);
+    const size_t len = stream_bytes(cmpBytes, nbEle, w);
     if (!len) return;
     *dst = host_decompress(cmpBytes, len, nbEle, w, 0);
 }
@@ -643,38 +712,6 @@ float SZx_estimateErrorBoundbasedonCR(int dataType, float targetCompressionRatio
                           initErrorBound, (uint8_t *)c.fx.p, c.stream, &eb, nullptr))
         return 0.0f;
     return eb;
-}
-
-// Length of a fastMode-2 stream from its own structure (header, bitmap,
-// constant medians, then each record's size from its lead codes).
-size_t szx_amd_stream_bytes(const unsigned char *s, size_t nbEle, int w)
-{
-    const size_t bs = s[4] ? s[4] : 256;
-    const size_t nb = (nbEle + bs - 1) / bs, nbit = (nb + 7) / 8;
-    uint64_t nconst = 0;
-    for (int k = 0; k < 8; k++) nconst = (nconst << 8) | s[5 + k];
-    if (nconst > nb) {
-        set_err("corrupt stream header");
-        return 0;
-    }
-    size_t q = 13 + nbit + 4 * nconst;
-    for (size_t b = 0; b < nb; b++) {
-        if (!((s[13 + (b >> 3)] >> (7 - (b & 7))) & 1)) continue;
-        const size_t nel = (b + 1) * bs <= nbEle ? bs : nbEle - b * bs;
-        const int rl = s[q];
-        if (rl == 0 || rl > 8 * w) {
-            set_err("corrupt record at byte %zu", q);
-            return 0;
-        }
-        const int rb = (rl + 7) / 8;
-        size_t sz = 5 + (nel + 3) / 4;
-        for (size_t i = 0; i < nel; i++) {
-            const int ld = (s[q + 5 + (i >> 2)] >> (6 - 2 * (int)(i & 3))) & 3;
-            if (ld < rb) sz += (size_t)(rb - ld);
-        }
-        q += sz;
-    }
-    return q;
 }
 
 size_t szx_amd_compress_bound(size_t n, int dataType, int blockSize)
@@ -1150,6 +1187,252 @@ int szx_amd_mlp_adam_dev(float *d_params, const float *d_grad, float *d_m, float
     }
     mlp_adam_launch(d_params, d_grad, d_m, d_v, lr, beta1, beta2, eps, step, (hipStream_t)hipStream);
     return SZx_SCES;
+}
+
+// ---- the rest of szx.h (szx.h:74-98) ------------------------------------------
+
+int SZx_computeDimension(size_t r5, size_t r4, size_t r3, size_t r2, size_t r1)
+{
+    // szx.cc:41-71: the number of leading non-zero dims, r1 first
+    if (r1 == 0) return 0;
+    if (r2 == 0) return 1;
+    if (r3 == 0) return 2;
+    if (r4 == 0) return 3;
+    if (r5 == 0) return 4;
+    return 5;
+}
+
+int SZx_filterDimension(size_t r5, size_t r4, size_t r3, size_t r2, size_t r1, size_t *correctedDimension)
+{
+    // szx.cc:109-229: c[] = {r1..r5}; each dim of extent 1 is dropped (tested
+    // from the slowest present dim down to r1), the dims above it shifting
+    // down one place.  Returns 1 if a dim was dropped, 0 if not, 2 for a 1-D
+    // array of no values.  The reference's shifts are kept as they are
+    // (which slot a removal clears differs between ranks).
+    const int dim = SZx_computeDimension(r5, r4, r3, r2, r1);
+    size_t *c = correctedDimension;
+    c[0] = r1;
+    c[1] = r2;
+    c[2] = r3;
+    c[3] = r4;
+    c[4] = r5;
+    const size_t r[5] = {r1, r2, r3, r4, r5};
+    if (dim == 1) return r1 < 1 ? 2 : 0;
+    if (dim < 2) return 0;
+    int changed = 0;
+    // the slowest dim of extent 1 is cleared in place
+    if (r[dim - 1] == 1) {
+        c[dim - 1] = 0;
+        changed = 1;
+    }
+    // every faster dim of extent 1 is removed by shifting the slower ones down
+    // over it; the vacated top slot becomes 0 (for ranks below 5 the
+    // reference shifts the absent dim's 0 into it)
+    for (int d = dim - 2; d >= 0; d--) {
+        if (r[d] != 1) continue;
+        for (int k = d; k < dim - 1; k++) c[k] = c[k + 1];
+        c[dim - 1] = 0;
+        changed = 1;
+    }
+    return changed;
+}
+
+size_t SZx_fast_compress_args_unpredictable_blocked_args(void *oriData, int dataType, unsigned char *outputBytes,
+                                                         float absErrBound, size_t nbEle, int blockSize)
+{
+    // szx.cc:501-512 -> szx_float.cc:980-1057: the blocked stream written to
+    // the caller's buffer, its size returned.  The reference's double branch
+    // returns 0 ("not implemented yet", with the double call commented out);
+    // here it writes the double stream (szx_double.cc:1160-1238 bytes).
+    g_err.clear();
+    const int w = width_of(dataType);
+    if (!w || !outputBytes) {
+        set_err("unsupported dataType %d or NULL output", dataType);
+        return 0;
+    }
+    size_t out = 0;
+    if (!host_compress(oriData, nbEle, w, 0, absErrBound, 0.0f, blockSize, &out, nullptr,
+                       SZx_WITH_BLOCK_FAST_CMPR, outputBytes))
+        return 0;
+    return out;
+}
+
+unsigned char *SZx_fast_compress_args_unpredictable_blocked_randomaccess_openmp(void *oriData, int dataType,
+                                                                               size_t *outSize, float absErrBound,
+                                                                               size_t nbEle, int blockSize)
+{
+    // szx.cc:441-448 -> szx_float.cc:1218-1405 / szx_double.cc:1320-1507: the
+    // fastMode-4 container at an explicit blockSize (powers of two 4..256).
+    g_err.clear();
+    if (outSize) *outSize = 0;
+    const int w = width_of(dataType);
+    if (!w) {
+        set_err("unsupported dataType %d", dataType);
+        return nullptr;
+    }
+    if (!bs_tiled(blockSize)) {
+        set_err("the random-access container takes blockSize 4..256, powers of two (got %d)", blockSize);
+        return nullptr;
+    }
+    return host_compress(oriData, nbEle, w, 0, absErrBound, 0.0f, blockSize, outSize, nullptr,
+                         SZx_OPENMP_FAST_CMPR);
+}
+
+void *SZx_fast_decompress_args_unpredictable_blocked_randomaccess_openmp(int dataType, size_t nbEle,
+                                                                        unsigned char *cmpBytes)
+{
+    // szx.cc:450-456 -> szxd_float.cc:313-399 / szxd_double.cc:518-605.  The
+    // signature carries no length: it is read from the container's size table.
+    g_err.clear();
+    const int w = width_of(dataType);
+    if (!w || !cmpBytes || nbEle == 0) {
+        set_err("unsupported dataType %d, NULL stream or no values", dataType);
+        return nullptr;
+    }
+    const size_t len = ra_stream_bytes(cmpBytes, nbEle, w);
+    if (!len) return nullptr;
+    return host_decompress(cmpBytes, len, nbEle, w, 0, SZx_OPENMP_FAST_CMPR);
+}
+
+void *SZx_fast_decompress_pred(int dataType, float *preData, unsigned char *curBytes, size_t byteLength,
+                               size_t r5, size_t r4, size_t r3, size_t r2, size_t r1)
+{
+    // szx.cc:353-376 -> szxd_float.cc:25-32: decode a non-blocked (fastMode 1)
+    // stream and add the prediction preData[i] to every value.  The reference
+    // returns NULL for double (its branch is commented out), and so does this.
+    g_err.clear();
+    if (dataType != SZx_FLOAT) {
+        set_err("SZx_fast_decompress_pred decodes float only (the reference returns NULL for double)");
+        return nullptr;
+    }
+    const size_t n = data_length(r5, r4, r3, r2, r1);
+    if (!preData || !curBytes || n == 0) {
+        set_err("NULL prediction or stream, or no values");
+        return nullptr;
+    }
+    HostCtx &c = g_ctx;
+    if (!c.ready()) return nullptr;
+    if (!c.in.reserve(byteLength ? byteLength : 1) || !c.out.reserve(n * 4) || !c.ws.reserve(nb_workspace(n)) ||
+        !c.res.reserve(64) || !c.fx.reserve(n * 4))
+        return nullptr;
+    if (!hip_ok(hipMemcpyAsync(c.in.p, curBytes, byteLength, hipMemcpyHostToDevice, c.stream), "H2D") ||
+        !hip_ok(hipMemcpyAsync(c.fx.p, preData, n * 4, hipMemcpyHostToDevice, c.stream), "H2D prediction"))
+        return nullptr;
+    nb_decompress_launch((const uint8_t *)c.in.p, byteLength, n, 4, c.out.p, (uint8_t *)c.ws.p, 0,
+                         (int32_t *)c.res.p, c.stream);
+    add_pred_launch((float *)c.out.p, (const float *)c.fx.p, n, c.stream);
+    int32_t status = 0;
+    if (!hip_ok(hipMemcpyAsync(&status, c.res.p, 4, hipMemcpyDeviceToHost, c.stream), "D2H") ||
+        !hip_ok(hipStreamSynchronize(c.stream), "decompress"))
+        return nullptr;
+    if (status != 0) {
+        set_err("malformed stream (code %d)", status);
+        return nullptr;
+    }
+    float *out = (float *)malloc(n * 4);
+    if (!out) {
+        set_err("malloc failed");
+        return nullptr;
+    }
+    if (!hip_ok(hipMemcpy(out, c.out.p, n * 4, hipMemcpyDeviceToHost), "D2H output")) {
+        free(out);
+        return nullptr;
+    }
+    return out;
+}
+
+float computeRadiusBuffer(void *oriData, int dataType, size_t nbEle, int samplingRate, int blockSize,
+                          float **radiusArray, float **mediusArray, void *buffer)
+{
+    // szx.cc:475-485 -> szx_float.cc:254-312 / szx_double.cc:437-495: copy
+    // every samplingRate-th block into *buffer (malloc'd), each sampled
+    // block's radius and median (as float) into *radiusArray / *mediusArray
+    // (malloc'd, one spare entry as in the reference), and return the range
+    // of the sampled values (extremes starting from oriData[0]).  The block
+    // statistics run on the GPU (k_fxr_sample, k_fxr_range); the sampled
+    // blocks come back with one strided copy.
+    g_err.clear();
+    const int w = width_of(dataType);
+    void **buf = (void **)buffer;
+    if (!w || !oriData || !radiusArray || !mediusArray || !buf || nbEle == 0 || samplingRate < 1 ||
+        blockSize < 1 || blockSize > 256) {
+        set_err("invalid arguments");
+        return 0.0f;
+    }
+    const size_t stride = (size_t)samplingRate * blockSize, ns = nbEle / stride;
+    float *rad = (float *)malloc(sizeof(float) * (ns + 1));
+    float *med = (float *)malloc(sizeof(float) * (ns + 1));
+    void *smp = malloc((size_t)w * (ns + 1) * blockSize);
+    if (!rad || !med || !smp) {
+        free(rad);
+        free(med);
+        free(smp);
+        set_err("malloc failed");
+        return 0.0f;
+    }
+    *radiusArray = rad;
+    *mediusArray = med;
+    *buf = smp;
+    if (ns == 0) return 0.0f;   // no block sampled: both extremes are oriData[0]
+    HostCtx &c = g_ctx;
+    const FxrLayout L = fxr_layout(nbEle, w, blockSize, samplingRate);
+    float range = 0.0f;
+    if (!c.ready() || !c.in.reserve(nbEle * (size_t)w) || !c.fx.reserve(L.ws_bytes) ||
+        !hip_ok(hipMemcpyAsync(c.in.p, oriData, nbEle * (size_t)w, hipMemcpyHostToDevice, c.stream), "H2D"))
+        return 0.0f;
+    fxr_sample_launch(c.in.p, nbEle, w, blockSize, samplingRate, (uint8_t *)c.fx.p, c.stream);
+    if (!hip_ok(hipMemcpyAsync(rad, (uint8_t *)c.fx.p + L.ws_rad, 4 * ns, hipMemcpyDeviceToHost, c.stream), "D2H") ||
+        !hip_ok(hipMemcpyAsync(med, (uint8_t *)c.fx.p + L.ws_med, 4 * ns, hipMemcpyDeviceToHost, c.stream), "D2H") ||
+        !hip_ok(hipMemcpyAsync(&range, (uint8_t *)c.fx.p + L.ws_range, 4, hipMemcpyDeviceToHost, c.stream), "D2H") ||
+        !hip_ok(hipMemcpy2DAsync(smp, (size_t)w * blockSize, c.in.p, (size_t)w * stride, (size_t)w * blockSize, ns,
+                                 hipMemcpyDeviceToHost, c.stream), "D2H sample") ||
+        !hip_ok(hipStreamSynchronize(c.stream), "radius buffer"))
+        return 0.0f;
+    return range;
+}
+
+float estimateCRbasedonErrorBound_buffered(int dataType, float errorBound, void *buffer, float *medianArray,
+                                           float *radiusArray, int samplingRate, int blockSize, size_t nbEle,
+                                           size_t *sumReqNbBytes, size_t *sum_actual_leadNumbers)
+{
+    // szx.cc:487-499 -> szx_float.cc:501-584 / szx_double.cc:497-582: the
+    // modelled compression ratio of the nbEle / (blockSize * samplingRate)
+    // buffered blocks at errorBound.  The three sums (constant blocks, sum of
+    // reqBytes, sum of min(lead, reqBytes)) are computed on the GPU by the
+    // same kernel the fixed-rate search uses; the ratio formula is evaluated
+    // here in the reference's float arithmetic.
+    g_err.clear();
+    const int w = width_of(dataType);
+    if (!w || !buffer || !medianArray || !radiusArray || nbEle == 0 || samplingRate < 1 || blockSize < 1 ||
+        blockSize > 256) {
+        set_err("invalid arguments");
+        return 0.0f;
+    }
+    const size_t ns = nbEle / ((size_t)blockSize * samplingRate);
+    const size_t nv = ns * (size_t)blockSize;
+    // the buffered blocks are contiguous: a 1-in-1 sample of ns blocks
+    const FxrLayout L = fxr_layout(nv, w, blockSize, 1);
+    HostCtx &c = g_ctx;
+    uint64_t sums[3] = {0, 0, 0};
+    if (ns) {
+        if (!c.ready() || !c.in.reserve(nv * (size_t)w) || !c.fx.reserve(L.ws_bytes) ||
+            !hip_ok(hipMemcpyAsync(c.in.p, buffer, nv * (size_t)w, hipMemcpyHostToDevice, c.stream), "H2D") ||
+            !hip_ok(hipMemcpyAsync((uint8_t *)c.fx.p + L.ws_med, medianArray, 4 * ns, hipMemcpyHostToDevice,
+                                   c.stream), "H2D") ||
+            !hip_ok(hipMemcpyAsync((uint8_t *)c.fx.p + L.ws_rad, radiusArray, 4 * ns, hipMemcpyHostToDevice,
+                                   c.stream), "H2D") ||
+            !hip_ok(hipMemcpyAsync((uint8_t *)c.fx.p + L.ws_ebs, &errorBound, 4, hipMemcpyHostToDevice, c.stream),
+                    "H2D"))
+            return 0.0f;
+        fxr_estimate_launch(c.in.p, nv, w, blockSize, 1, (uint8_t *)c.fx.p, 1, c.stream);
+        if (!hip_ok(hipMemcpyAsync(sums, (uint8_t *)c.fx.p + L.ws_sums, 24, hipMemcpyDeviceToHost, c.stream),
+                    "D2H sums") ||
+            !hip_ok(hipStreamSynchronize(c.stream), "model"))
+            return 0.0f;
+    }
+    if (sumReqNbBytes) *sumReqNbBytes = sums[1];
+    if (sum_actual_leadNumbers) *sum_actual_leadNumbers = sums[2];
+    return fxr_model_cr(w, blockSize, nbEle, ns, FxrSums{sums[0], sums[1], sums[2]});
 }
 
 }  // extern "C"

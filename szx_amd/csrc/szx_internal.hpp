@@ -111,6 +111,8 @@ void nb_compress_launch(const void *d_in, uint64_t n, int w, int rel_mode, float
                         uint8_t *d_ws, uint64_t *d_result, hipStream_t st);
 void nb_decompress_launch(const uint8_t *d_stream, uint64_t len, uint64_t n, int w, void *d_out, uint8_t *d_ws,
                           int exact, int32_t *d_status, hipStream_t st);
+// d_x[i] += d_pred[i] (float), the prediction step of SZx_fast_decompress_pred.
+void add_pred_launch(float *d_x, const float *d_pred, uint64_t n, hipStream_t st);
 
 // Checked header of a stream (k_check_header): block size, blocks, bitmap
 // bytes, constant blocks, first payload byte.

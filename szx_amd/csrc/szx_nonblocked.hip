@@ -535,4 +535,19 @@ void nb_decompress_launch(const uint8_t *d_stream, uint64_t len, uint64_t n, int
     timing_mark_end(KK_DECODE, st);
 }
 
+// SZx_fast_decompress_pred (szxd_float.cc:25-32): x[i] += pred[i] after a
+// non-blocked decode, one IEEE float addition per value (no contraction).
+__global__ __launch_bounds__(256) void k_add_pred(float *x, const float *pred, uint64_t n)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += stride) x[i] = x[i] + pred[i];
+}
+
+void add_pred_launch(float *d_x, const float *d_pred, uint64_t n, hipStream_t st)
+{
+    uint64_t g = (n + 255) / 256;
+    if (g > 8192) g = 8192;
+    if (g) hipLaunchKernelGGL(k_add_pred, dim3((unsigned)g), dim3(256), 0, st, d_x, d_pred, n);
+}
+
 }  // namespace szxamd

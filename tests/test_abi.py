@@ -91,3 +91,59 @@ def test_block_size_range_without_gpu():
     # the fixed-rate solver takes the tiled block sizes only
     assert L.szx_amd_fixed_rate_workspace(n, F, 100) == 0
     assert L.szx_amd_fixed_rate_workspace(n, F, 128) > 0
+
+
+def test_library_exports_nothing_undeclared():
+    """Every dynamic symbol the library defines is declared in the header
+    (the C++ internals stay local)."""
+    import subprocess
+    out = subprocess.run(["nm", "-D", "--defined-only", szx_amd.LIB_PATH], check=True,
+                         capture_output=True, text=True).stdout
+    defined = sorted({ln.split()[-1] for ln in out.splitlines() if ln.strip()})
+    extra = [s for s in defined if s not in declared_functions()]
+    assert not extra, extra
+
+
+def _dim_grid():
+    import itertools
+    for r in itertools.product((0, 1, 2, 3), repeat=5):
+        yield r   # (r5, r4, r3, r2, r1), zeros anywhere
+
+
+def test_dimension_helpers_match_the_compiled_reference():
+    """SZx_computeDimension / SZx_filterDimension / SZx_computeDataLength
+    against the reference's own functions (szx.cc:41-229), every dims tuple
+    over {0,1,2,3}^5 (no GPU call)."""
+    import pytest
+    import oracle as O
+    if not O.ref_available():
+        pytest.skip("compiled reference not built (oracle/Makefile ref)")
+    R = O.ref()
+    L = szx_amd.lib()
+    sz = C.c_size_t
+    for f in (R.SZx_computeDimension, R.SZx_filterDimension, R.SZx_computeDataLength):
+        f.argtypes = [sz] * 5 + ([C.POINTER(sz)] if f is R.SZx_filterDimension else [])
+    R.SZx_computeDataLength.restype = sz
+    for r in _dim_grid():
+        assert L.SZx_computeDimension(*r) == R.SZx_computeDimension(*r), r
+        assert L.SZx_computeDataLength(*r) == R.SZx_computeDataLength(*r), r
+        a, b = (sz * 5)(), (sz * 5)()
+        assert L.SZx_filterDimension(*r, a) == R.SZx_filterDimension(*r, b), r
+        assert list(a) == list(b), r
+
+
+def test_c_caller_builds_against_both_headers_and_runs_host_only_modes():
+    """The C caller compiled against include/szx_amd.h (and against the
+    reference's szx.h, where it was built) links to libszx_amd.so; its
+    host-only modes (unions, dims) run without a GPU and agree."""
+    import subprocess
+    import pytest
+    here = os.path.join(os.path.dirname(__file__), "c_abi")
+    exe = os.path.join(here, "szx_caller")
+    assert os.path.exists(exe), "tests/c_abi/szx_caller is not built (make -C tests/c_abi)"
+    assert subprocess.run([exe, "union"], capture_output=True).returncode == 0
+    dims = subprocess.run([exe, "dims"], check=True, capture_output=True, text=True).stdout
+    ref_exe = os.path.join(here, "szx_caller_refhdr")
+    if not os.path.exists(ref_exe):
+        pytest.skip("caller against the reference's szx.h not built")
+    assert subprocess.run([ref_exe, "dims"], check=True, capture_output=True, text=True).stdout == dims
