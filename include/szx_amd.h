@@ -258,6 +258,21 @@ int szx_amd_compress_dev(const void *d_in, size_t n, int dataType, int errBoundM
                          size_t outCapacity, void *d_work, size_t workBytes, uint64_t *d_result,
                          uint64_t *d_index, void *hipStream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops that forbid hipMalloc):
+ * szx_amd_compress_dev without a workspace argument.  The workspace lives in
+ * a static device array of SZX_AMD_NOALLOC_SCRATCH_BYTES that each GPU holds
+ * until the process ends; the call allocates nothing.  A compress whose
+ * szx_amd_compress_workspace() exceeds the array returns SZx_MERR (16 MiB,
+ * about 6 bytes per block: 2^27 values at blockSize 64..256, 2^26 at
+ * 32..256, 2^24 at 8..256).  Calls on one GPU share its array: calls that
+ * overlap in time on one GPU race (one at a time per GPU).  Other arguments
+ * and results as szx_amd_compress_dev. */
+#define SZX_AMD_NOALLOC_SCRATCH_BYTES (16u << 20)
+int szx_amd_compress_dev_noalloc(const void *d_in, size_t n, int dataType, int errBoundMode,
+                                 float absErrBound, float relBoundRatio, int blockSize, void *d_out,
+                                 size_t outCapacity, uint64_t *d_result, uint64_t *d_index, void *hipStream);
+
 /* Workspace bytes for szx_amd_decompress_dev (0 for an unsupported blockSize). */
 size_t szx_amd_decompress_workspace(size_t n, int dataType, int blockSize, size_t streamBytes);
 

@@ -28,6 +28,7 @@ SZx_NO_BLOCK_FAST_CMPR = 1
 SZx_WITH_BLOCK_FAST_CMPR = 2
 SZx_RANDOMACCESS_FAST_CMPR = 3
 SZx_OPENMP_FAST_CMPR = 4
+SZx_SCES, SZx_NSCS, SZx_FERR, SZx_TERR, SZx_DERR, SZx_MERR, SZx_BERR = 0, -1, -2, -3, -4, -5, -6
 DECODE_REFERENCE = 0
 DECODE_EXACT = 1
 
@@ -59,6 +60,7 @@ EXPORTS = (
     "SZx_fast_compress_args_unpredictable_blocked_randomaccess_openmp",
     "SZx_fast_decompress_args_unpredictable_blocked_randomaccess_openmp",
     "SZx_fast_decompress_pred", "computeRadiusBuffer", "estimateCRbasedonErrorBound_buffered",
+    "szx_amd_compress_dev_noalloc",
 )
 BATCH_FUSED, BATCH_PER_FIELD = 0, 1
 OCTAVE_BINS = 512
@@ -66,6 +68,7 @@ RATE_BISECT, RATE_LEARNED = 0, 1
 RATE_MODEL_PARAMS = 5825
 RATE_MODEL_INPUTS = 24
 RATE_CHUNK = 524288
+NOALLOC_SCRATCH_BYTES = 16 << 20
 RATE_MODEL_PATH = os.path.join(HERE, "rate_model_fp32.bin")
 
 
@@ -146,6 +149,8 @@ def lib() -> C.CDLL:
     L.szx_amd_compress_dev.restype = i32
     L.szx_amd_compress_dev.argtypes = [vp, sz, i32, i32, f32, f32, i32, vp, sz, vp, sz, vp, vp,
                                        vp]
+    L.szx_amd_compress_dev_noalloc.restype = i32
+    L.szx_amd_compress_dev_noalloc.argtypes = [vp, sz, i32, i32, f32, f32, i32, vp, sz, vp, vp, vp]
     L.szx_amd_index_words.restype = sz
     L.szx_amd_index_words.argtypes = [sz, i32, i32]
     L.szx_amd_decompress_workspace.restype = sz

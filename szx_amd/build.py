@@ -27,7 +27,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HOSTCXX = os.environ.get("CXX", "g++")
 ARCH = os.environ.get("SZX_AMD_ARCH", "gfx950")
 
-SOURCES = ["szx_compress.hip", "szx_decompress.hip", "szx_index.hip", "szx_generic.hip", "szx_nonblocked.hip", "szx_cuszx.hip", "szx_fxr.hip", "szx_mlp.hip",
+SOURCES = ["szx_compress.hip", "szx_decompress.hip", "szx_index.hip", "szx_generic.hip", "szx_nonblocked.hip", "szx_cuszx.hip", "szx_fxr.hip", "szx_mlp.hip", "szx_noalloc.hip",
            "szx_api.cpp", "szx_fxr_host.cpp", "szx_rate.cpp", "szx_batch.cpp"]
 HEADERS = ["szx_common.hpp", "szx_internal.hpp", "szx_wave.hpp", "szx_fxr.hpp",
            "szx_fxr_host.hpp", "szx_mlp.hpp", "szx_rate.hpp", "szx_index.hpp", "szx_batch.hpp"]
@@ -51,7 +51,7 @@ def build(verbose: bool = False) -> str:
         path = os.path.join(CSRC, src)
         obj = os.path.join(objdir, src + ".o")
         if src.endswith(".hip"):
-            cmd = [HIPCC, "-x", "hip", *COMMON, *DEVICE, "-c", path, "-o", obj]
+            cmd = [HIPCC, "-x", "hip", *COMMON, *DEVICE, "-I" + INCLUDE, "-c", path, "-o", obj]
         else:   # host-only C++ against the HIP runtime headers
             cmd = [HOSTCXX, *COMMON, "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
                    "-I" + INCLUDE, "-c", path, "-o", obj]

@@ -481,6 +481,12 @@ void api_error(const char *fmt, ...)
 }
 void api_clear_error() { g_err.clear(); }
 
+bool timing_set(bool on)
+{
+    const bool was = g_tim.on;
+    g_tim.on = on;
+    return was;
+}
 void timing_mark_begin(int kind, hipStream_t st)
 {
     TimingState &t = g_tim;

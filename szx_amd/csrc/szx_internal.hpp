@@ -54,6 +54,8 @@ enum KernelKind { KK_COMPRESS = 0, KK_DECODE = 1, KK_INDEX = 2, KK_COUNT = 3 };
 void api_error(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
 void api_clear_error();
 
+// Turn this thread's kernel timing on or off; returns the previous setting.
+bool timing_set(bool on);
 void timing_mark_begin(int kind, hipStream_t st);
 void timing_mark_end(int kind, hipStream_t st);
 
