@@ -61,7 +61,7 @@ def test_codec_from_c(caller, name, tmp_path, device):
     g = load(name)
     x = g["x"]
     x.tofile(tmp_path / "in.bin")
-    mode, bound = ("rel", g["rel"]) if "rel" in g else ("abs", g["eb"])
+    mode, bound = ("rel", g["rel"]) if "_rel" in name else ("abs", g["eb"])
     run(caller, "codec", dt(x), mode, f9(bound), x.size, tmp_path / "in.bin", tmp_path / "s.bin", tmp_path / "d.bin")
     assert (tmp_path / "s.bin").read_bytes() == g["stream"].tobytes()
     assert (tmp_path / "d.bin").read_bytes() == g["decoded"].tobytes()
