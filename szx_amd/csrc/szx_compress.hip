@@ -37,6 +37,11 @@ namespace szxamd {
 
 template <typename T> struct SlabLoad;
 template <> struct SlabLoad<float> {
+    __device__ static void load4_full(const float *x, float *v)
+    {
+        const float4 f = *reinterpret_cast<const float4 *>(x);
+        v[0] = f.x; v[1] = f.y; v[2] = f.z; v[3] = f.w;
+    }
     __device__ static void load4(const float *x, uint64_t e0, uint64_t n, float *v)
     {
         if (e0 + 4 <= n) {
@@ -48,6 +53,12 @@ template <> struct SlabLoad<float> {
     }
 };
 template <> struct SlabLoad<double> {
+    __device__ static void load4_full(const double *x, double *v)
+    {
+        const double2 a = *reinterpret_cast<const double2 *>(x);
+        const double2 b = *reinterpret_cast<const double2 *>(x + 2);
+        v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+    }
     __device__ static void load4(const double *x, uint64_t e0, uint64_t n, double *v)
     {
         if (e0 + 4 <= n) {
@@ -140,12 +151,26 @@ __device__ inline void minmax_float(const T (&v)[LV], const bool (&valid)[LV], T
             mx = vmax(mx, v[k]);
         }
     }
-    mn = from_bits<T>(seg_reduce<SEG>(to_bits<T>(mn), [](U a, U b) {
-        return to_bits<T>(vmin(from_bits<T>(a), from_bits<T>(b)));
-    }));
-    mx = from_bits<T>(seg_reduce<SEG>(to_bits<T>(mx), [](U a, U b) {
-        return to_bits<T>(vmax(from_bits<T>(a), from_bits<T>(b)));
-    }));
+    if constexpr (sizeof(T) == 4) {
+        // across lanes on order-preserving integer keys (negative floats'
+        // magnitude bits flipped), so each DPP step is one integer min / max;
+        // the lane extremes are numbers here (the per-lane fmin / fmax above
+        // already dropped the NaNs) and the map is its own inverse
+        auto key = [](uint32_t u) { return u ^ ((uint32_t)((int32_t)u >> 31) & 0x7FFFFFFFu); };
+        mn = from_bits<T>(key(seg_reduce<SEG>(key(to_bits<T>(mn)), [](uint32_t a, uint32_t b) {
+            return (uint32_t)((int32_t)a < (int32_t)b ? (int32_t)a : (int32_t)b);
+        })));
+        mx = from_bits<T>(key(seg_reduce<SEG>(key(to_bits<T>(mx)), [](uint32_t a, uint32_t b) {
+            return (uint32_t)((int32_t)a > (int32_t)b ? (int32_t)a : (int32_t)b);
+        })));
+    } else {
+        mn = from_bits<T>(seg_reduce<SEG>(to_bits<T>(mn), [](U a, U b) {
+            return to_bits<T>(vmin(from_bits<T>(a), from_bits<T>(b)));
+        }));
+        mx = from_bits<T>(seg_reduce<SEG>(to_bits<T>(mx), [](U a, U b) {
+            return to_bits<T>(vmax(from_bits<T>(a), from_bits<T>(b)));
+        }));
+    }
     const T first = from_bits<T>(seg_first<SEG>(to_bits<T>(v[0])));
     if (first != first) { mn = first; mx = first; }
 }
@@ -219,7 +244,14 @@ __device__ __forceinline__ void encode_slab(const T (&v)[LV], uint64_t e0, uint6
         const int ld8 = lead8<U>(w[k] ^ prev);
         int m8 = rb8 - ld8;
         m8 = m8 > 0 ? m8 : 0;
-        U mid = w[k] >> lo8;
+        // the value's mid bytes as the low bytes of `mid`; fp32 keeps exactly
+        // those m bytes (the bytes above them are zero), so the stage can OR
+        // the word in without a mask
+        U mid;
+        if constexpr (W == 4)
+            mid = m8 >= 32 ? w[k] : __builtin_amdgcn_ubfe(w[k], (uint32_t)lo8, (uint32_t)m8);
+        else
+            mid = w[k] >> lo8;
         uint32_t ldv = (uint32_t)ld8;
         if constexpr (!FULL) {
             if (!valid[k]) { m8 = 0; ldv = 0; mid = 0; }
@@ -311,6 +343,14 @@ __device__ __forceinline__ void load_tile(const CompressParams &p, uint64_t tile
     constexpr int SLAB = 256 * LV;
     const T *x = static_cast<const T *>(p.x);
     const uint64_t t0 = tile * (uint64_t)(SLAB * SLABS);
+    if (t0 + (uint64_t)(SLAB * SLABS) <= p.n) {   // the whole tile exists: plain 16-byte loads
+        const T *xt = x + t0 + (uint64_t)LV * threadIdx.x;
+#pragma unroll
+        for (int u = 0; u < SLABS; u++)
+#pragma unroll
+            for (int q = 0; q < LV; q += 4) SlabLoad<T>::load4_full(xt + u * SLAB + q, &v[u][q]);
+        return;
+    }
 #pragma unroll
     for (int u = 0; u < SLABS; u++)
 #pragma unroll
@@ -419,28 +459,30 @@ __device__ __forceinline__ void stage_slab(const SlabState<T, LV> &st, int u, co
         if ((uint32_t)(lseg * NLB + b) < nlead)
             s_stage[base + 5 + lseg * NLB + b] = (uint8_t)(st.code >> (8 * b));
     if constexpr (LDS && sizeof(U) == 4) {
-        // Mid bytes packed into aligned words: each value's m bytes are
-        // appended to a 64-bit accumulator whose low word is OR-ed into the
-        // zeroed stage after every value and dropped once complete.  OR is
-        // idempotent and every stage byte has one writer, so the words a run
-        // shares with its neighbours need no care, and nothing branches.
-        const uint32_t q = base + 5 + nlead + (meta & 0xFFFu);
+        // Mid bytes OR-ed into the zeroed stage as aligned words: value k's
+        // m bytes (the low bytes of mid[k], zero above) start at bit offset
+        // o8 of the stage, so the 64-bit word mid[k] << (o8 mod 32) covers the
+        // stage words it touches: the first one if m > 0, the second one only
+        // if the bytes cross into it.  LDS atomics are the expensive part here
+        // (measured: two unconditional ORs per value ran 0.28 ms against 0.22
+        // ms for these predicated ones at 512^3), so the lanes with nothing to
+        // add sit out.  OR is idempotent and every stage byte has one writer,
+        // so the words a run shares with its neighbours need no care.
         uint32_t *sw = reinterpret_cast<uint32_t *>(s_stage);
-        uint32_t wq = q >> 2, pend = q & 3;   // pending bytes: placeholders first
-        uint64_t acc = 0;
+        uint32_t o8 = 8u * (base + 5 + nlead + (meta & 0xFFFu));
 #pragma unroll
         for (int k = 0; k < LV; k++) {
-            const uint32_t m = (st.mb >> (8 * k + 3)) & 31u;
-            const uint32_t x = (uint32_t)st.mid[k] & (uint32_t)(0xFFFFFFFFull >> (32 - 8 * m));
-            acc |= (uint64_t)x << (8 * pend);
-            pend += m;
-            __hip_atomic_fetch_or(&sw[wq], (uint32_t)acc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            const bool adv = pend >= 4;
-            acc = adv ? acc >> 32 : acc;
-            wq += adv ? 1u : 0u;
-            pend -= adv ? 4u : 0u;
+            const uint32_t m8 = (uint32_t)(st.mb >> (8 * k)) & 0xFFu;
+            const uint64_t sh = (uint64_t)(uint32_t)st.mid[k] << (o8 & 31u);
+            uint32_t *pw = sw + (o8 >> 5);
+            if (m8) {
+                __hip_atomic_fetch_or(pw, (uint32_t)sh, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                if ((o8 & 31u) + m8 > 32u)
+                    __hip_atomic_fetch_or(pw + 1, (uint32_t)(sh >> 32), __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+            o8 += m8;
         }
-        __hip_atomic_fetch_or(&sw[wq], (uint32_t)acc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     } else {
         uint8_t *q = s_stage + base + 5 + nlead + (meta & 0xFFFu);
 #pragma unroll
