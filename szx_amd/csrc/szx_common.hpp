@@ -109,9 +109,6 @@ __device__ inline void store_relaxed(unsigned long long *p, unsigned long long v
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-struct ChunkState;
-__device__ inline void publish_pair(ChunkState *t, unsigned long long flag, unsigned long long a,
-                                    unsigned long long b);
 
 // Two-level decoupled look-back (compressor tiles over one field).
 //
@@ -132,15 +129,21 @@ __device__ inline void publish_pair(ChunkState *t, unsigned long long flag, unsi
 // that work.  A tile waits only for lower-numbered tiles, which start first
 // (in-order dispatch), so every wait ends; the waits are bounded all the same
 // (LB_SPIN_LIMIT polls) and report failure instead of hanging.
+// A chunk's aggregate and its inclusive prefix live in separate word pairs,
+// each written once: the inclusive prefix of chunk j can be known (by the
+// first tile of chunk j + 1, from chunk j's tile words) before the tile that
+// completes chunk j has published the aggregate, and one shared pair could
+// then end with one word of each kind.
 struct ChunkState {
-    unsigned long long w[2];   // flag | bytes, flag | constants
+    unsigned long long agg[2];   // FLAG_AGG | bytes, FLAG_AGG | constants
+    unsigned long long inc[2];   // FLAG_INC | bytes, FLAG_INC | constants
 };
 
-__device__ inline void publish_pair(ChunkState *t, unsigned long long flag, unsigned long long a,
+__device__ inline void publish_pair(unsigned long long *w, unsigned long long flag, unsigned long long a,
                                     unsigned long long b)
 {
-    store_relaxed(&t->w[1], flag | b);
-    store_relaxed(&t->w[0], flag | a);
+    store_relaxed(&w[1], flag | b);
+    store_relaxed(&w[0], flag | a);
 }
 
 constexpr int LB_CHUNK = 64;
@@ -182,7 +185,7 @@ __device__ inline unsigned long long wave_sum(unsigned long long v)
 
 struct LookbackRefs {
     unsigned long long *tiles;   // per tile: flag | constants << 40 | bytes
-    ChunkState *chunks;          // per chunk: aggregate or inclusive prefix
+    ChunkState *chunks;          // per chunk: aggregate and inclusive prefix
     unsigned long long *cacc;    // per chunk: bytes | constants << 32 | tiles << 56
     uint64_t ntiles;
 };
@@ -213,30 +216,51 @@ __device__ inline void lookback_chunk_done(const LookbackRefs &L, uint64_t tile,
     if ((old >> 56) + 1 == count) {
         const unsigned long long sum =
             old + ((unsigned long long)bytes | ((unsigned long long)consts << 32));
-        publish_pair(&L.chunks[c], FLAG_AGG, sum & 0xFFFFFFFFull, (sum >> 32) & 0xFFFFFFull);
+        publish_pair(L.chunks[c].agg, FLAG_AGG, sum & 0xFFFFFFFFull, (sum >> 32) & 0xFFFFFFull);
     }
 }
 
-// The words a tile's look-back reads first, loaded by the 64 lanes of a wave:
-// lane l holds tile (chunk start + l) for l below the tile's place in its
-// chunk, and, for l < LB_WIN, chunk (c - 1 - l) (an inclusive zero before
-// chunk 0).
+// The words a tile's look-back reads, loaded by the 64 lanes of a wave: lane
+// l holds tile (chunk start + l) for l below the tile's place in its chunk;
+// the tile words of the LB_PREV chunks before it (tile l of each); and, for
+// l < LB_WIN, chunk (c - 1 - LB_PREV - l) (an inclusive zero before chunk 0).
+//
+// The preceding chunks are read tile by tile because a chunk's own aggregate
+// appears only after its last tile's accumulator atomic has come back and
+// that tile has published it: for the chunks just before a tile that is
+// several microseconds after their tiles' aggregates, while each tile word is
+// visible about one memory round trip after its tile published it.  Chunk
+// words are used from LB_PREV + 1 chunks back, where they are long published.
+constexpr int LB_PREV = 4;
+
 struct LookbackPoll {
-    unsigned long long t, ca, cb;
+    unsigned long long t, ca, cb, ia, ib;
+    unsigned long long p[LB_PREV];
 };
 
 // Chunk words are read LB_WIN at a time (the nearest inclusive value is
-// normally one or two chunks back); lanes past the window hold zeros that
-// neither stop nor add to the scan.
+// normally a few chunks back); lanes past the window hold zeros that neither
+// stop nor add to the scan.
 constexpr int LB_WIN = 16;
 
 __device__ inline void lookback_chunk_words(const LookbackRefs &L, int64_t j, LookbackPoll &q)
 {
     const bool in = __lane_id() < LB_WIN;
-    q.ca = !in ? FLAG_AGG : j >= 0 ? load_relaxed(&L.chunks[j].w[0]) : FLAG_INC;
-    q.cb = !in ? FLAG_AGG : j >= 0 ? load_relaxed(&L.chunks[j].w[1]) : FLAG_INC;
+    q.ca = !in ? FLAG_AGG : j >= 0 ? load_relaxed(&L.chunks[j].agg[0]) : 0ull;
+    q.cb = !in ? FLAG_AGG : j >= 0 ? load_relaxed(&L.chunks[j].agg[1]) : 0ull;
+    q.ia = !in ? 0ull : j >= 0 ? load_relaxed(&L.chunks[j].inc[0]) : FLAG_INC;
+    q.ib = !in ? 0ull : j >= 0 ? load_relaxed(&L.chunks[j].inc[1]) : FLAG_INC;
 }
 
+// Tile word `lane` of chunk j (a ready zero for j < 0).
+__device__ inline unsigned long long prev_tile_word(const LookbackRefs &L, int64_t j, int lane)
+{
+    return j >= 0 ? load_relaxed(&L.tiles[(uint64_t)j * LB_CHUNK + lane]) : FLAG_AGG;
+}
+
+// Issue a tile's look-back reads (one wave).  Calling it again later refreshes
+// them: a word that was ready stays ready, with the same value or a later
+// inclusive one.
 __device__ inline LookbackPoll lookback_poll(const LookbackRefs &L, uint64_t tile)
 {
     const int lane = __lane_id();
@@ -244,7 +268,9 @@ __device__ inline LookbackPoll lookback_poll(const LookbackRefs &L, uint64_t til
     const int li = (int)(tile % LB_CHUNK);
     LookbackPoll q;
     q.t = lane < li ? load_relaxed(&L.tiles[c * LB_CHUNK + lane]) : FLAG_AGG;
-    lookback_chunk_words(L, (int64_t)c - 1 - lane, q);
+#pragma unroll
+    for (int k = 0; k < LB_PREV; k++) q.p[k] = prev_tile_word(L, (int64_t)c - 1 - k, lane);
+    lookback_chunk_words(L, (int64_t)c - 1 - LB_PREV - lane, q);
     return q;
 }
 
@@ -260,39 +286,58 @@ __device__ inline bool lookback_finish(const LookbackRefs &L, uint64_t tile, Loo
     const int li = (int)(tile % LB_CHUNK);
     const uint64_t c0 = c * LB_CHUNK;
 
-    // 1. the aggregates before the tile in its chunk
+    // 1. the aggregates before the tile in its chunk and in the LB_PREV
+    //    chunks before that
     int round = 0;
     while (true) {
-        const bool ready = (q.t & FLAG_MASK) != 0;
+        bool ready = (q.t & FLAG_MASK) != 0;
+#pragma unroll
+        for (int k = 0; k < LB_PREV; k++) ready = ready && (q.p[k] & FLAG_MASK) != 0;
         if (__ballot(!ready) == 0) break;
         if (!poll_backoff(round)) return false;
-        if (!ready) q.t = load_relaxed(&L.tiles[c0 + lane]);
+        if ((q.t & FLAG_MASK) == 0) q.t = load_relaxed(&L.tiles[c0 + lane]);
+#pragma unroll
+        for (int k = 0; k < LB_PREV; k++)
+            if ((q.p[k] & FLAG_MASK) == 0) q.p[k] = prev_tile_word(L, (int64_t)c - 1 - k, lane);
     }
-    const unsigned long long v = lane < li ? (q.t & VAL_MASK) : 0ull;
-    const unsigned long long in_b = wave_sum(v & T_BYTES);
-    const unsigned long long in_c = wave_sum(v >> 40);
+    unsigned long long lb = 0, lc = 0;
+    const unsigned long long own = lane < li ? (q.t & VAL_MASK) : 0ull;
+    lb += own & T_BYTES;
+    lc += own >> 40;
+#pragma unroll
+    for (int k = 0; k < LB_PREV; k++) {
+        const unsigned long long w = q.p[k] & VAL_MASK;
+        lb += w & T_BYTES;
+        lc += w >> 40;
+    }
+    const unsigned long long in_b = wave_sum(lb);
+    const unsigned long long in_c = wave_sum(lc);
 
-    // 2. the chunk's prefix
+    // 2. the prefix of the chunks before those
     unsigned long long cx_b = 0, cx_c = 0;
-    int64_t base = (int64_t)c - 1;
+    int64_t base = (int64_t)c - 1 - LB_PREV;
     round = 0;
     while (base >= 0) {
-        const unsigned long long fa = q.ca & FLAG_MASK, fb = q.cb & FLAG_MASK;
+        const bool inc = (q.ia & FLAG_MASK) == FLAG_INC && (q.ib & FLAG_MASK) == FLAG_INC;
+        const bool agg = (q.ca & FLAG_MASK) == FLAG_AGG && (q.cb & FLAG_MASK) == FLAG_AGG;
         int stop;
-        if (!window_ready(fa != 0 && fa == fb, fa == FLAG_INC && fb == FLAG_INC, stop)) {
+        if (!window_ready(inc || agg, inc, stop)) {
             if (!poll_backoff(round)) return false;
             lookback_chunk_words(L, base - lane, q);
             continue;
         }
-        cx_b += wave_sum(lane <= stop ? (q.ca & VAL_MASK) : 0ull);
-        cx_c += wave_sum(lane <= stop ? (q.cb & VAL_MASK) : 0ull);
+        const unsigned long long vb = inc ? q.ia : q.ca, vc = inc ? q.ib : q.cb;
+        cx_b += wave_sum(lane <= stop ? (vb & VAL_MASK) : 0ull);
+        cx_c += wave_sum(lane <= stop ? (vc & VAL_MASK) : 0ull);
         if (stop < 64) break;
         base -= LB_WIN;
         lookback_chunk_words(L, base - lane, q);
     }
-    if (publish && li == 0 && c > 0 && lane == 0) publish_pair(&L.chunks[c - 1], FLAG_INC, cx_b, cx_c);
     ex_bytes = cx_b + in_b;
     ex_cnt = cx_c + in_c;
+    // a chunk-leading tile's exclusive prefix is the previous chunk's
+    // inclusive one
+    if (publish && li == 0 && c > 0 && lane == 0) publish_pair(L.chunks[c - 1].inc, FLAG_INC, ex_bytes, ex_cnt);
     return true;
 }
 

@@ -494,12 +494,16 @@ __device__ __forceinline__ void stage_slab(const SlabState<T, LV> &st, int u, co
     }
 }
 
-template <typename T, int BS, int LV, int SLABS>
+// Stage every slab's records; after(u) runs after slab u is staged.
+template <typename T, int BS, int LV, int SLABS, typename After>
 __device__ __forceinline__ void stage_records(const SlabState<T, LV> (&st)[SLABS],
-                                              const TileMeta<T, BS, LV, SLABS> &M, uint8_t *s_stage)
+                                              const TileMeta<T, BS, LV, SLABS> &M, uint8_t *s_stage, After after)
 {
 #pragma unroll
-    for (int u = 0; u < SLABS; u++) stage_slab<T, BS, LV, SLABS, true>(st[u], u, M, s_stage);
+    for (int u = 0; u < SLABS; u++) {
+        stage_slab<T, BS, LV, SLABS, true>(st[u], u, M, s_stage);
+        after(u);
+    }
 }
 
 // Write out a tile once its exclusive prefix (ebytes, econst) is known (all
@@ -634,10 +638,7 @@ __device__ __forceinline__ void tile_prefix(const CompressParams &p, const Lookb
 }
 
 // One tile per workgroup.  The tile publishes its aggregate right after
-// its scan and issues its look-back reads at once; staging the records then
-// overlaps their round trip.  Every wave issues the same few look-back reads
-// (no wave's wait for its own loads has to include them); wave 0 consumes
-// them.
+// its scan; staging the records then overlaps the look-back.
 template <typename T, int BS, int LV, int SLABS, int IPT>
 __device__ __forceinline__ void compress_tile(const CompressParams &p, uint64_t tile, CompressLds<T, BS, LV, SLABS> &lds)
 {
@@ -651,10 +652,18 @@ __device__ __forceinline__ void compress_tile(const CompressParams &p, uint64_t 
     }
     SlabState<T, LV> st[SLABS];
     encode_scan<T, BS, LV, SLABS, IPT>(p, L, tile, v, st, lds.meta, pend);
-    const LookbackPoll q = lookback_poll(L, tile);
+    // wave 0 issues the look-back reads halfway through staging: the words it
+    // reads appear about a memory round trip after their tiles publish them,
+    // and the rest of staging covers the round trip
+    LookbackPoll q{};
     const bool staged = (lds.meta.total & 0xFFFFFu) <= (uint32_t)TileShape<T, BS, LV, SLABS>::STAGE_CAP;
-    if (staged) stage_records<T, BS, LV, SLABS>(st, lds.meta, reinterpret_cast<uint8_t *>(lds.stage.w));
-    if (threadIdx.x == 0) lookback_chunk_done(L, tile, pend, lds.meta.total & 0xFFFFFu, lds.meta.total >> 20);
+    if (staged) {
+        stage_records<T, BS, LV, SLABS>(st, lds.meta, reinterpret_cast<uint8_t *>(lds.stage.w), [&](int u) {
+            if (u == (SLABS - 1) / 2 && threadIdx.x < 64) q = lookback_poll(L, tile);
+        });
+    } else {
+        if (threadIdx.x < 64) q = lookback_poll(L, tile);
+    }
     tile_prefix(p, L, tile, q, lds.excl);
     __syncthreads();
     if (!staged) {
@@ -680,10 +689,17 @@ __device__ __forceinline__ void compress_tile(const CompressParams &p, uint64_t 
         }
     }
     copy_out<T, BS, LV, SLABS, IPT>(p, tile, lds.meta, lds.stage, lds.excl[0], lds.excl[1], staged);
+    // the chunk aggregate is the look-back's fallback for chunks far back
+    // (nearer ones are read tile by tile, and chunk inclusive prefixes appear
+    // sooner): publishing it last keeps the accumulator atomic's round trip
+    // off the tile's path
+    if (threadIdx.x == 0) lookback_chunk_done(L, tile, pend, lds.meta.total & 0xFFFFFu, lds.meta.total >> 20);
 }
 
+// Three workgroups per CU (the LDS stage allows three): ask the register
+// allocator for the 168 registers that three waves per SIMD leave each.
 template <typename T, int BS, int LV, int SLABS, int IPT>
-__global__ __launch_bounds__(256) void k_compress(CompressParams p)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_compress(CompressParams p)
 {
     __shared__ CompressLds<T, BS, LV, SLABS> lds;
     compress_tile<T, BS, LV, SLABS, IPT>(p, blockIdx.x, lds);
@@ -954,7 +970,7 @@ __global__ void k_batch_range_final(BatchField *bf, const uint64_t *rbase, int n
 // (fields in order, tbase[f] = first global tile of field f).  Workgroups
 // start in index order, so every tile a look-back waits for has started.
 template <int BS, int LV, int IPT>
-__global__ __launch_bounds__(256) void k_compress_batch(const BatchField *bf, const uint64_t *tbase, int nf)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_compress_batch(const BatchField *bf, const uint64_t *tbase, int nf)
 {
     constexpr int SF = IPT * TILE_VALUES_F32 / (256 * LV);
     constexpr int SD = IPT * TILE_VALUES_F64 / (256 * LV);
