@@ -394,6 +394,41 @@ def _torch():
     return torch
 
 
+def _torch_code(t) -> int:
+    """SZx dataType of a torch tensor (float32 / float64 only)."""
+    torch = _torch()
+    if t.dtype == torch.float32:
+        return SZx_FLOAT
+    if t.dtype == torch.float64:
+        return SZx_DOUBLE
+    raise SZxError(f"unsupported tensor dtype {t.dtype} (float32 and float64 are)")
+
+
+def _dev_tensor(t, name: str, n: int, dtype=None):
+    """Check, before any launch, that t is a contiguous device tensor of at
+    least n elements (of `dtype`, a numpy float dtype, when given): the
+    kernels read and write n elements from its data pointer."""
+    torch = _torch()
+    if not isinstance(t, torch.Tensor):
+        raise SZxError(f"{name} must be a torch tensor")
+    if not t.is_contiguous():
+        raise SZxError(f"{name} must be contiguous")
+    if dtype is not None and _torch_code(t) != _dtype_code(dtype):
+        raise SZxError(f"{name} is {t.dtype}, expected {np.dtype(dtype)}")
+    if t.numel() < n:
+        raise SZxError(f"{name} holds {t.numel()} elements, fewer than {n}")
+    if not t.is_cuda:
+        raise SZxError(f"{name} must be a device tensor")
+
+
+def _dev_bytes(t, name: str, nbytes: int):
+    """Check that t is a contiguous uint8 device tensor of at least nbytes."""
+    torch = _torch()
+    if isinstance(t, torch.Tensor) and t.dtype != torch.uint8:
+        raise SZxError(f"{name} must be uint8, not {t.dtype}")
+    _dev_tensor(t, name, nbytes)
+
+
 def _stream_handle(stream):
     torch = _torch()
     s = stream if stream is not None else torch.cuda.current_stream()
@@ -424,6 +459,7 @@ class DeviceCompressor:
     def __call__(self, x, errBoundMode: int = ABS, absErrBound: float = 0.0,
                  relBoundRatio: float = 0.0, stream=None) -> None:
         """Enqueue one compress of tensor x (asynchronous)."""
+        _dev_tensor(x, "x", self.n, self.dtype)
         args = (x.data_ptr(), self.n, _dtype_code(self.dtype), errBoundMode, absErrBound,
                 relBoundRatio, self.bs, self.out.data_ptr(), self.capacity, self.ws.data_ptr(),
                 self.wsbytes, self.result.data_ptr(), self.index.data_ptr(), _stream_handle(stream))
@@ -460,6 +496,11 @@ class DeviceDecompressor:
         """Enqueue one decode; `index` is the compressor's record index
         (DeviceCompressor.index) or None to rebuild it from the stream."""
         nb = stream_tensor.numel() if nbytes is None else nbytes
+        if index is not None:
+            if index.element_size() != 8:
+                raise SZxError("index must hold 64-bit words")
+            _dev_tensor(index, "index", lib().szx_amd_index_words(self.n, _dtype_code(self.dtype), self.bs))
+        _dev_bytes(stream_tensor, "stream_tensor", nb)
         args = (stream_tensor.data_ptr(), nb, self.n, _dtype_code(self.dtype), self.bs,
                 self.out.data_ptr(), self.ws.data_ptr(), self.wsbytes,
                 None if index is None else index.data_ptr(), flags, self.status.data_ptr(),
@@ -497,6 +538,7 @@ class DeviceNonBlocked:
 
     def compress(self, x, errBoundMode: int = ABS, absErrBound: float = 0.0, relBoundRatio: float = 0.0,
                  stream=None) -> None:
+        _dev_tensor(x, "x", self.n, self.dtype)
         rc = lib().szx_amd_compress_nb_dev(x.data_ptr(), self.n, _dtype_code(self.dtype), errBoundMode,
                                            absErrBound, relBoundRatio, self.out.data_ptr(), self.capacity,
                                            self.ws.data_ptr(), self.wsbytes, self.result.data_ptr(),
@@ -511,6 +553,7 @@ class DeviceNonBlocked:
         return self.out[off:off + size], size, eb
 
     def decompress(self, stream_tensor, nbytes: int, flags: int = DECODE_REFERENCE, stream=None) -> None:
+        _dev_bytes(stream_tensor, "stream_tensor", nbytes)
         rc = lib().szx_amd_decompress_nb_dev(stream_tensor.data_ptr(), nbytes, self.n, _dtype_code(self.dtype),
                                              self.dec.data_ptr(), self.ws.data_ptr(), self.wsbytes, flags,
                                              self.status.data_ptr(), _stream_handle(stream))
@@ -538,6 +581,7 @@ class DeviceCuszxWriter:
         self.result = torch.zeros(4, dtype=torch.int64, device=device)
 
     def compress(self, x, absErrBound: float, stream=None) -> None:
+        _dev_tensor(x, "x", self.n, np.float32)
         rc = lib().szx_amd_compress_cuszx_dev(x.data_ptr(), self.n, absErrBound, self.bs, self.out.data_ptr(),
                                               self.capacity, self.ws.data_ptr(), self.wsbytes,
                                               self.result.data_ptr(), _stream_handle(stream))
@@ -566,6 +610,8 @@ class DeviceCuszxReader:
 
     def decompress(self, stream_tensor, nbytes: int, out=None, stream=None) -> None:
         dst = self.dec if out is None else out
+        _dev_bytes(stream_tensor, "stream_tensor", nbytes)
+        _dev_tensor(dst, "out", self.n, np.float32)
         rc = lib().szx_amd_decompress_cuszx_dev(stream_tensor.data_ptr(), nbytes, self.n, dst.data_ptr(),
                                                 self.ws.data_ptr(), self.wsbytes, self.status.data_ptr(),
                                                 _stream_handle(stream))
@@ -593,7 +639,8 @@ class BatchCodec:
         arr = (Field * nf)()
         self.outs, self.decs = [], []
         for i, (x, mode, bound) in enumerate(fields):
-            code = _dtype_code(np.float32 if x.dtype == torch.float32 else np.float64)
+            _dev_tensor(x, f"field {i}", x.numel())
+            code = _torch_code(x)
             cap = L.szx_amd_compress_capacity(x.numel(), code, blockSize)
             if not cap:
                 raise SZxError(f"unsupported parameters (blockSize={blockSize})")
@@ -681,6 +728,7 @@ class DeviceFixedRateBound:
         """(bound, model evaluations); initErrorBound <= 0 starts at 1% of the
         sampled range, as the reference's fixed-rate compressor does."""
         eb, ne = C.c_float(0), C.c_int(0)
+        _dev_tensor(x, "x", self.n, self.dtype)
         rc = lib().szx_amd_fxr_bound_dev(x.data_ptr(), self.n, _dtype_code(self.dtype), self.bs,
                                          targetCompressionRatio, tolerance, initErrorBound,
                                          self.ws.data_ptr(), self.wsbytes, C.byref(eb),
@@ -741,7 +789,8 @@ def rate_stats(x, blockSize: int = 128, stream=None):
     """Per-chunk statistics of device array x (raw bytes tensor)."""
     torch = _torch()
     n = x.numel()
-    code = SZx_FLOAT if x.dtype == torch.float32 else SZx_DOUBLE
+    code = _torch_code(x)
+    _dev_tensor(x, "x", n)
     st = torch.empty(lib().szx_amd_rate_stats_bytes(n), dtype=torch.uint8, device=x.device)
     ws = torch.empty(lib().szx_amd_rate_stats_workspace(n), dtype=torch.uint8, device=x.device)
     _check(lib().szx_amd_rate_stats_dev(x.data_ptr(), n, code, blockSize, st.data_ptr(),
@@ -764,7 +813,8 @@ def rate_octave(x, reqExpo: int, blockSize: int = 128, stream=None):
     (numpy sizes, numpy float32 bounds)."""
     torch = _torch()
     n = x.numel()
-    code = SZx_FLOAT if x.dtype == torch.float32 else SZx_DOUBLE
+    code = _torch_code(x)
+    _dev_tensor(x, "x", n)
     acc = torch.zeros(1 + 2 * OCTAVE_BINS, dtype=torch.int64, device=x.device)
     ws = torch.empty(lib().szx_amd_rate_octave_workspace(), dtype=torch.uint8, device=x.device)
     _check(lib().szx_amd_rate_octave_dev(x.data_ptr(), n, code, blockSize, reqExpo,
@@ -802,6 +852,7 @@ class FixedRateCompressor:
     def __call__(self, x, targetBitsPerValue: float, tolerance: float = 0.02,
                  method: int = RATE_LEARNED, stream=None) -> RateInfo:
         info = RateInfo()
+        _dev_tensor(x, "x", self.n, self.dtype)
         params = self.model.params.data_ptr() if (method == RATE_LEARNED and self.model) else None
         _check(lib().szx_amd_fixed_rate_dev(x.data_ptr(), self.n, _dtype_code(self.dtype), self.bs,
                                             targetBitsPerValue, tolerance, method, params,
