@@ -100,6 +100,125 @@ def cpu_baseline(x_host, eb, sample_values):
     return kind, out
 
 
+def host_cpu_info():
+    """Host cores this process may use, all cores, and the CPU model name."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count()
+    return {"nproc": usable, "cpu_count": os.cpu_count(), "model": model,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
+def cfg1_eval(dev, steps, parity=True, cpu=True):
+    """The CPU-runnable configuration at full size: one fp32 100x500x500
+    field (32 waves, integer k in [1,16], |k|^-2, noise 1e-3 std, a 25% box of
+    0.0), blockSize 128, ABS eb = 1e-3 x (max - min).  GPU: compress (ABS and
+    REL 1e-3, which yields the same eb) and decompress with the record index,
+    with the stream and the decode compared with the oracle's.  CPU: the
+    reference compiled from its sources (the oracle if that is absent), serial
+    fastMode 2 on one core and OpenMP fastMode 4 on every core, each with the
+    eb given (ABS) and with the REL range pass, best of 5 after one warm-up."""
+    import numpy as np
+    import torch
+
+    import szx_amd as Z
+    from szx_amd.fields import sinusoid_field_torch
+
+    shape = (100, 500, 500)
+    x = sinusoid_field_torch(shape, seed=1, n_waves=32, kmax=16, slope=2.0, noise=1e-3, n_boxes=1,
+                             box_frac=0.25, box_value="zero", device=dev).contiguous().view(-1)
+    n = x.numel()
+    nbytes = 4 * n
+    comp = Z.DeviceCompressor(n, np.float32, 128, device=dev)
+    comp(x, Z.REL, relBoundRatio=1e-3)
+    view, _, csize, eb = comp.stream_view()
+    dec = Z.DeviceDecompressor(n, np.float32, 128, comp.capacity, device=dev)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    t_rel = timed(lambda: comp(x, Z.REL, relBoundRatio=1e-3))
+    t_abs = timed(lambda: comp(x, Z.ABS, absErrBound=eb))
+    view, _, csize, eb2 = comp.stream_view()
+    assert eb2 == eb
+    t_dec = timed(lambda: dec(view, csize, Z.DECODE_REFERENCE, index=comp.index))
+    assert dec.check() == 0
+    out = {"workload": "fp32 100x500x500 sinusoid field (32 waves, k<=16, |k|^-2, noise 1e-3 std, "
+                       "25% zero box), blockSize 128, ABS eb = 1e-3 x range",
+           "eb": eb, "compression_ratio": nbytes / csize,
+           "gpu": {"compress_rel_gbs": nbytes / t_rel / 1e9, "compress_abs_gbs": nbytes / t_abs / 1e9,
+                   "decompress_gbs": nbytes / t_dec / 1e9,
+                   "note": "wall clock per call, inputs resident in HBM"}}
+    xh = x.cpu().numpy() if (parity or cpu) else None
+    if parity or cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+    if parity:
+        s_gpu = view.cpu().numpy().tobytes()
+        eb_o = O.rel_eb(xh, 1e-3)
+        s_ora = O.compress(xh, eb_o, 128)
+        d_ora = O.decompress(s_ora, n, np.float32)
+        d_gpu = dec.out.cpu().numpy()
+        err = float(np.max(np.abs(d_gpu.astype(np.float64) - xh)))
+        out["parity"] = {"eb_identical": bool(np.float32(eb_o) == np.float32(eb)),
+                         "stream_identical": s_gpu == s_ora,
+                         "decode_identical": d_gpu.tobytes() == d_ora.tobytes(),
+                         "max_abs_err": err, "err_within_eb": err <= eb}
+        log(f"cfg1 parity {out['parity']}")
+    if cpu:
+        def best(fn, reps=5):
+            fn()
+            ts = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                fn()
+                ts.append(time.perf_counter() - t0)
+            return min(ts)
+
+        rows = {}
+        if O.ref_available():
+            kind = "reference"
+            threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+            for name, fm, cores in (("serial_fastmode2", 2, 1), ("openmp_fastmode4", 4, threads)):
+                tc_abs = best(lambda: O.ref_compress_args(xh, mode=0, abs_eb=eb, fast_mode=fm))
+                tc_rel = best(lambda: O.ref_compress_args(xh, mode=1, rel=1e-3, fast_mode=fm))
+                st = O.ref_compress_args(xh, mode=1, rel=1e-3, fast_mode=fm)
+                td = best(lambda: O.ref_decompress(st, n, np.float32, fast_mode=fm))
+                rows[name] = {"cores": cores, "compress_abs_gbs": nbytes / tc_abs / 1e9,
+                              "compress_rel_gbs": nbytes / tc_rel / 1e9,
+                              "decompress_gbs": nbytes / td / 1e9,
+                              "stream_equals_gpu": (st == s_gpu) if (parity and fm == 2) else None}
+        else:
+            kind = "port"
+            tc = best(lambda: O.compress(xh, eb, 128))
+            td = best(lambda: O.decompress(O.compress(xh, eb, 128), n, np.float32))
+            rows["serial_fastmode2"] = {"cores": 1, "compress_abs_gbs": nbytes / tc / 1e9,
+                                        "decompress_gbs": nbytes / td / 1e9}
+        out["cpu"] = {"kind": kind, "host": host_cpu_info(), **rows,
+                      "method": "best of 5 after 1 warm-up, in memory; compress_rel includes the "
+                                "reference's serial range pass (computeValueRange)"}
+        log(f"cfg1 CPU {out['cpu']}")
+    del comp, dec, x
+    torch.cuda.empty_cache()
+    return out
+
+
 def measured_traffic(shape, rel):
     """HBM bytes per k_compress launch on the default workload, from the
     committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a
@@ -245,10 +364,9 @@ def fp64_eval(dev, steps, parity=True):
                 eb_o = O.rel_eb(xh, rel)
                 rec["stream_identical"] = bool(np.float32(eb_o) == np.float32(eb)) and \
                     s_gpu == O.compress(xh, eb_o, bs)
-                if rel == 1e-6:
-                    d_ora = O.decompress(s_gpu, n, np.float64, ref_quirk=True)
-                    rec["decode_identical"] = bool(np.array_equal(dec.out.cpu().numpy().view(np.uint64),
-                                                                  d_ora.view(np.uint64)))
+                d_ora = O.decompress(s_gpu, n, np.float64, ref_quirk=True)
+                rec["decode_identical"] = bool(np.array_equal(dec.out.cpu().numpy().view(np.uint64),
+                                                              d_ora.view(np.uint64)))
             log(f"fp64 bs {bs} rel {rel:g}: {rec}")
             out.append(rec)
         del comp, dec
@@ -487,7 +605,7 @@ def _extents(r, n, nd):
     return (a, b, max(1, rest // b))
 
 
-def batched_eval(n_fields, dev, reps, single_gbs):
+def batched_eval(n_fields, dev, reps, single_gbs, parity=True):
     """Many fields in one launch sequence per direction, captured in a HIP
     graph, against one launch sequence per field on the same buffers.
 
@@ -567,9 +685,11 @@ def batched_eval(n_fields, dev, reps, single_gbs):
     assert not st.any(), f"decode status {st[st != 0][:4]}"
     comp_bytes = int(res_b[:, 1].sum())
 
-    # a sample of streams, to compare with the per-field launches' output
+    # a sample of streams, to compare with the per-field launches' output and
+    # with the oracle
     sample = r.choice(n_fields, size=min(n_fields, 64), replace=False)
     kept = {int(i): bc.stream(int(i), res_b).clone() for i in sample}
+    kept_dec = {int(i): bc.decs[int(i)].cpu().numpy() for i in sample}   # DECODE_REFERENCE
 
     # every field within its bound (exact decode: for fp64 the reference
     # decoder misreads one byte of some records, which DECODE_REFERENCE keeps)
@@ -579,6 +699,7 @@ def batched_eval(n_fields, dev, reps, single_gbs):
     # max(|median|, block range).  The same bytes come out of the oracle.
     ebs = np.array(res_b[:, 2], np.uint32).view(np.float32)
     worst, over, within_ulp = 0.0, 0, True
+    over_idx = []
     for i, (x, _, _) in enumerate(fields):
         d = bc.decs[i]
         e = (d.double() - x.double()).abs()
@@ -587,11 +708,55 @@ def batched_eval(n_fields, dev, reps, single_gbs):
         worst = max(worst, m / eb if eb > 0 else (0.0 if m == 0 else float("inf")))
         if m > eb:
             over += 1
+            over_idx.append(i)
             a = torch.clamp(d.double().abs(), min=2 * eb).float()
             ulp = (torch.nextafter(a, torch.full_like(a, float("inf"))) - a).double()
             within_ulp &= bool((e <= eb + ulp).all())
     log(f"max |x - x'| / eb over the batch: {worst:.7f} ({over} fields above eb, "
         f"all within one fp32 ulp of the median: {within_ulp})")
+
+    # the sampled fields against the oracle (eb, stream bytes, decoded bytes),
+    # and every field above eb against the reference's own decoder: the same
+    # bytes, so the same elements above eb
+    oracle_cmp = None
+    if parity:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        log(f"batch parity: {len(sample)} sampled fields and {len(over_idx)} fields above eb vs the oracle")
+        cnt = {"eb_identical": 0, "stream_identical": 0, "decode_identical": 0}
+        for i in sorted(kept):
+            x, _, rel = fields[i]
+            xh = x.cpu().numpy()
+            eb_o = O.rel_eb(xh, rel)
+            s_ora = O.compress(xh, eb_o, 128)
+            d_ora = O.decompress(s_ora, xh.size, xh.dtype)
+            cnt["eb_identical"] += int(np.float32(eb_o).view(np.uint32) == np.uint32(res_b[i, 2]))
+            cnt["stream_identical"] += int(kept[i].cpu().numpy().tobytes() == s_ora)
+            cnt["decode_identical"] += int(kept_dec[i].tobytes() == d_ora.tobytes())
+        bc.decompress(flags=Z.DECODE_REFERENCE)
+        above = {"fields": len(over_idx), "decode_identical_to_reference_decoder": 0,
+                 "same_elements_above_eb": 0, "elements_above_eb": 0,
+                 "decoder": "oracle/_ref (the reference compiled from its sources)" if O.ref_available()
+                 else "oracle"}
+        for i in over_idx:
+            x = fields[i][0]
+            xh = x.cpu().numpy()
+            s_i = bc.stream(i, res_b).cpu().numpy().tobytes()
+            d_ref = (O.ref_decompress(s_i, xh.size, xh.dtype) if O.ref_available()
+                     else O.decompress(s_i, xh.size, xh.dtype))
+            d_i = bc.decs[i].cpu().numpy()
+            eb = float(ebs[i])
+            m_gpu = np.abs(d_i.astype(np.float64) - xh) > eb
+            m_ref = np.abs(d_ref.astype(np.float64) - xh) > eb
+            above["decode_identical_to_reference_decoder"] += int(d_i.tobytes() == d_ref.tobytes())
+            above["same_elements_above_eb"] += int(np.array_equal(m_gpu, m_ref))
+            above["elements_above_eb"] += int(m_gpu.sum())
+        oracle_cmp = {"sampled_fields": len(kept), **cnt, "fields_above_eb": above}
+        oracle_cmp["all_identical"] = (cnt["eb_identical"] == cnt["stream_identical"] == cnt["decode_identical"]
+                                       == len(kept)
+                                       and above["decode_identical_to_reference_decoder"] == len(over_idx)
+                                       and above["same_elements_above_eb"] == len(over_idx))
+        log(f"batch parity {oracle_cmp}")
 
     ms_pc = timed(lambda: bc.compress(mode=Z.BATCH_PER_FIELD), reps)
     res_p = bc.results()
@@ -615,6 +780,7 @@ def batched_eval(n_fields, dev, reps, single_gbs):
         "single_field_compress_gbs": single_gbs,
         "single_over_batched_compress": single_gbs / gbs(ms_c) if single_gbs else None,
         "per_field_streams_identical": same,
+        "oracle_parity": oracle_cmp,
         "bound_check": {"max_err_over_eb": worst, "fields_above_eb": over,
                         "all_within_eb_plus_fp32_ulp_of_median": within_ulp,
                         "decode": "exact (fp64: DECODE_EXACT)",
@@ -640,6 +806,8 @@ def main():
     ap.add_argument("--no-noindex", action="store_true", help="skip the index-free decode")
     ap.add_argument("--fxr-fields", type=int, default=64,
                     help="fixed-rate evaluation fields (0 skips it)")
+    ap.add_argument("--no-cfg1", action="store_true",
+                    help="skip the fp32 100x500x500 field (GPU, parity and reference CPU rows)")
     ap.add_argument("--no-fp64", action="store_true",
                     help="skip the fp64 8192^2 field (REL 1e-4/1e-6/1e-8 x blockSize 64/128/256)")
     ap.add_argument("--batch-fields", type=int, default=1000,
@@ -762,26 +930,39 @@ def main():
         log("parity check against the oracle at full size")
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
-        comp(x, Z.REL, relBoundRatio=args.rel)
-        s_gpu = comp.stream_view()[0].cpu().numpy().tobytes()
-        eb_o = O.rel_eb(xh, args.rel)
-        s_ora = O.compress(xh, eb_o, 128)
-        dec(view, csize, Z.DECODE_REFERENCE, index=comp.index)
-        d_gpu = dec.out.cpu().numpy()
-        d_ora = O.decompress(s_ora, n, np.float32)
-        dec(view, csize, Z.DECODE_REFERENCE)
-        d_gpu2 = dec.out.cpu().numpy() if not args.no_noindex else d_gpu
-        parity = {
-            "eb_identical": bool(np.float32(eb_o) == np.float32(eb)),
-            "stream_identical": s_gpu == s_ora,
-            "decode_identical": bool(np.array_equal(d_gpu.view(np.uint32), d_ora.view(np.uint32))),
-            "decode_noindex_identical": bool(np.array_equal(d_gpu2.view(np.uint32),
-                                                            d_ora.view(np.uint32))),
-            "max_abs_err": float(np.max(np.abs(d_gpu.astype(np.float64) - xh))),
-            "eb": eb,
-        }
-        parity["err_within_eb"] = parity["max_abs_err"] <= eb
-        log(f"parity {parity}")
+
+        def parity_at(rel):
+            comp(x, Z.REL, relBoundRatio=rel)
+            v, _, cs, e = comp.stream_view()
+            s_gpu = v.cpu().numpy().tobytes()
+            eb_o = O.rel_eb(xh, rel)
+            s_ora = O.compress(xh, eb_o, 128)
+            dec(v, cs, Z.DECODE_REFERENCE, index=comp.index)
+            d_gpu = dec.out.cpu().numpy()
+            d_ora = O.decompress(s_ora, n, np.float32)
+            dec(v, cs, Z.DECODE_REFERENCE)
+            d_gpu2 = dec.out.cpu().numpy()
+            assert dec.check() == 0
+            rec = {"eb_identical": bool(np.float32(eb_o) == np.float32(e)),
+                   "stream_identical": s_gpu == s_ora,
+                   "decode_identical": bool(np.array_equal(d_gpu.view(np.uint32), d_ora.view(np.uint32))),
+                   "decode_noindex_identical": bool(np.array_equal(d_gpu2.view(np.uint32),
+                                                                   d_ora.view(np.uint32))),
+                   "max_abs_err": float(np.max(np.abs(d_gpu.astype(np.float64) - xh))),
+                   "eb": e, "compressed_bytes": cs}
+            rec["err_within_eb"] = rec["max_abs_err"] <= e
+            log(f"parity REL {rel:g}: {rec}")
+            return rec
+
+        # the other REL ratios first: the headline's stream stays in comp's
+        # buffer for the legs below
+        others = {f"rel_{r:g}": parity_at(r) for r in (1e-2, 1e-3, 1e-4) if r != args.rel}
+        parity = parity_at(args.rel)
+        assert comp.stream_view()[2] == csize
+        parity["other_rel"] = others
+        parity["all_identical"] = all(
+            r["eb_identical"] and r["stream_identical"] and r["decode_identical"]
+            and r["decode_noindex_identical"] for r in [parity, *others.values()])
 
     cpu = None
     if not args.no_cpu:
@@ -795,6 +976,7 @@ def main():
                          f"ABS eb {eb:.6g}, blockSize 128, best of 3 after 1 warm-up; "
                          + ("reference fastMode-4 OpenMP compress" if "openmp" in res
                             else "oracle serial compress"),
+               "host": host_cpu_info(),
                "serial_fastmode2": res["serial"], "openmp_fastmode4": res.get("openmp"),
                "other_modes": res.get("other_modes")}
 
@@ -813,6 +995,11 @@ def main():
                     "decompress_gbs": nbytes / (t2 - t1) / 1e9,
                     "note": "one call each, pageable host buffers, H2D + kernels + D2H + malloc"}
         log(f"host API {host_api}")
+
+    cfg1 = None
+    if not args.no_cfg1:
+        log("fp32 100x500x500 field, ABS 1e-3 x range (GPU, parity, reference CPU codec)")
+        cfg1 = cfg1_eval(dev, max(2, args.steps // 4), parity=not args.no_parity, cpu=not args.no_cpu)
 
     fixed = None
     if args.fxr_fields > 0:
@@ -856,7 +1043,7 @@ def main():
         del view, comp, dec
         torch.cuda.empty_cache()
         batched = batched_eval(args.batch_fields, dev, args.batch_reps,
-                               nbytes / (ms_rel * 1e-3) / 1e9)
+                               nbytes / (ms_rel * 1e-3) / 1e9, parity=not args.no_parity)
         log(f"batched {batched}")
 
     alg_c = nbytes + csize
@@ -900,6 +1087,7 @@ def main():
         "eb": eb,
         "parity": parity,
         "fixed_rate": fixed,
+        "cfg1_100x500x500": cfg1,
         "fp64_grf_8192sq": fp64,
         "batched": batched,
         "generic_block_sizes": generic,

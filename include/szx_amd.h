@@ -530,9 +530,15 @@ int szx_amd_batch_decompress(szx_amd_batch *batch, int flags, int mode, void *hi
  * 8192 or 16384 fp32 values; 4 needs 8 values per lane and blockSize 16..128).
  * szx_amd_batch_tune times every shape on this batch
  * (reps timed passes each, after one untimed pass; synchronizes), keeps the
- * fastest and reports it with its mean milliseconds per compress.  The
- * environment variable SZX_AMD_BATCH_SHAPE="lanes,tiles" sets the initial
- * shape of new plans. */
+ * fastest and reports it with its mean milliseconds per compress.  The choice
+ * is stored per batch class (GPU arch and CU count, blockSize, fp32 share of
+ * the bytes, log2 of the bytes and of the field count) in a JSON file,
+ * szx_amd_tune.json beside libszx_amd.so or the path in SZX_AMD_TUNE_CACHE
+ * ("off" disables it): for a class already there, szx_amd_batch_tune and
+ * szx_amd_batch_create take the stored shape without timing (and the tuner
+ * reports the stored milliseconds).  If timing fails, the plan keeps the shape
+ * it had.  The environment variable SZX_AMD_BATCH_SHAPE="lanes,tiles" sets the
+ * initial shape of new plans (before the file). */
 int szx_amd_batch_set_shape(szx_amd_batch *batch, int valuesPerLane, int tilesPerGroup);
 int szx_amd_batch_tune(szx_amd_batch *batch, int reps, int *valuesPerLane, int *tilesPerGroup,
                        float *ms, void *hipStream);

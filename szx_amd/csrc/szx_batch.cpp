@@ -20,14 +20,25 @@
 //
 // The tile shape of k_compress_batch (values per lane, index tiles per
 // workgroup tile) is chosen per batch by szx_amd_batch_tune, which times every
-// shape on the batch itself; SZX_AMD_BATCH_SHAPE="lv,ipt" overrides it.
+// shape on the batch itself; SZX_AMD_BATCH_SHAPE="lv,ipt" overrides it.  The
+// tuner's choice persists in a small JSON file, one entry per batch class
+// (GPU arch and CU count, blockSize, fp32 share of the bytes, log2 of the
+// bytes and of the field count): new plans of a known class start with its
+// shape and the tuner does not time them again.  SZX_AMD_TUNE_CACHE names
+// the file ("off" disables it); by default it is szx_amd_tune.json beside
+// libszx_amd.so.
 #include "szx_amd.h"
 
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <unistd.h>
 
+#include <map>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "szx_batch.hpp"
@@ -156,6 +167,111 @@ bool hip_ok(hipError_t e, const char *what)
     return false;
 }
 
+// ---- tuner cache --------------------------------------------------------
+
+struct TunedShape {
+    int lv = 0, ipt = 0;
+    float ms = 0.0f;
+};
+
+std::mutex g_tune_mu;
+
+// Path of the cache file, "" when disabled.
+std::string tune_cache_path()
+{
+    if (const char *e = getenv("SZX_AMD_TUNE_CACHE")) {
+        if (!strcmp(e, "off") || !strcmp(e, "0") || !*e) return "";
+        return e;
+    }
+    Dl_info info{};
+    if (!dladdr(reinterpret_cast<void *>(&tune_cache_path), &info) || !info.dli_fname) return "";
+    std::string so = info.dli_fname;
+    const size_t slash = so.rfind('/');
+    return (slash == std::string::npos ? std::string(".") : so.substr(0, slash)) + "/szx_amd_tune.json";
+}
+
+// Class of a batch: the shapes' relative speed depends on the device, the
+// block size, the dtype mix and the batch's size.
+std::string tune_key(const szx_amd_batch *b)
+{
+    hipDeviceProp_t prop{};
+    char arch[64] = "unknown";
+    int cus = 0;
+    if (hipGetDeviceProperties(&prop, b->dev) == hipSuccess) {
+        snprintf(arch, sizeof arch, "%.60s", prop.gcnArchName);
+        for (char *c = arch; *c; c++)
+            if (*c == ':' || *c == '"' || *c == ' ') *c = '\0';   // feature suffixes off
+        cus = prop.multiProcessorCount;
+    }
+    uint64_t bytes = 0, bytes32 = 0;
+    for (const szx_amd_field &f : b->fields) {
+        const uint64_t nb = f.n * (f.dataType == SZx_FLOAT ? 4ull : 8ull);
+        bytes += nb;
+        if (f.dataType == SZx_FLOAT) bytes32 += nb;
+    }
+    const int share = bytes ? (int)((10 * bytes32 + bytes / 2) / bytes) : 0;
+    int lb = 0, lf = 0;
+    while ((2ull << lb) <= bytes) lb++;
+    while ((2 << lf) <= b->nf) lf++;
+    char key[160];
+    snprintf(key, sizeof key, "%s/%dcu/bs%d/f32share%d/bytes2^%d/fields2^%d", arch, cus, b->bs, share, lb, lf);
+    return key;
+}
+
+// Entries of the file: lines of the form  "key": {"lanes": L, "tiles": T, "ms": M}
+std::map<std::string, TunedShape> tune_cache_read(const std::string &path)
+{
+    std::map<std::string, TunedShape> m;
+    FILE *f = fopen(path.c_str(), "r");
+    if (!f) return m;
+    char line[512];
+    while (fgets(line, sizeof line, f)) {
+        char *q0 = strchr(line, '"');
+        char *q1 = q0 ? strchr(q0 + 1, '"') : nullptr;
+        char *br = q1 ? strchr(q1, '{') : nullptr;
+        if (!br) continue;
+        TunedShape t;
+        if (sscanf(br, "{\"lanes\": %d, \"tiles\": %d, \"ms\": %f}", &t.lv, &t.ipt, &t.ms) == 3)
+            m[std::string(q0 + 1, q1)] = t;
+    }
+    fclose(f);
+    return m;
+}
+
+bool tune_cache_lookup(const szx_amd_batch *b, TunedShape &t)
+{
+    const std::string path = tune_cache_path();
+    if (path.empty()) return false;
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    const auto m = tune_cache_read(path);
+    const auto it = m.find(tune_key(b));
+    if (it == m.end()) return false;
+    t = it->second;
+    return true;
+}
+
+// Adds or replaces the batch's entry (whole file rewritten through a
+// temporary and a rename); a file that cannot be written is left alone.
+void tune_cache_store(const szx_amd_batch *b, const TunedShape &t)
+{
+    const std::string path = tune_cache_path();
+    if (path.empty()) return;
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    auto m = tune_cache_read(path);
+    m[tune_key(b)] = t;
+    const std::string tmp = path + ".tmp" + std::to_string((long)getpid());
+    FILE *f = fopen(tmp.c_str(), "w");
+    if (!f) return;
+    fputs("{\n", f);
+    size_t k = 0;
+    for (const auto &e : m)
+        fprintf(f, "  \"%s\": {\"lanes\": %d, \"tiles\": %d, \"ms\": %.6g}%s\n", e.first.c_str(), e.second.lv,
+                e.second.ipt, e.second.ms, ++k < m.size() ? "," : "");
+    fputs("}\n", f);
+    const bool ok = fclose(f) == 0;
+    if (!ok || rename(tmp.c_str(), path.c_str()) != 0) (void)remove(tmp.c_str());
+}
+
 bool shape_ok(int bs, int lv, int ipt)
 {
     if (ipt == 4) return lv == 8 && bs >= 16 && bs <= 128;
@@ -231,12 +347,16 @@ szx_amd_batch *szx_amd_batch_create(const szx_amd_field *fields, int nfields, in
     b->fields.assign(fields, fields + nfields);
     b->can_decode = dec_all;
     (void)hipGetDevice(&b->dev);
+    TunedShape t;
     if (const char *e = getenv("SZX_AMD_BATCH_SHAPE")) {
         int lv = 0, ipt = 0;
         if (sscanf(e, "%d,%d", &lv, &ipt) == 2 && shape_ok(blockSize, lv, ipt)) {
             b->lv = lv;
             b->ipt = ipt;
         }
+    } else if (tune_cache_lookup(b, t) && shape_ok(blockSize, t.lv, t.ipt)) {
+        b->lv = t.lv;
+        b->ipt = t.ipt;
     }
     if (!shape_ok(blockSize, b->lv, b->ipt)) b->lv = 4;
 
@@ -407,9 +527,19 @@ int szx_amd_batch_tune(szx_amd_batch *b, int reps, int *valuesPerLane, int *tile
 {
     api_clear_error();
     if (!b) { api_error("NULL batch"); return SZx_NSCS; }
+    TunedShape cached;
+    if (tune_cache_lookup(b, cached) && shape_ok(b->bs, cached.lv, cached.ipt)) {
+        b->lv = cached.lv;
+        b->ipt = cached.ipt;
+        if (valuesPerLane) *valuesPerLane = cached.lv;
+        if (tilesPerGroup) *tilesPerGroup = cached.ipt;
+        if (ms) *ms = cached.ms;
+        return SZx_SCES;
+    }
     if (reps < 1) reps = 1;
     hipStream_t st = static_cast<hipStream_t>(hipStream);
     const int shapes[5][2] = {{8, 4}, {8, 2}, {8, 1}, {4, 2}, {4, 1}};
+    const int lv0 = b->lv, ipt0 = b->ipt;   // restored if timing fails
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (!hip_ok(hipEventCreate(&e0), "event") || !hip_ok(hipEventCreate(&e1), "event")) {
         if (e0) (void)hipEventDestroy(e0);
@@ -440,9 +570,18 @@ int szx_amd_batch_tune(szx_amd_batch *b, int reps, int *valuesPerLane, int *tile
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    if (rc != SZx_SCES || !blv) return rc != SZx_SCES ? rc : SZx_NSCS;
+    if (rc != SZx_SCES || !blv) {
+        b->lv = lv0;
+        b->ipt = ipt0;
+        return rc != SZx_SCES ? rc : SZx_NSCS;
+    }
     b->lv = blv;
     b->ipt = bipt;
+    TunedShape t;
+    t.lv = blv;
+    t.ipt = bipt;
+    t.ms = best;
+    tune_cache_store(b, t);
     if (valuesPerLane) *valuesPerLane = blv;
     if (tilesPerGroup) *tilesPerGroup = bipt;
     if (ms) *ms = best;

@@ -11,20 +11,19 @@
 // subtract nothing).  Following the chain from byte 0 is inherently serial,
 // so it is cut into chunks of IX_C payload bytes:
 //
-//   A  k_ix_map     per chunk: size(p) for every byte p, then chains walked
-//                   from every candidate entry (merging chains stop where they
-//                   meet) give, for every possible entry offset e (the first
-//                   record start at or after the chunk start lies less than one
-//                   record size R into the chunk), the exit offset into the
-//                   next chunk.  Chains from different entries merge as soon
-//                   as they meet a common record boundary, so almost every
-//                   chunk maps all valid entries to one exit ("converged") and
-//                   the next chunk's entry is known without this one's.
-//   B  k_ix_entry   entries: a chunk after a converged chunk starts at its
-//                   exit; a run of unconverged chunks is followed through their
-//                   stored exit tables by one thread.
-//   C  k_ix_walk    per chunk (one wave): walk the chain from the entry, one
-//                   step per record, keep the record starts, count them.
+//   A  k_ix_spec    per chunk (one wave): a speculative chain from the lowest
+//                   candidate entry (the first record start at or after the
+//                   chunk start lies less than one record size R into the
+//                   chunk) whose chain gets through the chunk.  Chains from
+//                   different record boundaries merge after a few records,
+//                   so it almost always joins the true chain.
+//   B  k_ix_verify  per chunk (one wave): from the entry the chunk before
+//                   predicts (its speculative exit), walk until the chain
+//                   meets the speculative one; record starts = walked ones +
+//                   the speculative ones from there.  A chunk whose exit
+//                   differs from its speculative exit is flagged.
+//   C  k_ix_repair  (only when a chunk was flagged) one wave redoes, in chunk
+//                   order, the chunks whose entries were wrong.
 //   D  scans        records before each chunk; non-constant blocks before
 //                   each compressor tile.
 //   E  k_ix_scatter record starts in record order.
@@ -38,261 +37,37 @@
 // stream or meets an impossible reqLength is marked invalid.  A stream whose
 // chain does not produce exactly the records its bitmap needs is rejected
 // (status -30..-33); the decoder re-checks every record against the index.
+#include <stdlib.h>
+
 #include "szx_common.hpp"
 #include "szx_index.hpp"
 #include "szx_internal.hpp"
+#include "szx_wave.hpp"
 
 namespace szxamd {
 
 constexpr uint32_t IX_C = IX_CHUNK;        // payload bytes per chunk
 constexpr uint16_t IX_INV = 0xFFFF;        // invalid chain / position
 constexpr int IX_PAD = 96;                 // staged bytes past the chunk (5 + NL <= 69)
-constexpr uint32_t IX_PRE_PER = 17;                            // prefix entries per thread
-constexpr uint32_t IX_PRE = 256 * IX_PRE_PER;                  // >= IX_C + 5 + 64 + 1
-// staged words: 16-byte alignment head + every byte the prefix sums read
-constexpr uint32_t IX_WORDS = (16 + IX_PRE) / 4 + 2;
-static_assert(IX_PRE >= IX_C + IX_PAD, "prefix covers the staged chunk");
-constexpr int IX_PER = IX_C / 256;                             // chunk positions per thread
 
-__device__ inline uint32_t ld_u32_unaligned(const uint32_t *w, uint32_t byte)
-{
-    const uint32_t a = w[byte >> 2], b = w[(byte >> 2) + 1];
-    return __builtin_amdgcn_alignbyte(b, a, byte & 3);
-}
+__global__ void k_ix_zero(uint32_t *w) { *w = 0; }
 
-// Stage bytes [src, src + cnt) into LDS words, zero-padded, at byte offset
-// (src & 15): aligned 16-byte chunks lying wholly inside the stream
-// [lo, hi) are copied with one load (their bytes outside the range are
-// stream bytes no step reads), the rest byte by byte; returns the offset.
-__device__ inline uint32_t ix_stage(uint32_t *bw, const uint8_t *src, uint32_t cnt, uint32_t words,
-                                    const uint8_t *lo, const uint8_t *hi)
-{
-    uint8_t *b = reinterpret_cast<uint8_t *>(bw);
-    for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) bw[i] = 0;
-    __syncthreads();
-    const uintptr_t a = (uintptr_t)src & ~(uintptr_t)15;
-    const uint32_t head = (uint32_t)((uintptr_t)src - a);
-    const uint32_t nchunk = (head + cnt + 15) / 16;
-    for (uint32_t c = threadIdx.x; c < nchunk; c += blockDim.x) {
-        const uint8_t *g = reinterpret_cast<const uint8_t *>(a + 16 * (uintptr_t)c);
-        if (g >= lo && g + 16 <= hi)
-            *reinterpret_cast<uint4 *>(b + 16 * c) = *reinterpret_cast<const uint4 *>(g);
-    }
-    // the first and last chunks may be partial: one byte per thread
-    if (threadIdx.x < 32 && nchunk) {
-        const uint32_t c = threadIdx.x < 16 ? 0 : nchunk - 1;
-        const uint32_t k = threadIdx.x & 15;
-        const uint8_t *g = reinterpret_cast<const uint8_t *>(a + 16 * (uintptr_t)c) + k;
-        if (g >= lo && g < hi) b[16 * c + k] = *g;
-    }
-    __syncthreads();
-    return head;
-}
-
-// Per lead-code byte c, packed in 16-bit fields: the sum over its four 2-bit
-// codes of min(code, 1), of min(code, 2) and of the code itself -- what a
-// record with rb = 1, 2 or >= 3 subtracts from 4 rb per lead byte.
-__device__ inline uint64_t lead_counts(uint32_t c)
+// Per lead-code byte c: the sum over its four 2-bit codes of min(code, 1),
+// of min(code, 2) and of the code itself -- what a record with rb = 1, 2 or
+// >= 3 subtracts from 4 rb per lead byte -- packed at bits 0, 10 and 21 of a
+// u32.  A record's NL <= 64 lead bytes sum to at most 256 / 512 / 768 per
+// field, below 2^10 / 2^11 / 2^11, so the fields of a sum never carry.
+__device__ inline uint32_t lead_counts(uint32_t c)
 {
     const uint32_t lo = c & 0x55u, hi = (c >> 1) & 0x55u;
     const uint32_t f1 = __builtin_popcount(lo | hi);
     const uint32_t f3 = __builtin_popcount(lo) + 2 * __builtin_popcount(hi);
     const uint32_t f2 = f3 - __builtin_popcount(lo & hi);
-    return (uint64_t)f1 | ((uint64_t)f2 << 16) | ((uint64_t)f3 << 32);
+    return f1 | (f2 << 10) | (f3 << 21);
 }
 
-// Chain step for every chunk byte: J[p] = p + size(p), or IX_INV; positions
-// at or past IX_C absorb.  The NL-byte window sums of lead counts come from
-// an exclusive prefix sum over the staged bytes (pre, IX_PRE entries), so a
-// step costs two reads.  avail: payload bytes present from the chunk start.
-__device__ inline void ix_steps(const uint32_t *bw, uint32_t base, uint64_t *pre, uint16_t *J,
-                                uint32_t dom, uint32_t avail, int nl, int bs, int w)
-{
-    const uint8_t *b = reinterpret_cast<const uint8_t *>(bw) + base;
-    __shared__ uint64_t s_part[256];
-    // per-thread runs of IX_PRE_PER bytes
-    const uint32_t q0 = threadIdx.x * IX_PRE_PER;
-    uint64_t cnt[IX_PRE_PER];   // loads first, then arithmetic (LDS stores below may alias)
-#pragma unroll
-    for (uint32_t i = 0; i < IX_PRE_PER; i++) cnt[i] = b[q0 + i];
-    uint64_t run = 0;
-#pragma unroll
-    for (uint32_t i = 0; i < IX_PRE_PER; i++) {
-        cnt[i] = lead_counts((uint32_t)cnt[i]);
-        run += cnt[i];
-    }
-    s_part[threadIdx.x] = run;
-    __syncthreads();
-    for (int d = 1; d < 256; d <<= 1) {
-        const uint64_t v = threadIdx.x >= (unsigned)d ? s_part[threadIdx.x - d] : 0;
-        __syncthreads();
-        s_part[threadIdx.x] += v;
-        __syncthreads();
-    }
-    run = s_part[threadIdx.x] - run;
-#pragma unroll
-    for (uint32_t i = 0; i < IX_PRE_PER; i++) {
-        pre[q0 + i] = run;
-        run += cnt[i];
-    }
-    __syncthreads();
-    // chunk positions: IX_PER per thread, all loads before any store
-    uint32_t rl[IX_PER];
-    uint64_t lo[IX_PER], hi[IX_PER];
-#pragma unroll
-    for (int i = 0; i < IX_PER; i++) {
-        const uint32_t p = threadIdx.x + 256 * i;
-        rl[i] = b[p];
-        lo[i] = pre[p + 5];
-        hi[i] = pre[p + 5 + nl];
-    }
-#pragma unroll
-    for (int i = 0; i < IX_PER; i++) {
-        const uint32_t p = threadIdx.x + 256 * i;
-        uint16_t j = IX_INV;
-        if (p + 5 + (uint32_t)nl <= avail && rl[i] != 0 && rl[i] <= 8u * w) {
-            const uint32_t rb = (rl[i] + 7) >> 3;
-            const uint32_t sh = rb >= 3 ? 32 : (rb == 2 ? 16 : 0);
-            const uint32_t sub = (uint32_t)((hi[i] - lo[i]) >> sh) & 0xFFFFu;
-            // nonzero unused codes of a partial lead byte (a damaged stream)
-            // could subtract more than the block holds: no record there
-            if (sub <= rb * (uint32_t)bs) j = (uint16_t)(p + 5 + (uint32_t)nl + rb * (uint32_t)bs - sub);
-        }
-        J[p] = j;
-    }
-    for (uint32_t p = IX_C + threadIdx.x; p < dom; p += blockDim.x) J[p] = (uint16_t)p;
-}
-
-// A: exit map of every chunk.  After the step table, one chain per candidate
-// entry e < rmax is walked, each position claimed by the first chain to reach
-// it (an LDS compare-and-swap): a chain that meets a claimed position stops
-// and links to its owner, whose chain continues from there exactly as its own
-// would.  Chains merge after a few records, so the walk costs about one step
-// per distinct record start; links are then resolved by pointer jumping (link
-// positions increase along a link chain, so there are no cycles).
-constexpr uint32_t IX_NONE = 0xFFFFFFFFu;
-constexpr uint32_t IX_RES_INV = 0x40000000u;
-constexpr uint32_t IX_RES_LINK = 0x80000000u;
-
-__global__ __launch_bounds__(256) void k_ix_map(IxParams x)
-{
-    __shared__ uint32_t bw[IX_WORDS];
-    __shared__ uint16_t J0[IX_C];
-    __shared__ union {
-        uint64_t pre[IX_PRE];
-        struct {
-            uint32_t owner[IX_C];
-            uint32_t res[IX_RMAX];
-        } w;
-    } u;
-    __shared__ unsigned s_min, s_max, s_more;
-    const uint64_t c = blockIdx.x;
-    const DecodeHeader &h = *x.h;
-    if (*x.status != 0) return;
-    const uint64_t P = x.len - h.q0;       // payload bytes present in the buffer
-    const uint64_t g0 = c * IX_C;
-    if (g0 >= P) {
-        if (threadIdx.x == 0) {
-            x.conv[c] = 1;
-            x.exits[c] = IX_INV;
-        }
-        return;
-    }
-    const uint32_t avail = (uint32_t)(P - g0 < IX_C + IX_PAD ? P - g0 : IX_C + IX_PAD);
-    const uint32_t base = ix_stage(bw, x.s + h.q0 + g0, avail, IX_WORDS, x.s, x.s + x.len);
-    ix_steps(bw, base, u.pre, J0, IX_C, avail, x.nl, x.bs, x.w);
-    __syncthreads();   // the prefix sums die here: their space holds the claims
-    for (uint32_t p = threadIdx.x; p < IX_C; p += blockDim.x) u.w.owner[p] = IX_NONE;
-    if (threadIdx.x == 0) {
-        s_min = 0xFFFFFFFFu;
-        s_max = 0;
-    }
-    __syncthreads();
-    const uint32_t rmax = x.rmax;
-    for (uint32_t e = threadIdx.x; e < rmax; e += blockDim.x) {
-        uint32_t p = e, r;
-        for (;;) {
-            if (p >= IX_C) {
-                r = p - IX_C;
-                break;
-            }
-            const uint32_t o = atomicCAS(&u.w.owner[p], IX_NONE, e);
-            if (o != IX_NONE) {
-                r = IX_RES_LINK | o;
-                break;
-            }
-            const uint16_t j = J0[p];
-            if (j == IX_INV) {
-                r = IX_RES_INV;
-                break;
-            }
-            p = j;
-        }
-        u.w.res[e] = r;
-    }
-    __syncthreads();
-    for (int round = 0; round < 16; round++) {   // link depth < 2^16
-        if (threadIdx.x == 0) s_more = 0;
-        __syncthreads();
-        for (uint32_t e = threadIdx.x; e < rmax; e += blockDim.x) {
-            const uint32_t r = u.w.res[e];
-            if (r & IX_RES_LINK) {
-                const uint32_t t = u.w.res[r & 0xFFFFu];
-                u.w.res[e] = t;
-                if (t & IX_RES_LINK) s_more = 1;
-            }
-        }
-        __syncthreads();
-        const bool more = s_more != 0;
-        __syncthreads();   // every lane has read the flag before it is cleared
-        if (!more) break;
-    }
-    // entries e < rmax: exit offset into the next chunk (wave min / max, then
-    // one LDS atomic per wave)
-    unsigned mn = 0xFFFFFFFFu, mx = 0;
-    for (uint32_t e = threadIdx.x; e < rmax; e += blockDim.x) {
-        const uint32_t r = u.w.res[e];
-        if (!(r & (IX_RES_INV | IX_RES_LINK))) {
-            mn = r < mn ? r : mn;
-            mx = r > mx ? r : mx;
-        }
-    }
-#pragma unroll
-    for (int m = 1; m < 64; m <<= 1) {
-        const unsigned a = __shfl_xor(mn, m, 64), b = __shfl_xor(mx, m, 64);
-        mn = a < mn ? a : mn;
-        mx = b > mx ? b : mx;
-    }
-    if ((threadIdx.x & 63) == 0 && mn != 0xFFFFFFFFu) {
-        atomicMin(&s_min, mn);
-        atomicMax(&s_max, mx);
-    }
-    __syncthreads();
-    const bool conv = s_min == s_max || s_min == 0xFFFFFFFFu;
-    if (threadIdx.x == 0) {
-        x.conv[c] = conv ? 1 : 0;
-        x.exits[c] = s_min == 0xFFFFFFFFu ? IX_INV : (uint16_t)s_min;
-    }
-    if (!conv)
-        for (uint32_t e = threadIdx.x; e < rmax; e += blockDim.x) {
-            const uint32_t r = u.w.res[e];
-            x.tables[c * rmax + e] = (r & (IX_RES_INV | IX_RES_LINK)) ? IX_INV : (uint16_t)r;
-        }
-}
-
-// B: entry offset of every chunk (one thread per run of unconverged chunks).
-__global__ void k_ix_entry(IxParams x)
-{
-    const uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    if (c >= x.nch || *x.status != 0) return;
-    if (c != 0 && !x.conv[c - 1]) return;   // inside a run: its start thread writes it
-    uint16_t e = c == 0 ? 0 : x.exits[c - 1];
-    for (uint64_t k = c; k < x.nch; k++) {
-        x.entry[k] = e;
-        if (x.conv[k]) break;
-        e = e == IX_INV || e >= x.rmax ? IX_INV : x.tables[k * x.rmax + e];
-    }
-}
+constexpr uint32_t IX_MAXR = IX_C / 6 + 2;                      // record starts per chunk, any bs
+constexpr uint32_t IXW_WORDS = (16 + IX_C + IX_PAD) / 4 + 2;     // one wave's staged chunk
 
 // Stage bytes [src, src + cnt) into one wave's LDS words at byte offset
 // (src & 15): aligned 16-byte chunks lying wholly inside the stream [lo, hi)
@@ -316,51 +91,261 @@ __device__ inline uint32_t ix_stage_wave(uint32_t *bw, const uint8_t *src, uint3
     return head;
 }
 
-constexpr uint32_t IXW_WORDS = (16 + IX_C + IX_PAD) / 4 + 2;   // one wave's staged chunk
+// One step of the record chain inside a chunk, for the whole wave: the
+// position after the record at p (wave-uniform, < IX_C), or IX_INV where no
+// record can start (the reqLength byte out of range, the record's lead codes
+// past the bytes present, or lead codes subtracting more than the block
+// holds).  byte(i) reads chunk byte i; lanes 0..31 sum the packed lead counts
+// of two lead-code bytes each (NL <= 64) with a DPP reduction.
+template <typename Byte>
+__device__ inline uint32_t chain_step(const Byte &byte, const uint32_t *lut, uint32_t p, uint32_t avail,
+                                      uint32_t nl, uint32_t bsz, uint32_t wmax, int lane)
+{
+    if (p + 5 + nl > avail) return IX_INV;
+    const uint32_t rl = byte(p);
+    uint32_t c = 0;
+    if (lane < 32) {
+        const uint32_t i0 = 2 * (uint32_t)lane;
+        if (i0 < nl) c = lut[byte(p + 5 + i0)];
+        if (i0 + 1 < nl) c += lut[byte(p + 6 + i0)];
+    }
+    c = seg_reduce<32>(c, [](uint32_t u, uint32_t v) { return u + v; });
+    const uint32_t d = __builtin_amdgcn_readfirstlane(c);
+    const uint32_t rlu = __builtin_amdgcn_readfirstlane(rl);
+    if (rlu == 0 || rlu > wmax) return IX_INV;
+    const uint32_t rb = (rlu + 7) >> 3;
+    const uint32_t sub = rb >= 3 ? d >> 21 : rb == 2 ? (d >> 10) & 0x7FFu : d & 0x3FFu;
+    // nonzero unused codes of a partial lead byte (a damaged stream) could
+    // subtract more than the block holds: no record there
+    if (sub > rb * bsz) return IX_INV;
+    return p + 5 + nl + rb * bsz - sub;
+}
 
-// C: record starts inside every chunk, from its entry.  One wave per chunk
-// follows the chain: a record's size comes from its reqLength byte and its
-// lead-code bytes (one byte per lane, a wave sum of the same per-byte counts
-// the map kernel's prefix sums hold), so the chain costs one step per record.
-// Marks the entry and every chain position below IX_C, stopping where the
-// map kernel's step table would be invalid.
-__global__ __launch_bounds__(256) void k_ix_walk(IxParams x)
+struct IxChunk {
+    uint64_t c, g0;
+    uint32_t avail;
+    bool has;    // the chunk starts inside the payload
+    bool ends;   // the payload ends inside the chunk
+};
+
+__device__ inline IxChunk ix_chunk(const IxParams &x, uint64_t c)
+{
+    IxChunk k;
+    const uint64_t P = x.len - x.h->q0;   // payload bytes present in the buffer
+    k.c = c;
+    k.g0 = c * IX_C;
+    k.has = k.g0 < P;
+    k.ends = P - k.g0 <= IX_C;
+    k.avail = k.has ? (uint32_t)(P - k.g0 < IX_C + IX_PAD ? P - k.g0 : IX_C + IX_PAD) : 0u;
+    return k;
+}
+
+// Walk from `entry` to the end of the chunk, keeping every position (the
+// record starts; a position whose record is invalid is kept and ends the
+// walk); `stop(p)` ends the walk early at a position other knowledge covers.
+// Returns the positions kept; *last is the position the walk ended at (>=
+// IX_C past the chunk, IX_INV after an invalid record, or the stop position).
+template <typename Byte, typename Stop>
+__device__ inline uint32_t chain_walk(const Byte &byte, const uint32_t *lut, uint32_t entry, const IxChunk &k,
+                                      const IxParams &x, uint16_t *keep, uint32_t cap, const Stop &stop,
+                                      uint32_t *last, int lane)
+{
+    const uint32_t nl = (uint32_t)x.nl, bsz = (uint32_t)x.bs, wmax = 8u * (uint32_t)x.w;
+    uint32_t p = entry, n = 0;
+    while (p < IX_C) {
+        if (stop(p)) break;
+        if (lane == 0 && n < cap) keep[n] = (uint16_t)p;
+        n++;
+        p = chain_step(byte, lut, p, k.avail, nl, bsz, wmax, lane);
+    }
+    *last = p;
+    return n < cap ? n : cap;
+}
+
+// Lead-count table (lead_counts of every byte), one entry per thread.
+__device__ inline void lead_lut_fill(uint32_t *lut)
+{
+    for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) lut[i] = lead_counts(i);
+}
+
+// A: speculative chains.  One wave per chunk walks from the lowest candidate
+// entry (a position below rmax with a plausible reqLength byte) whose chain
+// reaches the end of the chunk, keeping its record starts.  Chains from
+// different record boundaries merge after a few records, so this chain
+// almost always joins the true one inside the chunk; B checks it.
+// spec[c] = start | exit << 16 (IX_INV halves when no chain gets through).
+// The chunk where the payload ends is not speculated (its chains end inside
+// it): B walks it from its entry.
+__global__ __launch_bounds__(256) void k_ix_spec(IxParams x)
 {
     __shared__ uint32_t bw[4][IXW_WORDS];
+    __shared__ uint32_t lut[256];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const uint64_t c = blockIdx.x * 4ull + wv;
+    lead_lut_fill(lut);
     if (*x.status != 0) return;   // workgroup-uniform: written by earlier kernels only
-    const DecodeHeader &h = *x.h;
-    const uint64_t P = x.len - h.q0;
-    const uint64_t g0 = c * IX_C;
+    const uint64_t c = blockIdx.x * 4ull + wv;
     const bool live = c < x.nch;
-    const bool has = live && g0 < P;
-    const uint32_t avail = has ? (uint32_t)(P - g0 < IX_C + IX_PAD ? P - g0 : IX_C + IX_PAD) : 0u;
+    const IxChunk k = ix_chunk(x, live ? c : 0);
     uint32_t base = 0;
-    if (has) base = ix_stage_wave(bw[wv], x.s + h.q0 + g0, avail, x.s, x.s + x.len, lane);
+    if (live && k.has) base = ix_stage_wave(bw[wv], x.s + x.h->q0 + k.g0, k.avail, x.s, x.s + x.len, lane);
     __syncthreads();
     if (!live) return;
     const uint8_t *b = reinterpret_cast<const uint8_t *>(bw[wv]) + base;
-    const uint32_t nl = (uint32_t)x.nl, bsz = (uint32_t)x.bs, wmax = 8u * (uint32_t)x.w;
-    uint32_t p = has ? x.entry[c] : IX_INV;
-    uint32_t k = 0;
+    const auto byte = [b](uint32_t i) -> uint32_t { return b[i]; };
+    const uint32_t nl = (uint32_t)x.nl, wmax = 8u * (uint32_t)x.w, rmax = x.rmax;
     uint16_t *pos = x.pos + c * x.maxr;
-    if (p != IX_INV) {
-        while (p < IX_C) {
-            if (lane == 0 && k < x.maxr) pos[k] = (uint16_t)p;
-            k++;
-            const uint32_t rl = b[p];
-            if (p + 5 + nl > avail || rl == 0 || rl > wmax) break;
-            const uint32_t rb = (rl + 7) >> 3;
-            const uint32_t sh = rb >= 3 ? 32 : (rb == 2 ? 16 : 0);
-            uint32_t sub = (uint32_t)lane < nl ? (uint32_t)(lead_counts(b[p + 5 + lane]) >> sh) & 0xFFFFu : 0u;
-#pragma unroll
-            for (int d = 1; d < 64; d <<= 1) sub += __shfl_xor(sub, d, 64);
-            if (sub > rb * bsz) break;
-            p = p + 5 + nl + rb * bsz - sub;
+    uint32_t start = IX_INV, ex = IX_INV, n = 0;
+    const auto never = [](uint32_t) { return false; };
+    for (uint32_t q0 = 0; k.has && !k.ends && !x.no_spec && q0 < rmax && start == IX_INV; q0 += 64) {
+        const uint32_t q = q0 + (uint32_t)lane;
+        const uint32_t rl = q < IX_C + IX_PAD ? b[q] : 0u;
+        unsigned long long m = __ballot(q < rmax && q + 5 + nl <= k.avail && rl != 0 && rl <= wmax);
+        while (m) {
+            const uint32_t e = q0 + (uint32_t)__builtin_ctzll(m);
+            m &= m - 1;
+            uint32_t last;
+            n = chain_walk(byte, lut, e, k, x, pos, x.maxr, never, &last, lane);
+            if (last != IX_INV) {
+                start = e;
+                ex = last - IX_C;
+                break;
+            }
         }
     }
-    if (lane == 0) x.cnt[c] = k < x.maxr ? k : x.maxr;
+    if (lane == 0) {
+        x.spec[c] = start | (ex << 16);
+        x.cnt[c] = start == IX_INV ? 0u : n;
+    }
+}
+
+// Position p of the chunk's speculative chain (sorted starts in lst[0..n)):
+// its index, found by the lanes, or -1.
+__device__ inline int spec_rank(const uint16_t *lst, uint32_t n, uint32_t p, int lane)
+{
+    int r = -1;
+    for (uint32_t i0 = 0; i0 < n; i0 += 64) {
+        const uint32_t i = i0 + (uint32_t)lane;
+        const unsigned long long m = __ballot(i < n && lst[i] == p);
+        if (m) {
+            r = (int)(i0 + __builtin_ctzll(m));
+            break;
+        }
+    }
+    return r;
+}
+
+// B: true record starts.  One wave per chunk walks from its entry -- the
+// speculative exit of the chunk before it (0 for the first) -- reading the
+// stream, until the chain meets the speculative chain (almost always within
+// a few records); the record starts are the walked ones followed by the
+// speculative ones from the meeting point.  tex[c] = the chunk's exit for
+// that entry, with bit 31 set and nfail counted when it differs from the
+// speculative exit (then the entries of the chunks after it are wrong, and C
+// redoes them).
+__global__ __launch_bounds__(256) void k_ix_verify(IxParams x)
+{
+    __shared__ uint32_t lut[256];
+    __shared__ uint32_t bits[4][IX_C / 32];
+    __shared__ uint16_t lst[4][IX_MAXR], pre[4][IX_MAXR];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    lead_lut_fill(lut);
+    for (uint32_t i = lane; i < IX_C / 32; i += 64) bits[wv][i] = 0;
+    __syncthreads();
+    if (*x.status != 0) return;
+    const uint64_t c = blockIdx.x * 4ull + wv;
+    if (c >= x.nch) return;
+    const IxChunk k = ix_chunk(x, c);
+    const uint32_t sp = x.spec[c];
+    const uint32_t sex = sp >> 16;
+    const uint32_t n = x.cnt[c];
+    uint16_t *pos = x.pos + c * x.maxr;
+    for (uint32_t i = lane; i < n; i += 64) {
+        const uint32_t q = pos[i];
+        lst[wv][i] = (uint16_t)q;
+        atomicOr(&bits[wv][q >> 5], 1u << (q & 31));
+    }
+    const uint32_t entry = c == 0 ? 0u : x.spec[c - 1] >> 16;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const uint8_t *g = x.s + x.h->q0 + k.g0;
+    const auto byte = [g](uint32_t i) -> uint32_t { return g[i]; };
+    const uint32_t *bw = bits[wv];
+    const auto on_spec = [bw](uint32_t p) { return ((bw[p >> 5] >> (p & 31)) & 1u) != 0; };
+    uint32_t last = IX_INV, m = 0;
+    if (k.has && entry != IX_INV) m = chain_walk(byte, lut, entry, k, x, pre[wv], IX_MAXR, on_spec, &last, lane);
+    // the walked starts, then the speculative ones from the meeting point
+    uint32_t from = n, ex;
+    const int r = last < IX_C ? spec_rank(lst[wv], n, last, lane) : -1;
+    if (r >= 0) {   // met the speculative chain at `last`
+        from = (uint32_t)r;
+        ex = sex;
+    } else {
+        ex = (last == IX_INV || last < IX_C) ? IX_INV : last - IX_C;
+    }
+    const uint32_t total = m + (n - from) < x.maxr ? m + (n - from) : x.maxr;
+    for (uint32_t i = lane; i < total; i += 64) pos[i] = i < m ? pre[wv][i] : lst[wv][from + i - m];
+    if (lane == 0) {
+        x.cnt[c] = total;
+        const bool bad = ex != sex;
+        x.tex[c] = ex | (bad ? 0x80000000u : 0u);
+        if (bad) atomicAdd(x.nfail, 1u);
+    }
+}
+
+// C: repair after speculative exits that were wrong (rare: a chunk whose
+// speculative chain never met the true one).  One wave, in chunk order: from
+// the first chunk whose exit differed, each following chunk is walked in full
+// from the true exit of the one before, until a chunk's true exit equals its
+// speculative one (its successors' entries were right); then the next chunk
+// whose exit differed, and so on.
+__global__ __launch_bounds__(64) void k_ix_repair(IxParams x)
+{
+    __shared__ uint32_t lut[256];
+    __shared__ uint16_t keep[IX_MAXR];
+    const int lane = threadIdx.x;
+    lead_lut_fill(lut);
+    __syncthreads();
+    if (*x.status != 0 || *x.nfail == 0) return;
+    const auto never = [](uint32_t) { return false; };
+    uint64_t c = 0;
+    for (;;) {
+        // next chunk whose exit differed from its speculative exit
+        uint64_t f = x.nch;
+        for (uint64_t i0 = c; i0 < x.nch; i0 += 64) {
+            const uint64_t i = i0 + (uint64_t)lane;
+            const unsigned long long m = __ballot(i < x.nch && (x.tex[i] & 0x80000000u));
+            if (m) {
+                f = i0 + __builtin_ctzll(m);
+                break;
+            }
+        }
+        if (f + 1 >= x.nch) return;
+        uint32_t t = x.tex[f] & 0xFFFFu;   // true exit of f (its entry was right)
+        c = f + 1;
+        for (; c < x.nch; c++) {
+            if (t == (x.spec[c - 1] >> 16)) break;   // entries right again from here
+            const IxChunk k = ix_chunk(x, c);
+            const uint8_t *g = x.s + x.h->q0 + k.g0;
+            const auto byte = [g](uint32_t i) -> uint32_t { return g[i]; };
+            uint32_t last = IX_INV, n = 0;
+            if (k.has && t != IX_INV) n = chain_walk(byte, lut, t, k, x, keep, IX_MAXR, never, &last, lane);
+            const uint32_t cap = n < x.maxr ? n : x.maxr;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            uint16_t *pos = x.pos + c * x.maxr;
+            for (uint32_t i = lane; i < cap; i += 64) pos[i] = keep[i];
+            const uint32_t tn = last == IX_INV ? IX_INV : last - IX_C;
+            if (lane == 0) {
+                x.cnt[c] = cap;
+                x.tex[c] = tn;
+            }
+            __builtin_amdgcn_wave_barrier();
+            t = tn;
+        }
+        if (c >= x.nch) return;
+    }
 }
 
 // Exclusive scan of n u32 values in three steps: per 1024-value block an
@@ -433,7 +418,17 @@ __global__ void k_ix_scatter(IxParams x)
         if (base + i < K) x.rec[base + i] = c * IX_C + x.pos[c * x.maxr + i];
 }
 
-// Non-constant blocks per compressor tile (bitmap popcount).
+// Bitmap byte j of a block range [b0, b1) with b0 a multiple of 8: the bits of
+// blocks at or past b1 cleared (the bitmap is MSB first).
+__device__ inline uint32_t bitmap_byte(const IxParams &x, const DecodeHeader &h, uint64_t j, uint64_t b1)
+{
+    uint32_t v = x.s[h.qbit + j];
+    if (8 * j + 8 > b1) v &= (0xFF00u >> (b1 - 8 * j)) & 0xFFu;
+    return v;
+}
+
+// Non-constant blocks per index tile (popcount of its bitmap bytes: a tile
+// holds a multiple of 8 blocks).
 __global__ void k_ix_tile_counts(IxParams x)
 {
     const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
@@ -441,75 +436,116 @@ __global__ void k_ix_tile_counts(IxParams x)
     const DecodeHeader &h = *x.h;
     const uint64_t b0 = t * x.tile_blocks, b1 = b0 + x.tile_blocks < h.nb ? b0 + x.tile_blocks : h.nb;
     uint32_t c = 0;
-    for (uint64_t b = b0; b < b1; b++) c += (x.s[h.qbit + (b >> 3)] >> (7 - (b & 7))) & 1;
+    if (x.tile_blocks % 8 == 0) {
+        for (uint64_t j = b0 / 8; 8 * j < b1; j++) c += __builtin_popcount(bitmap_byte(x, h, j, b1));
+    } else {
+        for (uint64_t b = b0; b < b1; b++) c += (x.s[h.qbit + (b >> 3)] >> (7 - (b & 7))) & 1;
+    }
     x.tcnt[t] = c;
 }
 
-// F: the last record's true size and the payload end; checks the record count.
+// F: the last record's true size and the payload end; checks the record
+// count.  One wave: the bitmap is searched from its end 64 bytes at a time,
+// and the last record's lead codes are summed one byte per lane.
 __global__ void k_ix_tail(IxParams x)
 {
-    if (threadIdx.x != 0 || *x.status != 0) return;
+    const int lane = threadIdx.x;
+    if (*x.status != 0) return;
     const DecodeHeader &h = *x.h;
     const uint64_t K = h.nb - h.nconst;
     const uint64_t P = x.len - h.q0;
-    if (*x.found < K) {
-        *x.status = -30;
-        return;
+    int32_t st = 0;
+    uint64_t pend = 0;
+    if (*x.found < K) st = -30;
+    if (st == 0 && K > 0) {
+        // the last non-constant block (one exists: K > 0)
+        const uint64_t nbyte = (h.nb + 7) / 8;
+        uint64_t b = 0;
+        for (uint64_t top = nbyte; top > 0; top = top > 64 ? top - 64 : 0) {
+            uint32_t v = 0;
+            if ((uint64_t)lane < top) {
+                const uint64_t j = top - 1 - lane;
+                v = x.s[h.qbit + j];
+                if (8 * j + 8 > h.nb) v &= (0xFF00u >> (h.nb - 8 * j)) & 0xFFu;
+            }
+            const unsigned long long m = __ballot(v != 0);
+            if (m) {
+                const int l = __builtin_ctzll(m);   // the lowest lane holds the highest byte
+                const uint32_t vb = __shfl(v, l, 64);
+                b = 8 * (top - 1 - l) + 7 - __builtin_ctz(vb);
+                break;
+            }
+        }
+        const uint64_t nel = (b + 1) * (uint64_t)x.bs <= x.n ? (uint64_t)x.bs : x.n - b * (uint64_t)x.bs;
+        const uint64_t nl = (nel + 3) / 4;
+        const uint64_t q = x.rec[K - 1];
+        if (q + 5 + nl > P) {
+            st = -31;
+        } else {
+            const uint8_t *r = x.s + h.q0 + q;
+            const uint32_t rl = r[0];
+            if (rl == 0 || rl > 8u * x.w) {
+                st = -32;
+            } else {
+                const uint32_t rb = (rl + 7) >> 3;
+                uint32_t add = 0;
+                for (uint64_t l = lane; l < nl; l += 64) {
+                    const uint32_t by = r[5 + l];
+                    for (uint32_t k = 0; k < 4 && 4 * l + k < nel; k++) {
+                        const uint32_t ld = (by >> (6 - 2 * k)) & 3;
+                        if (ld < rb) add += rb - ld;
+                    }
+                }
+#pragma unroll
+                for (int m = 1; m < 64; m <<= 1) add += __shfl_xor(add, m, 64);
+                const uint64_t sz = 5 + nl + add;
+                if (q + sz > P) st = -33;
+                else pend = q + sz;
+            }
+        }
     }
-    if (K == 0) {
-        *x.pend = 0;
-        return;
+    if (lane == 0) {
+        if (st) *x.status = st;
+        else *x.pend = pend;
     }
-    // the last non-constant block
-    uint64_t b = h.nb;
-    while (b > 0) {
-        b--;
-        if ((x.s[h.qbit + (b >> 3)] >> (7 - (b & 7))) & 1) break;
-    }
-    const uint64_t nel = (b + 1) * (uint64_t)x.bs <= x.n ? (uint64_t)x.bs : x.n - b * (uint64_t)x.bs;
-    const uint64_t nl = (nel + 3) / 4;
-    const uint64_t q = x.rec[K - 1];
-    if (q + 5 + nl > P) {
-        *x.status = -31;
-        return;
-    }
-    const uint8_t *r = x.s + h.q0 + q;
-    const uint32_t rl = r[0];
-    if (rl == 0 || rl > 8u * x.w) {
-        *x.status = -32;
-        return;
-    }
-    const uint32_t rb = (rl + 7) >> 3;
-    uint64_t sz = 5 + nl;
-    for (uint64_t i = 0; i < nel; i++) {
-        const uint32_t ld = (r[5 + (i >> 2)] >> (6 - 2 * (i & 3))) & 3;
-        if (ld < rb) sz += rb - ld;
-    }
-    if (q + sz > P) {
-        *x.status = -33;
-        return;
-    }
-    *x.pend = q + sz;
 }
 
-// G: tile pairs and per-block entries.  One workgroup per tile.
+// G: tile pairs and per-block entries.  One workgroup per tile; the
+// non-constant blocks before each block of the tile come from a wave scan of
+// the popcounts of the tile's bitmap bytes (at most 128: 1024 blocks).
 __global__ __launch_bounds__(256) void k_ix_tiles(IxParams x)
 {
-    __shared__ uint32_t s_nc[1024 + 1];   // non-constant blocks before each block of the tile
+    __shared__ uint32_t s_byte[128], s_pre[128];
     const uint64_t t = blockIdx.x;
     if (*x.status != 0) return;
     const DecodeHeader &h = *x.h;
     const uint64_t K = h.nb - h.nconst;
     const uint64_t b0 = t * x.tile_blocks;
     const uint64_t nbt = b0 + x.tile_blocks <= h.nb ? x.tile_blocks : h.nb - b0;
+    const uint64_t b1 = b0 + nbt;
+    const uint32_t nbytes = (uint32_t)((nbt + 7) / 8);
     const uint64_t ord0 = x.tbase[t];
-    if (threadIdx.x == 0) {
-        uint32_t a = 0;
-        for (uint64_t i = 0; i < nbt; i++) {
-            s_nc[i] = a;
-            a += (x.s[h.qbit + ((b0 + i) >> 3)] >> (7 - ((b0 + i) & 7))) & 1;
+    if (threadIdx.x < 64) {
+        // two bytes per lane, inclusive wave scan of their popcounts
+        const uint32_t j0 = 2 * threadIdx.x, j1 = j0 + 1;
+        const uint32_t v0 = j0 < nbytes ? bitmap_byte(x, h, b0 / 8 + j0, b1) : 0u;
+        const uint32_t v1 = j1 < nbytes ? bitmap_byte(x, h, b0 / 8 + j1, b1) : 0u;
+        const uint32_t c0 = __builtin_popcount(v0), c1 = __builtin_popcount(v1);
+        uint32_t inc = c0 + c1;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t u = __shfl_up(inc, d, 64);
+            if ((int)threadIdx.x >= d) inc += u;
         }
-        s_nc[nbt] = a;
+        const uint32_t ex = inc - c0 - c1;
+        if (j0 < 128) {
+            s_byte[j0] = v0;
+            s_byte[j1] = v1;
+            s_pre[j0] = ex;
+            s_pre[j1] = ex + c0;
+        }
+    }
+    if (threadIdx.x == 0) {
         const uint64_t off = ord0 < K ? x.rec[ord0] : *x.pend;
         x.index[2 * t] = off;
         x.index[2 * t + 1] = b0 - ord0;
@@ -521,16 +557,18 @@ __global__ __launch_bounds__(256) void k_ix_tiles(IxParams x)
     __syncthreads();
     const uint64_t off = ord0 < K ? x.rec[ord0] : *x.pend;
     for (uint64_t i = threadIdx.x; i < nbt; i += blockDim.x) {
-        const uint64_t b = b0 + i;
-        const bool nc = (x.s[h.qbit + (b >> 3)] >> (7 - (b & 7))) & 1;
+        const uint32_t j = (uint32_t)(i >> 3), k = (uint32_t)(i & 7);
+        const uint32_t by = s_byte[j];
+        const uint32_t before = s_pre[j] + (k ? (uint32_t)__builtin_popcount(by >> (8 - k)) : 0u);
+        const bool nc = (by >> (7 - k)) & 1;
         uint64_t v;
         if (nc) {
-            const uint64_t ord = ord0 + s_nc[i];
+            const uint64_t ord = ord0 + before;
             v = ord < K ? x.rec[ord] - off : 0xFFFF;
         } else {
-            v = i - s_nc[i];
+            v = i - before;
         }
-        x.bidx[b] = (uint16_t)(v > 0xFFFF ? 0xFFFF : v);
+        x.bidx[b0 + i] = (uint16_t)(v > 0xFFFF ? 0xFFFF : v);
     }
 }
 
@@ -591,12 +629,10 @@ IxLayout ix_layout(uint64_t n, int w, int bs, uint64_t len)
     L.ntiles = bs_tiled(bs) ? index_tiles(n, w) : (nb ? nb : 1);
     auto al = [](uint64_t v) { return (v + 255) & ~255ull; };
     uint64_t off = 0;
-    L.o_exits = off;  off += al(2 * L.nch);
-    L.o_conv = off;   off += al(L.nch);
-    L.o_entry = off;  off += al(2 * L.nch);
+    L.o_spec = off;   off += al(4 * L.nch);
+    L.o_tex = off;    off += al(4 * L.nch);
     L.o_cnt = off;    off += al(4 * L.nch);
     L.o_cbase = off;  off += al(8 * L.nch);
-    L.o_tables = off; off += al(2 * L.nch * L.rmax);
     L.o_pos = off;    off += al(2 * L.nch * L.maxr);
     L.o_rec = off;    off += al(8 * (nb ? nb : 1));
     L.o_tcnt = off;   off += al(4 * L.ntiles);
@@ -627,18 +663,17 @@ static IxParams ix_params(const uint8_t *s, uint64_t len, uint64_t n, int w, int
     x.tile_blocks = bs_tiled(bs) ? tile_values(w) / (uint64_t)bs : 1;
     x.h = h;
     x.status = status;
-    x.exits = reinterpret_cast<uint16_t *>(ws + L.o_exits);
-    x.conv = ws + L.o_conv;
-    x.entry = reinterpret_cast<uint16_t *>(ws + L.o_entry);
+    x.spec = reinterpret_cast<uint32_t *>(ws + L.o_spec);
+    x.tex = reinterpret_cast<uint32_t *>(ws + L.o_tex);
     x.cnt = reinterpret_cast<uint32_t *>(ws + L.o_cnt);
     x.cbase = reinterpret_cast<uint64_t *>(ws + L.o_cbase);
-    x.tables = reinterpret_cast<uint16_t *>(ws + L.o_tables);
     x.pos = reinterpret_cast<uint16_t *>(ws + L.o_pos);
     x.rec = reinterpret_cast<uint64_t *>(ws + L.o_rec);
     x.tcnt = reinterpret_cast<uint32_t *>(ws + L.o_tcnt);
     x.tbase = reinterpret_cast<uint64_t *>(ws + L.o_tbase);
     x.found = reinterpret_cast<uint64_t *>(ws + L.o_misc);
     x.pend = reinterpret_cast<uint64_t *>(ws + L.o_misc + 8);
+    x.nfail = reinterpret_cast<uint32_t *>(ws + L.o_misc + 24);
     x.index = index;
     x.bidx = bidx;
     return x;
@@ -665,11 +700,16 @@ void index_build_launch(const uint8_t *s, uint64_t len, uint64_t n, int w, int b
 {
     const IxLayout L = ix_layout(n, w, bs, len);
     IxParams x = ix_params(s, len, n, w, bs, h, index, bidx, ws, status, L);
+    // SZX_AMD_INDEX_NO_SPECULATION=1 (for tests): no speculative chains, so
+    // every chunk after the first goes through the in-order repair
+    const char *ns = getenv("SZX_AMD_INDEX_NO_SPECULATION");
+    x.no_spec = ns && *ns && *ns != '0';
     uint64_t *bsum = reinterpret_cast<uint64_t *>(ws + L.o_bsum);
     const unsigned nch = (unsigned)L.nch;
-    hipLaunchKernelGGL(k_ix_map, dim3(nch), dim3(256), 0, st, x);
-    hipLaunchKernelGGL(k_ix_entry, dim3((nch + 255) / 256), dim3(256), 0, st, x);
-    hipLaunchKernelGGL(k_ix_walk, dim3((nch + 3) / 4), dim3(256), 0, st, x);
+    hipLaunchKernelGGL(k_ix_zero, dim3(1), dim3(1), 0, st, x.nfail);
+    hipLaunchKernelGGL(k_ix_spec, dim3((nch + 3) / 4), dim3(256), 0, st, x);
+    hipLaunchKernelGGL(k_ix_verify, dim3((nch + 3) / 4), dim3(256), 0, st, x);
+    hipLaunchKernelGGL(k_ix_repair, dim3(1), dim3(64), 0, st, x);
     scan_u32(x.cnt, L.nch, x.cbase, bsum, x.found, st);
     hipLaunchKernelGGL(k_ix_scatter, dim3(nch), dim3(256), 0, st, x);
     index_tiles_launch(x, L, ws, bs_tiled(bs), true, st);

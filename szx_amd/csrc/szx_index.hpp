@@ -15,16 +15,16 @@ struct IxParams {
     const uint8_t *s;
     uint64_t len, n;
     int bs, w, nl;
+    int no_spec;        // speculative chains off (every chunk repaired in order)
     uint32_t rmax, maxr;
     uint64_t nch, ntiles, tile_blocks;
     const DecodeHeader *h;
     int32_t *status;
-    uint16_t *exits;    // per chunk: exit offset when converged
-    uint8_t *conv;      // per chunk: all valid entries exit at one offset
-    uint16_t *entry;    // per chunk: offset of its first record
+    uint32_t *spec;     // per chunk: speculative chain start | its exit << 16
+    uint32_t *tex;      // per chunk: exit from its entry (bit 31: differs from the speculative one)
+    uint32_t *nfail;    // chunks whose exit differs
     uint32_t *cnt;      // per chunk: records found
     uint64_t *cbase;    // per chunk: records before it
-    uint16_t *tables;   // per unconverged chunk: exit for every entry
     uint16_t *pos;      // per chunk: record starts
     uint64_t *rec;      // per record: payload offset
     uint32_t *tcnt;     // per tile: non-constant blocks
@@ -38,8 +38,7 @@ struct IxParams {
 struct IxLayout {
     uint64_t nch, ntiles, nl;
     uint32_t rmax, maxr;
-    uint64_t o_exits, o_conv, o_entry, o_cnt, o_cbase, o_tables, o_pos, o_rec, o_tcnt, o_tbase,
-        o_misc, o_osz, o_bsum, bytes;
+    uint64_t o_spec, o_tex, o_cnt, o_cbase, o_pos, o_rec, o_tcnt, o_tbase, o_misc, o_osz, o_bsum, bytes;
 };
 
 IxLayout ix_layout(uint64_t n, int w, int bs, uint64_t len);

@@ -222,6 +222,30 @@ def test_index_free_decode(bs, device):
             assert bits_equal(dec.out.cpu().numpy(), want)
 
 
+@pytest.mark.parametrize("bs", [8, 128, 256])
+def test_index_free_decode_in_order_repair(bs, device, monkeypatch):
+    """The index builder's in-order repair (normally only for chunks whose
+    speculative chain never met the true one) on every chunk: speculation
+    switched off, each chunk's entry comes from the repair walking the chunk
+    before it.  Same bytes as the oracle's decoder."""
+    import torch
+    monkeypatch.setenv("SZX_AMD_INDEX_NO_SPECULATION", "1")
+    r = np.random.default_rng(2000 + bs)
+    for dtype in (np.float32, np.float64):
+        n = 300007
+        x = _field(r, n, dtype, int(r.integers(0, 5)))
+        x[n // 3: n // 2] = 0.25
+        eb = float(np.float32((float(np.ptp(x)) + 1e-30) * 1e-5))
+        s = O.compress(x, eb, bs)
+        say("in-order repair", bs, dtype.__name__, n, len(s))
+        want = O.decompress(s, n, dtype, ref_quirk=True)
+        st = torch.from_numpy(np.frombuffer(s, np.uint8).copy()).to(device)
+        dec = Z.DeviceDecompressor(n, dtype, bs, len(s))
+        dec(st, len(s), Z.DECODE_REFERENCE)
+        assert dec.check() == 0
+        assert bits_equal(dec.out.cpu().numpy(), want)
+
+
 def test_index_free_rejects_damage(device):
     import torch
     x = np.sin(np.arange(300000) * 0.001).astype(np.float32)

@@ -439,3 +439,35 @@ def test_mlp_oracle_bf16_rounding():
     # bf16 keeps 8 significant bits: 1 + 2^-8 is a tie and rounds to even (1.0)
     assert b[0] == 1.0 and b[1] == 1.0 and b[2] == 1.0078125
     assert np.all(np.abs(b - a) <= np.abs(a) * 2.0 ** -8)
+
+
+def test_mlp_oracle_forward_matches_torch():
+    """The forward pass against torch's own layers: F.linear for each layer
+    and F.gelu(approximate="tanh"), in float64 with rounding off, and with
+    rounding on, torch's bfloat16 cast at the same points.  A wrong GELU
+    constant, a transposed weight or a swapped layer order fails here."""
+    import torch
+    import torch.nn.functional as F
+
+    import mlp_oracle as M
+    r = np.random.default_rng(11)
+    P = M.init_params(3).astype(np.float64)
+    P += 0.05 * r.standard_normal(P.size)
+    X = r.standard_normal((64, M.N_IN)) * 2.0
+    p = {k: torch.from_numpy(v.copy()) for k, v in M.unpack(P).items()}
+    xt = torch.from_numpy(X)
+
+    def torch_forward(rnd):
+        q = (lambda t: t.to(torch.bfloat16).to(torch.float64)) if rnd else (lambda t: t)
+        g1 = F.gelu(F.linear(q(xt), q(p["W1"]), p["b1"]), approximate="tanh")
+        g2 = F.gelu(F.linear(q(g1), q(p["W2"]), p["b2"]), approximate="tanh")
+        return F.linear(g2, p["w3"].view(1, -1), p["b3"]).view(-1).numpy()
+
+    y, _ = M.forward(P, X, rounding=False)
+    np.testing.assert_allclose(y, torch_forward(False), rtol=1e-12, atol=1e-12)
+    y, _ = M.forward(P, X, rounding=True)
+    np.testing.assert_allclose(y, torch_forward(True), rtol=1e-9, atol=1e-9)
+    # bf16 rounding itself: torch's float32 -> bfloat16 cast (round to nearest even)
+    a = (r.standard_normal(4096) * 10.0 ** r.uniform(-20, 20, 4096)).astype(np.float32)
+    tb = torch.from_numpy(a).to(torch.bfloat16).to(torch.float64).numpy()
+    assert np.array_equal(M.bf16(a), tb)
