@@ -95,22 +95,23 @@ __device__ inline uint32_t ix_stage_wave(uint32_t *bw, const uint8_t *src, uint3
 // position after the record at p (wave-uniform, < IX_C), or IX_INV where no
 // record can start (the reqLength byte out of range, the record's lead codes
 // past the bytes present, or lead codes subtracting more than the block
-// holds).  byte(i) reads chunk byte i; lanes 0..31 sum the packed lead counts
-// of two lead-code bytes each (NL <= 64) with a DPP reduction.
+// holds).  byte(i) reads chunk byte i; lane l < NL takes lead-code byte l, the
+// packed counts are summed inside each 16-lane row (DPP) and the row sums
+// read out as scalars.
 template <typename Byte>
-__device__ inline uint32_t chain_step(const Byte &byte, const uint32_t *lut, uint32_t p, uint32_t avail,
-                                      uint32_t nl, uint32_t bsz, uint32_t wmax, int lane)
+__device__ inline uint32_t chain_step(const Byte &byte, uint32_t p, uint32_t avail, uint32_t nl, uint32_t bsz,
+                                      uint32_t wmax, int lane)
 {
     if (p + 5 + nl > avail) return IX_INV;
     const uint32_t rl = byte(p);
-    uint32_t c = 0;
-    if (lane < 32) {
-        const uint32_t i0 = 2 * (uint32_t)lane;
-        if (i0 < nl) c = lut[byte(p + 5 + i0)];
-        if (i0 + 1 < nl) c += lut[byte(p + 6 + i0)];
-    }
-    c = seg_reduce<32>(c, [](uint32_t u, uint32_t v) { return u + v; });
-    const uint32_t d = __builtin_amdgcn_readfirstlane(c);
+    uint32_t c = (uint32_t)lane < nl ? lead_counts(byte(p + 5 + (uint32_t)lane)) : 0u;
+    c += partner<1>(c);
+    c += partner<2>(c);
+    c += partner<4>(c);
+    c += partner<8>(c);
+    uint32_t d = __builtin_amdgcn_readlane(c, 0);
+    if (nl > 16) d += __builtin_amdgcn_readlane(c, 16);
+    if (nl > 32) d += __builtin_amdgcn_readlane(c, 32) + __builtin_amdgcn_readlane(c, 48);
     const uint32_t rlu = __builtin_amdgcn_readfirstlane(rl);
     if (rlu == 0 || rlu > wmax) return IX_INV;
     const uint32_t rb = (rlu + 7) >> 3;
@@ -146,7 +147,7 @@ __device__ inline IxChunk ix_chunk(const IxParams &x, uint64_t c)
 // Returns the positions kept; *last is the position the walk ended at (>=
 // IX_C past the chunk, IX_INV after an invalid record, or the stop position).
 template <typename Byte, typename Stop>
-__device__ inline uint32_t chain_walk(const Byte &byte, const uint32_t *lut, uint32_t entry, const IxChunk &k,
+__device__ inline uint32_t chain_walk(const Byte &byte, uint32_t entry, const IxChunk &k,
                                       const IxParams &x, uint16_t *keep, uint32_t cap, const Stop &stop,
                                       uint32_t *last, int lane)
 {
@@ -156,16 +157,10 @@ __device__ inline uint32_t chain_walk(const Byte &byte, const uint32_t *lut, uin
         if (stop(p)) break;
         if (lane == 0 && n < cap) keep[n] = (uint16_t)p;
         n++;
-        p = chain_step(byte, lut, p, k.avail, nl, bsz, wmax, lane);
+        p = chain_step(byte, p, k.avail, nl, bsz, wmax, lane);
     }
     *last = p;
     return n < cap ? n : cap;
-}
-
-// Lead-count table (lead_counts of every byte), one entry per thread.
-__device__ inline void lead_lut_fill(uint32_t *lut)
-{
-    for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) lut[i] = lead_counts(i);
 }
 
 // A: speculative chains.  One wave per chunk walks from the lowest candidate
@@ -179,9 +174,8 @@ __device__ inline void lead_lut_fill(uint32_t *lut)
 __global__ __launch_bounds__(256) void k_ix_spec(IxParams x)
 {
     __shared__ uint32_t bw[4][IXW_WORDS];
-    __shared__ uint32_t lut[256];
+    __shared__ uint32_t pfx[4][128];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    lead_lut_fill(lut);
     if (*x.status != 0) return;   // workgroup-uniform: written by earlier kernels only
     const uint64_t c = blockIdx.x * 4ull + wv;
     const bool live = c < x.nch;
@@ -192,19 +186,48 @@ __global__ __launch_bounds__(256) void k_ix_spec(IxParams x)
     if (!live) return;
     const uint8_t *b = reinterpret_cast<const uint8_t *>(bw[wv]) + base;
     const auto byte = [b](uint32_t i) -> uint32_t { return b[i]; };
-    const uint32_t nl = (uint32_t)x.nl, wmax = 8u * (uint32_t)x.w, rmax = x.rmax;
+    const uint32_t nl = (uint32_t)x.nl, bsz = (uint32_t)x.bs, wmax = 8u * (uint32_t)x.w, rmax = x.rmax;
     uint16_t *pos = x.pos + c * x.maxr;
     uint32_t start = IX_INV, ex = IX_INV, n = 0;
     const auto never = [](uint32_t) { return false; };
     for (uint32_t q0 = 0; k.has && !k.ends && !x.no_spec && q0 < rmax && start == IX_INV; q0 += 64) {
+        // Candidates q0 + lane, kept only if their record and the next
+        // record start look valid (or the record leaves the chunk): the
+        // records' lead-count window sums come from an inclusive prefix over
+        // the 64 + NL bytes from q0 + 5.  Bytes past `avail` may be stale;
+        // only candidates whose windows lie before it use them.
         const uint32_t q = q0 + (uint32_t)lane;
-        const uint32_t rl = q < IX_C + IX_PAD ? b[q] : 0u;
-        unsigned long long m = __ballot(q < rmax && q + 5 + nl <= k.avail && rl != 0 && rl <= wmax);
+        const uint32_t ca = lead_counts(b[q0 + 5 + lane]);
+        const uint32_t cb = (uint32_t)lane < nl ? lead_counts(b[q0 + 69 + lane]) : 0u;
+        const uint32_t sa = seg_scan_incl<64>(ca, lane), sb = seg_scan_incl<64>(cb, lane);
+        pfx[wv][lane] = sa;
+        pfx[wv][64 + lane] = __builtin_amdgcn_readlane(sa, 63) + sb;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const uint32_t rl = b[q];
+        bool ok = q < rmax && q + 5 + nl <= k.avail && rl != 0 && rl <= wmax;
+        if (ok) {
+            const uint32_t d = pfx[wv][lane + nl - 1] - (lane ? pfx[wv][lane - 1] : 0u);
+            const uint32_t rb = (rl + 7) >> 3;
+            const uint32_t sub = rb >= 3 ? d >> 21 : rb == 2 ? (d >> 10) & 0x7FFu : d & 0x3FFu;
+            const uint32_t nx = q + 5 + nl + rb * bsz - sub;
+            ok = sub <= rb * bsz;
+            if (ok && nx < IX_C) {
+                ok = nx + 5 + nl <= k.avail;
+                if (ok) {
+                    const uint32_t r2 = b[nx];
+                    ok = r2 != 0 && r2 <= wmax;
+                }
+            }
+        }
+        unsigned long long m = __ballot(ok);
+        __builtin_amdgcn_wave_barrier();   // pfx is rewritten by the next batch
         while (m) {
             const uint32_t e = q0 + (uint32_t)__builtin_ctzll(m);
             m &= m - 1;
             uint32_t last;
-            n = chain_walk(byte, lut, e, k, x, pos, x.maxr, never, &last, lane);
+            n = chain_walk(byte, e, k, x, pos, x.maxr, never, &last, lane);
             if (last != IX_INV) {
                 start = e;
                 ex = last - IX_C;
@@ -244,11 +267,9 @@ __device__ inline int spec_rank(const uint16_t *lst, uint32_t n, uint32_t p, int
 // redoes them).
 __global__ __launch_bounds__(256) void k_ix_verify(IxParams x)
 {
-    __shared__ uint32_t lut[256];
     __shared__ uint32_t bits[4][IX_C / 32];
     __shared__ uint16_t lst[4][IX_MAXR], pre[4][IX_MAXR];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    lead_lut_fill(lut);
     for (uint32_t i = lane; i < IX_C / 32; i += 64) bits[wv][i] = 0;
     __syncthreads();
     if (*x.status != 0) return;
@@ -273,7 +294,7 @@ __global__ __launch_bounds__(256) void k_ix_verify(IxParams x)
     const uint32_t *bw = bits[wv];
     const auto on_spec = [bw](uint32_t p) { return ((bw[p >> 5] >> (p & 31)) & 1u) != 0; };
     uint32_t last = IX_INV, m = 0;
-    if (k.has && entry != IX_INV) m = chain_walk(byte, lut, entry, k, x, pre[wv], IX_MAXR, on_spec, &last, lane);
+    if (k.has && entry != IX_INV) m = chain_walk(byte, entry, k, x, pre[wv], IX_MAXR, on_spec, &last, lane);
     // the walked starts, then the speculative ones from the meeting point
     uint32_t from = n, ex;
     const int r = last < IX_C ? spec_rank(lst[wv], n, last, lane) : -1;
@@ -301,11 +322,8 @@ __global__ __launch_bounds__(256) void k_ix_verify(IxParams x)
 // whose exit differed, and so on.
 __global__ __launch_bounds__(64) void k_ix_repair(IxParams x)
 {
-    __shared__ uint32_t lut[256];
     __shared__ uint16_t keep[IX_MAXR];
     const int lane = threadIdx.x;
-    lead_lut_fill(lut);
-    __syncthreads();
     if (*x.status != 0 || *x.nfail == 0) return;
     const auto never = [](uint32_t) { return false; };
     uint64_t c = 0;
@@ -329,7 +347,7 @@ __global__ __launch_bounds__(64) void k_ix_repair(IxParams x)
             const uint8_t *g = x.s + x.h->q0 + k.g0;
             const auto byte = [g](uint32_t i) -> uint32_t { return g[i]; };
             uint32_t last = IX_INV, n = 0;
-            if (k.has && t != IX_INV) n = chain_walk(byte, lut, t, k, x, keep, IX_MAXR, never, &last, lane);
+            if (k.has && t != IX_INV) n = chain_walk(byte, t, k, x, keep, IX_MAXR, never, &last, lane);
             const uint32_t cap = n < x.maxr ? n : x.maxr;
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
@@ -406,15 +424,16 @@ static void scan_u32(const uint32_t *in, uint64_t n, uint64_t *out, uint64_t *bs
     hipLaunchKernelGGL(k_scan_add, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, out, n, bsum);
 }
 
-// E: record starts (payload offsets) in record order.
-__global__ void k_ix_scatter(IxParams x)
+// E: record starts (payload offsets) in record order.  One wave per chunk.
+__global__ __launch_bounds__(256) void k_ix_scatter(IxParams x)
 {
-    const uint64_t c = blockIdx.x;
-    if (*x.status != 0) return;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint64_t c = blockIdx.x * 4ull + wv;
+    if (c >= x.nch || *x.status != 0) return;
     const uint64_t base = x.cbase[c];
     const uint32_t k = x.cnt[c];
     const uint64_t K = x.h->nb - x.h->nconst;
-    for (uint32_t i = threadIdx.x; i < k; i += blockDim.x)
+    for (uint32_t i = lane; i < k; i += 64)
         if (base + i < K) x.rec[base + i] = c * IX_C + x.pos[c * x.maxr + i];
 }
 
@@ -510,14 +529,15 @@ __global__ void k_ix_tail(IxParams x)
     }
 }
 
-// G: tile pairs and per-block entries.  One workgroup per tile; the
-// non-constant blocks before each block of the tile come from a wave scan of
-// the popcounts of the tile's bitmap bytes (at most 128: 1024 blocks).
+// G: tile pairs and per-block entries.  One wave per tile; the non-constant
+// blocks before each block of the tile come from a wave scan of the
+// popcounts of the tile's bitmap bytes (at most 128: 1024 blocks).
 __global__ __launch_bounds__(256) void k_ix_tiles(IxParams x)
 {
-    __shared__ uint32_t s_byte[128], s_pre[128];
-    const uint64_t t = blockIdx.x;
-    if (*x.status != 0) return;
+    __shared__ uint32_t s_byte[4][128], s_pre[4][128];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint64_t t = blockIdx.x * 4ull + wv;
+    if (t >= x.ntiles || *x.status != 0) return;
     const DecodeHeader &h = *x.h;
     const uint64_t K = h.nb - h.nconst;
     const uint64_t b0 = t * x.tile_blocks;
@@ -525,28 +545,21 @@ __global__ __launch_bounds__(256) void k_ix_tiles(IxParams x)
     const uint64_t b1 = b0 + nbt;
     const uint32_t nbytes = (uint32_t)((nbt + 7) / 8);
     const uint64_t ord0 = x.tbase[t];
-    if (threadIdx.x < 64) {
+    {
         // two bytes per lane, inclusive wave scan of their popcounts
-        const uint32_t j0 = 2 * threadIdx.x, j1 = j0 + 1;
+        const uint32_t j0 = 2 * (uint32_t)lane, j1 = j0 + 1;
         const uint32_t v0 = j0 < nbytes ? bitmap_byte(x, h, b0 / 8 + j0, b1) : 0u;
         const uint32_t v1 = j1 < nbytes ? bitmap_byte(x, h, b0 / 8 + j1, b1) : 0u;
         const uint32_t c0 = __builtin_popcount(v0), c1 = __builtin_popcount(v1);
-        uint32_t inc = c0 + c1;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const uint32_t u = __shfl_up(inc, d, 64);
-            if ((int)threadIdx.x >= d) inc += u;
-        }
+        const uint32_t inc = seg_scan_incl<64>(c0 + c1, lane);
         const uint32_t ex = inc - c0 - c1;
-        if (j0 < 128) {
-            s_byte[j0] = v0;
-            s_byte[j1] = v1;
-            s_pre[j0] = ex;
-            s_pre[j1] = ex + c0;
-        }
+        s_byte[wv][j0] = v0;
+        s_byte[wv][j1] = v1;
+        s_pre[wv][j0] = ex;
+        s_pre[wv][j1] = ex + c0;
     }
-    if (threadIdx.x == 0) {
-        const uint64_t off = ord0 < K ? x.rec[ord0] : *x.pend;
+    const uint64_t off = ord0 < K ? x.rec[ord0] : *x.pend;
+    if (lane == 0) {
         x.index[2 * t] = off;
         x.index[2 * t + 1] = b0 - ord0;
         if (t == x.ntiles - 1) {
@@ -554,12 +567,13 @@ __global__ __launch_bounds__(256) void k_ix_tiles(IxParams x)
             x.index[2 * x.ntiles + 1] = h.nconst;
         }
     }
-    __syncthreads();
-    const uint64_t off = ord0 < K ? x.rec[ord0] : *x.pend;
-    for (uint64_t i = threadIdx.x; i < nbt; i += blockDim.x) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (uint64_t i = lane; i < nbt; i += 64) {
         const uint32_t j = (uint32_t)(i >> 3), k = (uint32_t)(i & 7);
-        const uint32_t by = s_byte[j];
-        const uint32_t before = s_pre[j] + (k ? (uint32_t)__builtin_popcount(by >> (8 - k)) : 0u);
+        const uint32_t by = s_byte[wv][j];
+        const uint32_t before = s_pre[wv][j] + (k ? (uint32_t)__builtin_popcount(by >> (8 - k)) : 0u);
         const bool nc = (by >> (7 - k)) & 1;
         uint64_t v;
         if (nc) {
@@ -690,7 +704,7 @@ static void index_tiles_launch(IxParams &x, const IxLayout &L, uint8_t *ws, bool
     scan_u32(x.tcnt, L.ntiles, x.tbase, bsum, scratch, st);
     if (chain_tail) hipLaunchKernelGGL(k_ix_tail, dim3(1), dim3(64), 0, st, x);
     if (tiled)
-        hipLaunchKernelGGL(k_ix_tiles, dim3((unsigned)L.ntiles), dim3(256), 0, st, x);
+        hipLaunchKernelGGL(k_ix_tiles, dim3((unsigned)((L.ntiles + 3) / 4)), dim3(256), 0, st, x);
     else
         hipLaunchKernelGGL(k_ix_pairs, dim3((unsigned)((L.ntiles + 256) / 256)), dim3(256), 0, st, x);
 }
@@ -711,7 +725,7 @@ void index_build_launch(const uint8_t *s, uint64_t len, uint64_t n, int w, int b
     hipLaunchKernelGGL(k_ix_verify, dim3((nch + 3) / 4), dim3(256), 0, st, x);
     hipLaunchKernelGGL(k_ix_repair, dim3(1), dim3(64), 0, st, x);
     scan_u32(x.cnt, L.nch, x.cbase, bsum, x.found, st);
-    hipLaunchKernelGGL(k_ix_scatter, dim3(nch), dim3(256), 0, st, x);
+    hipLaunchKernelGGL(k_ix_scatter, dim3((nch + 3) / 4), dim3(256), 0, st, x);
     index_tiles_launch(x, L, ws, bs_tiled(bs), true, st);
 }
 
