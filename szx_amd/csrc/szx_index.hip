@@ -50,8 +50,6 @@ constexpr uint32_t IX_C = IX_CHUNK;        // payload bytes per chunk
 constexpr uint16_t IX_INV = 0xFFFF;        // invalid chain / position
 constexpr int IX_PAD = 96;                 // staged bytes past the chunk (5 + NL <= 69)
 
-__global__ void k_ix_zero(uint32_t *w) { *w = 0; }
-
 // Per lead-code byte c: the sum over its four 2-bit codes of min(code, 1),
 // of min(code, 2) and of the code itself -- what a record with rb = 1, 2 or
 // >= 3 subtracts from 4 rb per lead byte -- packed at bits 0, 10 and 21 of a
@@ -176,6 +174,7 @@ __global__ __launch_bounds__(256) void k_ix_spec(IxParams x)
     __shared__ uint32_t bw[4][IXW_WORDS];
     __shared__ uint32_t pfx[4][128];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *x.nfail = 0;   // counted by B
     if (*x.status != 0) return;   // workgroup-uniform: written by earlier kernels only
     const uint64_t c = blockIdx.x * 4ull + wv;
     const bool live = c < x.nch;
@@ -430,7 +429,7 @@ __global__ __launch_bounds__(256) void k_ix_scatter(IxParams x)
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint64_t c = blockIdx.x * 4ull + wv;
     if (c >= x.nch || *x.status != 0) return;
-    const uint64_t base = x.cbase[c];
+    const uint64_t base = x.cbase[c] + x.csum[c >> 10];
     const uint32_t k = x.cnt[c];
     const uint64_t K = x.h->nb - x.h->nconst;
     for (uint32_t i = lane; i < k; i += 64)
@@ -447,12 +446,9 @@ __device__ inline uint32_t bitmap_byte(const IxParams &x, const DecodeHeader &h,
 }
 
 // Non-constant blocks per index tile (popcount of its bitmap bytes: a tile
-// holds a multiple of 8 blocks).
-__global__ void k_ix_tile_counts(IxParams x)
+// holds a multiple of 8 blocks, or is one block).
+__device__ inline uint32_t tile_count(const IxParams &x, const DecodeHeader &h, uint64_t t)
 {
-    const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    if (t >= x.ntiles || *x.status != 0) return;
-    const DecodeHeader &h = *x.h;
     const uint64_t b0 = t * x.tile_blocks, b1 = b0 + x.tile_blocks < h.nb ? b0 + x.tile_blocks : h.nb;
     uint32_t c = 0;
     if (x.tile_blocks % 8 == 0) {
@@ -460,8 +456,24 @@ __global__ void k_ix_tile_counts(IxParams x)
     } else {
         for (uint64_t b = b0; b < b1; b++) c += (x.s[h.qbit + (b >> 3)] >> (7 - (b & 7))) & 1;
     }
-    x.tcnt[t] = c;
+    return c;
 }
+
+// Non-constant blocks before each tile, in two steps like the scans above:
+// per 1024 tiles the exclusive scan of their counts (tbase) and the block
+// sums (tsum, then scanned by k_scan_tops); readers add tsum[t >> 10].
+__global__ __launch_bounds__(1024) void k_ix_tile_scan(IxParams x)
+{
+    __shared__ uint64_t s[1024];
+    const uint64_t t = blockIdx.x * 1024ull + threadIdx.x;
+    if (*x.status != 0) return;   // workgroup-uniform
+    const uint64_t v = t < x.ntiles ? tile_count(x, *x.h, t) : 0;
+    const uint64_t e = block_excl_scan_1024(v, s);
+    if (t < x.ntiles) x.tbase[t] = e;
+    if (threadIdx.x == 1023) x.tsum[blockIdx.x] = s[1023];
+}
+
+__device__ inline uint64_t tile_base(const IxParams &x, uint64_t t) { return x.tbase[t] + x.tsum[t >> 10]; }
 
 // F: the last record's true size and the payload end; checks the record
 // count.  One wave: the bitmap is searched from its end 64 bytes at a time,
@@ -544,7 +556,7 @@ __global__ __launch_bounds__(256) void k_ix_tiles(IxParams x)
     const uint64_t nbt = b0 + x.tile_blocks <= h.nb ? x.tile_blocks : h.nb - b0;
     const uint64_t b1 = b0 + nbt;
     const uint32_t nbytes = (uint32_t)((nbt + 7) / 8);
-    const uint64_t ord0 = x.tbase[t];
+    const uint64_t ord0 = tile_base(x, t);
     {
         // two bytes per lane, inclusive wave scan of their popcounts
         const uint32_t j0 = 2 * (uint32_t)lane, j1 = j0 + 1;
@@ -601,7 +613,7 @@ __global__ void k_ix_pairs(IxParams x)
         return;
     }
     const uint64_t K = h.nb - h.nconst;
-    const uint64_t ord = x.tbase[b];
+    const uint64_t ord = tile_base(x, b);
     x.index[2 * b] = ord < K ? x.rec[ord] : *x.pend;
     x.index[2 * b + 1] = b - ord;
 }
@@ -649,8 +661,9 @@ IxLayout ix_layout(uint64_t n, int w, int bs, uint64_t len)
     L.o_cbase = off;  off += al(8 * L.nch);
     L.o_pos = off;    off += al(2 * L.nch * L.maxr);
     L.o_rec = off;    off += al(8 * (nb ? nb : 1));
-    L.o_tcnt = off;   off += al(4 * L.ntiles);
     L.o_tbase = off;  off += al(8 * L.ntiles);
+    L.o_tsum = off;   off += al(8 * (L.ntiles / 1024 + 1));
+    L.o_csum = off;   off += al(8 * (L.nch / 1024 + 1));
     L.o_misc = off;   off += 256;   // found, pend, scratch total
     L.o_osz = off;    off += al(4 * (nb ? nb : 1));   // random-access container sizes
     uint64_t mx = L.nch > L.ntiles ? L.nch : L.ntiles;
@@ -683,8 +696,9 @@ static IxParams ix_params(const uint8_t *s, uint64_t len, uint64_t n, int w, int
     x.cbase = reinterpret_cast<uint64_t *>(ws + L.o_cbase);
     x.pos = reinterpret_cast<uint16_t *>(ws + L.o_pos);
     x.rec = reinterpret_cast<uint64_t *>(ws + L.o_rec);
-    x.tcnt = reinterpret_cast<uint32_t *>(ws + L.o_tcnt);
     x.tbase = reinterpret_cast<uint64_t *>(ws + L.o_tbase);
+    x.tsum = reinterpret_cast<uint64_t *>(ws + L.o_tsum);
+    x.csum = reinterpret_cast<uint64_t *>(ws + L.o_csum);
     x.found = reinterpret_cast<uint64_t *>(ws + L.o_misc);
     x.pend = reinterpret_cast<uint64_t *>(ws + L.o_misc + 8);
     x.nfail = reinterpret_cast<uint32_t *>(ws + L.o_misc + 24);
@@ -699,9 +713,9 @@ static void index_tiles_launch(IxParams &x, const IxLayout &L, uint8_t *ws, bool
                                hipStream_t st)
 {
     uint64_t *scratch = reinterpret_cast<uint64_t *>(ws + L.o_misc + 16);
-    uint64_t *bsum = reinterpret_cast<uint64_t *>(ws + L.o_bsum);
-    hipLaunchKernelGGL(k_ix_tile_counts, dim3((unsigned)((L.ntiles + 255) / 256)), dim3(256), 0, st, x);
-    scan_u32(x.tcnt, L.ntiles, x.tbase, bsum, scratch, st);
+    const uint64_t nblk = (L.ntiles + 1023) / 1024;
+    hipLaunchKernelGGL(k_ix_tile_scan, dim3((unsigned)nblk), dim3(1024), 0, st, x);
+    hipLaunchKernelGGL(k_scan_tops, dim3(1), dim3(1024), 0, st, x.tsum, nblk, scratch);
     if (chain_tail) hipLaunchKernelGGL(k_ix_tail, dim3(1), dim3(64), 0, st, x);
     if (tiled)
         hipLaunchKernelGGL(k_ix_tiles, dim3((unsigned)((L.ntiles + 3) / 4)), dim3(256), 0, st, x);
@@ -718,13 +732,14 @@ void index_build_launch(const uint8_t *s, uint64_t len, uint64_t n, int w, int b
     // every chunk after the first goes through the in-order repair
     const char *ns = getenv("SZX_AMD_INDEX_NO_SPECULATION");
     x.no_spec = ns && *ns && *ns != '0';
-    uint64_t *bsum = reinterpret_cast<uint64_t *>(ws + L.o_bsum);
     const unsigned nch = (unsigned)L.nch;
-    hipLaunchKernelGGL(k_ix_zero, dim3(1), dim3(1), 0, st, x.nfail);
     hipLaunchKernelGGL(k_ix_spec, dim3((nch + 3) / 4), dim3(256), 0, st, x);
     hipLaunchKernelGGL(k_ix_verify, dim3((nch + 3) / 4), dim3(256), 0, st, x);
     hipLaunchKernelGGL(k_ix_repair, dim3(1), dim3(64), 0, st, x);
-    scan_u32(x.cnt, L.nch, x.cbase, bsum, x.found, st);
+    // records before each chunk: cbase + csum[c >> 10] (read by the scatter)
+    const uint64_t nblk = (L.nch + 1023) / 1024;
+    hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nblk), dim3(1024), 0, st, x.cnt, L.nch, x.cbase, x.csum);
+    hipLaunchKernelGGL(k_scan_tops, dim3(1), dim3(1024), 0, st, x.csum, nblk, x.found);
     hipLaunchKernelGGL(k_ix_scatter, dim3((nch + 3) / 4), dim3(256), 0, st, x);
     index_tiles_launch(x, L, ws, bs_tiled(bs), true, st);
 }

@@ -24,11 +24,12 @@ struct IxParams {
     uint32_t *tex;      // per chunk: exit from its entry (bit 31: differs from the speculative one)
     uint32_t *nfail;    // chunks whose exit differs
     uint32_t *cnt;      // per chunk: records found
-    uint64_t *cbase;    // per chunk: records before it
+    uint64_t *cbase;    // per chunk: records before it in its group of 1024 chunks
+    uint64_t *csum;     // per 1024 chunks: records before them
     uint16_t *pos;      // per chunk: record starts
     uint64_t *rec;      // per record: payload offset
-    uint32_t *tcnt;     // per tile: non-constant blocks
-    uint64_t *tbase;    // per tile: non-constant blocks before it
+    uint64_t *tbase;    // per tile: non-constant blocks before it in its group of 1024 tiles
+    uint64_t *tsum;     // per 1024 tiles: non-constant blocks before them
     uint64_t *found;    // records found in total
     uint64_t *pend;     // payload end
     uint64_t *index;    // output: tile pairs
@@ -38,7 +39,7 @@ struct IxParams {
 struct IxLayout {
     uint64_t nch, ntiles, nl;
     uint32_t rmax, maxr;
-    uint64_t o_spec, o_tex, o_cnt, o_cbase, o_pos, o_rec, o_tcnt, o_tbase, o_misc, o_osz, o_bsum, bytes;
+    uint64_t o_spec, o_tex, o_cnt, o_cbase, o_pos, o_rec, o_tbase, o_tsum, o_csum, o_misc, o_osz, o_bsum, bytes;
 };
 
 IxLayout ix_layout(uint64_t n, int w, int bs, uint64_t len);
