@@ -546,10 +546,17 @@ int szx_amd_batch_tune(szx_amd_batch *batch, int reps, int *valuesPerLane, int *
 /* Kernel timing for benchmarks: after szx_amd_timing_begin(), this thread's
  * launches bracket their main kernels with HIP events on the launch stream;
  * szx_amd_timing_end() waits for them and returns, per kernel kind
- * (0 = compress, 1 = decode, 2 = record-index build), the summed milliseconds
- * and the number of launches.  Arrays hold 3 entries. */
+ * (0 = compress, 1 = decode, 2 = record-index build, 3 = szx_amd_copy_dev),
+ * the summed milliseconds and the number of launches.  Arrays hold 4
+ * entries. */
 void szx_amd_timing_begin(void);
 int szx_amd_timing_end(double *ms_sum, int *count);
+
+/* Device copy of `bytes` bytes (pointers and size multiples of 16) with
+ * 16-byte loads and non-temporal 16-byte stores, one pass: the copy rate
+ * that bounds the codec kernels on this GPU.  No counterpart in the
+ * reference: a measurement helper for benchmarks. */
+int szx_amd_copy_dev(const void *d_src, void *d_dst, size_t bytes, void *hipStream);
 
 #ifdef __cplusplus
 }

@@ -43,7 +43,7 @@ EXPORTS = (
     "szx_amd_decompress_dev", "szx_amd_compress_ra_dev", "szx_amd_decompress_ra_dev",
     "szx_amd_nb_capacity", "szx_amd_nb_workspace", "szx_amd_compress_nb_dev", "szx_amd_decompress_nb_dev",
     "szx_amd_index_words", "szx_amd_timing_begin",
-    "szx_amd_timing_end", "szx_amd_fxr_workspace", "szx_amd_fxr_bound_dev",
+    "szx_amd_timing_end", "szx_amd_copy_dev", "szx_amd_fxr_workspace", "szx_amd_fxr_bound_dev",
     "szx_amd_fixed_rate_workspace", "szx_amd_fixed_rate_dev", "szx_amd_rate_stats_bytes",
     "szx_amd_rate_stats_workspace", "szx_amd_rate_stats_dev", "szx_amd_rate_inputs_dev",
     "szx_amd_mlp_forward_dev", "szx_amd_mlp_grad_workspace", "szx_amd_mlp_grad_dev",
@@ -187,6 +187,8 @@ def lib() -> C.CDLL:
     L.szx_amd_timing_begin.argtypes = []
     L.szx_amd_timing_end.restype = i32
     L.szx_amd_timing_end.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_int)]
+    L.szx_amd_copy_dev.restype = i32
+    L.szx_amd_copy_dev.argtypes = [vp, vp, sz, vp]
     L.szx_amd_fxr_workspace.restype = sz
     L.szx_amd_fxr_workspace.argtypes = [sz, i32, i32]
     L.szx_amd_fxr_bound_dev.restype = i32
@@ -244,7 +246,7 @@ def lib() -> C.CDLL:
     return L
 
 
-KERNEL_KINDS = ("compress", "decode", "index")
+KERNEL_KINDS = ("compress", "decode", "index", "copy")
 
 
 def timing_begin() -> None:
@@ -254,11 +256,25 @@ def timing_begin() -> None:
 
 def timing_end() -> dict:
     """{kind: (summed ms, launches)} for the launches since timing_begin()."""
-    ms = (C.c_double * 3)()
-    cnt = (C.c_int * 3)()
+    ms = (C.c_double * len(KERNEL_KINDS))()
+    cnt = (C.c_int * len(KERNEL_KINDS))()
     if lib().szx_amd_timing_end(ms, cnt) != 0:
         raise SZxError(last_error())
     return {k: (ms[i], cnt[i]) for i, k in enumerate(KERNEL_KINDS)}
+
+
+def copy_dev(src, dst, stream=None) -> None:
+    """Copy device tensor src into dst (same byte size, a multiple of 16) with
+    the library's plain copy kernel: the copy rate that bounds the codec."""
+    torch = _torch()
+    if not isinstance(src, torch.Tensor) or not isinstance(dst, torch.Tensor):
+        raise SZxError("copy_dev takes two torch tensors")
+    nb = src.numel() * src.element_size()
+    if not src.is_contiguous() or not dst.is_contiguous() or dst.numel() * dst.element_size() != nb:
+        raise SZxError("copy_dev needs contiguous tensors of the same byte size")
+    if not src.is_cuda or not dst.is_cuda:
+        raise SZxError("copy_dev takes device tensors")
+    _check(lib().szx_amd_copy_dev(src.data_ptr(), dst.data_ptr(), nb, _stream_handle(stream)))
 
 
 def last_error() -> str:

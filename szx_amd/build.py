@@ -27,7 +27,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HOSTCXX = os.environ.get("CXX", "g++")
 ARCH = os.environ.get("SZX_AMD_ARCH", "gfx950")
 
-SOURCES = ["szx_compress.hip", "szx_decompress.hip", "szx_index.hip", "szx_generic.hip", "szx_nonblocked.hip", "szx_cuszx.hip", "szx_fxr.hip", "szx_mlp.hip", "szx_noalloc.hip",
+SOURCES = ["szx_compress.hip", "szx_decompress.hip", "szx_index.hip", "szx_generic.hip", "szx_nonblocked.hip", "szx_cuszx.hip", "szx_fxr.hip", "szx_mlp.hip", "szx_noalloc.hip", "szx_probe.hip",
            "szx_api.cpp", "szx_fxr_host.cpp", "szx_rate.cpp", "szx_batch.cpp"]
 HEADERS = ["szx_common.hpp", "szx_internal.hpp", "szx_wave.hpp", "szx_fxr.hpp",
            "szx_fxr_host.hpp", "szx_mlp.hpp", "szx_rate.hpp", "szx_index.hpp", "szx_batch.hpp"]

@@ -49,7 +49,7 @@ inline uint64_t index_words(uint64_t n, int w, int bs)
 // Optional kernel timing: when enabled on this thread, the launchers bracket
 // each main kernel (compress, decode, index build) with a pair of HIP events
 // on the launch stream.
-enum KernelKind { KK_COMPRESS = 0, KK_DECODE = 1, KK_INDEX = 2, KK_COUNT = 3 };
+enum KernelKind { KK_COMPRESS = 0, KK_DECODE = 1, KK_INDEX = 2, KK_COPY = 3, KK_COUNT = 4 };
 // Error text of the calling thread (szx_amd_last_error).
 void api_error(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
 void api_clear_error();
