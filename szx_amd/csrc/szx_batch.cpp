@@ -15,8 +15,8 @@
 //   k_compress_batch  one workgroup per tile of the global tile list
 //   k_assemble_batch  header, bitmap, constant medians (flattened)
 // and of a batched decompress:
-//   k_decode_header_batch, k_decode_batch (persistent grid over every field's
-//   index tiles, located through the result words).
+//   k_decode_header_batch, k_decode_batch (one workgroup per index tile of
+//   every field, the streams located through the result words).
 //
 // The tile shape of k_compress_batch (values per lane, index tiles per
 // workgroup tile) is chosen per batch by szx_amd_batch_tune, which times every

@@ -464,9 +464,12 @@ __global__ __launch_bounds__(256) void k_decode_tiles(DecodeParams p)
 }
 
 // ---------------------------------------------------------------------------
-// Batched decode: every field of a batched compress, one persistent launch.
-// The stream of a field is found on the device from the compressor's result
-// words, so a compress and its decode can be enqueued (or captured) together.
+// Batched decode: every field of a batched compress in one launch, one
+// workgroup per index tile of every field (the field found by a binary search
+// of the tile bases), so tiles of all fields overlap on the CUs exactly as a
+// single field's tiles do.  The stream of a field is found on the device from
+// the compressor's result words, so a compress and its decode can be enqueued
+// (or captured) together.
 // ---------------------------------------------------------------------------
 
 __global__ void k_decode_header_batch(const BatchDecodeField *f, int nf, int bs)
@@ -480,23 +483,24 @@ __global__ void k_decode_header_batch(const BatchDecodeField *f, int nf, int bs)
 
 template <int BS, int LV>
 __global__ __launch_bounds__(256) void k_decode_batch(const BatchDecodeField *f, const uint64_t *tbase, int nf,
-                                                      uint64_t total, int exact)
+                                                      int exact)
 {
     __shared__ union Lds {
         DecodeLds<float, BS, LV> f;
         DecodeLds<double, BS, LV> d;
     } lds;
-    int fi = 0;
-    for (uint64_t g = blockIdx.x; g < total; g += gridDim.x) {
-        while (fi + 1 < nf && tbase[fi + 1] <= g) fi++;
-        const BatchDecodeField &d = f[fi];
-        const DecodeParams p{d.out + d.result[0], d.result[1], d.n, d.h, d.index,
-                             reinterpret_cast<const uint16_t *>(d.index + 2 * (d.ntidx + 1)), d.status, d.dst,
-                             exact};
-        if (d.w == 4) decode_tile<float, BS, LV>(p, g - tbase[fi], lds.f);
-        else decode_tile<double, BS, LV>(p, g - tbase[fi], lds.d);
-        __syncthreads();
+    const uint64_t g = blockIdx.x;
+    int lo = 0, hi = nf - 1;   // the field holding tile g
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (tbase[mid] <= g) lo = mid;
+        else hi = mid - 1;
     }
+    const BatchDecodeField &d = f[lo];
+    const DecodeParams p{d.out + d.result[0], d.result[1], d.n, d.h, d.index,
+                         reinterpret_cast<const uint16_t *>(d.index + 2 * (d.ntidx + 1)), d.status, d.dst, exact};
+    if (d.w == 4) decode_tile<float, BS, LV>(p, g - tbase[lo], lds.f);
+    else decode_tile<double, BS, LV>(p, g - tbase[lo], lds.d);
 }
 
 template <int BS>
@@ -504,17 +508,8 @@ static void launch_decode_batch(const BatchDecodeField *f, const uint64_t *tbase
                                 int exact, hipStream_t st)
 {
     constexpr int LV = BS >= 8 ? 8 : 4;
-    auto kern = k_decode_batch<BS, LV>;
-    static int resident = 0;
-    if (!resident) {
-        int dev = 0, per_cu = 0, cus = 0;
-        (void)hipGetDevice(&dev);
-        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0);
-        resident = per_cu > 0 && cus > 0 ? per_cu * cus : 256;
-    }
-    const uint64_t grid = total < (uint64_t)resident ? total : (uint64_t)resident;
-    if (grid) hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(256), 0, st, f, tbase, nf, total, exact);
+    if (total) hipLaunchKernelGGL((k_decode_batch<BS, LV>), dim3((unsigned)total), dim3(256), 0, st, f, tbase, nf,
+                                  exact);
 }
 
 int batch_decode_launch(int bs, const BatchDecodeField *f, const uint64_t *tbase, int nf, uint64_t total,
