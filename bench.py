@@ -236,6 +236,44 @@ def measured_traffic(shape, rel):
     return None
 
 
+def rate_reachability(x, targets, tol, bs=128, depth=40):
+    """Which target rates any bound can reach, from the exact stream sizes
+    of every octave of bounds between range * 2^-depth and the range
+    (k_rate_octave: 512 bounds per octave, and the size each octave tends to
+    at its top edge).  Inside an octave the size falls in steps of single
+    blocks turning constant; at each octave edge every block's reqLength
+    drops one bit and the size jumps.  Per target: 'hit' when a size on the
+    ladder is within tolerance, 'gap' when the tolerance window lies wholly
+    inside a jump or outside every reachable size (no bound reaches it), and
+    'between' otherwise (the window falls between two ladder sizes of one
+    octave: reachable unless a single step crosses it)."""
+    import math
+
+    import numpy as np
+    import szx_amd as Z
+    n = x.numel()
+    rng = float(x.max()) - float(x.min())
+    top = math.floor(math.log2(rng)) if rng > 0 else 0
+    pts, spans = [], []
+    for e in range(top - depth, top + 2):
+        sizes, _, end = Z.rate_octave(x, e, bs, with_end=True)
+        bpv = 8.0 * sizes.astype(np.float64) / n
+        pts.append(bpv)
+        pts.append(np.array([8.0 * end / n]))
+        spans.append((8.0 * end / n, float(bpv[0])))
+    pts = np.concatenate(pts)
+    out = []
+    for t in targets:
+        lo, hi = t * (1 - tol), t * (1 + tol)
+        if np.any((pts >= lo) & (pts <= hi)):
+            out.append("hit")
+        elif any(a <= hi and b >= lo for a, b in spans):
+            out.append("between")
+        else:
+            out.append("gap")
+    return out
+
+
 def fixed_rate_eval(n_fields, dev, tol=0.02, targets=(1.0, 2.0, 4.0, 8.0)):
     """Fixed-rate solve on fp32 256^3 fields (32 waves, k in [1,16], slope
     U[1.5,4], noise sigma/std log-uniform [1e-6,1e-2], a 25% zero box; seeds
@@ -264,9 +302,11 @@ def fixed_rate_eval(n_fields, dev, tol=0.02, targets=(1.0, 2.0, 4.0, 8.0)):
     x = gen(0)
     fr(x, 2.0, tol, method=Z.RATE_LEARNED)   # warm-up
     fr(x, 2.0, tol, method=Z.RATE_BISECT)
+    reach = []
     for i in range(n_fields):
         x = gen(i)
         torch.cuda.synchronize()
+        reach += rate_reachability(x, targets, tol)
         for t in targets:
             for name, meth in (("learned", Z.RATE_LEARNED), ("bisection", Z.RATE_BISECT)):
                 t0 = time.perf_counter()
@@ -301,6 +341,16 @@ def fixed_rate_eval(n_fields, dev, tol=0.02, targets=(1.0, 2.0, 4.0, 8.0)):
                      "hit_rate": float(np.mean(d["hit"])),
                      "mean_rate_err": float(np.mean(d["err"])),
                      "max_passes": int(np.max(d["passes"]))}
+    feas = [r == "hit" for r in reach]
+    out["reachability"] = {
+        "targets": len(reach), "reachable_on_ladder": int(sum(feas)),
+        "between_ladder_sizes": int(sum(r == "between" for r in reach)),
+        "in_a_gap": int(sum(r == "gap" for r in reach)),
+        "note": "exact sizes of every octave of bounds (512 per octave): a target in a gap is "
+                "reached by no bound; 'between' targets fall between two ladder sizes of one octave"}
+    for name, d in res.items():
+        hits = [h for h, f in zip(d["hit"], feas) if f]
+        out[name]["hit_rate_on_reachable"] = float(np.mean(hits)) if hits else None
     both = [a == b for a, b in zip(res["learned"]["hit"], res["bisection"]["hit"])]
     out["learned_hits_where_bisection_hits"] = float(np.mean(
         [a for a, b in zip(res["learned"]["hit"], res["bisection"]["hit"]) if b] or [0]))

@@ -824,9 +824,12 @@ def rate_inputs(stats, n: int, ebs, stream=None):
     return X
 
 
-def rate_octave(x, reqExpo: int, blockSize: int = 128, stream=None):
+def rate_octave(x, reqExpo: int, blockSize: int = 128, stream=None, with_end: bool = False):
     """Exact stream sizes (bytes) for the OCTAVE_BINS bounds of one octave:
-    (numpy sizes, numpy float32 bounds)."""
+    (numpy sizes, numpy float32 bounds).  with_end adds the size the octave
+    tends to as the bound approaches 2^(reqExpo+1) from below (every block in
+    the octave's radius bins constant), the last size before the rate jumps
+    at the octave edge."""
     torch = _torch()
     n = x.numel()
     code = _torch_code(x)
@@ -844,6 +847,9 @@ def rate_octave(x, reqExpo: int, blockSize: int = 128, stream=None):
     sizes = 13 + (nb + 7) // 8 + a[0] + 4 * const_before + rec_from
     ebs = np.ldexp(np.float32(1) + np.arange(OCTAVE_BINS, dtype=np.float32) / np.float32(OCTAVE_BINS),
                    reqExpo).astype(np.float32)
+    if with_end:
+        end = int(13 + (nb + 7) // 8 + a[0] + 4 * int(cnt.sum()))
+        return sizes.astype(np.int64), ebs, end
     return sizes.astype(np.int64), ebs
 
 

@@ -305,12 +305,18 @@ __device__ inline uint64_t record_bytes(const T (&v)[FX_MAXC], int n, int lane, 
 //       record)
 //   [1 + 2k], [2 + 2k]: blocks with radius in bin k of (2^e, 2^(e+1)]:
 //       count, record bytes.
+// With EDGES, the two bounds just outside the octave as well (every block's
+// reqLength moves by one bit there, so the rate jumps):
+//   [OCT_ACC + 0], [OCT_ACC + 1]: at 2^(e+1) (exponent e+1): constant
+//       blocks, record bytes;
+//   [OCT_ACC + 2], [OCT_ACC + 3]: at the float below 2^e (exponent e-1):
+//       constant blocks, record bytes.
 // Each workgroup bins in LDS and adds its non-zero bins to acc.
-template <typename T>
+template <typename T, bool EDGES>
 __global__ __launch_bounds__(256) void k_rate_octave(const T *x, uint64_t n, int bs, int reqexp,
                                                      unsigned long long *acc)
 {
-    constexpr int WIDTH = 1 + 2 * OCT_BINS;
+    constexpr int WIDTH = OCT_ACC + (EDGES ? 4 : 0);
     __shared__ unsigned long long s_acc[WIDTH];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int k = threadIdx.x; k < WIDTH; k += 256) s_acc[k] = 0;
@@ -318,7 +324,9 @@ __global__ __launch_bounds__(256) void k_rate_octave(const T *x, uint64_t n, int
     const uint64_t nb = (n + bs - 1) / bs;
     // 2^reqexp and 2^(reqexp+1) as doubles (exact for every float exponent)
     const double lo_edge = ldexp(1.0, reqexp), hi_edge = 2.0 * lo_edge;
-    unsigned long long fixed = 0;
+    // the float bound just below 2^e (bounds are floats for both dtypes)
+    const double below = (double)nextafterf(ldexpf(1.0f, reqexp), 0.0f);
+    unsigned long long fixed = 0, a_const = 0, a_rec = 0, b_const = 0, b_rec = 0;
     const uint64_t step = gridDim.x * 4ull;
     for (uint64_t i0 = blockIdx.x * 4ull + wv; i0 < nb; i0 += PF * step) {
         T vp[PF][FX_MAXC];
@@ -346,9 +354,19 @@ __global__ __launch_bounds__(256) void k_rate_octave(const T *x, uint64_t n, int
             const double rd = (double)r;
             if (rd <= lo_edge) {
                 fixed += 4;
+                if (EDGES) {
+                    a_const++;
+                    if (rd <= below) b_const++;
+                    else b_rec += record_bytes<T>(v, cnt, lane, (float)r, (float)m, reqexp - 1);
+                }
                 continue;
             }
             const uint64_t rec = record_bytes<T>(v, cnt, lane, (float)r, (float)m, reqexp);
+            if (EDGES) {
+                b_rec += record_bytes<T>(v, cnt, lane, (float)r, (float)m, reqexp - 1);
+                if (rd <= hi_edge) a_const++;
+                else a_rec += record_bytes<T>(v, cnt, lane, (float)r, (float)m, reqexp + 1);
+            }
             if (!(rd <= hi_edge)) {   // also NaN radii: never constant
                 fixed += rec;
                 continue;
@@ -363,6 +381,12 @@ __global__ __launch_bounds__(256) void k_rate_octave(const T *x, uint64_t n, int
         }
     }
     if (lane == 0 && fixed) atomicAdd(&s_acc[0], fixed);
+    if (EDGES && lane == 0) {
+        if (a_const) atomicAdd(&s_acc[OCT_ACC + 0], a_const);
+        if (a_rec) atomicAdd(&s_acc[OCT_ACC + 1], a_rec);
+        if (b_const) atomicAdd(&s_acc[OCT_ACC + 2], b_const);
+        if (b_rec) atomicAdd(&s_acc[OCT_ACC + 3], b_rec);
+    }
     __syncthreads();
     for (int k = threadIdx.x; k < WIDTH; k += 256)
         if (s_acc[k]) atomicAdd(&acc[k], s_acc[k]);
@@ -696,21 +720,25 @@ void rate_inputs_launch(const RateStats *d_stats, int nchunks, const float *d_eb
 uint64_t octave_workspace_bytes() { return 256; }
 
 void rate_octave_launch(const void *x, uint64_t n, int w, int bs, int reqexp, uint8_t *ws,
-                        unsigned long long *d_acc, hipStream_t st)
+                        unsigned long long *d_acc, hipStream_t st, bool edges)
 {
     (void)ws;
-    constexpr int WIDTH = 1 + 2 * OCT_BINS;
-    hipLaunchKernelGGL(k_fill_u64, dim3((WIDTH + 255) / 256), dim3(256), 0, st, d_acc, WIDTH);
+    const int width = OCT_ACC + (edges ? 4 : 0);
+    hipLaunchKernelGGL(k_fill_u64, dim3((width + 255) / 256), dim3(256), 0, st, d_acc, width);
     const uint64_t nb = (n + bs - 1) / bs;
     uint64_t g = (nb + 4 * PF - 1) / (4 * PF);
     if (g < 1) g = 1;
     if (g > OCT_WG) g = OCT_WG;
-    if (w == 4)
-        hipLaunchKernelGGL(k_rate_octave<float>, dim3((unsigned)g), dim3(256), 0, st,
-                           static_cast<const float *>(x), n, bs, reqexp, d_acc);
+    const float *xf = static_cast<const float *>(x);
+    const double *xd = static_cast<const double *>(x);
+    if (w == 4 && edges)
+        hipLaunchKernelGGL((k_rate_octave<float, true>), dim3((unsigned)g), dim3(256), 0, st, xf, n, bs, reqexp, d_acc);
+    else if (w == 4)
+        hipLaunchKernelGGL((k_rate_octave<float, false>), dim3((unsigned)g), dim3(256), 0, st, xf, n, bs, reqexp, d_acc);
+    else if (edges)
+        hipLaunchKernelGGL((k_rate_octave<double, true>), dim3((unsigned)g), dim3(256), 0, st, xd, n, bs, reqexp, d_acc);
     else
-        hipLaunchKernelGGL(k_rate_octave<double>, dim3((unsigned)g), dim3(256), 0, st,
-                           static_cast<const double *>(x), n, bs, reqexp, d_acc);
+        hipLaunchKernelGGL((k_rate_octave<double, false>), dim3((unsigned)g), dim3(256), 0, st, xd, n, bs, reqexp, d_acc);
 }
 
 }  // namespace szxamd

@@ -10,6 +10,7 @@ constexpr int FXR_MAX_CAND = 32;    // candidate bounds per model pass
 constexpr int FXR_MAX_WG = 1024;    // workgroups of a model pass
 constexpr int OCT_BINS = 512;       // radius bins per octave of bounds
 constexpr int OCT_WG = 2048;        // workgroups of an octave pass
+constexpr int OCT_ACC = 1 + 2 * OCT_BINS;   // u64 accumulators of an octave pass (+4 with edges)
 
 // Workspace of the sampled model: per sampled block median and radius,
 // per-workgroup partial sums, the candidate bounds, the folded sums and the
@@ -28,12 +29,13 @@ void fxr_estimate_launch(const void *x, uint64_t n, int w, int bs, int rate, uin
                          hipStream_t st);
 
 // Exact rate over the octave of bounds with exponent reqexp: d_acc receives
-// 1 + 2 * OCT_BINS u64 (fixed bytes, then per radius bin count and record
-// bytes); ws holds octave_workspace_bytes() (unused scratch, kept for the
-// interface).
+// OCT_ACC u64 (fixed bytes, then per radius bin count and record bytes), and
+// with `edges` four more: constant blocks and record bytes at 2^(reqexp+1)
+// and at the float bound below 2^reqexp; ws holds octave_workspace_bytes()
+// (unused scratch, kept for the interface).
 uint64_t octave_workspace_bytes();
 void rate_octave_launch(const void *x, uint64_t n, int w, int bs, int reqexp, uint8_t *ws,
-                        unsigned long long *d_acc, hipStream_t st);
+                        unsigned long long *d_acc, hipStream_t st, bool edges = false);
 
 // Per-chunk statistics feeding the rate model (RATE_CHUNK values per chunk;
 // the last chunk may be shorter).  d: min, max, sum x, sum x^2, sum |d1|,

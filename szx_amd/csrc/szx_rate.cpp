@@ -53,7 +53,7 @@ RateLayout rate_layout(uint64_t n, int w, int bs)
     uint64_t off = 0;
     L.comp = off;     off += al(C.ws_bytes);
     L.oct = off;      off += al(octave_workspace_bytes());
-    L.acc = off;      off += al((1 + 2 * OCT_BINS) * 8);
+    L.acc = off;      off += al((OCT_ACC + 4) * 8);
     L.stats_ws = off; off += al(rate_stats_workspace_bytes(n));
     L.stats = off;    off += al(L.nchunks * sizeof(RateStats));
     L.ebs = off;      off += al(K_MAX * 4);
@@ -118,12 +118,16 @@ struct Solver {
         return true;
     }
 
-    // Exact stream sizes for the OCT_BINS bounds of octave e.
+    // Exact stream sizes for the OCT_BINS bounds of octave e, and for the
+    // bounds just outside it: edge_up at 2^(e+1), edge_dn at the float below
+    // 2^e (eb_dn).
+    double edge_up = 0.0, edge_dn = 0.0;
+    float eb_up = 0.0f, eb_dn = 0.0f;
     bool octave(int e, std::vector<double> &bpv, std::vector<float> &ebs)
     {
-        e = e < -1074 ? -1074 : (e > 1023 ? 1023 : e);
-        rate_octave_launch(x, n, w, bs, e, ws + L.oct, (unsigned long long *)(ws + L.acc), st);
-        std::vector<unsigned long long> acc(1 + 2 * OCT_BINS);
+        e = e < -148 ? -148 : (e > 126 ? 126 : e);   // float bounds, both edges inside
+        rate_octave_launch(x, n, w, bs, e, ws + L.oct, (unsigned long long *)(ws + L.acc), st, true);
+        std::vector<unsigned long long> acc(OCT_ACC + 4);
         if (!ok(hipMemcpyAsync(acc.data(), ws + L.acc, acc.size() * 8, hipMemcpyDeviceToHost, st),
                 "D2H octave") ||
             !ok(hipStreamSynchronize(st), "octave pass"))
@@ -144,6 +148,10 @@ struct Solver {
             const_part += 4 * acc[1 + 2 * k];
             rec_part -= acc[2 + 2 * k];
         }
+        edge_up = 8.0 * (double)(STREAM_HEADER + nbit + 4 * acc[OCT_ACC] + acc[OCT_ACC + 1]) / (double)n;
+        edge_dn = 8.0 * (double)(STREAM_HEADER + nbit + 4 * acc[OCT_ACC + 2] + acc[OCT_ACC + 3]) / (double)n;
+        eb_up = ldexpf(1.0f, e + 1);
+        eb_dn = nextafterf(ldexpf(1.0f, e), 0.0f);
         return true;
     }
 
@@ -172,6 +180,24 @@ struct Solver {
             else if (bpv[0] < target * (1.0 - tol)) dir = -1;         // bounds too loose
             if (dir == 0) {
                 *eb = ebs[bk];
+                *found = true;
+                return true;
+            }
+            // the bound just past the octave edge in that direction: the
+            // answer itself, or the far side of a jump that holds the target
+            // (no bound reaches it: take the nearer side), or else the next
+            // octave is needed
+            const double edge = dir > 0 ? edge_up : edge_dn;
+            const float eb_edge = dir > 0 ? eb_up : eb_dn;
+            const double near = dir > 0 ? bpv[OCT_BINS - 1] : bpv[0];
+            const float eb_near = dir > 0 ? ebs[OCT_BINS - 1] : ebs[0];
+            if (within(edge)) {
+                *eb = eb_edge;
+                *found = true;
+                return true;
+            }
+            if (dir > 0 ? edge < target * (1.0 - tol) : edge > target * (1.0 + tol)) {
+                *eb = fabs(edge - target) < fabs(near - target) ? eb_edge : eb_near;
                 *found = true;
                 return true;
             }
