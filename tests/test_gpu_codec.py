@@ -300,6 +300,64 @@ def test_generic_block_sizes(bs, device):
             assert bits_equal(Z.decompress_blocked(want_s, n, dtype), want)
 
 
+def _tile_at_stage_cap(extra):
+    """An fp32 field whose first 16384-value tile (block size 128: one
+    compressor tile, 48 KiB stage) has records of 49152 + extra bytes in all.
+    117 blocks alternate +-1.5 (eb 2^-14: reqLength 24, 3 bytes a value, lead
+    code 0: 421-byte records), 11 blocks are constant, and block 0 trades
+    bytes for lead codes: a repeated value (code 3) saves 3 bytes, a value of
+    the same sign and exponent as the one before (code 1) saves 1."""
+    def block(codes):
+        v, prev = [], None
+        for c in codes:
+            if prev is None:
+                x = 1.5
+            elif c == 3:
+                x = prev
+            elif c == 1:
+                x = np.copysign(1.25 if abs(prev) == 1.5 else 1.5, prev)
+            else:
+                x = -np.copysign(1.5, prev)
+            v.append(x)
+            prev = x
+        return np.array(v, np.float32)
+
+    saved = 117 * 421 - (49152 + extra)
+    n3, n1 = saved // 3, saved % 3
+    codes = [0] * 128
+    for j in range(n3):
+        codes[1 + j] = 3
+    for j in range(n1):
+        codes[n3 + 4 + 2 * j] = 1
+    alt = block([0] * 128)
+    tile = np.concatenate([block(codes)] + [alt] * 116 + [np.zeros(128 * 11, np.float32)])
+    rest = np.sin(np.arange(2 * 16384 + 777) * 0.01).astype(np.float32)
+    return np.concatenate([tile, rest])
+
+
+@pytest.mark.parametrize("extra", [0, 1])
+def test_stage_cap_edge(extra, device):
+    """A tile whose record bytes are exactly the stage capacity is staged;
+    one byte more and it is written straight to the payload.  Both streams
+    match the oracle's, and the tile sizes are checked on the oracle's stream
+    first."""
+    import torch
+    x = _tile_at_stage_cap(extra)
+    eb = 2.0 ** -14
+    want = O.compress(x, eb, 128)
+    offs = O.record_offsets(want, x.size, 4)
+    assert int(offs[117]) - int(offs[0]) == 49152 + extra
+    say("stage cap edge", extra)
+    comp = Z.DeviceCompressor(x.size, np.float32, 128)
+    comp(torch.from_numpy(x).to(device), Z.ABS, eb)
+    view, off, size, _ = comp.stream_view()
+    assert view.cpu().numpy().tobytes() == want
+    dec = Z.DeviceDecompressor(x.size, np.float32, 128, size)
+    dec(view, size, Z.DECODE_REFERENCE, index=comp.index)
+    assert dec.check() == 0
+    assert bits_equal(dec.out.cpu().numpy(), O.decompress(want, x.size, np.float32))
+
+
 def test_generic_rel_and_damage(device):
     import torch
     x = (np.sin(np.arange(100003) * 0.003) * 3).astype(np.float32)
@@ -324,12 +382,15 @@ def test_generic_rel_and_damage(device):
     assert dec.check() == 0
 
 
-@pytest.mark.parametrize("bs", [16, 64, 128, 256])
+@pytest.mark.parametrize("bs", [4, 8, 16, 32, 64, 128, 256])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_incompressible_tiles(dtype, bs, device):
-    """Tiles whose records outgrow the compressor's LDS stage (48 KiB for the
-    64 KiB tiles of block sizes 16..128, 20 KiB otherwise) are written
-    straight to the payload; mixed with ordinary tiles in one field."""
+    """Tiles whose records outgrow the compressor's LDS stage are written
+    straight to the payload; mixed with ordinary tiles in one field.  The
+    stage holds 48 KiB for the 64 KiB tiles of block sizes 16..128 with fewer
+    than 512 blocks per tile, 44 KiB for those with 512 or more (fp32 block
+    sizes 16 and 32, fp64 16), and 20 KiB for the smaller tiles of block sizes
+    4, 8 and 256."""
     import torch
     r = np.random.default_rng(77)
     n = 300_001
