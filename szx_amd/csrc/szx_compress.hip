@@ -111,8 +111,9 @@ __device__ inline uint32_t wave0_exclusive_scan(uint32_t *a, int lane)
 //           block (7) << 16
 //   mb    = per value 8 x (mid bytes), one byte each
 template <typename T, int LV> struct SlabState {
+    static_assert(LV * 4 <= 32, "a 4-bit mid-byte count per value fits one word");
     typename Word<T>::U mid[LV];
-    uint64_t mb;
+    uint32_t mb;   // mid bytes of value k (0..8) in bits 4k..4k+3
     uint32_t meta, code;
     float hmed;
 };
@@ -238,7 +239,7 @@ __device__ __forceinline__ void encode_slab(const T (&v)[LV], uint64_t e0, uint6
     // [lo, lo + m), lo = W - rb, are its low bytes (only those are staged).
     const int rb8 = 8 * rb, lo8 = 8 * W - rb8;
     uint32_t codex = 0, tot8 = 0;
-    uint64_t mb = 0;
+    uint32_t mb = 0;
 #pragma unroll
     for (int k = 0; k < LV; k++) {
         const int ld8 = lead8<U>(w[k] ^ prev);
@@ -259,7 +260,7 @@ __device__ __forceinline__ void encode_slab(const T (&v)[LV], uint64_t e0, uint6
         // element k -> byte k/4 of the lane's lead codes, bits 7-6 first
         // (8 ld here; the sum is divided by 8 below)
         codex |= ldv << (8 * (k >> 2) + 6 - 2 * (k & 3));
-        mb |= (uint64_t)(uint32_t)m8 << (8 * k);
+        mb |= ((uint32_t)m8 >> 3) << (4 * k);
         tot8 += (uint32_t)m8;
         st.mid[k] = mid;
         prev = w[k];
@@ -472,7 +473,7 @@ __device__ __forceinline__ void stage_slab(const SlabState<T, LV> &st, int u, co
         uint32_t o8 = 8u * (base + 5 + nlead + (meta & 0xFFFu));
 #pragma unroll
         for (int k = 0; k < LV; k++) {
-            const uint32_t m8 = (uint32_t)(st.mb >> (8 * k)) & 0xFFu;
+            const uint32_t m8 = ((st.mb >> (4 * k)) & 0xFu) << 3;
             const uint64_t sh = (uint64_t)(uint32_t)st.mid[k] << (o8 & 31u);
             uint32_t *pw = sw + (o8 >> 5);
             if (m8) {
@@ -487,7 +488,7 @@ __device__ __forceinline__ void stage_slab(const SlabState<T, LV> &st, int u, co
         uint8_t *q = s_stage + base + 5 + nlead + (meta & 0xFFFu);
 #pragma unroll
         for (int k = 0; k < LV; k++) {
-            const int m = (int)(st.mb >> (8 * k + 3)) & 31;
+            const int m = (int)(st.mb >> (4 * k)) & 0xF;
             stage_bytes<U>(q, st.mid[k], m);
             q += m;
         }
