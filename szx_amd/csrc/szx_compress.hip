@@ -566,7 +566,12 @@ __device__ __forceinline__ void copy_out(const CompressParams &p, uint64_t tile,
             } else {
                 val = make_uint4(a0, a1, a2, a3);
             }
-            *reinterpret_cast<uint4 *>(dst + o) = val;
+            // non-temporal: the stream is not read again by this launch, and
+            // keeping it out of the caches leaves them to the input (REL
+            // step 0.305 -> 0.294 ms at 512^3 fp32)
+            typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+            const u32x4_t wv = {val.x, val.y, val.z, val.w};
+            __builtin_nontemporal_store(wv, reinterpret_cast<u32x4_t *>(dst + o));
         }
         const uint32_t tail0 = h + 16 * nchunk;
         if (tid < (int)(tbytes - tail0)) dst[tail0 + tid] = s_stage[tail0 + tid];
@@ -766,6 +771,11 @@ __global__ void k_assemble(AssembleParams a)
 // ---------------------------------------------------------------------------
 // REL: global value range and bound (szx.cc:249-253, 303-308).
 // ---------------------------------------------------------------------------
+// The grid sweeps the array from its end to its start: the front of the
+// array, which the compressor reads first, is then the part most recently
+// read, so the compressor finds up to the last-level cache's worth of it
+// there (REL step 0.317 -> 0.305 ms at 512^3 fp32).  Min and max do not
+// depend on the order (NaNs lose every comparison either way).
 template <typename T>
 __global__ __launch_bounds__(256) void k_range_partial(const T *x, uint64_t n, T *part)
 {
@@ -776,7 +786,7 @@ __global__ __launch_bounds__(256) void k_range_partial(const T *x, uint64_t n, T
     for (; q + 3 * stride < nq; q += 4 * stride) {
         T v[4][4];
 #pragma unroll
-        for (int u = 0; u < 4; u++) SlabLoad<T>::load4(x, 4 * (q + u * stride), n, v[u]);
+        for (int u = 0; u < 4; u++) SlabLoad<T>::load4(x, 4 * (nq - 1 - (q + u * stride)), n, v[u]);
 #pragma unroll
         for (int u = 0; u < 4; u++)
 #pragma unroll
@@ -787,7 +797,7 @@ __global__ __launch_bounds__(256) void k_range_partial(const T *x, uint64_t n, T
     }
     for (; q < nq; q += stride) {
         T v[4];
-        SlabLoad<T>::load4(x, 4 * q, n, v);
+        SlabLoad<T>::load4(x, 4 * (nq - 1 - q), n, v);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             mn = v[k] < mn ? v[k] : mn;

@@ -379,7 +379,9 @@ __device__ inline void cz_decode_units(const CzParams &p, const CzHeader &H, uin
         }
         if (act) {
             if (aligned) {
-                *reinterpret_cast<float4 *>(y) = make_float4(v[0], v[1], v[2], v[3]);
+                typedef float f32x4_t __attribute__((ext_vector_type(4)));
+                const f32x4_t w4 = {v[0], v[1], v[2], v[3]};
+                __builtin_nontemporal_store(w4, reinterpret_cast<f32x4_t *>(y));   // streamed past the caches
             } else {
 #pragma unroll
                 for (int k = 0; k < 4; k++) y[k] = v[k];

