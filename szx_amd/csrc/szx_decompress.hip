@@ -105,11 +105,10 @@ __global__ void k_check_header(const uint8_t *s, uint64_t len, uint64_t n, int b
     }
 }
 
-// Whole-vector output stores are non-temporal: the decoded array is written
-// once and not read back by this kernel, so it streams past the caches
-// (k_decode_tiles 0.178 -> 0.136 ms at 512^3 fp32).
+// Whole-vector fp32 output stores are non-temporal: the decoded array is
+// written once and not read back by this kernel, so it streams past the
+// caches (k_decode_tiles 0.178 -> 0.136 ms at 512^3 fp32).
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
-typedef double f64x2_t __attribute__((ext_vector_type(2)));
 template <typename T> struct Store4;
 template <> struct Store4<float> {
     __device__ static void store_full(float *o, uint64_t e0, const float v[4])
@@ -128,11 +127,13 @@ template <> struct Store4<float> {
     }
 };
 template <> struct Store4<double> {
+    // plain stores: a lane's 32 bytes go out as two 16-byte halves, so each
+    // store instruction leaves every line half written, which non-temporal
+    // stores handle badly (fp64 decode 2.2-2.5 -> 1.3-1.4 TB/s measured)
     __device__ static void store_full(double *o, uint64_t e0, const double v[4])
     {
-        f64x2_t a = {v[0], v[1]}, b = {v[2], v[3]};
-        __builtin_nontemporal_store(a, reinterpret_cast<f64x2_t *>(o + e0));
-        __builtin_nontemporal_store(b, reinterpret_cast<f64x2_t *>(o + e0 + 2));
+        *reinterpret_cast<double2 *>(o + e0) = make_double2(v[0], v[1]);
+        *reinterpret_cast<double2 *>(o + e0 + 2) = make_double2(v[2], v[3]);
     }
     __device__ static void store(double *o, uint64_t e0, uint64_t n, const double v[4])
     {

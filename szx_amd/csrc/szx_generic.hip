@@ -429,7 +429,7 @@ __device__ inline void decode_segment(const GenericDecode &p, uint64_t b, uint64
         const T med = (T)bits_f32(h->medle ? bswap32(be) : be);
 #pragma unroll
         for (int k = 0; k < 4; k++)
-            if (val[k]) __builtin_nontemporal_store(med, &out[4 * l + k]);
+            if (val[k]) out[4 * l + k] = med;
         return;
     }
     const int nlead = (m + 3) >> 2;
@@ -506,7 +506,7 @@ __device__ inline void decode_segment(const GenericDecode &p, uint64_t b, uint64
     for (int k = 0; k < 4; k++) {
         const U top = lk[k] ? ~(U)0 << (8 * (W - lk[k])) : (U)0;
         const U cur = wv[k] | (prev & top);
-        if (val[k]) __builtin_nontemporal_store(from_bits<T>(cur << shift) + med, &out[4 * l + k]);
+        if (val[k]) out[4 * l + k] = from_bits<T>(cur << shift) + med;
         prev = cur;
     }
 }

@@ -434,7 +434,7 @@ __global__ __launch_bounds__(256) void k_nb_decode(const uint8_t *s, uint64_t le
         if (e0 + k >= n) break;
         const U top = lk[k] ? ~(U)0 << (8 * (W - lk[k])) : (U)0;
         const U cur = wv[k] | (prev & top);
-        __builtin_nontemporal_store(from_bits<T>(cur << P.shift) + med, &out[e0 + k]);
+        out[e0 + k] = from_bits<T>(cur << P.shift) + med;
         prev = cur;
     }
 }
