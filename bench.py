@@ -950,7 +950,7 @@ def main():
     ycopy.copy_(x)
     ms_copy = timed(lambda: ycopy.copy_(x), 10)
     copy_gbs = 2 * nbytes / (ms_copy * 1e-3) / 1e9
-    # the library's own copy kernel (16-byte loads, non-temporal 16-byte
+    # the library's own copy kernel (non-temporal 16-byte loads and
     # stores), timed by the kernel timer: the ceiling the codec kernels share
     Z.copy_dev(x, ycopy)
     Z.timing_begin()

@@ -552,10 +552,10 @@ int szx_amd_batch_tune(szx_amd_batch *batch, int reps, int *valuesPerLane, int *
 void szx_amd_timing_begin(void);
 int szx_amd_timing_end(double *ms_sum, int *count);
 
-/* Device copy of `bytes` bytes (pointers and size multiples of 16) with
- * 16-byte loads and non-temporal 16-byte stores, one pass: the copy rate
- * that bounds the codec kernels on this GPU.  No counterpart in the
- * reference: a measurement helper for benchmarks. */
+/* Device copy of `bytes` bytes (pointers and size multiples of 16), one
+ * non-temporal 16-byte load and store per lane: the copy rate that bounds
+ * the codec kernels on this GPU.  No counterpart in the reference: a
+ * measurement helper for benchmarks. */
 int szx_amd_copy_dev(const void *d_src, void *d_dst, size_t bytes, void *hipStream);
 
 #ifdef __cplusplus

@@ -407,3 +407,19 @@ def test_incompressible_tiles(dtype, bs, device):
         dec(view, size, Z.DECODE_REFERENCE, index=comp.index)
         assert dec.check() == 0
         assert bits_equal(dec.out.cpu().numpy(), O.decompress(s, n, dtype, ref_quirk=True))
+
+
+def test_copy_kernel(device):
+    """The measurement helper copies exactly (sizes that are not multiples of
+    the 4 KiB a workgroup moves included) and refuses what it cannot copy."""
+    import torch
+    for n in (4, 1024, 1024 + 4, (1 << 20) + 12):
+        x = torch.randint(-2**31, 2**31 - 1, (n,), dtype=torch.int32, device=device)
+        y = torch.zeros_like(x)
+        Z.copy_dev(x, y)
+        torch.cuda.synchronize()
+        assert torch.equal(x, y), n
+    with pytest.raises(Z.SZxError):
+        Z.copy_dev(torch.zeros(6, dtype=torch.int8, device=device), torch.zeros(6, dtype=torch.int8, device=device))
+    with pytest.raises(Z.SZxError):
+        Z.copy_dev(torch.zeros(16, device=device), torch.zeros(8, device=device))
