@@ -437,6 +437,19 @@ size_t szx_amd_rate_octave_workspace(void);
 int szx_amd_rate_octave_dev(const void *d_in, size_t n, int dataType, int blockSize, int reqExpo,
                             uint64_t *d_acc, void *d_work, size_t workBytes, void *hipStream);
 
+/* The same sizes over nOctaves (1 or 2) adjacent octaves in one pass, with
+ * the bounds just outside them: octave reqExpo - j (j < nOctaves) fills
+ * d_acc[j * (1 + 2B) ...] as above; then 4 u64 follow -- constant blocks
+ * and record bytes at 2^(reqExpo+1), and constant blocks and record bytes
+ * at the float bound just below 2^(reqExpo-nOctaves+1).  The stream at such
+ * an edge bound has 13 + ceil(nb/8) + 4 * constant + record bytes.
+ * reqExpo in [nOctaves - 149, 126] (float bounds on both sides).  The
+ * fixed-rate solver's learned search sizes a pair when the model's bound
+ * lies close to an octave edge.  No counterpart in the reference. */
+int szx_amd_rate_octave_span_dev(const void *d_in, size_t n, int dataType, int blockSize, int reqExpo,
+                                 int nOctaves, uint64_t *d_acc, void *d_work, size_t workBytes,
+                                 void *hipStream);
+
 /* The rate model: an MLP 24 -> 64 -> 64 -> 1 (GELU) predicting a chunk's
  * log2(bits per value) at a bound from the chunk's statistics.  Parameters
  * are one FP32 array: W1[64][24], b1[64], W2[64][64] (out, in), b2[64],

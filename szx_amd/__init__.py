@@ -48,6 +48,7 @@ EXPORTS = (
     "szx_amd_rate_stats_workspace", "szx_amd_rate_stats_dev", "szx_amd_rate_inputs_dev",
     "szx_amd_mlp_forward_dev", "szx_amd_mlp_grad_workspace", "szx_amd_mlp_grad_dev",
     "szx_amd_mlp_adam_dev", "szx_amd_rate_octave_workspace", "szx_amd_rate_octave_dev",
+    "szx_amd_rate_octave_span_dev",
     "szx_amd_batch_workspace", "szx_amd_batch_create", "szx_amd_batch_destroy",
     "szx_amd_batch_compress", "szx_amd_batch_results_dev", "szx_amd_batch_status_dev",
     "szx_amd_batch_index_dev", "szx_amd_batch_results", "szx_amd_batch_decompress",
@@ -203,6 +204,8 @@ def lib() -> C.CDLL:
     L.szx_amd_rate_octave_workspace.argtypes = []
     L.szx_amd_rate_octave_dev.restype = i32
     L.szx_amd_rate_octave_dev.argtypes = [vp, sz, i32, i32, i32, vp, vp, sz, vp]
+    L.szx_amd_rate_octave_span_dev.restype = i32
+    L.szx_amd_rate_octave_span_dev.argtypes = [vp, sz, i32, i32, i32, i32, vp, vp, sz, vp]
     L.szx_amd_rate_stats_bytes.restype = sz
     L.szx_amd_rate_stats_bytes.argtypes = [sz]
     L.szx_amd_rate_stats_workspace.restype = sz
@@ -851,6 +854,41 @@ def rate_octave(x, reqExpo: int, blockSize: int = 128, stream=None, with_end: bo
         end = int(13 + (nb + 7) // 8 + a[0] + 4 * int(cnt.sum()))
         return sizes.astype(np.int64), ebs, end
     return sizes.astype(np.int64), ebs
+
+
+def rate_octave_span(x, reqExpo: int, nOctaves: int = 2, blockSize: int = 128, stream=None):
+    """Exact stream sizes (bytes) for the OCTAVE_BINS bounds of each of the
+    nOctaves octaves reqExpo, reqExpo - 1, ... sized in one pass, lowest
+    bound first, and for the bounds just outside them: (numpy sizes, numpy
+    float32 bounds, (size at 2^(reqExpo+1), its bound), (size at the float
+    just below the lowest octave, its bound))."""
+    torch = _torch()
+    n = x.numel()
+    code = _torch_code(x)
+    _dev_tensor(x, "x", n)
+    width = 1 + 2 * OCTAVE_BINS
+    acc = torch.zeros(nOctaves * width + 4, dtype=torch.int64, device=x.device)
+    ws = torch.empty(lib().szx_amd_rate_octave_workspace(), dtype=torch.uint8, device=x.device)
+    _check(lib().szx_amd_rate_octave_span_dev(x.data_ptr(), n, code, blockSize, reqExpo, nOctaves,
+                                              acc.data_ptr(), ws.data_ptr(), ws.numel(),
+                                              _stream_handle(stream)))
+    a = acc.cpu().numpy().astype(np.uint64)
+    nb = (n + blockSize - 1) // blockSize
+    head = 13 + (nb + 7) // 8
+    sizes, ebs = [], []
+    for j in reversed(range(nOctaves)):
+        o = a[j * width:(j + 1) * width]
+        cnt, rec = o[1::2], o[2::2]
+        const_before = np.concatenate([[0], np.cumsum(cnt)[:-1]])
+        rec_from = np.cumsum(rec[::-1])[::-1]
+        sizes.append(head + o[0] + 4 * const_before + rec_from)
+        ebs.append(np.ldexp(np.float32(1) + np.arange(OCTAVE_BINS, dtype=np.float32) / np.float32(OCTAVE_BINS),
+                            reqExpo - j).astype(np.float32))
+    g = a[nOctaves * width:]
+    up = (int(head + 4 * g[0] + g[1]), np.float32(np.ldexp(np.float32(1), reqExpo + 1)))
+    dn = (int(head + 4 * g[2] + g[3]),
+          np.nextafter(np.float32(np.ldexp(np.float32(1), reqExpo - nOctaves + 1)), np.float32(0)))
+    return np.concatenate(sizes).astype(np.int64), np.concatenate(ebs), up, dn
 
 
 class FixedRateCompressor:

@@ -1118,6 +1118,23 @@ int szx_amd_rate_octave_dev(const void *d_in, size_t n, int dataType, int blockS
     return SZx_SCES;
 }
 
+int szx_amd_rate_octave_span_dev(const void *d_in, size_t n, int dataType, int blockSize, int reqExpo,
+                                 int nOctaves, uint64_t *d_acc, void *d_work, size_t workBytes,
+                                 void *hipStream)
+{
+    g_err.clear();
+    const int w = width_of(dataType);
+    if (!w || !d_in || n == 0 || !d_acc || blockSize < 1 || blockSize > 256 ||
+        workBytes < octave_workspace_bytes() || nOctaves < 1 || nOctaves > 2 ||
+        reqExpo < nOctaves - 149 || reqExpo > 126) {
+        set_err("invalid arguments");
+        return SZx_NSCS;
+    }
+    rate_octave_launch(d_in, n, w, blockSize, reqExpo, (uint8_t *)d_work, (unsigned long long *)d_acc,
+                       (hipStream_t)hipStream, true, nOctaves);
+    return SZx_SCES;
+}
+
 size_t szx_amd_rate_stats_bytes(size_t n)
 {
     const size_t c = (n + RATE_CHUNK - 1) / RATE_CHUNK;

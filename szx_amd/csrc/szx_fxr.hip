@@ -299,34 +299,44 @@ __device__ inline uint64_t record_bytes(const T (&v)[FX_MAXC], int n, int lane, 
     return 5 + (uint64_t)(n + 3) / 4 + (uint64_t)rb * n - leads;
 }
 
-// One wave per codec block.  acc layout (u64, zeroed before the launch):
+// One wave per codec block, over NOCT adjacent octaves: octave j (0 <= j <
+// NOCT) holds the bounds with exponent reqexp - j.  acc layout (u64, zeroed
+// before the launch), per octave j at j * OCT_ACC:
 //   [0] bytes of constant blocks and records fixed for the whole octave
-//       (radius <= 2^e: always constant, 4 bytes; radius > 2^(e+1): always a
-//       record)
-//   [1 + 2k], [2 + 2k]: blocks with radius in bin k of (2^e, 2^(e+1)]:
-//       count, record bytes.
-// With EDGES, the two bounds just outside the octave as well (every block's
-// reqLength moves by one bit there, so the rate jumps):
-//   [OCT_ACC + 0], [OCT_ACC + 1]: at 2^(e+1) (exponent e+1): constant
-//       blocks, record bytes;
-//   [OCT_ACC + 2], [OCT_ACC + 3]: at the float below 2^e (exponent e-1):
+//       (radius <= 2^(reqexp-j): always constant, 4 bytes; radius >
+//       2^(reqexp-j+1): always a record)
+//   [1 + 2k], [2 + 2k]: blocks with radius in bin k of (2^(reqexp-j),
+//       2^(reqexp-j+1)]: count, record bytes.
+// With EDGES, the two bounds just outside the span as well (every block's
+// reqLength moves by one bit there, so the rate jumps), at NOCT * OCT_ACC:
+//   [0], [1]: at 2^(reqexp+1) (exponent reqexp+1): constant blocks, record
+//       bytes;
+//   [2], [3]: at the float below 2^(reqexp-NOCT+1) (exponent reqexp-NOCT):
 //       constant blocks, record bytes.
+// The record bytes of a block depend on the exponent only, so each octave
+// costs one record size per block whose radius exceeds its lower edge.
 // Each workgroup bins in LDS and adds its non-zero bins to acc.
-template <typename T, bool EDGES>
+template <typename T, bool EDGES, int NOCT>
 __global__ __launch_bounds__(256) void k_rate_octave(const T *x, uint64_t n, int bs, int reqexp,
                                                      unsigned long long *acc)
 {
-    constexpr int WIDTH = OCT_ACC + (EDGES ? 4 : 0);
+    constexpr int EDGE = NOCT * OCT_ACC;
+    constexpr int WIDTH = EDGE + (EDGES ? 4 : 0);
     __shared__ unsigned long long s_acc[WIDTH];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int k = threadIdx.x; k < WIDTH; k += 256) s_acc[k] = 0;
     __syncthreads();
     const uint64_t nb = (n + bs - 1) / bs;
-    // 2^reqexp and 2^(reqexp+1) as doubles (exact for every float exponent)
-    const double lo_edge = ldexp(1.0, reqexp), hi_edge = 2.0 * lo_edge;
-    // the float bound just below 2^e (bounds are floats for both dtypes)
-    const double below = (double)nextafterf(ldexpf(1.0f, reqexp), 0.0f);
-    unsigned long long fixed = 0, a_const = 0, a_rec = 0, b_const = 0, b_rec = 0;
+    // 2^(reqexp-j) as doubles (exact for every float exponent)
+    double lo_edge[NOCT];
+#pragma unroll
+    for (int j = 0; j < NOCT; j++) lo_edge[j] = ldexp(1.0, reqexp - j);
+    const double top = 2.0 * lo_edge[0];
+    // the float bound just below the span (bounds are floats for both dtypes)
+    const double below = (double)nextafterf(ldexpf(1.0f, reqexp - NOCT + 1), 0.0f);
+    unsigned long long fixed[NOCT], a_const = 0, a_rec = 0, b_const = 0, b_rec = 0;
+#pragma unroll
+    for (int j = 0; j < NOCT; j++) fixed[j] = 0;
     const uint64_t step = gridDim.x * 4ull;
     for (uint64_t i0 = blockIdx.x * 4ull + wv; i0 < nb; i0 += PF * step) {
         T vp[PF][FX_MAXC];
@@ -350,42 +360,46 @@ __global__ __launch_bounds__(256) void k_rate_octave(const T *x, uint64_t n, int
             const T r = (hi - lo) / 2;
             const T m = lo + r;
             // the constant test compares the block radius (double for fp64)
-            // with the float bound widened to T
+            // with the float bound widened to T; a NaN radius is never
+            // constant
             const double rd = (double)r;
-            if (rd <= lo_edge) {
-                fixed += 4;
-                if (EDGES) {
-                    a_const++;
-                    if (rd <= below) b_const++;
-                    else b_rec += record_bytes<T>(v, cnt, lane, (float)r, (float)m, reqexp - 1);
-                }
-                continue;
-            }
-            const uint64_t rec = record_bytes<T>(v, cnt, lane, (float)r, (float)m, reqexp);
             if (EDGES) {
-                b_rec += record_bytes<T>(v, cnt, lane, (float)r, (float)m, reqexp - 1);
-                if (rd <= hi_edge) a_const++;
+                if (rd <= top) a_const++;
                 else a_rec += record_bytes<T>(v, cnt, lane, (float)r, (float)m, reqexp + 1);
+                if (rd <= below) b_const++;
+                else b_rec += record_bytes<T>(v, cnt, lane, (float)r, (float)m, reqexp - NOCT);
             }
-            if (!(rd <= hi_edge)) {   // also NaN radii: never constant
-                fixed += rec;
-                continue;
-            }
-            // bin k holds radii in (2^e (1 + k/B), 2^e (1 + (k+1)/B)]
-            const double t = (rd / lo_edge - 1.0) * OCT_BINS;   // in (0, OCT_BINS]
-            const int k = t >= OCT_BINS ? OCT_BINS - 1 : (t <= 1.0 ? 0 : (int)ceil(t) - 1);
-            if (lane == 0) {
-                atomicAdd(&s_acc[1 + 2 * k], 1ull);
-                atomicAdd(&s_acc[2 + 2 * k], (unsigned long long)rec);
+#pragma unroll
+            for (int j = 0; j < NOCT; j++) {
+                if (rd <= lo_edge[j]) {
+                    fixed[j] += 4;
+                    continue;
+                }
+                const uint64_t rec = record_bytes<T>(v, cnt, lane, (float)r, (float)m, reqexp - j);
+                if (!(rd <= 2.0 * lo_edge[j])) {
+                    fixed[j] += rec;
+                    continue;
+                }
+                // bin k holds radii in (2^e (1 + k/B), 2^e (1 + (k+1)/B)]
+                const double t = (rd / lo_edge[j] - 1.0) * OCT_BINS;   // in (0, OCT_BINS]
+                const int k = t >= OCT_BINS ? OCT_BINS - 1 : (t <= 1.0 ? 0 : (int)ceil(t) - 1);
+                if (lane == 0) {
+                    atomicAdd(&s_acc[j * OCT_ACC + 1 + 2 * k], 1ull);
+                    atomicAdd(&s_acc[j * OCT_ACC + 2 + 2 * k], (unsigned long long)rec);
+                }
             }
         }
     }
-    if (lane == 0 && fixed) atomicAdd(&s_acc[0], fixed);
+    if (lane == 0) {
+#pragma unroll
+        for (int j = 0; j < NOCT; j++)
+            if (fixed[j]) atomicAdd(&s_acc[j * OCT_ACC], fixed[j]);
+    }
     if (EDGES && lane == 0) {
-        if (a_const) atomicAdd(&s_acc[OCT_ACC + 0], a_const);
-        if (a_rec) atomicAdd(&s_acc[OCT_ACC + 1], a_rec);
-        if (b_const) atomicAdd(&s_acc[OCT_ACC + 2], b_const);
-        if (b_rec) atomicAdd(&s_acc[OCT_ACC + 3], b_rec);
+        if (a_const) atomicAdd(&s_acc[EDGE + 0], a_const);
+        if (a_rec) atomicAdd(&s_acc[EDGE + 1], a_rec);
+        if (b_const) atomicAdd(&s_acc[EDGE + 2], b_const);
+        if (b_rec) atomicAdd(&s_acc[EDGE + 3], b_rec);
     }
     __syncthreads();
     for (int k = threadIdx.x; k < WIDTH; k += 256)
@@ -720,25 +734,32 @@ void rate_inputs_launch(const RateStats *d_stats, int nchunks, const float *d_eb
 uint64_t octave_workspace_bytes() { return 256; }
 
 void rate_octave_launch(const void *x, uint64_t n, int w, int bs, int reqexp, uint8_t *ws,
-                        unsigned long long *d_acc, hipStream_t st, bool edges)
+                        unsigned long long *d_acc, hipStream_t st, bool edges, int nocts)
 {
     (void)ws;
-    const int width = OCT_ACC + (edges ? 4 : 0);
+    const int width = nocts * OCT_ACC + (edges || nocts == 2 ? 4 : 0);
     hipLaunchKernelGGL(k_fill_u64, dim3((width + 255) / 256), dim3(256), 0, st, d_acc, width);
     const uint64_t nb = (n + bs - 1) / bs;
     uint64_t g = (nb + 4 * PF - 1) / (4 * PF);
     if (g < 1) g = 1;
     if (g > OCT_WG) g = OCT_WG;
+    const dim3 grid((unsigned)g), blk(256);
     const float *xf = static_cast<const float *>(x);
     const double *xd = static_cast<const double *>(x);
-    if (w == 4 && edges)
-        hipLaunchKernelGGL((k_rate_octave<float, true>), dim3((unsigned)g), dim3(256), 0, st, xf, n, bs, reqexp, d_acc);
-    else if (w == 4)
-        hipLaunchKernelGGL((k_rate_octave<float, false>), dim3((unsigned)g), dim3(256), 0, st, xf, n, bs, reqexp, d_acc);
-    else if (edges)
-        hipLaunchKernelGGL((k_rate_octave<double, true>), dim3((unsigned)g), dim3(256), 0, st, xd, n, bs, reqexp, d_acc);
-    else
-        hipLaunchKernelGGL((k_rate_octave<double, false>), dim3((unsigned)g), dim3(256), 0, st, xd, n, bs, reqexp, d_acc);
+    if (nocts == 2) {   // the searches' octave pairs always carry the edges
+        if (w == 4)
+            hipLaunchKernelGGL((k_rate_octave<float, true, 2>), grid, blk, 0, st, xf, n, bs, reqexp, d_acc);
+        else
+            hipLaunchKernelGGL((k_rate_octave<double, true, 2>), grid, blk, 0, st, xd, n, bs, reqexp, d_acc);
+    } else if (w == 4 && edges) {
+        hipLaunchKernelGGL((k_rate_octave<float, true, 1>), grid, blk, 0, st, xf, n, bs, reqexp, d_acc);
+    } else if (w == 4) {
+        hipLaunchKernelGGL((k_rate_octave<float, false, 1>), grid, blk, 0, st, xf, n, bs, reqexp, d_acc);
+    } else if (edges) {
+        hipLaunchKernelGGL((k_rate_octave<double, true, 1>), grid, blk, 0, st, xd, n, bs, reqexp, d_acc);
+    } else {
+        hipLaunchKernelGGL((k_rate_octave<double, false, 1>), grid, blk, 0, st, xd, n, bs, reqexp, d_acc);
+    }
 }
 
 }  // namespace szxamd

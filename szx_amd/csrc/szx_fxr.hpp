@@ -32,10 +32,12 @@ void fxr_estimate_launch(const void *x, uint64_t n, int w, int bs, int rate, uin
 // OCT_ACC u64 (fixed bytes, then per radius bin count and record bytes), and
 // with `edges` four more: constant blocks and record bytes at 2^(reqexp+1)
 // and at the float bound below 2^reqexp; ws holds octave_workspace_bytes()
-// (unused scratch, kept for the interface).
+// (unused scratch, kept for the interface).  nocts = 2 sizes the octaves
+// reqexp and reqexp - 1 in the same pass (2 * OCT_ACC u64, reqexp's first;
+// always with the edges, the lower one then below 2^(reqexp-1)).
 uint64_t octave_workspace_bytes();
 void rate_octave_launch(const void *x, uint64_t n, int w, int bs, int reqexp, uint8_t *ws,
-                        unsigned long long *d_acc, hipStream_t st, bool edges = false);
+                        unsigned long long *d_acc, hipStream_t st, bool edges = false, int nocts = 1);
 
 // Per-chunk statistics feeding the rate model (RATE_CHUNK values per chunk;
 // the last chunk may be shorter).  d: min, max, sum x, sum x^2, sum |d1|,

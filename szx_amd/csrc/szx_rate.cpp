@@ -13,9 +13,11 @@
 //    bounds and the predicted field rate picks a bound.  Its exponent names
 //    an octave of bounds; one octave pass (k_rate_octave) gives the exact
 //    stream size for OCT_BINS bounds spread over that octave, so the best
-//    bound in the octave is known exactly before compressing.  If the
-//    target lies outside the octave's sizes the neighbouring octave is
-//    tried (at most OCT_TRIES octave passes), then bisection takes over.
+//    bound in the octave is known exactly before compressing (a prediction
+//    close to an octave edge sizes the octaves on both sides of it in the
+//    same pass).  If the target lies outside those sizes the neighbouring
+//    octave is tried (at most OCT_TRIES octave passes), then bisection
+//    takes over.
 //    The final compress runs at the chosen bound and its size is checked.
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -35,6 +37,7 @@ namespace {
 
 constexpr int K_MAX = 256;      // ladder of candidate bounds (quarter octaves)
 constexpr int OCT_TRIES = 3;
+constexpr double OCT_PAIR = 0.1;   // log2 distance from an octave edge
 constexpr int BISECT_MAX = 60;
 constexpr int STREAM_HEADER = 13;   // szx_float.cc:1091-1098
 
@@ -53,7 +56,7 @@ RateLayout rate_layout(uint64_t n, int w, int bs)
     uint64_t off = 0;
     L.comp = off;     off += al(C.ws_bytes);
     L.oct = off;      off += al(octave_workspace_bytes());
-    L.acc = off;      off += al((OCT_ACC + 4) * 8);
+    L.acc = off;      off += al((2 * OCT_ACC + 4) * 8);
     L.stats_ws = off; off += al(rate_stats_workspace_bytes(n));
     L.stats = off;    off += al(L.nchunks * sizeof(RateStats));
     L.ebs = off;      off += al(K_MAX * 4);
@@ -118,48 +121,59 @@ struct Solver {
         return true;
     }
 
-    // Exact stream sizes for the OCT_BINS bounds of octave e, and for the
-    // bounds just outside it: edge_up at 2^(e+1), edge_dn at the float below
-    // 2^e (eb_dn).
+    // Exact stream sizes for the OCT_BINS bounds of octave e (nocts = 2:
+    // also those of octave e - 1, in the same pass; bpv and ebs then run
+    // from octave e - 1's lowest bound up), and for the bounds just outside
+    // the span: edge_up at 2^(e+1), edge_dn at the float below the span's
+    // lowest bound (eb_dn).
     double edge_up = 0.0, edge_dn = 0.0;
     float eb_up = 0.0f, eb_dn = 0.0f;
-    bool octave(int e, std::vector<double> &bpv, std::vector<float> &ebs)
+    bool octave(int e, int nocts, std::vector<double> &bpv, std::vector<float> &ebs)
     {
-        e = e < -148 ? -148 : (e > 126 ? 126 : e);   // float bounds, both edges inside
-        rate_octave_launch(x, n, w, bs, e, ws + L.oct, (unsigned long long *)(ws + L.acc), st, true);
-        std::vector<unsigned long long> acc(OCT_ACC + 4);
+        // float bounds, both edges inside
+        const int e_min = -149 + nocts;
+        e = e < e_min ? e_min : (e > 126 ? 126 : e);
+        rate_octave_launch(x, n, w, bs, e, ws + L.oct, (unsigned long long *)(ws + L.acc), st, true, nocts);
+        std::vector<unsigned long long> acc((size_t)nocts * OCT_ACC + 4);
         if (!ok(hipMemcpyAsync(acc.data(), ws + L.acc, acc.size() * 8, hipMemcpyDeviceToHost, st),
                 "D2H octave") ||
             !ok(hipStreamSynchronize(st), "octave pass"))
             return false;
         passes++;
         const uint64_t nb = (n + bs - 1) / bs, nbit = (nb + 7) / 8;
-        // size(eb_k) = header + bitmap + fixed + 4 * (count of bins < k)
-        //              + records of bins >= k
-        uint64_t rec_all = 0;
-        for (int k = 0; k < OCT_BINS; k++) rec_all += acc[2 + 2 * k];
-        uint64_t const_part = 0, rec_part = rec_all;
-        bpv.assign(OCT_BINS, 0.0);
-        ebs.assign(OCT_BINS, 0.0f);
-        for (int k = 0; k < OCT_BINS; k++) {
-            const uint64_t size = STREAM_HEADER + nbit + acc[0] + const_part + rec_part;
-            bpv[k] = 8.0 * (double)size / (double)n;
-            ebs[k] = ldexpf(1.0f + (float)k / OCT_BINS, e);
-            const_part += 4 * acc[1 + 2 * k];
-            rec_part -= acc[2 + 2 * k];
+        bpv.assign((size_t)nocts * OCT_BINS, 0.0);
+        ebs.assign((size_t)nocts * OCT_BINS, 0.0f);
+        for (int j = 0; j < nocts; j++) {
+            // octave e - j fills entries [(nocts - 1 - j) * OCT_BINS, ...)
+            const unsigned long long *a = acc.data() + (size_t)j * OCT_ACC;
+            const size_t at = (size_t)(nocts - 1 - j) * OCT_BINS;
+            // size(eb_k) = header + bitmap + fixed + 4 * (count of bins < k)
+            //              + records of bins >= k
+            uint64_t rec_all = 0;
+            for (int k = 0; k < OCT_BINS; k++) rec_all += a[2 + 2 * k];
+            uint64_t const_part = 0, rec_part = rec_all;
+            for (int k = 0; k < OCT_BINS; k++) {
+                const uint64_t size = STREAM_HEADER + nbit + a[0] + const_part + rec_part;
+                bpv[at + k] = 8.0 * (double)size / (double)n;
+                ebs[at + k] = ldexpf(1.0f + (float)k / OCT_BINS, e - j);
+                const_part += 4 * a[1 + 2 * k];
+                rec_part -= a[2 + 2 * k];
+            }
         }
-        edge_up = 8.0 * (double)(STREAM_HEADER + nbit + 4 * acc[OCT_ACC] + acc[OCT_ACC + 1]) / (double)n;
-        edge_dn = 8.0 * (double)(STREAM_HEADER + nbit + 4 * acc[OCT_ACC + 2] + acc[OCT_ACC + 3]) / (double)n;
+        const unsigned long long *g = acc.data() + (size_t)nocts * OCT_ACC;
+        edge_up = 8.0 * (double)(STREAM_HEADER + nbit + 4 * g[0] + g[1]) / (double)n;
+        edge_dn = 8.0 * (double)(STREAM_HEADER + nbit + 4 * g[2] + g[3]) / (double)n;
         eb_up = ldexpf(1.0f, e + 1);
-        eb_dn = nextafterf(ldexpf(1.0f, e), 0.0f);
+        eb_dn = nextafterf(ldexpf(1.0f, e - nocts + 1), 0.0f);
         return true;
     }
 
-    // Octave search from exponent e; on success *eb is the best bound of the
-    // octave that holds the target (or the best seen when none does).
-    // Returns false only on a device error; *found says whether an octave
-    // bracketed the target.
-    bool octave_search(int e, float *eb, bool *found, int *lo_e, int *hi_e)
+    // Octave search from the octave(s) e (nocts = 2: e and e - 1) on; on
+    // success *eb is the best bound of the span that holds the target (or
+    // the best seen when none does).  Later tries size one octave each, next
+    // to the span tried before.  Returns false only on a device error;
+    // *found says whether a span bracketed the target.
+    bool octave_search(int e, int nocts, float *eb, bool *found, int *lo_e, int *hi_e)
     {
         *found = false;
         *lo_e = -100000;
@@ -167,30 +181,31 @@ struct Solver {
         std::vector<double> bpv;
         std::vector<float> ebs;
         int last_dir = 0;           // +1: moved to looser bounds, -1: tighter
-        float edge_eb = 0.0f;       // nearest bound of the previous octave
+        float edge_eb = 0.0f;       // nearest bound of the previous span
         double edge_bpv = -1.0;
         for (int t = 0; t < OCT_TRIES; t++) {
-            if (!octave(e, bpv, ebs)) return false;
+            if (!octave(e, nocts, bpv, ebs)) return false;
+            const int nk = (int)bpv.size();
             // bpv is non-increasing in k
             int bk = 0;
-            for (int k = 1; k < OCT_BINS; k++)
+            for (int k = 1; k < nk; k++)
                 if (fabs(bpv[k] - target) < fabs(bpv[bk] - target)) bk = k;
             int dir = 0;
-            if (bpv[OCT_BINS - 1] > target * (1.0 + tol)) dir = +1;   // bounds too tight
-            else if (bpv[0] < target * (1.0 - tol)) dir = -1;         // bounds too loose
+            if (bpv[nk - 1] > target * (1.0 + tol)) dir = +1;   // bounds too tight
+            else if (bpv[0] < target * (1.0 - tol)) dir = -1;   // bounds too loose
             if (dir == 0) {
                 *eb = ebs[bk];
                 *found = true;
                 return true;
             }
-            // the bound just past the octave edge in that direction: the
+            // the bound just past the span's edge in that direction: the
             // answer itself, or the far side of a jump that holds the target
             // (no bound reaches it: take the nearer side), or else the next
             // octave is needed
             const double edge = dir > 0 ? edge_up : edge_dn;
             const float eb_edge = dir > 0 ? eb_up : eb_dn;
-            const double near = dir > 0 ? bpv[OCT_BINS - 1] : bpv[0];
-            const float eb_near = dir > 0 ? ebs[OCT_BINS - 1] : ebs[0];
+            const double near = dir > 0 ? bpv[nk - 1] : bpv[0];
+            const float eb_near = dir > 0 ? ebs[nk - 1] : ebs[0];
             if (within(edge)) {
                 *eb = eb_edge;
                 *found = true;
@@ -216,9 +231,11 @@ struct Solver {
                 *lo_e = e + 1 > *lo_e ? e + 1 : *lo_e;
                 e++;
             } else {
-                *hi_e = e < *hi_e ? e : *hi_e;
-                e--;
+                const int bottom = e - nocts + 1;
+                *hi_e = bottom < *hi_e ? bottom : *hi_e;
+                e = bottom - 1;
             }
+            nocts = 1;
         }
         return true;
     }
@@ -336,10 +353,15 @@ int fixed_rate_solve(const void *d_in, uint64_t n, int w, int bs, float target, 
                 break;
             }
         }
+        // a prediction within OCT_PAIR of an octave edge sizes the octaves
+        // on both sides of that edge in its first pass
         const int e0 = (int)floor(lg_eb);
+        const double frac = lg_eb - e0;
+        const int e_top = frac < OCT_PAIR ? e0 : (frac > 1.0 - OCT_PAIR ? e0 + 1 : e0);
+        const int nocts = frac < OCT_PAIR || frac > 1.0 - OCT_PAIR ? 2 : 1;
         bool found = false;
         int lo_e, hi_e;
-        if (!S.octave_search(e0, &eb, &found, &lo_e, &hi_e)) {
+        if (!S.octave_search(e_top, nocts, &eb, &found, &lo_e, &hi_e)) {
             *err = S.err;
             return -1;
         }

@@ -95,6 +95,35 @@ def test_rate_octave_is_exact(dtype, device):
         assert np.all(np.diff(sizes) <= 0)
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_rate_octave_span_is_exact(dtype, device):
+    """One pass over an octave pair gives each octave's single-pass sizes,
+    and the bounds just outside the pair size like the oracle's streams."""
+    import torch
+    r = np.random.default_rng(16)
+    n = 150001
+    t = np.arange(n)
+    x = (np.cos(t * 0.002) * 20 + r.standard_normal(n) * 3e-3).astype(dtype)
+    x[20000:60000] = -2.25
+    xt = torch.from_numpy(x).to(device)
+    top = int(np.floor(np.log2(np.ptp(x))))
+    for e in (top - 5, top - 12):
+        for nocts in (1, 2):
+            say("octave span", dtype.__name__, e, nocts)
+            sizes, ebs, up, dn = Z.rate_octave_span(xt, e, nocts, 128)
+            assert sizes.size == nocts * Z.OCTAVE_BINS
+            for j in range(nocts):
+                s1, e1 = Z.rate_octave(xt, e - j, 128)
+                at = (nocts - 1 - j) * Z.OCTAVE_BINS
+                assert np.array_equal(sizes[at:at + Z.OCTAVE_BINS], s1), (e, j)
+                assert np.array_equal(ebs[at:at + Z.OCTAVE_BINS], e1), (e, j)
+            assert np.all(np.diff(sizes) <= 0)
+            for size, eb in (up, dn):
+                assert size == len(O.compress(x, float(eb), 128)), (e, nocts, float(eb))
+            assert sizes[0] == len(O.compress(x, float(ebs[0]), 128))
+            assert sizes[-1] == len(O.compress(x, float(ebs[-1]), 128))
+
+
 def test_rate_stats_match_numpy(device):
     import torch
     r = np.random.default_rng(15)
