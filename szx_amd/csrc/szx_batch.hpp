@@ -31,6 +31,12 @@ struct CompressParams {
     uint64_t nidx;        // index tiles
     uint16_t *osz;        // random-access container: record size by non-constant rank, or nullptr
     int fm4;              // fastMode-4 fp64 block statistics (radius rounded to float first)
+    // REL with the range pass fused into the launch: the pass's per-workgroup
+    // {min, max} (RANGE_LB_PARTS pairs of the field's type), folded into the
+    // bound by every workgroup; nullptr otherwise
+    const void *rparts;
+    float rel;            // the bound ratio
+    float *eb_out;        // where tile 0 leaves the bound for the assembly
 };
 
 // One field's header / bitmap / median assembly (device pointers).
@@ -50,6 +56,10 @@ struct AssembleParams {
     uint64_t nb;          // blocks (random-access container)
     const uint16_t *osz;  // random-access container: record sizes
 };
+
+// Workgroups (of 1024 lanes) of the single-field REL range pass whose
+// partials the compressor folds itself.
+constexpr int RANGE_LB_PARTS = 256;
 
 // Values per REL range item of the batched range pass.
 constexpr uint64_t RANGE_ITEM = 1ull << 20;

@@ -337,26 +337,27 @@ template <typename T, int BS, int LV, int SLABS> struct CompressLds {
     unsigned long long excl[2];   // the tile's exclusive (bytes, constants)
 };
 
-// Issue every load of a tile (values past n read as zero and are masked).
-template <typename T, int LV, int SLABS>
+// Issue every load of a tile (values past n read as zero and are masked);
+// FULL: the whole tile exists (plain 16-byte loads).
+template <typename T, int LV, int SLABS, bool FULL>
 __device__ __forceinline__ void load_tile(const CompressParams &p, uint64_t tile, T (&v)[SLABS][LV])
 {
     constexpr int SLAB = 256 * LV;
     const T *x = static_cast<const T *>(p.x);
     const uint64_t t0 = tile * (uint64_t)(SLAB * SLABS);
-    if (t0 + (uint64_t)(SLAB * SLABS) <= p.n) {   // the whole tile exists: plain 16-byte loads
+    if constexpr (FULL) {
         const T *xt = x + t0 + (uint64_t)LV * threadIdx.x;
 #pragma unroll
         for (int u = 0; u < SLABS; u++)
 #pragma unroll
             for (int q = 0; q < LV; q += 4) SlabLoad<T>::load4_full(xt + u * SLAB + q, &v[u][q]);
-        return;
+    } else {
+#pragma unroll
+        for (int u = 0; u < SLABS; u++)
+#pragma unroll
+            for (int q = 0; q < LV; q += 4)
+                SlabLoad<T>::load4(x, t0 + (uint64_t)u * SLAB + (uint64_t)LV * threadIdx.x + q, p.n, &v[u][q]);
     }
-#pragma unroll
-    for (int u = 0; u < SLABS; u++)
-#pragma unroll
-        for (int q = 0; q < LV; q += 4)
-            SlabLoad<T>::load4(x, t0 + (uint64_t)u * SLAB + (uint64_t)LV * threadIdx.x + q, p.n, &v[u][q]);
 }
 
 // Block statistics and records of a tile whose values are in v, in
@@ -364,7 +365,8 @@ __device__ __forceinline__ void load_tile(const CompressParams &p, uint64_t tile
 // M for the scan.
 template <typename T, int BS, int LV, int SLABS>
 __device__ __forceinline__ void encode_slabs(const CompressParams &p, uint64_t tile, const T (&v)[SLABS][LV],
-                                             SlabState<T, LV> (&st)[SLABS], TileMeta<T, BS, LV, SLABS> &M)
+                                             SlabState<T, LV> (&st)[SLABS], TileMeta<T, BS, LV, SLABS> &M,
+                                             float eb)
 {
     using S = TileShape<T, BS, LV, SLABS>;
     constexpr int SEG = BS / LV;          // lanes per block
@@ -374,7 +376,6 @@ __device__ __forceinline__ void encode_slabs(const CompressParams &p, uint64_t t
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int lseg = tid & (SEG - 1);
-    const float eb = p.eb_dev ? *p.eb_dev : p.eb;
     const int reqexp = req_expo(eb);
     const uint64_t n = p.n;
     const uint64_t t0 = tile * TILE;
@@ -406,7 +407,7 @@ __device__ __forceinline__ void encode_slabs(const CompressParams &p, uint64_t t
 template <typename T, int BS, int LV, int SLABS, int IPT>
 __device__ __forceinline__ void encode_scan(const CompressParams &p, const LookbackRefs &L, uint64_t tile,
                                             const T (&v)[SLABS][LV], SlabState<T, LV> (&st)[SLABS],
-                                            TileMeta<T, BS, LV, SLABS> &M, unsigned long long &pend)
+                                            TileMeta<T, BS, LV, SLABS> &M, unsigned long long &pend, float eb)
 {
     using S = TileShape<T, BS, LV, SLABS>;
     constexpr int NBLK = S::NBLK;
@@ -415,7 +416,7 @@ __device__ __forceinline__ void encode_scan(const CompressParams &p, const Lookb
     static_assert((NBLK / IPT) % 8 == 0, "index tiles own whole bitmap bytes");
     static_assert(S::STAGE / IPT < (1 << 16), "record starts in an index tile fit the 16-bit block index");
     const int lane = threadIdx.x & 63;
-    encode_slabs<T, BS, LV, SLABS>(p, tile, v, st, M);
+    encode_slabs<T, BS, LV, SLABS>(p, tile, v, st, M, eb);
     __syncthreads();
 
     if (threadIdx.x < 64) {
@@ -462,27 +463,36 @@ __device__ __forceinline__ void stage_slab(const SlabState<T, LV> &st, int u, co
     if constexpr (LDS && sizeof(U) == 4) {
         // Mid bytes OR-ed into the zeroed stage as aligned words: value k's
         // m bytes (the low bytes of mid[k], zero above) start at bit offset
-        // o8 of the stage, so the 64-bit word mid[k] << (o8 mod 32) covers the
-        // stage words it touches: the first one if m > 0, the second one only
-        // if the bytes cross into it.  LDS atomics are the expensive part here
+        // o8 of the stage.  LDS atomics are the expensive part here
         // (measured: two unconditional ORs per value ran 0.28 ms against 0.22
-        // ms for these predicated ones at 512^3), so the lanes with nothing to
-        // add sit out.  OR is idempotent and every stage byte has one writer,
-        // so the words a run shares with its neighbours need no care.
+        // ms for predicated ones at 512^3), so the lanes with nothing to add
+        // sit out.  OR is idempotent and every stage byte has one writer, so
+        // the words a run shares with its neighbours need no care.
         uint32_t *sw = reinterpret_cast<uint32_t *>(s_stage);
         uint32_t o8 = 8u * (base + 5 + nlead + (meta & 0xFFFu));
+        static_assert(LV % 2 == 0, "values are staged in pairs");
 #pragma unroll
-        for (int k = 0; k < LV; k++) {
-            const uint32_t m8 = ((st.mb >> (4 * k)) & 0xFu) << 3;
-            const uint64_t sh = (uint64_t)(uint32_t)st.mid[k] << (o8 & 31u);
-            uint32_t *pw = sw + (o8 >> 5);
-            if (m8) {
+        for (int k = 0; k < LV; k += 2) {
+            // two neighbouring values as one run of mp <= 64 bits (each mid
+            // word is zero above its own bytes), OR-ed into the one to three
+            // stage words it touches
+            const uint32_t ma = ((st.mb >> (4 * k)) & 0xFu) << 3;
+            const uint32_t mp = ma + (((st.mb >> (4 * k + 4)) & 0xFu) << 3);
+            if (mp) {
+                const uint64_t run = (uint64_t)(uint32_t)st.mid[k] | ((uint64_t)(uint32_t)st.mid[k + 1] << ma);
+                const uint32_t s = o8 & 31u;
+                const uint64_t sh = run << s;
+                uint32_t *pw = sw + (o8 >> 5);
                 __hip_atomic_fetch_or(pw, (uint32_t)sh, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                if ((o8 & 31u) + m8 > 32u)
+                if (s + mp > 32u)
                     __hip_atomic_fetch_or(pw + 1, (uint32_t)(sh >> 32), __ATOMIC_RELAXED,
                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                // bits past 64 (only when s > 0): the run's top s bits
+                if (s + mp > 64u)
+                    __hip_atomic_fetch_or(pw + 2, __builtin_amdgcn_alignbit(0u, (uint32_t)(run >> 32), 32u - s),
+                                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             }
-            o8 += m8;
+            o8 += mp;
         }
     } else {
         uint8_t *q = s_stage + base + 5 + nlead + (meta & 0xFFFu);
@@ -622,6 +632,57 @@ __device__ __forceinline__ void copy_out(const CompressParams &p, uint64_t tile,
     }
 }
 
+// REL with the range pass fused into the launch: every wave folds the pass's
+// RANGE_LB_PARTS {min, max} pairs (four per lane) into the bound itself, so
+// no barrier is needed.  The loads are issued before the tile's and the
+// fold follows them in the same block, so it waits for its own loads only.
+template <typename T> struct RelParts {
+    T mn[4], mx[4];
+    T first;
+    __device__ void load(const CompressParams &p)
+    {
+        static_assert(RANGE_LB_PARTS == 4 * 64, "four range partials per lane");
+        const T *q = static_cast<const T *>(p.rparts) + 8 * (threadIdx.x & 63);
+        T t[8];
+        SlabLoad<T>::load4_full(q, t);
+        SlabLoad<T>::load4_full(q + 4, t + 4);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            mn[k] = t[2 * k];
+            mx[k] = t[2 * k + 1];
+        }
+        first = static_cast<const T *>(p.x)[0];
+    }
+    // Min and max skip NaNs (the partials hold none), a NaN x[0] makes the
+    // range NaN, and the bound is rounded as the reference rounds it:
+    // fl32(range * rel) for fp32, (float)(range * (double)rel) for fp64
+    // (szx.cc:249-253, 303-308).
+    __device__ float fold(float rel) const
+    {
+        using U = typename Word<T>::U;
+        T a = mn[0], b = mx[0];
+#pragma unroll
+        for (int k = 1; k < 4; k++) {
+            a = mn[k] < a ? mn[k] : a;
+            b = mx[k] > b ? mx[k] : b;
+        }
+        a = from_bits<T>(seg_reduce<64>(to_bits<T>(a), [](U x, U y) {
+            const T fx = from_bits<T>(x), fy = from_bits<T>(y);
+            return fy < fx ? y : x;
+        }));
+        b = from_bits<T>(seg_reduce<64>(to_bits<T>(b), [](U x, U y) {
+            const T fx = from_bits<T>(x), fy = from_bits<T>(y);
+            return fy > fx ? y : x;
+        }));
+        if (first != first) {
+            a = first;
+            b = first;
+        }
+        const T range = b - a;
+        return (float)(range * (T)rel);
+    }
+};
+
 // Finish a tile's look-back from its polled words (wave 0) and leave its
 // exclusive prefix in ex[0..1] for the workgroup (read after a barrier).  On a
 // wait that overran its bound the call is flagged failed and the prefix is
@@ -651,13 +712,31 @@ __device__ __forceinline__ void compress_tile(const CompressParams &p, uint64_t 
     const LookbackRefs L{p.tiles, p.chunks, p.cacc, p.ntiles};
     T v[SLABS][LV];
     unsigned long long pend = 0;
-    load_tile<T, LV, SLABS>(p, tile, v);
+    float eb = p.eb;
+    {
+        constexpr int TILE = 256 * LV * SLABS;
+        RelParts<T> rp;
+        if (tile * (uint64_t)TILE + TILE <= p.n) {
+            if (p.rparts) rp.load(p);
+            load_tile<T, LV, SLABS, true>(p, tile, v);
+            if (p.rparts) eb = rp.fold(p.rel);
+        } else {
+            if (p.rparts) rp.load(p);
+            load_tile<T, LV, SLABS, false>(p, tile, v);
+            if (p.rparts) eb = rp.fold(p.rel);
+        }
+    }
+    if (p.rparts) {
+        if (tile == 0 && threadIdx.x == 0) *p.eb_out = eb;
+    } else if (p.eb_dev) {
+        eb = *p.eb_dev;
+    }
     if constexpr (sizeof(T) == 4) {   // the fp32 stage is OR-ed into (stage_slab)
         constexpr int N4 = TileShape<T, BS, LV, SLABS>::STAGE_DW / 4;
         for (int i = threadIdx.x; i < N4; i += 256) reinterpret_cast<uint4 *>(lds.stage.w)[i] = make_uint4(0, 0, 0, 0);
     }
     SlabState<T, LV> st[SLABS];
-    encode_scan<T, BS, LV, SLABS, IPT>(p, L, tile, v, st, lds.meta, pend);
+    encode_scan<T, BS, LV, SLABS, IPT>(p, L, tile, v, st, lds.meta, pend, eb);
     // wave 0 issues the look-back reads halfway through staging: the words it
     // reads appear about a memory round trip after their tiles publish them,
     // and the rest of staging covers the round trip
@@ -678,7 +757,6 @@ __device__ __forceinline__ void compress_tile(const CompressParams &p, uint64_t 
         // final offsets
         constexpr int SLAB = 256 * LV;
         const uint64_t t0 = tile * (uint64_t)(SLAB * SLABS);
-        const float eb = p.eb_dev ? *p.eb_dev : p.eb;
         const int reqexp = req_expo(eb);
         const int lseg = threadIdx.x & (BS / LV - 1);
         for (int u = 0; u < SLABS; u++) {
@@ -776,8 +854,12 @@ __global__ void k_assemble(AssembleParams a)
 // read, so the compressor finds up to the last-level cache's worth of it
 // there (REL step 0.317 -> 0.305 ms at 512^3 fp32).  Min and max do not
 // depend on the order (NaNs lose every comparison either way).
-template <typename T>
-__global__ __launch_bounds__(256) void k_range_partial(const T *x, uint64_t n, T *part)
+// NT lanes per workgroup; `zero` (if set) has its nzero words cleared along
+// the way (the compressor's look-back state, when the pass is part of a
+// compress launch).
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void k_range_partial(const T *x, uint64_t n, T *part,
+                                                      unsigned long long *zero, uint64_t nzero)
 {
     T mn = pos_inf<T>(), mx = -pos_inf<T>();
     const uint64_t nq = n / 4;
@@ -815,12 +897,15 @@ __global__ __launch_bounds__(256) void k_range_partial(const T *x, uint64_t n, T
         o = __shfl_xor(mx, m, 64);
         mx = o > mx ? o : mx;
     }
-    __shared__ T sm[2][4];
+    if (zero)
+        for (uint64_t i = blockIdx.x * (uint64_t)NT + threadIdx.x; i < nzero; i += (uint64_t)gridDim.x * NT)
+            zero[i] = 0;
+    __shared__ T sm[2][NT / 64];
     const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) { sm[0][w] = mn; sm[1][w] = mx; }
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int k = 1; k < 4; k++) {
+        for (int k = 1; k < NT / 64; k++) {
             mn = sm[0][k] < mn ? sm[0][k] : mn;
             mx = sm[1][k] > mx ? sm[1][k] : mx;
         }
@@ -1095,14 +1180,14 @@ void range_bound_launch(const void *d_in, uint64_t n, int w, float rel, uint8_t 
 {
     if (w == 4) {
         float *part = reinterpret_cast<float *>(d_ws);
-        hipLaunchKernelGGL(k_range_partial<float>, dim3(kRangeParts), dim3(256), 0, st,
-                           static_cast<const float *>(d_in), n, part);
+        hipLaunchKernelGGL((k_range_partial<float, 256>), dim3(kRangeParts), dim3(256), 0, st,
+                           static_cast<const float *>(d_in), n, part, nullptr, 0);
         hipLaunchKernelGGL(k_range_final<float>, dim3(1), dim3(256), 0, st, static_cast<const float *>(d_in),
                            part, kRangeParts, rel, eb_out, static_cast<float *>(d_minmax));
     } else {
         double *part = reinterpret_cast<double *>(d_ws);
-        hipLaunchKernelGGL(k_range_partial<double>, dim3(kRangeParts), dim3(256), 0, st,
-                           static_cast<const double *>(d_in), n, part);
+        hipLaunchKernelGGL((k_range_partial<double, 256>), dim3(kRangeParts), dim3(256), 0, st,
+                           static_cast<const double *>(d_in), n, part, nullptr, 0);
         hipLaunchKernelGGL(k_range_final<double>, dim3(1), dim3(256), 0, st, static_cast<const double *>(d_in),
                            part, kRangeParts, rel, eb_out, static_cast<double *>(d_minmax));
     }
@@ -1112,15 +1197,15 @@ void range_launch(const void *d_in, uint64_t n, int w, uint8_t *d_ws, void *d_mi
 {
     if (w == 4) {
         float *part = reinterpret_cast<float *>(d_ws);
-        hipLaunchKernelGGL(k_range_partial<float>, dim3(kRangeParts), dim3(256), 0, st,
-                           static_cast<const float *>(d_in), n, part);
+        hipLaunchKernelGGL((k_range_partial<float, 256>), dim3(kRangeParts), dim3(256), 0, st,
+                           static_cast<const float *>(d_in), n, part, nullptr, 0);
         hipLaunchKernelGGL(k_range_final<float>, dim3(1), dim3(256), 0, st,
                            static_cast<const float *>(d_in), part, kRangeParts, 0.0f,
                            (float *)nullptr, static_cast<float *>(d_minmax));
     } else {
         double *part = reinterpret_cast<double *>(d_ws);
-        hipLaunchKernelGGL(k_range_partial<double>, dim3(kRangeParts), dim3(256), 0, st,
-                           static_cast<const double *>(d_in), n, part);
+        hipLaunchKernelGGL((k_range_partial<double, 256>), dim3(kRangeParts), dim3(256), 0, st,
+                           static_cast<const double *>(d_in), n, part, nullptr, 0);
         hipLaunchKernelGGL(k_range_final<double>, dim3(1), dim3(256), 0, st,
                            static_cast<const double *>(d_in), part, kRangeParts, 0.0f,
                            (float *)nullptr, static_cast<double *>(d_minmax));
@@ -1136,22 +1221,38 @@ int compress_launch(const void *d_in, uint64_t n, int w, int rel_mode, float abs
     const CompressLayout L = compress_layout(n, w, bs);
     unsigned long long *misc = reinterpret_cast<unsigned long long *>(d_ws + L.ws_misc);
     const uint64_t zero_words = (L.ws_misc + 64 * sizeof(uint64_t)) / 8;
-    hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((zero_words + 255) / 256 < 1024 ? (zero_words + 255) / 256 : 1024)),
-                       dim3(256), 0, st, reinterpret_cast<unsigned long long *>(d_ws), zero_words);
-    float *eb_dev = nullptr;
-    if (rel_mode) {
+    unsigned long long *zero = reinterpret_cast<unsigned long long *>(d_ws);
+    float *eb_dev = nullptr;        // REL: the bound, computed on the device
+    const void *rparts = nullptr;   // REL, tiled: the range partials k_compress folds
+    if (rel_mode && bs_tiled(bs)) {
+        // one launch for the range partials and the clear; every compressor
+        // workgroup folds the partials into the bound itself
+        eb_dev = reinterpret_cast<float *>(misc + 32);
+        rparts = d_ws + L.ws_parts;
+        if (w == 4)
+            hipLaunchKernelGGL((k_range_partial<float, 1024>), dim3(RANGE_LB_PARTS), dim3(1024), 0, st,
+                               static_cast<const float *>(d_in), n, (float *)(d_ws + L.ws_parts), zero, zero_words);
+        else
+            hipLaunchKernelGGL((k_range_partial<double, 1024>), dim3(RANGE_LB_PARTS), dim3(1024), 0, st,
+                               static_cast<const double *>(d_in), n, (double *)(d_ws + L.ws_parts), zero,
+                               zero_words);
+    } else {
+        hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((zero_words + 255) / 256 < 1024 ? (zero_words + 255) / 256 : 1024)),
+                           dim3(256), 0, st, zero, zero_words);
+    }
+    if (rel_mode && !rparts) {
         eb_dev = reinterpret_cast<float *>(misc + 32);
         if (w == 4) {
             float *part = reinterpret_cast<float *>(d_ws + L.ws_parts);
-            hipLaunchKernelGGL(k_range_partial<float>, dim3(kRangeParts), dim3(256), 0, st,
-                               static_cast<const float *>(d_in), n, part);
+            hipLaunchKernelGGL((k_range_partial<float, 256>), dim3(kRangeParts), dim3(256), 0, st,
+                               static_cast<const float *>(d_in), n, part, nullptr, 0);
             hipLaunchKernelGGL(k_range_final<float>, dim3(1), dim3(256), 0, st,
                                static_cast<const float *>(d_in), part, kRangeParts, rel, eb_dev,
                                (float *)nullptr);
         } else {
             double *part = reinterpret_cast<double *>(d_ws + L.ws_parts);
-            hipLaunchKernelGGL(k_range_partial<double>, dim3(kRangeParts), dim3(256), 0, st,
-                               static_cast<const double *>(d_in), n, part);
+            hipLaunchKernelGGL((k_range_partial<double, 256>), dim3(kRangeParts), dim3(256), 0, st,
+                               static_cast<const double *>(d_in), n, part, nullptr, 0);
             hipLaunchKernelGGL(k_range_final<double>, dim3(1), dim3(256), 0, st,
                                static_cast<const double *>(d_in), part, kRangeParts, rel, eb_dev,
                                (double *)nullptr);
@@ -1161,7 +1262,10 @@ int compress_launch(const void *d_in, uint64_t n, int w, int rel_mode, float abs
     p.x = d_in;
     p.n = n;
     p.eb = abs_eb;
-    p.eb_dev = eb_dev;
+    p.eb_dev = rparts ? nullptr : eb_dev;
+    p.rparts = rparts;
+    p.rel = rel;
+    p.eb_out = rparts ? eb_dev : nullptr;
     p.payload = d_out + L.region;
     p.bitmap = d_ws + L.ws_bitmap;
     p.cmed = reinterpret_cast<uint32_t *>(d_ws + L.ws_cmed);
