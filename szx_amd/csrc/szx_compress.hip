@@ -27,6 +27,8 @@
 // stream therefore starts at a data-dependent offset inside the output
 // buffer, which is returned with the size.  This keeps HBM traffic at one read
 // of the input plus one write of the stream.
+#include <type_traits>
+
 #include "szx_common.hpp"
 #include "szx_batch.hpp"
 #include "szx_internal.hpp"
@@ -110,10 +112,17 @@ __device__ inline uint32_t wave0_exclusive_scan(uint32_t *a, int lane)
 //   code  = the lane's 2-bit lead codes (LV/4 bytes) | lead bytes of the
 //           block (7) << 16
 //   mb    = per value 8 x (mid bytes), one byte each
+// fp32 keeps the mid bytes of two neighbouring values as one run (PAIRS):
+// value 2p's bytes, then value 2p+1's, as the low bytes of a 64-bit word
+// (zero above), so staging handles a pair at a time.
 template <typename T, int LV> struct SlabState {
-    static_assert(LV * 4 <= 32, "a 4-bit mid-byte count per value fits one word");
-    typename Word<T>::U mid[LV];
-    uint32_t mb;   // mid bytes of value k (0..8) in bits 4k..4k+3
+    static constexpr bool PAIRS = sizeof(T) == 4;
+    static constexpr int NM = PAIRS ? LV / 2 : LV;
+    static_assert(NM * 4 <= 32, "a 4-bit mid-byte count per run fits one word");
+    static_assert(!PAIRS || LV % 2 == 0, "values pair up");
+    using Mid = typename std::conditional<PAIRS, uint64_t, typename Word<T>::U>::type;
+    Mid mid[NM];
+    uint32_t mb;   // mid bytes of run r (0..8) in bits 4r..4r+3
     uint32_t meta, code;
     float hmed;
 };
@@ -240,6 +249,8 @@ __device__ __forceinline__ void encode_slab(const T (&v)[LV], uint64_t e0, uint6
     const int rb8 = 8 * rb, lo8 = 8 * W - rb8;
     uint32_t codex = 0, tot8 = 0;
     uint32_t mb = 0;
+    U mid_even = 0;      // PAIRS: the pair's first value, its mid bits
+    uint32_t m8_even = 0;
 #pragma unroll
     for (int k = 0; k < LV; k++) {
         const int ld8 = lead8<U>(w[k] ^ prev);
@@ -260,9 +271,19 @@ __device__ __forceinline__ void encode_slab(const T (&v)[LV], uint64_t e0, uint6
         // element k -> byte k/4 of the lane's lead codes, bits 7-6 first
         // (8 ld here; the sum is divided by 8 below)
         codex |= ldv << (8 * (k >> 2) + 6 - 2 * (k & 3));
-        mb |= ((uint32_t)m8 >> 3) << (4 * k);
         tot8 += (uint32_t)m8;
-        st.mid[k] = mid;
+        if constexpr (SlabState<T, LV>::PAIRS) {
+            if ((k & 1) == 0) {
+                mid_even = mid;
+                m8_even = (uint32_t)m8;
+            } else {
+                st.mid[k >> 1] = (uint64_t)(uint32_t)mid_even | ((uint64_t)(uint32_t)mid << m8_even);
+                mb |= ((m8_even + (uint32_t)m8) >> 3) << (4 * (k >> 1));
+            }
+        } else {
+            st.mid[k] = mid;
+            mb |= ((uint32_t)m8 >> 3) << (4 * k);
+        }
         prev = w[k];
     }
     const uint32_t code = codex >> 3;
@@ -470,16 +491,13 @@ __device__ __forceinline__ void stage_slab(const SlabState<T, LV> &st, int u, co
         // the words a run shares with its neighbours need no care.
         uint32_t *sw = reinterpret_cast<uint32_t *>(s_stage);
         uint32_t o8 = 8u * (base + 5 + nlead + (meta & 0xFFFu));
-        static_assert(LV % 2 == 0, "values are staged in pairs");
 #pragma unroll
-        for (int k = 0; k < LV; k += 2) {
-            // two neighbouring values as one run of mp <= 64 bits (each mid
-            // word is zero above its own bytes), OR-ed into the one to three
-            // stage words it touches
-            const uint32_t ma = ((st.mb >> (4 * k)) & 0xFu) << 3;
-            const uint32_t mp = ma + (((st.mb >> (4 * k + 4)) & 0xFu) << 3);
+        for (int r = 0; r < LV / 2; r++) {
+            // two neighbouring values as one run of mp <= 64 bits (zero
+            // above), OR-ed into the one to three stage words it touches
+            const uint32_t mp = ((st.mb >> (4 * r)) & 0xFu) << 3;
             if (mp) {
-                const uint64_t run = (uint64_t)(uint32_t)st.mid[k] | ((uint64_t)(uint32_t)st.mid[k + 1] << ma);
+                const uint64_t run = st.mid[r];
                 const uint32_t s = o8 & 31u;
                 const uint64_t sh = run << s;
                 uint32_t *pw = sw + (o8 >> 5);
@@ -497,9 +515,9 @@ __device__ __forceinline__ void stage_slab(const SlabState<T, LV> &st, int u, co
     } else {
         uint8_t *q = s_stage + base + 5 + nlead + (meta & 0xFFFu);
 #pragma unroll
-        for (int k = 0; k < LV; k++) {
-            const int m = (int)(st.mb >> (4 * k)) & 0xF;
-            stage_bytes<U>(q, st.mid[k], m);
+        for (int r = 0; r < SlabState<T, LV>::NM; r++) {
+            const int m = (int)(st.mb >> (4 * r)) & 0xF;
+            stage_bytes<typename SlabState<T, LV>::Mid>(q, st.mid[r], m);
             q += m;
         }
     }
