@@ -118,11 +118,13 @@ __device__ inline uint32_t wave0_exclusive_scan(uint32_t *a, int lane)
 template <typename T, int LV> struct SlabState {
     static constexpr bool PAIRS = sizeof(T) == 4;
     static constexpr int NM = PAIRS ? LV / 2 : LV;
-    static_assert(NM * 4 <= 32, "a 4-bit mid-byte count per run fits one word");
+    static_assert(PAIRS ? NM <= 4 : NM * 4 <= 32, "the mid counts fit one word");
     static_assert(!PAIRS || LV % 2 == 0, "values pair up");
     using Mid = typename std::conditional<PAIRS, uint64_t, typename Word<T>::U>::type;
     Mid mid[NM];
-    uint32_t mb;   // mid bytes of run r (0..8) in bits 4r..4r+3
+    // PAIRS: mid bits of run r (0..64) in byte r; else mid bytes of value k
+    // (0..8) in bits 4k..4k+3
+    uint32_t mb;
     uint32_t meta, code;
     float hmed;
 };
@@ -254,8 +256,8 @@ __device__ __forceinline__ void encode_slab(const T (&v)[LV], uint64_t e0, uint6
 #pragma unroll
     for (int k = 0; k < LV; k++) {
         const int ld8 = lead8<U>(w[k] ^ prev);
-        int m8 = rb8 - ld8;
-        m8 = m8 > 0 ? m8 : 0;
+        // max(rb - ld, 0) mid bytes (x 8): one clamped subtract
+        uint32_t m8 = __builtin_elementwise_sub_sat((uint32_t)rb8, (uint32_t)ld8);
         // the value's mid bytes as the low bytes of `mid`; fp32 keeps exactly
         // those m bytes (the bytes above them are zero), so the stage can OR
         // the word in without a mask
@@ -271,18 +273,20 @@ __device__ __forceinline__ void encode_slab(const T (&v)[LV], uint64_t e0, uint6
         // element k -> byte k/4 of the lane's lead codes, bits 7-6 first
         // (8 ld here; the sum is divided by 8 below)
         codex |= ldv << (8 * (k >> 2) + 6 - 2 * (k & 3));
-        tot8 += (uint32_t)m8;
         if constexpr (SlabState<T, LV>::PAIRS) {
             if ((k & 1) == 0) {
                 mid_even = mid;
-                m8_even = (uint32_t)m8;
+                m8_even = m8;
             } else {
                 st.mid[k >> 1] = (uint64_t)(uint32_t)mid_even | ((uint64_t)(uint32_t)mid << m8_even);
-                mb |= ((m8_even + (uint32_t)m8) >> 3) << (4 * (k >> 1));
+                const uint32_t mp = m8_even + m8;   // the pair's mid bits
+                mb |= mp << (8 * (k >> 1));
+                tot8 += mp;
             }
         } else {
             st.mid[k] = mid;
-            mb |= ((uint32_t)m8 >> 3) << (4 * k);
+            mb |= (m8 >> 3) << (4 * k);
+            tot8 += m8;
         }
         prev = w[k];
     }
@@ -495,7 +499,7 @@ __device__ __forceinline__ void stage_slab(const SlabState<T, LV> &st, int u, co
         for (int r = 0; r < LV / 2; r++) {
             // two neighbouring values as one run of mp <= 64 bits (zero
             // above), OR-ed into the one to three stage words it touches
-            const uint32_t mp = ((st.mb >> (4 * r)) & 0xFu) << 3;
+            const uint32_t mp = (st.mb >> (8 * r)) & 0xFFu;
             if (mp) {
                 const uint64_t run = st.mid[r];
                 const uint32_t s = o8 & 31u;
@@ -516,7 +520,8 @@ __device__ __forceinline__ void stage_slab(const SlabState<T, LV> &st, int u, co
         uint8_t *q = s_stage + base + 5 + nlead + (meta & 0xFFFu);
 #pragma unroll
         for (int r = 0; r < SlabState<T, LV>::NM; r++) {
-            const int m = (int)(st.mb >> (4 * r)) & 0xF;
+            const int m = SlabState<T, LV>::PAIRS ? (int)((st.mb >> (8 * r)) & 0xFFu) >> 3
+                                                  : (int)(st.mb >> (4 * r)) & 0xF;
             stage_bytes<typename SlabState<T, LV>::Mid>(q, st.mid[r], m);
             q += m;
         }
