@@ -486,33 +486,28 @@ __device__ __forceinline__ void stage_slab(const SlabState<T, LV> &st, int u, co
         if ((uint32_t)(lseg * NLB + b) < nlead)
             s_stage[base + 5 + lseg * NLB + b] = (uint8_t)(st.code >> (8 * b));
     if constexpr (LDS && sizeof(U) == 4) {
-        // Mid bytes OR-ed into the zeroed stage as aligned words: value k's
-        // m bytes (the low bytes of mid[k], zero above) start at bit offset
-        // o8 of the stage.  LDS atomics are the expensive part here
+        // Mid bytes OR-ed into the zeroed stage as aligned 8-byte words: a
+        // run's bytes start at bit offset o8 of the stage.  LDS atomics are the expensive part here
         // (measured: two unconditional ORs per value ran 0.28 ms against 0.22
         // ms for predicated ones at 512^3), so the lanes with nothing to add
         // sit out.  OR is idempotent and every stage byte has one writer, so
         // the words a run shares with its neighbours need no care.
-        uint32_t *sw = reinterpret_cast<uint32_t *>(s_stage);
+        static_assert(TileShape<T, BS, LV, SLABS>::STAGE_DW % 2 == 0, "the stage is whole 8-byte words");
+        uint64_t *sq = reinterpret_cast<uint64_t *>(s_stage);
         uint32_t o8 = 8u * (base + 5 + nlead + (meta & 0xFFFu));
 #pragma unroll
         for (int r = 0; r < LV / 2; r++) {
             // two neighbouring values as one run of mp <= 64 bits (zero
-            // above), OR-ed into the one to three stage words it touches
+            // above), OR-ed into the one or two 8-byte stage words it touches
             const uint32_t mp = (st.mb >> (8 * r)) & 0xFFu;
             if (mp) {
                 const uint64_t run = st.mid[r];
-                const uint32_t s = o8 & 31u;
-                const uint64_t sh = run << s;
-                uint32_t *pw = sw + (o8 >> 5);
-                __hip_atomic_fetch_or(pw, (uint32_t)sh, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                if (s + mp > 32u)
-                    __hip_atomic_fetch_or(pw + 1, (uint32_t)(sh >> 32), __ATOMIC_RELAXED,
-                                          __HIP_MEMORY_SCOPE_WORKGROUP);
-                // bits past 64 (only when s > 0): the run's top s bits
+                const uint32_t s = o8 & 63u;
+                uint64_t *pq = sq + (o8 >> 6);
+                __hip_atomic_fetch_or(pq, run << s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                // bits past the first word (only when s > 0): the run's top s
                 if (s + mp > 64u)
-                    __hip_atomic_fetch_or(pw + 2, __builtin_amdgcn_alignbit(0u, (uint32_t)(run >> 32), 32u - s),
-                                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    __hip_atomic_fetch_or(pq + 1, run >> (64u - s), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             }
             o8 += mp;
         }
