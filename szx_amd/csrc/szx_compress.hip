@@ -253,43 +253,53 @@ __device__ __forceinline__ void encode_slab(const T (&v)[LV], uint64_t e0, uint6
     uint32_t mb = 0;
     U mid_even = 0;      // PAIRS: the pair's first value, its mid bits
     uint32_t m8_even = 0;
+    // NARROW: no lane of the wave needs 4 bytes, so every m8 is at most 24
+    // and one bit-field extract gives the mid bits (4 required bytes with no
+    // lead byte, m8 = 32, is the width the extract cannot encode)
+    auto values = [&](auto narrow) {
+        constexpr bool NARROW = decltype(narrow)::value;
 #pragma unroll
-    for (int k = 0; k < LV; k++) {
-        const int ld8 = lead8<U>(w[k] ^ prev);
-        // max(rb - ld, 0) mid bytes (x 8): one clamped subtract
-        uint32_t m8 = __builtin_elementwise_sub_sat((uint32_t)rb8, (uint32_t)ld8);
-        // the value's mid bytes as the low bytes of `mid`; fp32 keeps exactly
-        // those m bytes (the bytes above them are zero), so the stage can OR
-        // the word in without a mask
-        U mid;
-        if constexpr (W == 4)
-            mid = m8 >= 32 ? w[k] : __builtin_amdgcn_ubfe(w[k], (uint32_t)lo8, (uint32_t)m8);
-        else
-            mid = w[k] >> lo8;
-        uint32_t ldv = (uint32_t)ld8;
-        if constexpr (!FULL) {
-            if (!valid[k]) { m8 = 0; ldv = 0; mid = 0; }
-        }
-        // element k -> byte k/4 of the lane's lead codes, bits 7-6 first
-        // (8 ld here; the sum is divided by 8 below)
-        codex |= ldv << (8 * (k >> 2) + 6 - 2 * (k & 3));
-        if constexpr (SlabState<T, LV>::PAIRS) {
-            if ((k & 1) == 0) {
-                mid_even = mid;
-                m8_even = m8;
-            } else {
-                st.mid[k >> 1] = (uint64_t)(uint32_t)mid_even | ((uint64_t)(uint32_t)mid << m8_even);
-                const uint32_t mp = m8_even + m8;   // the pair's mid bits
-                mb |= mp << (8 * (k >> 1));
-                tot8 += mp;
+        for (int k = 0; k < LV; k++) {
+            const int ld8 = lead8<U>(w[k] ^ prev);
+            // max(rb - ld, 0) mid bytes (x 8): one clamped subtract
+            uint32_t m8 = __builtin_elementwise_sub_sat((uint32_t)rb8, (uint32_t)ld8);
+            // the value's mid bytes as the low bytes of `mid`; fp32 keeps exactly
+            // those m bytes (the bytes above them are zero), so the stage can OR
+            // the word in without a mask
+            U mid;
+            if constexpr (W == 4)
+                mid = (NARROW || m8 < 32) ? __builtin_amdgcn_ubfe(w[k], (uint32_t)lo8, m8) : w[k];
+            else
+                mid = w[k] >> lo8;
+            uint32_t ldv = (uint32_t)ld8;
+            if constexpr (!FULL) {
+                if (!valid[k]) { m8 = 0; ldv = 0; mid = 0; }
             }
-        } else {
-            st.mid[k] = mid;
-            mb |= (m8 >> 3) << (4 * k);
-            tot8 += m8;
+            // element k -> byte k/4 of the lane's lead codes, bits 7-6 first
+            // (8 ld here; the sum is divided by 8 below)
+            codex |= ldv << (8 * (k >> 2) + 6 - 2 * (k & 3));
+            if constexpr (SlabState<T, LV>::PAIRS) {
+                if ((k & 1) == 0) {
+                    mid_even = mid;
+                    m8_even = m8;
+                } else {
+                    st.mid[k >> 1] = (uint64_t)(uint32_t)mid_even | ((uint64_t)(uint32_t)mid << m8_even);
+                    const uint32_t mp = m8_even + m8;   // the pair's mid bits
+                    mb |= mp << (8 * (k >> 1));
+                    tot8 += mp;
+                }
+            } else {
+                st.mid[k] = mid;
+                mb |= (m8 >> 3) << (4 * k);
+                tot8 += m8;
+            }
+            prev = w[k];
         }
-        prev = w[k];
-    }
+    };
+    if (W == 4 && __builtin_amdgcn_ballot_w64(rb == 4) == 0)
+        values(std::true_type{});
+    else
+        values(std::false_type{});
     const uint32_t code = codex >> 3;
     const uint32_t tot = tot8 >> 3;
     const uint32_t inc = seg_scan_incl<SEG>(tot, lane);
