@@ -118,13 +118,19 @@ __device__ inline uint32_t wave0_exclusive_scan(uint32_t *a, int lane)
 template <typename T, int LV> struct SlabState {
     static constexpr bool PAIRS = sizeof(T) == 4;
     static constexpr int NM = PAIRS ? LV / 2 : LV;
-    static_assert(PAIRS ? NM <= 4 : NM * 4 <= 32, "the mid counts fit one word");
+    static_assert(PAIRS ? NM <= 8 : NM * 4 <= 32, "the mid counts fit one or two words");
     static_assert(!PAIRS || LV % 2 == 0, "values pair up");
+    static_assert(LV <= 16, "the lead codes fit one word");
     using Mid = typename std::conditional<PAIRS, uint64_t, typename Word<T>::U>::type;
+    using MB = typename std::conditional<(PAIRS && NM > 4), uint64_t, uint32_t>::type;
     Mid mid[NM];
     // PAIRS: mid bits of run r (0..64) in byte r; else mid bytes of value k
     // (0..8) in bits 4k..4k+3
-    uint32_t mb;
+    MB mb;
+    // meta: lane's mid-byte offset in its block (bits 0-11), rb (12-15),
+    // non-constant (16), reqLength (17-23), lead-code bytes of the block
+    // (24-30); code: the lane's lead codes, 2 bits per value, first value in
+    // the top bits of byte 0
     uint32_t meta, code;
     float hmed;
 };
@@ -249,8 +255,9 @@ __device__ __forceinline__ void encode_slab(const T (&v)[LV], uint64_t e0, uint6
     // m = rb - ld (>= 0), and the word shifted so that its mid bytes
     // [lo, lo + m), lo = W - rb, are its low bytes (only those are staged).
     const int rb8 = 8 * rb, lo8 = 8 * W - rb8;
-    uint32_t codex = 0, tot8 = 0;
-    uint32_t mb = 0;
+    // lead codes of values 8h..8h+7 (x 8) in codex[h]
+    uint32_t codex[(LV + 7) / 8] = {}, tot8 = 0;
+    typename SlabState<T, LV>::MB mb = 0;
     U mid_even = 0;      // PAIRS: the pair's first value, its mid bits
     uint32_t m8_even = 0;
     // NARROW: no lane of the wave needs 4 bytes, so every m8 is at most 24
@@ -277,7 +284,7 @@ __device__ __forceinline__ void encode_slab(const T (&v)[LV], uint64_t e0, uint6
             }
             // element k -> byte k/4 of the lane's lead codes, bits 7-6 first
             // (8 ld here; the sum is divided by 8 below)
-            codex |= ldv << (8 * (k >> 2) + 6 - 2 * (k & 3));
+            codex[k >> 3] |= ldv << (8 * ((k & 7) >> 2) + 6 - 2 * (k & 3));
             if constexpr (SlabState<T, LV>::PAIRS) {
                 if ((k & 1) == 0) {
                     mid_even = mid;
@@ -285,7 +292,7 @@ __device__ __forceinline__ void encode_slab(const T (&v)[LV], uint64_t e0, uint6
                 } else {
                     st.mid[k >> 1] = (uint64_t)(uint32_t)mid_even | ((uint64_t)(uint32_t)mid << m8_even);
                     const uint32_t mp = m8_even + m8;   // the pair's mid bits
-                    mb |= mp << (8 * (k >> 1));
+                    mb |= (typename SlabState<T, LV>::MB)mp << (8 * (k >> 1));
                     tot8 += mp;
                 }
             } else {
@@ -300,15 +307,16 @@ __device__ __forceinline__ void encode_slab(const T (&v)[LV], uint64_t e0, uint6
         values(std::true_type{});
     else
         values(std::false_type{});
-    const uint32_t code = codex >> 3;
+    uint32_t code = codex[0] >> 3;
+    if constexpr (LV > 8) code |= (codex[1] >> 3) << 16;
     const uint32_t tot = tot8 >> 3;
     const uint32_t inc = seg_scan_incl<SEG>(tot, lane);
     const uint32_t midtot = seg_last<SEG>(inc);
     const uint32_t nlead = (uint32_t)(nel + 3) >> 2;
 
     st.meta = (inc - tot) | ((uint32_t)rb << 12) | ((nonconst ? 1u : 0u) << 16) |
-              ((uint32_t)len << 17);
-    st.code = code | (nlead << 16);
+              ((uint32_t)len << 17) | (nlead << 24);
+    st.code = code;
     st.mb = mb;
     st.hmed = hmed;
     recb = nonconst ? 5u + nlead + midtot : 0u;
@@ -482,10 +490,10 @@ __device__ __forceinline__ void stage_slab(const SlabState<T, LV> &st, int u, co
     if (!((meta >> 16) & 1)) return;
     const int lb = u * BPS + tid / SEG;
     const uint32_t base = M.pack[lb] & 0xFFFFFu;
-    const uint32_t nlead = st.code >> 16;
+    const uint32_t nlead = (meta >> 24) & 0x7Fu;
     if (lseg == 0) {   // reqLength byte + big-endian median
         const uint32_t hb = bswap32(f32_bits(st.hmed));
-        s_stage[base] = (uint8_t)(meta >> 17);
+        s_stage[base] = (uint8_t)((meta >> 17) & 0x7Fu);
         s_stage[base + 1] = (uint8_t)hb;
         s_stage[base + 2] = (uint8_t)(hb >> 8);
         s_stage[base + 3] = (uint8_t)(hb >> 16);
@@ -1177,9 +1185,12 @@ CompressLayout compress_layout(uint64_t n, int w, int bs)
 template <typename T, int BS>
 static void launch_main(const CompressParams &p, uint64_t ntiles, hipStream_t st)
 {
-    // 8 values per lane where a block spans at least 8 values, else 4
-    constexpr int LV = BS >= 8 ? 8 : 4;
-    constexpr int IPT = compress_wide((int)sizeof(T), BS) ? INDEX_PER_TILE_WIDE : INDEX_PER_TILE;
+    // 16 values per lane for the wide fp32 tiles (half the per-slab block
+    // statistics and scans of 8), else 8 where a block spans at least 8
+    // values, else 4
+    constexpr bool WIDE = compress_wide((int)sizeof(T), BS);
+    constexpr int LV = (WIDE && sizeof(T) == 4) ? 16 : (BS >= 8 ? 8 : 4);
+    constexpr int IPT = WIDE ? INDEX_PER_TILE_WIDE : INDEX_PER_TILE;
     constexpr int TILE = IPT * (sizeof(T) == 4 ? TILE_VALUES_F32 : TILE_VALUES_F64);
     constexpr int SLABS = TILE / (256 * LV);
     hipLaunchKernelGGL((k_compress<T, BS, LV, SLABS, IPT>), dim3((unsigned)ntiles), dim3(256), 0, st, p);
