@@ -73,7 +73,7 @@ struct FieldSlot {
 };
 
 struct Layout {
-    uint64_t bf, df, bases, res, status, part, fields, bytes;
+    uint64_t bf, df, bases, res, status, part, fields, dmap, bytes;
     uint64_t total[3] = {0, 0, 0}, nitems = 0, dtotal = 0, zitems = 0, aitems = 0;
     std::vector<FieldSlot> slot;
 };
@@ -131,6 +131,7 @@ bool plan_layout(const szx_amd_field *f, int nf, int bs, Layout &Y)
         s.index = off;  off = al(off + 8 * index_words(f[i].n, s.w, bs));
         s.hdr = off;    off += ALIGN;
     }
+    Y.dmap = off;   off = al(off + 4 * (Y.dtotal ? Y.dtotal : 1));
     Y.bytes = off;
     return true;
 }
@@ -154,6 +155,7 @@ struct szx_amd_batch {
         return reinterpret_cast<uint64_t *>(work + Y.bases) + (uint64_t)k * (nf + 1);
     }
     uint64_t *d_res() const { return reinterpret_cast<uint64_t *>(work + Y.res); }
+    uint32_t *d_dmap() const { return reinterpret_cast<uint32_t *>(work + Y.dmap); }
     int32_t *d_status() const { return reinterpret_cast<int32_t *>(work + Y.status); }
     uint64_t *d_index(int i) const { return reinterpret_cast<uint64_t *>(work + Y.slot[i].index); }
 };
@@ -432,6 +434,11 @@ szx_amd_batch *szx_amd_batch_create(const szx_amd_field *fields, int nfields, in
         }
     }
     for (int k = 0; k < NBASE; k++) base[k * (nfields + 1) + nfields] = acc[k];
+    // the field of every decode tile (one load per workgroup instead of a
+    // search of the tile bases)
+    std::vector<uint32_t> dmap(Y.dtotal ? Y.dtotal : 1, 0);
+    for (int i = 0; i < nfields; i++)
+        for (uint64_t t = 0; t < Y.slot[i].nidx; t++) dmap[base[3 * (nfields + 1) + i] + t] = (uint32_t)i;
 
     hipStream_t st = static_cast<hipStream_t>(hipStream);
     const bool ok = hip_ok(hipMemcpyAsync(b->d_bf(), bf.data(), bf.size() * sizeof(BatchField),
@@ -440,6 +447,8 @@ szx_amd_batch *szx_amd_batch_create(const szx_amd_field *fields, int nfields, in
                                           hipMemcpyHostToDevice, st), "H2D fields") &&
                     hip_ok(hipMemcpyAsync(b->d_base(0), base.data(), base.size() * sizeof(uint64_t),
                                           hipMemcpyHostToDevice, st), "H2D bases") &&
+                    hip_ok(hipMemcpyAsync(b->d_dmap(), dmap.data(), dmap.size() * sizeof(uint32_t),
+                                          hipMemcpyHostToDevice, st), "H2D tile map") &&
                     hip_ok(hipStreamSynchronize(st), "batch setup");
     if (!ok) {
         delete b;
@@ -504,7 +513,7 @@ int szx_amd_batch_decompress(szx_amd_batch *b, int flags, int mode, void *hipStr
     }
     if (mode != SZX_AMD_BATCH_FUSED) { api_error("unknown batch mode %d", mode); return SZx_NSCS; }
     timing_mark_begin(KK_DECODE, st);
-    const int rc = batch_decode_launch(b->bs, b->d_df(), b->d_base(3), b->nf, b->Y.dtotal, exact, st);
+    const int rc = batch_decode_launch(b->bs, b->d_df(), b->d_base(3), b->d_dmap(), b->nf, b->Y.dtotal, exact, st);
     timing_mark_end(KK_DECODE, st);
     if (rc) { api_error("unsupported blockSize"); return SZx_NSCS; }
     return hip_ok(hipGetLastError(), "launch") ? SZx_SCES : SZx_NSCS;

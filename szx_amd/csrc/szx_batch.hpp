@@ -111,8 +111,9 @@ struct BatchDecodeField {
 };
 
 // Batched decode (szx_decompress.hip): header checks, then every index tile
-// of every field.  tbase: first global index tile of each field (nf + 1).
-int batch_decode_launch(int bs, const BatchDecodeField *f, const uint64_t *tbase, int nf, uint64_t total,
-                        int exact, hipStream_t st);
+// of every field.  tbase: first global index tile of each field (nf + 1);
+// tmap: the field of every global index tile (total).
+int batch_decode_launch(int bs, const BatchDecodeField *f, const uint64_t *tbase, const uint32_t *tmap, int nf,
+                        uint64_t total, int exact, hipStream_t st);
 
 }  // namespace szxamd

@@ -512,7 +512,8 @@ __global__ __launch_bounds__(256) void k_decode_tiles(DecodeParams p)
 // Batched decode: every field of a batched compress in one launch, one
 // workgroup per index tile of every field (the field found by a binary search
 // of the tile bases), so tiles of all fields overlap on the CUs exactly as a
-// single field's tiles do.  The stream of a field is found on the device from
+// single field's tiles do (the field of each tile comes from a table built
+// with the plan).  The stream of a field is found on the device from
 // the compressor's result words, so a compress and its decode can be enqueued
 // (or captured) together.
 // ---------------------------------------------------------------------------
@@ -527,48 +528,43 @@ __global__ void k_decode_header_batch(const BatchDecodeField *f, int nf, int bs)
 }
 
 template <int BS, int LV>
-__global__ __launch_bounds__(256) void k_decode_batch(const BatchDecodeField *f, const uint64_t *tbase, int nf,
-                                                      int exact)
+__global__ __launch_bounds__(256) void k_decode_batch(const BatchDecodeField *f, const uint64_t *tbase,
+                                                      const uint32_t *tmap, int exact)
 {
     __shared__ union Lds {
         DecodeLds<float, BS, LV> f;
         DecodeLds<double, BS, LV> d;
     } lds;
     const uint64_t g = blockIdx.x;
-    int lo = 0, hi = nf - 1;   // the field holding tile g
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) / 2;
-        if (tbase[mid] <= g) lo = mid;
-        else hi = mid - 1;
-    }
-    const BatchDecodeField &d = f[lo];
+    const uint32_t fi = tmap[g];   // the field holding tile g
+    const BatchDecodeField &d = f[fi];
     const DecodeParams p{d.out + d.result[0], d.result[1], d.n, d.h, d.index,
                          reinterpret_cast<const uint16_t *>(d.index + 2 * (d.ntidx + 1)), d.status, d.dst, exact};
-    if (d.w == 4) decode_tile<float, BS, LV>(p, g - tbase[lo], lds.f);
-    else decode_tile<double, BS, LV>(p, g - tbase[lo], lds.d);
+    if (d.w == 4) decode_tile<float, BS, LV>(p, g - tbase[fi], lds.f);
+    else decode_tile<double, BS, LV>(p, g - tbase[fi], lds.d);
 }
 
 template <int BS>
-static void launch_decode_batch(const BatchDecodeField *f, const uint64_t *tbase, int nf, uint64_t total,
-                                int exact, hipStream_t st)
+static void launch_decode_batch(const BatchDecodeField *f, const uint64_t *tbase, const uint32_t *tmap,
+                                uint64_t total, int exact, hipStream_t st)
 {
     constexpr int LV = BS >= 8 ? 8 : 4;
-    if (total) hipLaunchKernelGGL((k_decode_batch<BS, LV>), dim3((unsigned)total), dim3(256), 0, st, f, tbase, nf,
+    if (total) hipLaunchKernelGGL((k_decode_batch<BS, LV>), dim3((unsigned)total), dim3(256), 0, st, f, tbase, tmap,
                                   exact);
 }
 
-int batch_decode_launch(int bs, const BatchDecodeField *f, const uint64_t *tbase, int nf, uint64_t total,
-                        int exact, hipStream_t st)
+int batch_decode_launch(int bs, const BatchDecodeField *f, const uint64_t *tbase, const uint32_t *tmap, int nf,
+                        uint64_t total, int exact, hipStream_t st)
 {
     hipLaunchKernelGGL(k_decode_header_batch, dim3((nf + 255) / 256), dim3(256), 0, st, f, nf, bs);
     switch (bs) {
-    case 4: launch_decode_batch<4>(f, tbase, nf, total, exact, st); break;
-    case 8: launch_decode_batch<8>(f, tbase, nf, total, exact, st); break;
-    case 16: launch_decode_batch<16>(f, tbase, nf, total, exact, st); break;
-    case 32: launch_decode_batch<32>(f, tbase, nf, total, exact, st); break;
-    case 64: launch_decode_batch<64>(f, tbase, nf, total, exact, st); break;
-    case 128: launch_decode_batch<128>(f, tbase, nf, total, exact, st); break;
-    case 256: launch_decode_batch<256>(f, tbase, nf, total, exact, st); break;
+    case 4: launch_decode_batch<4>(f, tbase, tmap, total, exact, st); break;
+    case 8: launch_decode_batch<8>(f, tbase, tmap, total, exact, st); break;
+    case 16: launch_decode_batch<16>(f, tbase, tmap, total, exact, st); break;
+    case 32: launch_decode_batch<32>(f, tbase, tmap, total, exact, st); break;
+    case 64: launch_decode_batch<64>(f, tbase, tmap, total, exact, st); break;
+    case 128: launch_decode_batch<128>(f, tbase, tmap, total, exact, st); break;
+    case 256: launch_decode_batch<256>(f, tbase, tmap, total, exact, st); break;
     default: return -1;
     }
     return 0;
