@@ -1263,18 +1263,15 @@ int compress_launch(const void *d_in, uint64_t n, int w, int rel_mode, float abs
     unsigned long long *zero = reinterpret_cast<unsigned long long *>(d_ws);
     float *eb_dev = nullptr;        // REL: the bound, computed on the device
     const void *rparts = nullptr;   // REL, tiled: the range partials k_compress folds
-    if (rel_mode && bs_tiled(bs)) {
+    if (rel_mode && bs_tiled(bs) && w == 4) {
         // one launch for the range partials and the clear; every compressor
-        // workgroup folds the partials into the bound itself
+        // workgroup folds the partials into the bound itself.  (fp64 keeps
+        // the separate fold: folding 256 pairs of doubles cost its kernel
+        // 3-4 %, about the two launches it saves.)
         eb_dev = reinterpret_cast<float *>(misc + 32);
         rparts = d_ws + L.ws_parts;
-        if (w == 4)
-            hipLaunchKernelGGL((k_range_partial<float, 1024>), dim3(RANGE_LB_PARTS), dim3(1024), 0, st,
-                               static_cast<const float *>(d_in), n, (float *)(d_ws + L.ws_parts), zero, zero_words);
-        else
-            hipLaunchKernelGGL((k_range_partial<double, 1024>), dim3(RANGE_LB_PARTS), dim3(1024), 0, st,
-                               static_cast<const double *>(d_in), n, (double *)(d_ws + L.ws_parts), zero,
-                               zero_words);
+        hipLaunchKernelGGL((k_range_partial<float, 1024>), dim3(RANGE_LB_PARTS), dim3(1024), 0, st,
+                           static_cast<const float *>(d_in), n, (float *)(d_ws + L.ws_parts), zero, zero_words);
     } else {
         hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((zero_words + 255) / 256 < 1024 ? (zero_words + 255) / 256 : 1024)),
                            dim3(256), 0, st, zero, zero_words);
