@@ -705,7 +705,7 @@ def batched_eval(n_fields, dev, reps, single_gbs, parity=True):
 
     bc = Z.BatchCodec(fields, 128)
     shapes = {}
-    for lv, ipt in ((8, 4), (8, 2), (8, 1), (4, 2), (4, 1)):
+    for lv, ipt in ((16, 4), (8, 4), (8, 2), (8, 1), (4, 2), (4, 1)):
         bc.set_shape(lv, ipt)
         bc.compress()
         torch.cuda.synchronize()

@@ -538,9 +538,11 @@ int szx_amd_batch_results(szx_amd_batch *batch, uint64_t *h_results, void *hipSt
  * PER_FIELD needs szx_amd_batch_results() after the compress. */
 int szx_amd_batch_decompress(szx_amd_batch *batch, int flags, int mode, void *hipStream);
 
-/* Tile shape of the fused compress: values per lane (4 or 8; 8 needs
- * blockSize >= 8) and index tiles per workgroup tile (1, 2 or 4, i.e. 4096,
- * 8192 or 16384 fp32 values; 4 needs 8 values per lane and blockSize 16..128).
+/* Tile shape of the fused compress: values per lane (4, 8 or 16; 8 needs
+ * blockSize >= 8; 16 applies to the fp32 fields, the fp64 fields taking 8) and
+ * index tiles per workgroup tile (1, 2 or 4, i.e. 4096, 8192 or 16384 fp32
+ * values; 4 needs 8 or 16 values per lane and blockSize 16..128, and 16 values
+ * per lane needs 4).
  * szx_amd_batch_tune times every shape on this batch
  * (reps timed passes each, after one untimed pass; synchronizes), keeps the
  * fastest and reports it with its mean milliseconds per compress.  The choice

@@ -276,7 +276,7 @@ void tune_cache_store(const szx_amd_batch *b, const TunedShape &t)
 
 bool shape_ok(int bs, int lv, int ipt)
 {
-    if (ipt == 4) return lv == 8 && bs >= 16 && bs <= 128;
+    if (ipt == 4) return (lv == 8 || lv == 16) && bs >= 16 && bs <= 128;
     return (ipt == 1 || ipt == 2) && (lv == 4 || (lv == 8 && bs >= 8));
 }
 
@@ -547,7 +547,7 @@ int szx_amd_batch_tune(szx_amd_batch *b, int reps, int *valuesPerLane, int *tile
     }
     if (reps < 1) reps = 1;
     hipStream_t st = static_cast<hipStream_t>(hipStream);
-    const int shapes[5][2] = {{8, 4}, {8, 2}, {8, 1}, {4, 2}, {4, 1}};
+    const int shapes[6][2] = {{16, 4}, {8, 4}, {8, 2}, {8, 1}, {4, 2}, {4, 1}};
     const int lv0 = b->lv, ipt0 = b->ipt;   // restored if timing fails
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (!hip_ok(hipEventCreate(&e0), "event") || !hip_ok(hipEventCreate(&e1), "event")) {

@@ -1101,21 +1101,22 @@ __global__ void k_batch_range_final(BatchField *bf, const uint64_t *rbase, int n
 // Many fields: one workgroup per tile of the batch's global tile list
 // (fields in order, tbase[f] = first global tile of field f).  Workgroups
 // start in index order, so every tile a look-back waits for has started.
-template <int BS, int LV, int IPT>
+// LVF / LVD: values per lane of the fp32 / fp64 fields' tiles.
+template <int BS, int LVF, int LVD, int IPT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_compress_batch(const BatchField *bf, const uint64_t *tbase, int nf)
 {
-    constexpr int SF = IPT * TILE_VALUES_F32 / (256 * LV);
-    constexpr int SD = IPT * TILE_VALUES_F64 / (256 * LV);
+    constexpr int SF = IPT * TILE_VALUES_F32 / (256 * LVF);
+    constexpr int SD = IPT * TILE_VALUES_F64 / (256 * LVD);
     __shared__ union Lds {
-        CompressLds<float, BS, LV, SF> f;
-        CompressLds<double, BS, LV, SD> d;
+        CompressLds<float, BS, LVF, SF> f;
+        CompressLds<double, BS, LVD, SD> d;
     } lds;
     const uint64_t g = blockIdx.x;
     const int f = field_of(tbase, nf, g);
     CompressParams p = bf[f].p;
     p.ntiles = bf[f].ntiles[IPT == 4 ? 2 : IPT - 1];
-    if (bf[f].w == 4) compress_tile<float, BS, LV, SF, IPT>(p, g - tbase[f], lds.f);
-    else compress_tile<double, BS, LV, SD, IPT>(p, g - tbase[f], lds.d);
+    if (bf[f].w == 4) compress_tile<float, BS, LVF, SF, IPT>(p, g - tbase[f], lds.f);
+    else compress_tile<double, BS, LVD, SD, IPT>(p, g - tbase[f], lds.d);
 }
 
 // Clear every field's look-back words, totals and error flag (one block per
@@ -1381,31 +1382,40 @@ void batch_range_launch(const BatchField *bf, const uint64_t *rbase, int nf, uin
 
 // Persistent grid: every workgroup resident at once (the round-robin walk
 // relies on it), at most one per tile.
-template <int BS, int LV, int IPT>
+template <int BS, int LVF, int LVD, int IPT>
 static void launch_batch(const BatchField *bf, const uint64_t *tbase, int nf, uint64_t total, hipStream_t st)
 {
-    if (total) hipLaunchKernelGGL((k_compress_batch<BS, LV, IPT>), dim3((unsigned)total), dim3(256), 0, st, bf, tbase, nf);
+    if (total)
+        hipLaunchKernelGGL((k_compress_batch<BS, LVF, LVD, IPT>), dim3((unsigned)total), dim3(256), 0, st, bf, tbase,
+                           nf);
 }
 
 template <int BS>
 static int batch_shape(int lv, int ipt, const BatchField *bf, const uint64_t *tbase, int nf, uint64_t total,
                        hipStream_t st)
 {
+    constexpr int L8 = BS >= 8 ? 8 : 4;
+    if (lv == 16) {   // the fp32 fields' wide tiles take 16 values per lane, fp64 8
+        if (ipt != 4) return -1;
+        if constexpr (BS >= 16 && BS <= 128) launch_batch<BS, 16, 8, 4>(bf, tbase, nf, total, st);
+        else return -1;
+        return 0;
+    }
     if (lv == 8 && BS >= 8) {
         if (ipt == 4) {
-            if constexpr (BS >= 16 && BS <= 128) launch_batch<BS, 8, 4>(bf, tbase, nf, total, st);
+            if constexpr (BS >= 16 && BS <= 128) launch_batch<BS, 8, 8, 4>(bf, tbase, nf, total, st);
             else return -1;
         } else if (ipt == 2) {
-            launch_batch<BS, (BS >= 8 ? 8 : 4), 2>(bf, tbase, nf, total, st);
+            launch_batch<BS, L8, L8, 2>(bf, tbase, nf, total, st);
         } else {
-            launch_batch<BS, (BS >= 8 ? 8 : 4), 1>(bf, tbase, nf, total, st);
+            launch_batch<BS, L8, L8, 1>(bf, tbase, nf, total, st);
         }
         return 0;
     }
     if (lv == 4) {
         if (ipt == 4) return -1;
-        if (ipt == 2) launch_batch<BS, 4, 2>(bf, tbase, nf, total, st);
-        else launch_batch<BS, 4, 1>(bf, tbase, nf, total, st);
+        if (ipt == 2) launch_batch<BS, 4, 4, 2>(bf, tbase, nf, total, st);
+        else launch_batch<BS, 4, 4, 1>(bf, tbase, nf, total, st);
         return 0;
     }
     return -1;

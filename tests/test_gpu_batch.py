@@ -96,7 +96,7 @@ def batch(device):
     return bc, xs, ebs, oracle_of(xs, ebs)
 
 
-@pytest.mark.parametrize("shape", [(8, 4), (8, 2), (8, 1), (4, 2), (4, 1)])
+@pytest.mark.parametrize("shape", [(16, 4), (8, 4), (8, 2), (8, 1), (4, 2), (4, 1)])
 def test_batch_shapes_match_oracle(batch, shape):
     bc, xs, ebs, want = batch
     say("shape", shape)
