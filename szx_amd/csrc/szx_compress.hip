@@ -1264,15 +1264,19 @@ int compress_launch(const void *d_in, uint64_t n, int w, int rel_mode, float abs
     unsigned long long *zero = reinterpret_cast<unsigned long long *>(d_ws);
     float *eb_dev = nullptr;        // REL: the bound, computed on the device
     const void *rparts = nullptr;   // REL, tiled: the range partials k_compress folds
-    if (rel_mode && bs_tiled(bs) && w == 4) {
+    if (rel_mode && bs_tiled(bs)) {
         // one launch for the range partials and the clear; every compressor
-        // workgroup folds the partials into the bound itself.  (fp64 keeps
-        // the separate fold: folding 256 pairs of doubles cost its kernel
-        // 3-4 %, about the two launches it saves.)
+        // workgroup folds the partials into the bound itself (fp64: the fold
+        // of 256 pairs of doubles costs its kernel 3-4 %, the two launches
+        // it saves more: REL call 0.292 -> 0.280 ms at 8192^2)
         eb_dev = reinterpret_cast<float *>(misc + 32);
         rparts = d_ws + L.ws_parts;
-        hipLaunchKernelGGL((k_range_partial<float, 1024>), dim3(RANGE_LB_PARTS), dim3(1024), 0, st,
-                           static_cast<const float *>(d_in), n, (float *)(d_ws + L.ws_parts), zero, zero_words);
+        if (w == 4)
+            hipLaunchKernelGGL((k_range_partial<float, 1024>), dim3(RANGE_LB_PARTS), dim3(1024), 0, st,
+                               static_cast<const float *>(d_in), n, (float *)(d_ws + L.ws_parts), zero, zero_words);
+        else
+            hipLaunchKernelGGL((k_range_partial<double, 1024>), dim3(RANGE_LB_PARTS), dim3(1024), 0, st,
+                               static_cast<const double *>(d_in), n, (double *)(d_ws + L.ws_parts), zero, zero_words);
     } else {
         hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((zero_words + 255) / 256 < 1024 ? (zero_words + 255) / 256 : 1024)),
                            dim3(256), 0, st, zero, zero_words);
