@@ -67,3 +67,21 @@ def test_other_wrappers_check_their_inputs():
         Z.rate_stats(torch.zeros(64, dtype=torch.float32)[::2])
     with pytest.raises(Z.SZxError, match="contiguous"):
         Z.BatchCodec([(torch.zeros(64, dtype=torch.float32)[::2], Z.ABS, 1e-3)])
+
+
+@pytest.mark.parametrize("n, reqexpo, noct", [
+    (0, 0, 1),        # no values
+    (100, 0, 3),      # one or two octaves only
+    (100, 0, 0),
+    (100, 127, 1),    # the bound above 2^(reqExpo+1) must be a float
+    (100, -149, 1),   # the float below the lowest octave must exist
+    (100, -148, 2),
+])
+def test_octave_span_refuses_bad_arguments(n, reqexpo, noct):
+    """The octave-span entry point rejects its arguments before any launch
+    (a dummy, never dereferenced device address stands in for the field)."""
+    L = Z.lib()
+    ws = L.szx_amd_rate_octave_workspace()
+    rc = L.szx_amd_rate_octave_span_dev(16, n, Z.SZx_FLOAT, 128, reqexpo, noct, 16, 16, ws, None)
+    assert rc != 0
+    assert "invalid arguments" in Z.last_error()
