@@ -73,7 +73,7 @@ struct FieldSlot {
 };
 
 struct Layout {
-    uint64_t bf, df, bases, res, status, part, fields, dmap, bytes;
+    uint64_t bf, df, bases, res, status, part, fields, dmap, drec, bytes;
     uint64_t total[3] = {0, 0, 0}, nitems = 0, dtotal = 0, zitems = 0, aitems = 0;
     std::vector<FieldSlot> slot;
 };
@@ -132,6 +132,7 @@ bool plan_layout(const szx_amd_field *f, int nf, int bs, Layout &Y)
         s.hdr = off;    off += ALIGN;
     }
     Y.dmap = off;   off = al(off + 4 * (Y.dtotal ? Y.dtotal : 1));
+    Y.drec = off;   off = al(off + batch_decode_records_bytes(Y.dtotal ? Y.dtotal : 1));
     Y.bytes = off;
     return true;
 }
@@ -513,7 +514,8 @@ int szx_amd_batch_decompress(szx_amd_batch *b, int flags, int mode, void *hipStr
     }
     if (mode != SZX_AMD_BATCH_FUSED) { api_error("unknown batch mode %d", mode); return SZx_NSCS; }
     timing_mark_begin(KK_DECODE, st);
-    const int rc = batch_decode_launch(b->bs, b->d_df(), b->d_base(3), b->d_dmap(), b->nf, b->Y.dtotal, exact, st);
+    const int rc = batch_decode_launch(b->bs, b->d_df(), b->d_base(3), b->d_dmap(), b->work + b->Y.drec, b->nf,
+                                       b->Y.dtotal, exact, st);
     timing_mark_end(KK_DECODE, st);
     if (rc) { api_error("unsupported blockSize"); return SZx_NSCS; }
     return hip_ok(hipGetLastError(), "launch") ? SZx_SCES : SZx_NSCS;

@@ -110,10 +110,12 @@ struct BatchDecodeField {
     int w;
 };
 
-// Batched decode (szx_decompress.hip): header checks, then every index tile
+// Batched decode (szx_decompress.hip): header checks, every tile's bytes
+// (into recs, batch_decode_records_bytes(total) bytes), then every index tile
 // of every field.  tbase: first global index tile of each field (nf + 1);
 // tmap: the field of every global index tile (total).
-int batch_decode_launch(int bs, const BatchDecodeField *f, const uint64_t *tbase, const uint32_t *tmap, int nf,
-                        uint64_t total, int exact, hipStream_t st);
+uint64_t batch_decode_records_bytes(uint64_t total);
+int batch_decode_launch(int bs, const BatchDecodeField *f, const uint64_t *tbase, const uint32_t *tmap,
+                        void *recs, int nf, uint64_t total, int exact, hipStream_t st);
 
 }  // namespace szxamd

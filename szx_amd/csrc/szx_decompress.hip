@@ -336,6 +336,23 @@ struct DecodeParams {
     int exact;
 };
 
+// Where one index tile's bytes are, from the checked header and the record
+// index: what a tile decode needs before it can stage its records.  flags:
+// 0 ready, 1 the header check failed (nothing to do), 2 the index entry is
+// out of range (status -20).
+struct TileSrc {
+    const uint8_t *rsrc;        // the tile's first record byte
+    const uint8_t *msrc;        // its first constant median
+    const uint8_t *lo, *hi;     // the stream's bytes
+    const uint8_t *bmp;         // the bitmap byte of its first block
+    const uint16_t *bidx;       // the record-index entry of its first block
+    uint32_t tb, nct;           // record bytes, constant blocks
+    uint32_t nvb;               // blocks of the tile inside the array
+    uint32_t flags;
+    uint32_t field, tile;       // batched decode: the field and its tile
+    uint32_t medle, pad;
+};
+
 template <typename T, int BS, int LV> struct DecodeShape {
     static constexpr int W = Word<T>::W;
     static constexpr int TILE = sizeof(T) == 4 ? TILE_VALUES_F32 : TILE_VALUES_F64;
@@ -355,8 +372,58 @@ template <typename T, int BS, int LV> struct DecodeLds {
 
 // Decode index tile `tile` of one stream (TILE values, NBLK blocks); each
 // lane decodes LV consecutive values per slab of 256 * LV values.
+// The bytes of index tile `tile` of one stream (NBLK blocks of the tile shape).
+template <int NBLK, int STAGE>
+__device__ inline TileSrc tile_src(const DecodeParams &p, uint64_t tile)
+{
+    TileSrc t{};
+    const DecodeHeader *h = p.h;
+    // gate on the header check, which an earlier kernel wrote
+    if (h->rc != 0) {
+        t.flags = 1;
+        return t;
+    }
+    const uint64_t *index = p.index;
+    const uint64_t nb = h->nb, q0 = h->q0, nconst = h->nconst;
+    const uint64_t gb0 = tile * NBLK;
+    const uint64_t off = q0 + index[2 * tile];
+    const uint64_t end = q0 + index[2 * tile + 2];
+    const uint64_t cbase = index[2 * tile + 1];
+    const uint64_t cend = index[2 * tile + 3];
+    if (end < off || end > p.len || end - off > (uint64_t)STAGE || cend < cbase ||
+        cend - cbase > (uint64_t)NBLK || cend > nconst) {
+        t.flags = 2;
+        return t;
+    }
+    t.rsrc = p.s + off;
+    t.msrc = p.s + h->qmed + 4 * cbase;   // inside [0, q0) <= len
+    t.lo = p.s;
+    t.hi = p.s + p.len;
+    t.bmp = p.s + h->qbit + (gb0 >> 3);   // NBLK is a multiple of 8
+    t.bidx = p.bidx + gb0;
+    t.tb = (uint32_t)(end - off);
+    t.nct = (uint32_t)(cend - cbase);
+    t.nvb = gb0 >= nb ? 0u : (nb - gb0 < (uint64_t)NBLK ? (uint32_t)(nb - gb0) : (uint32_t)NBLK);
+    t.medle = (uint32_t)h->medle;
+    t.flags = 0;
+    return t;
+}
+
+template <typename T, int BS, int LV>
+__device__ inline void decode_tile_core(const DecodeParams &p, uint64_t tile, const TileSrc &ts,
+                                        DecodeLds<T, BS, LV> &lds);
+
 template <typename T, int BS, int LV>
 __device__ inline void decode_tile(const DecodeParams &p, uint64_t tile, DecodeLds<T, BS, LV> &lds)
+{
+    using S = DecodeShape<T, BS, LV>;
+    decode_tile_core<T, BS, LV>(p, tile, tile_src<S::NBLK, S::STAGE>(p, tile), lds);
+}
+
+// Decode index tile `tile` from its bytes ts (tile_src).
+template <typename T, int BS, int LV>
+__device__ inline void decode_tile_core(const DecodeParams &p, uint64_t tile, const TileSrc &ts,
+                                        DecodeLds<T, BS, LV> &lds)
 {
     using S = DecodeShape<T, BS, LV>;
     constexpr int SEG = BS / LV;
@@ -375,50 +442,38 @@ __device__ inline void decode_tile(const DecodeParams &p, uint64_t tile, DecodeL
     uint16_t *s_size = lds.size;
     uint8_t *s_state = lds.state;
     int &s_err = lds.err;
-    const uint8_t *s = p.s;
-    const uint64_t len = p.len, n = p.n;
-    const DecodeHeader *h = p.h;
-    const uint64_t *index = p.index;
-    const uint16_t *bidx = p.bidx;
+    const uint64_t n = p.n;
     int32_t *status = p.status;
     T *out = static_cast<T *>(p.out);
     const int exact = p.exact;
 
-    // Gate on the header check, which an earlier kernel wrote, so every
-    // thread of the workgroup takes the same branch (the status word itself
-    // may change while this kernel runs: other tiles report into it).
-    if (h->rc != 0) return;
+    // The flags come from the header check and the index, which earlier
+    // kernels wrote, so every thread of the workgroup takes the same branch
+    // (the status word itself may change while this kernel runs: other
+    // tiles report into it).
+    if (ts.flags == 1) return;
     const int tid = threadIdx.x;
-    const int lane = tid & 63;
-    const int lseg = tid % SEG;
-    const uint64_t nb = h->nb, q0 = h->q0, nconst = h->nconst;
-    const uint64_t gb0 = tile * NBLK;
-    const uint64_t off = q0 + index[2 * tile];
-    const uint64_t end = q0 + index[2 * tile + 2];
-    const uint64_t cbase = index[2 * tile + 1];
-    const uint64_t cend = index[2 * tile + 3];
-    if (end < off || end > len || end - off > (uint64_t)STAGE || cend < cbase ||
-        cend - cbase > (uint64_t)NBLK || cend > nconst) {
+    if (ts.flags != 0) {
         if (tid == 0) atomicMin(status, -20);
         return;
     }
-    const uint32_t tb = (uint32_t)(end - off);
-    const uint32_t nct = (uint32_t)(cend - cbase);
-    const uint8_t *rsrc = s + off;
-    const uint8_t *msrc = s + h->qmed + 4 * cbase;   // inside [0, q0) <= len
+    const int lane = tid & 63;
+    const int lseg = tid % SEG;
+    const uint32_t tb = ts.tb, nct = ts.nct, nvb = ts.nvb;
+    const uint8_t *rsrc = ts.rsrc;
+    const uint8_t *msrc = ts.msrc;
     const uint32_t rdel = (uint32_t)((uintptr_t)rsrc & 15), mdel = (uint32_t)((uintptr_t)msrc & 15);
     if (tid == 0) s_err = 0;
-    stage_bytes(st, rsrc, tb, s, s + len, tid);
-    stage_bytes(s_med, msrc, 4 * nct, s, s + len, tid);
+    stage_bytes(st, rsrc, tb, ts.lo, ts.hi, tid);
+    stage_bytes(s_med, msrc, 4 * nct, ts.lo, ts.hi, tid);
     for (int lb = tid; lb < NBLK; lb += 256) {
-        const uint64_t b = gb0 + lb;
-        s_state[lb] = b < nb ? (s[h->qbit + (b >> 3)] >> (7 - (b & 7))) & 1 : 0;
-        s_start[lb] = b < nb ? bidx[b] : 0;
+        s_state[lb] = (uint32_t)lb < nvb ? (ts.bmp[lb >> 3] >> (7 - (lb & 7))) & 1 : 0;
+        s_start[lb] = (uint32_t)lb < nvb ? ts.bidx[lb] : 0;
         s_size[lb] = 0;
     }
     __syncthreads();
 
-    const TileLds L{st, rdel, s_med + mdel, nct, tb, s_start, s_size, s_state, &s_err, (int)h->medle};
+    const TileLds L{st, rdel, s_med + mdel, nct, tb, s_start, s_size, s_state, &s_err, (int)ts.medle};
     const bool full = (tile + 1) * (uint64_t)TILE <= n;
     if (full && sizeof(T) == 8) {
         // fp64: a lane's values are 8 LV consecutive bytes, and its own four
@@ -474,7 +529,7 @@ __device__ inline void decode_tile(const DecodeParams &p, uint64_t tile, DecodeL
 #pragma unroll
         for (int k = 0; k < PER_LANE; k++) {
             const int lb = lane * PER_LANE + k;
-            if (lb < NBLK && gb0 + lb < nb) {
+            if (lb < NBLK && (uint32_t)lb < nvb) {
                 if (s_state[lb]) bytes += s_size[lb];
                 else consts++;
             }
@@ -484,7 +539,7 @@ __device__ inline void decode_tile(const DecodeParams &p, uint64_t tile, DecodeL
 #pragma unroll
         for (int k = 0; k < PER_LANE; k++) {
             const int lb = lane * PER_LANE + k;
-            if (lb < NBLK && gb0 + lb < nb) {
+            if (lb < NBLK && (uint32_t)lb < nvb) {
                 if (s_state[lb]) {
                     bad |= s_start[lb] != eb;
                     eb += s_size[lb];
@@ -527,44 +582,69 @@ __global__ void k_decode_header_batch(const BatchDecodeField *f, int nf, int bs)
     check_header(s, d.result[1], d.n, bs, d.h, d.status);
 }
 
+// Every tile's bytes (tile_src), one thread per tile, so that a decode
+// workgroup starts staging its records after one load (the field's result
+// words, header and index entries are resolved here, off its path).
 template <int BS, int LV>
-__global__ __launch_bounds__(256) void k_decode_batch(const BatchDecodeField *f, const uint64_t *tbase,
-                                                      const uint32_t *tmap, int exact)
+__global__ __launch_bounds__(256) void k_decode_setup_batch(const BatchDecodeField *f, const uint64_t *tbase,
+                                                            const uint32_t *tmap, TileSrc *recs, uint64_t total)
+{
+    using S32 = DecodeShape<float, BS, LV>;
+    using S64 = DecodeShape<double, BS, LV>;
+    const uint64_t g = blockIdx.x * 256ull + threadIdx.x;
+    if (g >= total) return;
+    const uint32_t fi = tmap[g];   // the field holding tile g
+    const BatchDecodeField &d = f[fi];
+    const DecodeParams p{d.out + d.result[0], d.result[1], d.n, d.h, d.index,
+                         reinterpret_cast<const uint16_t *>(d.index + 2 * (d.ntidx + 1)), d.status, d.dst, 0};
+    const uint64_t t = g - tbase[fi];
+    TileSrc r = d.w == 4 ? tile_src<S32::NBLK, S32::STAGE>(p, t) : tile_src<S64::NBLK, S64::STAGE>(p, t);
+    r.field = fi;
+    r.tile = (uint32_t)t;
+    recs[g] = r;
+}
+
+template <int BS, int LV>
+__global__ __launch_bounds__(256) void k_decode_batch(const BatchDecodeField *f, const TileSrc *recs, int exact)
 {
     __shared__ union Lds {
         DecodeLds<float, BS, LV> f;
         DecodeLds<double, BS, LV> d;
     } lds;
-    const uint64_t g = blockIdx.x;
-    const uint32_t fi = tmap[g];   // the field holding tile g
-    const BatchDecodeField &d = f[fi];
-    const DecodeParams p{d.out + d.result[0], d.result[1], d.n, d.h, d.index,
-                         reinterpret_cast<const uint16_t *>(d.index + 2 * (d.ntidx + 1)), d.status, d.dst, exact};
-    if (d.w == 4) decode_tile<float, BS, LV>(p, g - tbase[fi], lds.f);
-    else decode_tile<double, BS, LV>(p, g - tbase[fi], lds.d);
+    const TileSrc r = recs[blockIdx.x];
+    const BatchDecodeField &d = f[r.field];
+    // the core reads only the values count, status, output and mode of p
+    const DecodeParams p{nullptr, 0, d.n, nullptr, nullptr, nullptr, d.status, d.dst, exact};
+    if (d.w == 4) decode_tile_core<float, BS, LV>(p, r.tile, r, lds.f);
+    else decode_tile_core<double, BS, LV>(p, r.tile, r, lds.d);
 }
 
 template <int BS>
 static void launch_decode_batch(const BatchDecodeField *f, const uint64_t *tbase, const uint32_t *tmap,
-                                uint64_t total, int exact, hipStream_t st)
+                                TileSrc *recs, uint64_t total, int exact, hipStream_t st)
 {
     constexpr int LV = BS >= 8 ? 8 : 4;
-    if (total) hipLaunchKernelGGL((k_decode_batch<BS, LV>), dim3((unsigned)total), dim3(256), 0, st, f, tbase, tmap,
-                                  exact);
+    if (!total) return;
+    hipLaunchKernelGGL((k_decode_setup_batch<BS, LV>), dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, f,
+                       tbase, tmap, recs, total);
+    hipLaunchKernelGGL((k_decode_batch<BS, LV>), dim3((unsigned)total), dim3(256), 0, st, f, recs, exact);
 }
 
-int batch_decode_launch(int bs, const BatchDecodeField *f, const uint64_t *tbase, const uint32_t *tmap, int nf,
-                        uint64_t total, int exact, hipStream_t st)
+uint64_t batch_decode_records_bytes(uint64_t total) { return total * sizeof(TileSrc); }
+
+int batch_decode_launch(int bs, const BatchDecodeField *f, const uint64_t *tbase, const uint32_t *tmap,
+                        void *recs, int nf, uint64_t total, int exact, hipStream_t st)
 {
+    TileSrc *r = static_cast<TileSrc *>(recs);
     hipLaunchKernelGGL(k_decode_header_batch, dim3((nf + 255) / 256), dim3(256), 0, st, f, nf, bs);
     switch (bs) {
-    case 4: launch_decode_batch<4>(f, tbase, tmap, total, exact, st); break;
-    case 8: launch_decode_batch<8>(f, tbase, tmap, total, exact, st); break;
-    case 16: launch_decode_batch<16>(f, tbase, tmap, total, exact, st); break;
-    case 32: launch_decode_batch<32>(f, tbase, tmap, total, exact, st); break;
-    case 64: launch_decode_batch<64>(f, tbase, tmap, total, exact, st); break;
-    case 128: launch_decode_batch<128>(f, tbase, tmap, total, exact, st); break;
-    case 256: launch_decode_batch<256>(f, tbase, tmap, total, exact, st); break;
+    case 4: launch_decode_batch<4>(f, tbase, tmap, r, total, exact, st); break;
+    case 8: launch_decode_batch<8>(f, tbase, tmap, r, total, exact, st); break;
+    case 16: launch_decode_batch<16>(f, tbase, tmap, r, total, exact, st); break;
+    case 32: launch_decode_batch<32>(f, tbase, tmap, r, total, exact, st); break;
+    case 64: launch_decode_batch<64>(f, tbase, tmap, r, total, exact, st); break;
+    case 128: launch_decode_batch<128>(f, tbase, tmap, r, total, exact, st); break;
+    case 256: launch_decode_batch<256>(f, tbase, tmap, r, total, exact, st); break;
     default: return -1;
     }
     return 0;
