@@ -73,7 +73,7 @@ struct FieldSlot {
 };
 
 struct Layout {
-    uint64_t bf, df, bases, res, status, part, fields, dmap, drec, bytes;
+    uint64_t bf, df, bases, res, status, part, fields, dmap, drec, cmap[3], bytes;
     uint64_t total[3] = {0, 0, 0}, nitems = 0, dtotal = 0, zitems = 0, aitems = 0;
     std::vector<FieldSlot> slot;
 };
@@ -133,6 +133,10 @@ bool plan_layout(const szx_amd_field *f, int nf, int bs, Layout &Y)
     }
     Y.dmap = off;   off = al(off + 4 * (Y.dtotal ? Y.dtotal : 1));
     Y.drec = off;   off = al(off + batch_decode_records_bytes(Y.dtotal ? Y.dtotal : 1));
+    for (int k = 0; k < 3; k++) {   // the field of every compressor tile, per tile size
+        Y.cmap[k] = off;
+        off = al(off + 4 * (Y.total[k] ? Y.total[k] : 1));
+    }
     Y.bytes = off;
     return true;
 }
@@ -157,6 +161,7 @@ struct szx_amd_batch {
     }
     uint64_t *d_res() const { return reinterpret_cast<uint64_t *>(work + Y.res); }
     uint32_t *d_dmap() const { return reinterpret_cast<uint32_t *>(work + Y.dmap); }
+    uint32_t *d_cmap(int k) const { return reinterpret_cast<uint32_t *>(work + Y.cmap[k]); }
     int32_t *d_status() const { return reinterpret_cast<int32_t *>(work + Y.status); }
     uint64_t *d_index(int i) const { return reinterpret_cast<uint64_t *>(work + Y.slot[i].index); }
 };
@@ -303,8 +308,8 @@ int enqueue_compress(szx_amd_batch *b, int mode, hipStream_t st)
     if (Y.nitems) batch_range_launch(b->d_bf(), b->d_base(2), b->nf, Y.nitems,
                                      reinterpret_cast<double *>(b->work + Y.part), st);
     timing_mark_begin(KK_COMPRESS, st);
-    const int rc = batch_compress_launch(b->bs, b->lv, b->ipt, b->d_bf(), b->d_base(tbase_slot(b->ipt)), b->nf,
-                                         Y.total[tile_slot(b->ipt)], st);
+    const int rc = batch_compress_launch(b->bs, b->lv, b->ipt, b->d_bf(), b->d_base(tbase_slot(b->ipt)),
+                                         b->d_cmap(tile_slot(b->ipt)), Y.total[tile_slot(b->ipt)], st);
     timing_mark_end(KK_COMPRESS, st);
     if (rc) { api_error("unsupported tile shape"); return SZx_NSCS; }
     batch_assemble_launch(b->d_bf(), b->d_base(5), b->nf, Y.aitems, st);
@@ -440,6 +445,15 @@ szx_amd_batch *szx_amd_batch_create(const szx_amd_field *fields, int nfields, in
     std::vector<uint32_t> dmap(Y.dtotal ? Y.dtotal : 1, 0);
     for (int i = 0; i < nfields; i++)
         for (uint64_t t = 0; t < Y.slot[i].nidx; t++) dmap[base[3 * (nfields + 1) + i] + t] = (uint32_t)i;
+    // the same for the compressor's tiles of each size (tile_slot order)
+    std::vector<uint32_t> cmap[3];
+    for (int k = 0; k < 3; k++) {
+        const int bslot = k == 2 ? 6 : k;   // tbase_slot of ipt 1, 2, 4
+        cmap[k].assign(Y.total[k] ? Y.total[k] : 1, 0);
+        for (int i = 0; i < nfields; i++)
+            for (uint64_t t = 0; t < Y.slot[i].ntiles[k]; t++)
+                cmap[k][base[bslot * (nfields + 1) + i] + t] = (uint32_t)i;
+    }
 
     hipStream_t st = static_cast<hipStream_t>(hipStream);
     const bool ok = hip_ok(hipMemcpyAsync(b->d_bf(), bf.data(), bf.size() * sizeof(BatchField),
@@ -449,6 +463,12 @@ szx_amd_batch *szx_amd_batch_create(const szx_amd_field *fields, int nfields, in
                     hip_ok(hipMemcpyAsync(b->d_base(0), base.data(), base.size() * sizeof(uint64_t),
                                           hipMemcpyHostToDevice, st), "H2D bases") &&
                     hip_ok(hipMemcpyAsync(b->d_dmap(), dmap.data(), dmap.size() * sizeof(uint32_t),
+                                          hipMemcpyHostToDevice, st), "H2D tile map") &&
+                    hip_ok(hipMemcpyAsync(b->d_cmap(0), cmap[0].data(), cmap[0].size() * sizeof(uint32_t),
+                                          hipMemcpyHostToDevice, st), "H2D tile map") &&
+                    hip_ok(hipMemcpyAsync(b->d_cmap(1), cmap[1].data(), cmap[1].size() * sizeof(uint32_t),
+                                          hipMemcpyHostToDevice, st), "H2D tile map") &&
+                    hip_ok(hipMemcpyAsync(b->d_cmap(2), cmap[2].data(), cmap[2].size() * sizeof(uint32_t),
                                           hipMemcpyHostToDevice, st), "H2D tile map") &&
                     hip_ok(hipStreamSynchronize(st), "batch setup");
     if (!ok) {

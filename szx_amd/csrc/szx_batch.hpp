@@ -88,8 +88,9 @@ struct BatchField {
 // the chosen tile shape; rbase: first range item of each field (nf + 1).
 void batch_range_launch(const BatchField *bf, const uint64_t *rbase, int nf, uint64_t nitems, double *part,
                         hipStream_t st);
-int batch_compress_launch(int bs, int lv, int ipt, const BatchField *bf, const uint64_t *tbase, int nf,
-                          uint64_t total, hipStream_t st);
+// tmap: the field of every tile of the global tile list (total).
+int batch_compress_launch(int bs, int lv, int ipt, const BatchField *bf, const uint64_t *tbase,
+                          const uint32_t *tmap, uint64_t total, hipStream_t st);
 // zbase / abase: first block of each field in the flattened clear / assembly
 // grids (nf + 1 entries; the last is the grid size).
 void batch_zero_launch(const BatchField *bf, const uint64_t *zbase, int nf, uint64_t nitems, hipStream_t st);

@@ -1103,7 +1103,8 @@ __global__ void k_batch_range_final(BatchField *bf, const uint64_t *rbase, int n
 // start in index order, so every tile a look-back waits for has started.
 // LVF / LVD: values per lane of the fp32 / fp64 fields' tiles.
 template <int BS, int LVF, int LVD, int IPT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_compress_batch(const BatchField *bf, const uint64_t *tbase, int nf)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_compress_batch(const BatchField *bf, const uint64_t *tbase,
+                                                                                                const uint32_t *tmap)
 {
     constexpr int SF = IPT * TILE_VALUES_F32 / (256 * LVF);
     constexpr int SD = IPT * TILE_VALUES_F64 / (256 * LVD);
@@ -1112,7 +1113,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         CompressLds<double, BS, LVD, SD> d;
     } lds;
     const uint64_t g = blockIdx.x;
-    const int f = field_of(tbase, nf, g);
+    const uint32_t f = tmap[g];   // the field of tile g (a table built with the plan)
     CompressParams p = bf[f].p;
     p.ntiles = bf[f].ntiles[IPT == 4 ? 2 : IPT - 1];
     if (bf[f].w == 4) compress_tile<float, BS, LVF, SF, IPT>(p, g - tbase[f], lds.f);
@@ -1387,56 +1388,57 @@ void batch_range_launch(const BatchField *bf, const uint64_t *rbase, int nf, uin
 // Persistent grid: every workgroup resident at once (the round-robin walk
 // relies on it), at most one per tile.
 template <int BS, int LVF, int LVD, int IPT>
-static void launch_batch(const BatchField *bf, const uint64_t *tbase, int nf, uint64_t total, hipStream_t st)
+static void launch_batch(const BatchField *bf, const uint64_t *tbase, const uint32_t *tmap, uint64_t total,
+                         hipStream_t st)
 {
     if (total)
         hipLaunchKernelGGL((k_compress_batch<BS, LVF, LVD, IPT>), dim3((unsigned)total), dim3(256), 0, st, bf, tbase,
-                           nf);
+                           tmap);
 }
 
 template <int BS>
-static int batch_shape(int lv, int ipt, const BatchField *bf, const uint64_t *tbase, int nf, uint64_t total,
-                       hipStream_t st)
+static int batch_shape(int lv, int ipt, const BatchField *bf, const uint64_t *tbase, const uint32_t *tmap,
+                       uint64_t total, hipStream_t st)
 {
     constexpr int L8 = BS >= 8 ? 8 : 4;
     if (lv == 16) {   // the fp32 fields' wide tiles take 16 values per lane, fp64 8
         if (ipt != 4) return -1;
-        if constexpr (BS >= 16 && BS <= 128) launch_batch<BS, 16, 8, 4>(bf, tbase, nf, total, st);
+        if constexpr (BS >= 16 && BS <= 128) launch_batch<BS, 16, 8, 4>(bf, tbase, tmap, total, st);
         else return -1;
         return 0;
     }
     if (lv == 8 && BS >= 8) {
         if (ipt == 4) {
-            if constexpr (BS >= 16 && BS <= 128) launch_batch<BS, 8, 8, 4>(bf, tbase, nf, total, st);
+            if constexpr (BS >= 16 && BS <= 128) launch_batch<BS, 8, 8, 4>(bf, tbase, tmap, total, st);
             else return -1;
         } else if (ipt == 2) {
-            launch_batch<BS, L8, L8, 2>(bf, tbase, nf, total, st);
+            launch_batch<BS, L8, L8, 2>(bf, tbase, tmap, total, st);
         } else {
-            launch_batch<BS, L8, L8, 1>(bf, tbase, nf, total, st);
+            launch_batch<BS, L8, L8, 1>(bf, tbase, tmap, total, st);
         }
         return 0;
     }
     if (lv == 4) {
         if (ipt == 4) return -1;
-        if (ipt == 2) launch_batch<BS, 4, 4, 2>(bf, tbase, nf, total, st);
-        else launch_batch<BS, 4, 4, 1>(bf, tbase, nf, total, st);
+        if (ipt == 2) launch_batch<BS, 4, 4, 2>(bf, tbase, tmap, total, st);
+        else launch_batch<BS, 4, 4, 1>(bf, tbase, tmap, total, st);
         return 0;
     }
     return -1;
 }
 
-int batch_compress_launch(int bs, int lv, int ipt, const BatchField *bf, const uint64_t *tbase, int nf,
-                          uint64_t total, hipStream_t st)
+int batch_compress_launch(int bs, int lv, int ipt, const BatchField *bf, const uint64_t *tbase,
+                          const uint32_t *tmap, uint64_t total, hipStream_t st)
 {
     if (ipt != 1 && ipt != 2 && ipt != 4) return -1;
     switch (bs) {
-    case 4: return batch_shape<4>(lv, ipt, bf, tbase, nf, total, st);
-    case 8: return batch_shape<8>(lv, ipt, bf, tbase, nf, total, st);
-    case 16: return batch_shape<16>(lv, ipt, bf, tbase, nf, total, st);
-    case 32: return batch_shape<32>(lv, ipt, bf, tbase, nf, total, st);
-    case 64: return batch_shape<64>(lv, ipt, bf, tbase, nf, total, st);
-    case 128: return batch_shape<128>(lv, ipt, bf, tbase, nf, total, st);
-    case 256: return batch_shape<256>(lv, ipt, bf, tbase, nf, total, st);
+    case 4: return batch_shape<4>(lv, ipt, bf, tbase, tmap, total, st);
+    case 8: return batch_shape<8>(lv, ipt, bf, tbase, tmap, total, st);
+    case 16: return batch_shape<16>(lv, ipt, bf, tbase, tmap, total, st);
+    case 32: return batch_shape<32>(lv, ipt, bf, tbase, tmap, total, st);
+    case 64: return batch_shape<64>(lv, ipt, bf, tbase, tmap, total, st);
+    case 128: return batch_shape<128>(lv, ipt, bf, tbase, tmap, total, st);
+    case 256: return batch_shape<256>(lv, ipt, bf, tbase, tmap, total, st);
     default: return -1;
     }
 }
