@@ -1170,6 +1170,7 @@ CompressLayout compress_layout(uint64_t n, int w, int bs)
     L.ws_chunks = off;  off += ceil_div(L.nidx, LB_CHUNK) * sizeof(ChunkState);
     L.ws_cacc = off;    off += ceil_div(L.nidx, LB_CHUNK) * sizeof(unsigned long long);
     L.ws_misc = off;    off += 64 * sizeof(uint64_t);   // totals at +128 B, eb at +256 B, error at +384 B
+    off = (off + 255) & ~(uint64_t)255;   // the range partials are read as 16-byte vectors
     L.ws_parts = off;   off += 2 * kRangeParts * sizeof(double);
     L.ws_bitmap = off;  off += (L.nbit + 15) & ~(uint64_t)15;
     L.ws_cmed = off;    off += (4 * L.nb + 15) & ~(uint64_t)15;
