@@ -673,17 +673,18 @@ __device__ __forceinline__ void copy_out(const CompressParams &p, uint64_t tile,
 // no barrier is needed.  The loads are issued before the tile's and the
 // fold follows them in the same block, so it waits for its own loads only.
 template <typename T> struct RelParts {
-    T mn[4], mx[4];
+    static constexpr int PL = RANGE_LB_PARTS / 64;   // partials per lane
+    static_assert(RANGE_LB_PARTS % 128 == 0, "whole 16-byte loads per lane");
+    T mn[PL], mx[PL];
     T first;
     __device__ void load(const CompressParams &p)
     {
-        static_assert(RANGE_LB_PARTS == 4 * 64, "four range partials per lane");
-        const T *q = static_cast<const T *>(p.rparts) + 8 * (threadIdx.x & 63);
-        T t[8];
-        SlabLoad<T>::load4_full(q, t);
-        SlabLoad<T>::load4_full(q + 4, t + 4);
+        const T *q = static_cast<const T *>(p.rparts) + 2 * PL * (threadIdx.x & 63);
+        T t[2 * PL];
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
+        for (int j = 0; j < 2 * PL; j += 4) SlabLoad<T>::load4_full(q + j, t + j);
+#pragma unroll
+        for (int k = 0; k < PL; k++) {
             mn[k] = t[2 * k];
             mx[k] = t[2 * k + 1];
         }
@@ -698,7 +699,7 @@ template <typename T> struct RelParts {
         using U = typename Word<T>::U;
         T a = mn[0], b = mx[0];
 #pragma unroll
-        for (int k = 1; k < 4; k++) {
+        for (int k = 1; k < PL; k++) {
             a = mn[k] < a ? mn[k] : a;
             b = mx[k] > b ? mx[k] : b;
         }
