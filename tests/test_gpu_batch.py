@@ -197,3 +197,37 @@ def test_batch_without_decode_buffers(device):
     check_batch(bc, xs, ebs, oracle_of(xs, ebs), "no decode")
     with pytest.raises(Z.SZxError):
         bc.decompress()
+
+
+def test_batch_decode_rejects_damaged_index(batch):
+    """A field whose record index points past its stream is reported (-20)
+    by the batched decode's per-tile setup; the other fields still decode."""
+    import torch
+    bc, xs, ebs, want = batch
+    bc.set_shape(8, 4)
+    bc.compress()
+    torch.cuda.synchronize()
+    victim = next(i for i, x in enumerate(xs) if x.size >= 3 * 4096)
+    ix = Z.lib().szx_amd_batch_index_dev(bc.handle, victim)
+    base = ix - bc.ws.data_ptr()   # the index lives in the plan's workspace
+    assert 0 <= base and base % 16 == 0
+    # index pair 1: the record offset (and constants before) of the second tile
+    pair = bc.ws[base + 16:base + 32].view(torch.int64)
+    saved = pair.clone()
+    pair.copy_(torch.tensor([1 << 40, 0], dtype=torch.int64, device=pair.device))
+    say("damaged index of field", victim)
+    for d in bc.decs:
+        d.zero_()
+    bc.decompress()
+    torch.cuda.synchronize()
+    st = bc.status().cpu().numpy()
+    assert st[victim] == -20, st[victim]
+    for i in range(len(xs)):
+        if i != victim:
+            assert st[i] == 0, (i, st[i])
+            assert bits_equal(bc.decs[i].cpu().numpy(), want[i][1]), i
+    pair.copy_(saved)
+    bc.decompress()
+    torch.cuda.synchronize()
+    assert bc.status().cpu().numpy()[victim] == 0
+    assert bits_equal(bc.decs[victim].cpu().numpy(), want[victim][1])
